@@ -1,0 +1,210 @@
+"""Benchmark of the assembly / operator-apply path on one MI355X.
+
+Workload (the headline configuration): Q2 triquadratic hexahedra on the unit
+cube, 64^3 elements (2,146,689 dofs), 3x3x3 Gauss, lognormal kappa = exp(g),
+g a Gaussian random field (std 1, correlation length 0.1, seed 2) at the dofs,
+homogeneous Dirichlet on all sides.  One step = one Builder pass:
+clear + buildLhs(AssemblyStiffnessFE(kappa)) + closeMatrix, i.e. geometry,
+element matrices, Dirichlet elimination, deterministic CSR scatter and prune.
+Reported beside it: CSR SpMV bandwidth on the assembled matrix, per-kernel
+times and the roofline of the dominant kernel, and the oracle's serial CPU
+assembly rate on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+
+def q2hex_flops_per_elem(nq=27, n=27, dim=3, g=27, kappa=True):
+    """Reference-formulation flops of one Q2-hex kappa-stiffness element:
+    geometry 2*g*3*dim + 6 dim^2 + 60 + 6 dim^2, gradients 6 n dim,
+    the n x n outer products 6 n^2 + 3 n, kappa interpolation 2 n (per q)."""
+    per_q = 2 * g * 3 * dim + 6 * dim * dim + 60 + 6 * dim * dim + 6 * n * dim + 6 * n * n + 3 * n
+    if kappa:
+        per_q += 2 * n
+    return nq * per_q
+
+
+def element_kernel_flops(nq=27, n=27, dim=3):
+    """Flops of the element-matrix contraction alone (what the element kernel
+    computes from the per-point metric): per q, u = M g_i (2*dim^2 per row) and
+    n dot products of length dim per row, accumulated (2*dim flops each)."""
+    return nq * n * (2 * dim * dim + 2 * dim * n)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=64, help="elements per axis")
+    ap.add_argument("--spmv", type=int, default=100, help="SpMV applications timed")
+    ap.add_argument("--cpu-sample", type=int, default=12, help="elements per axis of the CPU sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    import proxpde_amd.native as N
+    from proxpde_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    n = args.n
+    t0 = time.time()
+    prob = W.Problem("hex_q2", (n, n, n))
+    xd = prob.dof_coords()
+    kappa = np.exp(W.grf(xd, seed=2, std=1.0, ell=0.1, features=1024, device=dev))
+    dofs = prob.all_sides()
+    a = prob.assembler()
+    a.set_dirichlet(dofs, np.zeros(len(dofs)))
+    a.set_coefficient(kappa)
+    t_setup_host = time.time() - t0
+    torch.cuda.synchronize()
+    t1 = time.time()
+    nnz_s = a.build_pattern()
+    t_pattern = time.time() - t1
+    E = prob.mesh.nelem
+    terms = [("stiffness_fe", 1.0)]
+
+    def step():
+        a.clear()
+        a.assemble(terms)
+        a.close(1e-16)
+
+    for _ in range(args.warmup):
+        step()
+    a.profile(True)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - ts
+    ms_step = 1e3 * elapsed / args.steps
+    prof = {k: a.profile_get(k) for k in ("geometry", "element", "scatter", "rhs_scatter", "close")}
+    a.profile(False)
+    nnz = a.nnz
+
+    # --- SpMV on the assembled matrix (torch stream so torch events time it)
+    stream = torch.cuda.current_stream()
+    a.set_stream(stream.cuda_stream)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    x = torch.rand(prob.ndofs, dtype=torch.float64, device=dev, generator=gen) * 2 - 1
+    y = torch.empty_like(x)
+    for _ in range(5):
+        a.spmv_dev(x.data_ptr(), y.data_ptr())
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.spmv):
+        a.spmv_dev(x.data_ptr(), y.data_ptr())
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    spmv_ms = ev0.elapsed_time(ev1) / args.spmv
+    spmv_bytes = nnz * 12 + 8 * (prob.ndofs + 1) + 16 * prob.ndofs
+    spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    x32 = x.float()
+    y32 = torch.empty_like(x32)
+    for _ in range(5):
+        a.spmv_dev(x32.data_ptr(), y32.data_ptr(), precision=N.FP32)
+    ev0.record(stream)
+    for _ in range(args.spmv):
+        a.spmv_dev(x32.data_ptr(), y32.data_ptr(), precision=N.FP32)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    spmv32_ms = ev0.elapsed_time(ev1) / args.spmv
+    spmv32_gbs = (nnz * 8 + 8 * (prob.ndofs + 1) + 8 * prob.ndofs) / (spmv32_ms * 1e-3) / 1e9
+    a.set_stream(None)
+
+    # --- roofline of the dominant kernel
+    per = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in prof.items()}
+    dom = max(per, key=per.get)
+    if dom == "element":
+        fl = element_kernel_flops() * E
+        achieved = fl / (per["element"] * 1e-3) / 1e12
+        roof = {"kernel": "k_element_rows<double,27,3>", "bound": "mfma", "achieved": achieved,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                "traffic": None}
+    elif dom == "scatter":
+        by = E * 27 * 27 * (8 + 4) + nnz_s * (4 + 8 + 8)
+        achieved = by / (per["scatter"] * 1e-3) / 1e9
+        roof = {"kernel": "k_scatter<double>", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    else:
+        roof = {"kernel": dom, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": None, "traffic": None}
+
+    # --- CPU baseline: the oracle's serial Builder on a bounded sample
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        m = args.cpu_sample
+        s = O.Space("hex_q2", "cube_hex", (m, m, m))
+        ps = W.Problem("hex_q2", (m, m, m))
+        kap = np.exp(W.grf(ps.dof_coords(), seed=2, std=1.0, ell=0.1, features=1024))
+        flag = np.zeros(s.ndofs, np.uint8)
+        flag[ps.all_sides()] = 1
+        tc = time.perf_counter()
+        reps = 0
+        while True:
+            s.assemble(terms, kappa=kap, bc=(flag, np.zeros(s.ndofs)))
+            reps += 1
+            if time.perf_counter() - tc > 10.0:
+                break
+        tcpu = (time.perf_counter() - tc) / reps
+        cpu = {"value": (m ** 3) / tcpu, "unit": "elements/s", "cores": 1, "kind": "port",
+               "sample": f"oracle serial Builder (buildLhs+closeMatrix), Q2 hex {m}^3 = {m ** 3} elements, "
+                         f"same kappa recipe, {reps} passes, {tcpu:.3f} s/pass"}
+
+    value = E / (ms_step * 1e-3)
+    out = {
+        "metric": "elements assembled/s (Q2 hex 64^3, kappa stiffness, buildLhs+closeMatrix)",
+        "value": value,
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"Q2 hex {n}^3 unit cube, 3x3x3 Gauss, lognormal kappa (GRF std 1, ell 0.1, seed 2), "
+                               "all-sides Dirichlet", "elements": E, "dofs": prob.ndofs,
+                   "nnz_structural": nnz_s, "nnz": nnz, "parallelism": "single GPU"},
+        "kernels_ms": {k: per[k] for k in per},
+        "pattern_build_s": t_pattern,
+        "host_setup_s": t_setup_host,
+        "spmv": {"ms": spmv_ms, "GB/s": spmv_gbs, "frac_hbm": spmv_gbs / HBM_PEAK_GBS,
+                 "fp32_ms": spmv32_ms, "fp32_GB/s": spmv32_gbs, "bytes": spmv_bytes},
+        "reference_flops_per_elem": q2hex_flops_per_elem(),
+        "reference_gflops_per_s": q2hex_flops_per_elem() * value / 1e9,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,191 @@
+/*
+ * proxpde_amd: MI355X (gfx950) element assembly and operator-apply library.
+ *
+ * C ABI.  Plain pointers and sizes; no exceptions cross it; every call returns
+ * PXG_OK (0) or a negative status, and pxg_last_error() explains the last one.
+ * A context owns its device memory and one HIP stream; it is externally
+ * synchronised (one host thread at a time per context).
+ *
+ * The assembly entry points replace the CPU path of the reference's
+ * proxpde::Builder (include/builder.hpp:12-391):
+ *   Builder(uint size) / init(size)          -> pxg_create + pxg_set_dofmap
+ *   FESpace{mesh} / CurFE<RefFE, QR>         -> pxg_set_fe + pxg_set_mesh
+ *   BCEss{feSpace, side, fun}                -> pxg_set_dirichlet
+ *   buildLhs(tuple{AssemblyStiffness, ...})  -> pxg_assemble
+ *   buildRhs(tuple{AssemblyRhsAnalytic})     -> pxg_assemble_rhs
+ *   closeMatrix()                            -> pxg_close
+ *   A (row-major Eigen::SparseMatrix), b     -> pxg_get_csr
+ * Host-side mesh and dof-map construction (the mirror of the reference's
+ * buildHyperCube / DOF::setupElemMap / BCEss::setupDofMap) is declared in
+ * proxpde_amd_host.h.
+ */
+#ifndef PROXPDE_AMD_H
+#define PROXPDE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PXG_OK 0
+#define PXG_E_INVALID (-1) /* bad argument or shape                         */
+#define PXG_E_OOM (-2)     /* device or host allocation failed              */
+#define PXG_E_HIP (-3)     /* HIP runtime error                             */
+#define PXG_E_NUMERIC (-4) /* non-positive Jacobian, CG breakdown, NaN      */
+#define PXG_E_STATE (-5)   /* call out of order (e.g. assemble before mesh) */
+#define PXG_E_NOCONV (-6)  /* iterative solve hit its iteration limit       */
+
+/* ---- finite elements (reference include/fe.hpp:30-132) ----------------- */
+#define PXG_FE_LINE_P1 1 /* RefLineP1,        GaussQR<Line,2>          */
+#define PXG_FE_QUAD_Q1 2 /* RefQuadQ1,        GaussQR<Quad,4>          */
+#define PXG_FE_QUAD_Q2 3 /* RefQuadQ2,        GaussQR<Quad,9>          */
+#define PXG_FE_HEX_Q1 4  /* RefHexahedronQ1,  GaussQR<Hexahedron,8>    */
+#define PXG_FE_HEX_Q2 5  /* RefHexahedronQ2,  GaussQR<Hexahedron,27>   */
+#define PXG_FE_TET_P1 6  /* RefTetrahedronP1, GaussQR<Tetrahedron,4>   */
+#define PXG_FE_TET_P2 7  /* RefTetrahedronP2, GaussQR<Tetrahedron,11>  */
+
+/* ---- assembly terms (reference include/assembly_lhs.hpp) --------------- */
+#define PXG_TERM_STIFFNESS 1    /* AssemblyStiffness(coef)          :36-75   */
+#define PXG_TERM_STIFFNESS_FE 2 /* AssemblyStiffnessFE(kappa)       :77-113  */
+#define PXG_TERM_MASS 3         /* AssemblyMass / ScalarMass(coef)  :191-231 */
+#define PXG_TERM_MASS_FE 4      /* AssemblyMassFE(coef, kappa)      :233-283 */
+
+/* ---- analytic right-hand sides (AssemblyRhsAnalytic, assembly_rhs.hpp:34-91)
+ * The reference takes a host std::function; the closed forms used by its
+ * tests and by the workloads are evaluated on the device, any other function
+ * goes through PXG_RHS_QVALUES with values at pxg_get_qpoints() points.    */
+#define PXG_RHS_NONE 0
+#define PXG_RHS_CONST 1      /* p0                                      */
+#define PXG_RHS_SINSIN 2     /* p0 sin(p1 x) sin(p2 y)                  */
+#define PXG_RHS_SINSINSIN 3  /* p0 sin(p1 x) sin(p2 y) sin(p3 z)        */
+#define PXG_RHS_FEVAR1D 4    /* sin(p1 x) - p0 cos(p2 x)                */
+#define PXG_RHS_QVALUES 5    /* host values per (element, qpoint)       */
+
+#define PXG_FP64 0
+#define PXG_FP32 1
+
+typedef struct pxg_ctx pxg_ctx;
+
+typedef struct pxg_term
+{
+  int kind;    /* PXG_TERM_*                              */
+  double coef; /* scalar coefficient (FE terms: multiplier) */
+} pxg_term;
+
+/* context ----------------------------------------------------------------- */
+int pxg_create(pxg_ctx ** ctx, int device);
+int pxg_destroy(pxg_ctx * ctx);
+const char * pxg_last_error(const pxg_ctx * ctx);
+/* run the context's work on an existing HIP stream (NULL: own stream) */
+int pxg_set_stream(pxg_ctx * ctx, void * hip_stream);
+int pxg_synchronize(pxg_ctx * ctx);
+/* per-kernel timing with HIP events on the context stream: enable resets the
+ * totals; names: "geometry", "element", "element32", "scatter", "scatter32",
+ * "rhs_scatter", "close", "mf_apply", "mf_gradk", "spmv", "jacobi"            */
+int pxg_profile(pxg_ctx * ctx, int enable);
+int pxg_profile_get(pxg_ctx * ctx, const char * name, double * total_ms, int64_t * count);
+
+/* FESpace / CurFE: FE type and quadrature rule.  nq = 0 selects the
+ * reference's RecommendedQR; otherwise nodes (nq x dim) and weights (nq).
+ * Replaces CurFE<RefFE,QR>::CurFE (include/curfe.hpp:20-45).             */
+int pxg_set_fe(pxg_ctx * ctx, int fe_type, int nq, const double * qr_nodes, const double * qr_weights);
+
+/* mesh: point coordinates (n_pts x 3) and element-to-vertex connectivity
+ * (n_elem x npe, reference vertex order, include/geo.hpp).                 */
+int pxg_set_mesh(pxg_ctx * ctx, int64_t n_pts, const double * coords, int64_t n_elem, int npe,
+                 const uint32_t * conn);
+
+/* DOF::elemMap (include/dof.hpp:73-76): n_elem x n_local dof ids          */
+int pxg_set_dofmap(pxg_ctx * ctx, int64_t n_dofs, const uint32_t * elem_map);
+
+/* BCEss (include/bc.hpp:31-309): constrained dofs and their values.
+ * Several BCEss merge by later-wins; diag = 1 as BCEss::diag.             */
+int pxg_set_dirichlet(pxg_ctx * ctx, int64_t n_c, const uint32_t * dofs, const double * values);
+
+/* nodal coefficient field kappa (FEVar on the same space) for FE terms    */
+int pxg_set_coefficient(pxg_ctx * ctx, const double * kappa_host);
+
+/* CSR sparsity pattern and element->nnz map from the dof map (once per mesh):
+ * sorted segmented reduction of the E*n*n (row, col) keys.                */
+int pxg_build_pattern(pxg_ctx * ctx, int64_t * nnz_structural);
+
+/* Builder::buildLhs (include/builder.hpp:31-154): element matrices of the
+ * given terms, per-element Dirichlet elimination, deterministic scatter into
+ * the structural CSR values and the lifting into b.  Accumulates, as
+ * successive buildLhs calls accumulate triplets.  precision: PXG_FP64 or
+ * PXG_FP32 (element kernel and values in float).                          */
+int pxg_assemble(pxg_ctx * ctx, int n_terms, const pxg_term * terms, int precision);
+
+/* Builder::buildRhs with AssemblyRhsAnalytic (include/builder.hpp:279-364) */
+int pxg_assemble_rhs(pxg_ctx * ctx, int rhs_kind, const double * params, const double * qvalues);
+
+/* quadrature points x_q of every element (n_elem x nq x 3), for host
+ * evaluation of arbitrary right-hand sides (CurFE::qpoint, curfe.hpp:80)  */
+int pxg_get_qpoints(pxg_ctx * ctx, double * out);
+
+/* Builder::closeMatrix (include/builder.hpp:366-371): drop |v| <= eps      */
+int pxg_close(pxg_ctx * ctx, double prune_eps, int64_t * nnz);
+
+/* Builder::clear (include/builder.hpp:381-385): zero values and b          */
+int pxg_clear(pxg_ctx * ctx);
+
+/* closed matrix (row-major CSR, int32 column indices like Eigen's default
+ * StorageIndex) and b; host buffers, synchronous.  Any pointer may be NULL. */
+int pxg_get_csr(pxg_ctx * ctx, int64_t * rowptr, int32_t * col, double * val, float * val32, double * b);
+
+/* structural pattern and element->nnz map (n_elem x n x n)                 */
+int pxg_get_pattern(pxg_ctx * ctx, int64_t * rowptr, int32_t * col, uint32_t * emap);
+
+/* per element and quadrature point geometry of CurFE::reinit
+ * (include/curfe.hpp:50-103): detJ, JxW (n_elem x nq each)                 */
+int pxg_get_geometry(pxg_ctx * ctx, double * detJ, double * JxW);
+
+/* ---- operator apply (device pointers, the context's stream) ------------- */
+/* y = A x with the closed matrix (SparseMatrix<RowMajor> * Vec)            */
+int pxg_spmv(pxg_ctx * ctx, int precision, const void * x_dev, void * y_dev);
+/* sweeps of x <- x + omega D^-1 (b - A x)                                  */
+int pxg_jacobi(pxg_ctx * ctx, int precision, double omega, int sweeps, const void * b_dev, void * x_dev);
+/* device pointers to the closed CSR arrays (valid until the next close)    */
+int pxg_csr_device(pxg_ctx * ctx, const int64_t ** rowptr, const int32_t ** col, const double ** val,
+                   const float ** val32, int64_t * nnz);
+/* device pointer to b (n_dofs doubles)                                     */
+int pxg_rhs_device(pxg_ctx * ctx, double ** b_dev);
+
+/* ---- matrix-free y = K(kappa) x and its gradients ------------------------
+ * nb vectors interleaved as [n_dofs][nb]; use_bc applies the Dirichlet form
+ * C K C + D_c of the assembled matrix.  Geometry is recomputed per call.     */
+int pxg_mf_apply(pxg_ctx * ctx, const double * kappa_dev, int nb, const double * x_dev, double * y_dev,
+                 int use_bc);
+/* gk[n] = sum_b d/dkappa_n ( ybar_b . K(kappa) x_b )                         */
+int pxg_mf_grad_kappa(pxg_ctx * ctx, int nb, const double * x_dev, const double * ybar_dev, double * gk_dev,
+                      int use_bc);
+
+/* ---- geometric multigrid-preconditioned CG ------------------------------ */
+typedef struct pxg_mg pxg_mg;
+/* levels: closed contexts (finest first) on structured hex meshes whose
+ * element counts halve per level; dims: n_lev x 3 element counts.
+ * Weighted Jacobi (omega, nu1 pre / nu2 post sweeps), R = P^T, coarsest
+ * level solved by Jacobi-CG to coarse_rtol in FP64.                       */
+int pxg_mg_create(pxg_mg ** mg, int n_lev, pxg_ctx ** levels, const int * dims, double omega, int nu1, int nu2,
+                  double coarse_rtol);
+int pxg_mg_destroy(pxg_mg * mg);
+/* refresh level matrices after a re-assembly (same patterns)               */
+int pxg_mg_update(pxg_mg * mg);
+/* solve A x = b (device pointers, FP64); mode PXG_FP64: MG-PCG in FP64;
+ * PXG_FP32: FP32 MG-PCG inner solves inside FP64 iterative refinement
+ * (FP64 residual b - A x, FP64 CG inner products, FP64 coarsest solve).   */
+int pxg_mg_solve(pxg_mg * mg, int mode, const double * b_dev, double * x_dev, double rtol, int maxit,
+                 int * iters, double * relres);
+/* one V-cycle x = M^{-1} b (FP64), for testing                            */
+int pxg_mg_vcycle(pxg_mg * mg, const double * b_dev, double * x_dev);
+/* prolongation P_l (level l+1 -> l) as host CSR; returns nnz via *nnz     */
+int pxg_mg_prolongation(pxg_mg * mg, int level, int64_t * rowptr, int32_t * col, double * val, int64_t * nnz);
+const char * pxg_mg_last_error(const pxg_mg * mg);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PROXPDE_AMD_H */

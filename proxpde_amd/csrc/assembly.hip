@@ -1,0 +1,900 @@
+// Element assembly on gfx950: geometry, element matrices with Dirichlet
+// elimination, CSR pattern / element->nnz map, deterministic scatter, prune.
+//
+// Replaces the serial element loop of Builder::buildLhs / buildRhs /
+// closeMatrix (reference include/builder.hpp:31-154, :279-371) and
+// CurFE::reinit (include/curfe.hpp:50-103).
+//
+// Data layout in HBM (all row-major, element-major):
+//   xyz      [npts][3]   f64     conn   [E][npe]   u32
+//   elemMap  [E][n]      u32     kappa  [ndofs]    f64
+//   geoBuf   [E][nq][8]  f64     per quadrature point: s*(J^T J)^{-1} (6 sym.
+//                                entries), mass weight, JxW
+//   keBuf    [E][n][n]   T       element matrices after Dirichlet elimination
+//   feBuf    [E][n]      f64     element right-hand sides / liftings
+//   contrib  [E*n*n]     u32     element-matrix entry ids sorted by (row, col, elem)
+//   segPtr   [nnzS+1]    u32     segment of each structural nonzero in contrib
+//   emapNnz  [E*n*n]     u32     element-matrix entry -> structural nonzero
+#include "ctx.hpp"
+#include "kernels.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+namespace pxg
+{
+
+namespace
+{
+constexpr int kBlock = 256;
+
+inline unsigned gridFor(int64_t work, int block = kBlock)
+{
+  int64_t g = (work + block - 1) / block;
+  return (unsigned)std::max<int64_t>(g, 1);
+}
+
+enum : int
+{
+  kHasStiff = 1,
+  kHasStiffFE = 2,
+  kHasMass = 4,
+  kHasMassFE = 8,
+};
+
+// --------------------------------------------------------------------------
+// geometry: one thread per (element, quadrature point)
+// J = sum_v x_v (x) grad psi_v(xi_q); A = J^T J; detJ = sqrt(det A);
+// JxW = detJ w_q; kappa_q = sum_i phi_i(xi_q) kappa_i;
+// out: s (J^T J)^{-1} with s the stiffness weight, mass weight, JxW.
+template <int DIM, int NPE>
+__global__ void __launch_bounds__(kBlock)
+    k_geometry(int64_t E, int nq, int n, double const * __restrict__ xyz, uint32_t const * __restrict__ conn,
+               double const * __restrict__ geoGrad, double const * __restrict__ w, double const * __restrict__ phi,
+               uint32_t const * __restrict__ emap, double const * __restrict__ kappa, double cStiff,
+               double cStiffFE, double cMass, double cMassFE, int flags, double * __restrict__ geo,
+               int * __restrict__ status)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= E * nq)
+    return;
+  int64_t const e = t / nq;
+  int const q = (int)(t - e * nq);
+  double J[3][DIM];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < DIM; ++c)
+      J[r][c] = 0.;
+#pragma unroll
+  for (int v = 0; v < NPE; ++v)
+  {
+    uint32_t const pid = conn[e * NPE + v];
+    double const x0 = xyz[3ull * pid], x1 = xyz[3ull * pid + 1], x2 = xyz[3ull * pid + 2];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c)
+    {
+      double const g = geoGrad[(q * NPE + v) * DIM + c];
+      J[0][c] += x0 * g;
+      J[1][c] += x1 * g;
+      J[2][c] += x2 * g;
+    }
+  }
+  double A[DIM][DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a)
+#pragma unroll
+    for (int b = 0; b < DIM; ++b)
+      A[a][b] = J[0][a] * J[0][b] + J[1][a] * J[1][b] + J[2][a] * J[2][b];
+  double det, I00 = 0., I01 = 0., I02 = 0., I11 = 0., I12 = 0., I22 = 0.;
+  if constexpr (DIM == 1)
+  {
+    det = A[0][0];
+    I00 = 1. / det;
+  }
+  else if constexpr (DIM == 2)
+  {
+    det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+    I00 = A[1][1] / det;
+    I01 = -A[0][1] / det;
+    I11 = A[0][0] / det;
+  }
+  else
+  {
+    double const c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+    double const c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+    double const c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+    det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+    I00 = c00 / det;
+    I01 = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+    I02 = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+    I11 = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+    I12 = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;
+    I22 = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
+  }
+  if (!(det > 0.))
+    atomicOr(status, 1);
+  double const JxW = sqrt(det) * w[q];
+  double kq = 0.;
+  if (flags & (kHasStiffFE | kHasMassFE))
+    for (int i = 0; i < n; ++i)
+      kq += phi[q * n + i] * kappa[emap[e * n + i]];
+  double s = 0., m = 0.;
+  if (flags & kHasStiff)
+    s += cStiff * JxW;
+  if (flags & kHasStiffFE)
+    s += (kq * JxW) * cStiffFE;
+  if (flags & kHasMass)
+    m += cMass * JxW;
+  if (flags & kHasMassFE)
+    m += (cMassFE * kq) * JxW;
+  double * o = geo + t * 8;
+  o[0] = s * I00;
+  o[1] = s * I01;
+  o[2] = s * I02;
+  o[3] = s * I11;
+  o[4] = s * I12;
+  o[5] = s * I22;
+  o[6] = m;
+  o[7] = JxW;
+}
+
+// --------------------------------------------------------------------------
+// element matrices: one thread per (element, local row i).
+// Ke_ij = sum_q grad_i(q)^T M_q grad_j(q) + m_q phi_i(q) phi_j(q), with the
+// reference-element tables staged in LDS.  Then Dirichlet elimination
+// (include/builder.hpp:83-124): constrained rows/columns zeroed, diagonal 1,
+// Fe_i = -sum_j Ke_ij h_j on free rows, Fe_i = h_i on constrained rows.
+template <typename T, int N, int DIM>
+__global__ void __launch_bounds__(kBlock)
+    k_element_rows(int64_t E, int nq, double const * __restrict__ geo, T const * __restrict__ gradTab,
+                   T const * __restrict__ phiTab, int hasMass, uint32_t const * __restrict__ emap,
+                   uint8_t const * __restrict__ bcFlag, double const * __restrict__ bcVal, int hasBC,
+                   T * __restrict__ ke, double * __restrict__ fe)
+{
+  extern __shared__ unsigned char smem_raw[];
+  T * sGrad = reinterpret_cast<T *>(smem_raw);
+  T * sPhi = sGrad + nq * N * DIM;
+  for (int k = threadIdx.x; k < nq * N * DIM; k += blockDim.x)
+    sGrad[k] = gradTab[k];
+  for (int k = threadIdx.x; k < nq * N; k += blockDim.x)
+    sPhi[k] = phiTab[k];
+  __syncthreads();
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= E * N)
+    return;
+  int64_t const e = t / N;
+  int const i = (int)(t - e * N);
+  T acc[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+    acc[j] = T(0);
+  for (int q = 0; q < nq; ++q)
+  {
+    double const * G = geo + ((e * nq + q) << 3);
+    T const M00 = (T)G[0], M01 = (T)G[1], M02 = (T)G[2], M11 = (T)G[3], M12 = (T)G[4], M22 = (T)G[5];
+    T const * gq = sGrad + q * N * DIM;
+    T u0, u1 = T(0), u2 = T(0);
+    if constexpr (DIM == 1)
+      u0 = M00 * gq[i];
+    else if constexpr (DIM == 2)
+    {
+      T const g0 = gq[i * 2], g1 = gq[i * 2 + 1];
+      u0 = M00 * g0 + M01 * g1;
+      u1 = M01 * g0 + M11 * g1;
+    }
+    else
+    {
+      T const g0 = gq[i * 3], g1 = gq[i * 3 + 1], g2 = gq[i * 3 + 2];
+      u0 = M00 * g0 + M01 * g1 + M02 * g2;
+      u1 = M01 * g0 + M11 * g1 + M12 * g2;
+      u2 = M02 * g0 + M12 * g1 + M22 * g2;
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+    {
+      T v = u0 * gq[j * DIM];
+      if constexpr (DIM > 1)
+        v += u1 * gq[j * DIM + 1];
+      if constexpr (DIM > 2)
+        v += u2 * gq[j * DIM + 2];
+      acc[j] += v;
+    }
+    if (hasMass)
+    {
+      T const pm = (T)G[6] * sPhi[q * N + i];
+#pragma unroll
+      for (int j = 0; j < N; ++j)
+        acc[j] += pm * sPhi[q * N + j];
+    }
+  }
+  double f = 0.;
+  if (hasBC)
+  {
+    uint32_t const * ids = emap + e * N;
+    bool const ci = bcFlag[ids[i]] != 0;
+    if (ci)
+    {
+      f = bcVal[ids[i]];
+#pragma unroll
+      for (int j = 0; j < N; ++j)
+        acc[j] = (j == i) ? T(1) : T(0);
+    }
+    else
+    {
+      double s = 0.;
+#pragma unroll
+      for (int j = 0; j < N; ++j)
+        if (bcFlag[ids[j]])
+        {
+          s += (double)acc[j] * bcVal[ids[j]];
+          acc[j] = T(0);
+        }
+      f = 0. - s;
+    }
+  }
+  T * out = ke + t * N;
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+    out[j] = acc[j];
+  fe[t] = f;
+}
+
+// --------------------------------------------------------------------------
+// pattern construction: keys row*ndofs+col of every element-matrix entry
+__global__ void k_entry_keys(int64_t E, int n, int64_t ndofs, uint32_t const * __restrict__ emap,
+                             uint64_t * __restrict__ keys, uint32_t * __restrict__ ids)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t const nn = (int64_t)n * n;
+  if (t >= E * nn)
+    return;
+  int64_t const e = t / nn;
+  int const r = (int)(t - e * nn);
+  int const i = r / n, j = r - (r / n) * n;
+  keys[t] = (uint64_t)emap[e * n + i] * (uint64_t)ndofs + emap[e * n + j];
+  ids[t] = (uint32_t)t;
+}
+
+__global__ void k_head_flags(int64_t m, uint64_t const * __restrict__ keys, uint32_t * __restrict__ flags)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m)
+    return;
+  flags[t] = (t == 0 || keys[t] != keys[t - 1]) ? 1u : 0u;
+}
+
+// after an inclusive scan of the head flags: nnz id of every sorted entry
+__global__ void k_pattern_fill(int64_t m, int64_t ndofs, uint64_t const * __restrict__ keys,
+                               uint32_t const * __restrict__ scan, uint32_t const * __restrict__ sortedIds,
+                               uint32_t * __restrict__ segPtr, int32_t * __restrict__ colS,
+                               int64_t * __restrict__ rowptrS, uint32_t * __restrict__ emapNnz)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m)
+    return;
+  uint32_t const id = scan[t] - 1u;
+  emapNnz[sortedIds[t]] = id;
+  uint64_t const k = keys[t];
+  if (t == 0 || k != keys[t - 1])
+  {
+    segPtr[id] = (uint32_t)t;
+    uint64_t const row = k / (uint64_t)ndofs;
+    colS[id] = (int32_t)(k - row * (uint64_t)ndofs);
+    if (t == 0 || keys[t - 1] / (uint64_t)ndofs != row)
+      rowptrS[row] = id;
+  }
+}
+
+__global__ void k_dof_keys(int64_t m, uint32_t const * __restrict__ emap, uint32_t * __restrict__ keys,
+                           uint32_t * __restrict__ ids)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m)
+    return;
+  keys[t] = emap[t];
+  ids[t] = (uint32_t)t;
+}
+
+__global__ void k_dof_seg(int64_t m, uint32_t const * __restrict__ keys, uint32_t * __restrict__ dofSeg)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m)
+    return;
+  if (t == 0 || keys[t] != keys[t - 1])
+    dofSeg[keys[t]] = (uint32_t)t;
+}
+
+// --------------------------------------------------------------------------
+// deterministic scatter: each structural nonzero sums its contributions in
+// element order onto the value it already holds (successive buildLhs calls
+// accumulate like the reference's triplet list; no float atomics)
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_scatter(int64_t nnzS, uint32_t const * __restrict__ segPtr, uint32_t const * __restrict__ contrib,
+              T const * __restrict__ ke, T * __restrict__ val)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnzS)
+    return;
+  T acc = val[t];
+  uint32_t const k1 = segPtr[t + 1];
+  for (uint32_t k = segPtr[t]; k < k1; ++k)
+    acc += ke[contrib[k]];
+  val[t] = acc;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_rhs_scatter(int64_t ndofs, uint32_t const * __restrict__ dofSeg, uint32_t const * __restrict__ dofContrib,
+                  double const * __restrict__ fe, double * __restrict__ b)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ndofs)
+    return;
+  double acc = b[t];
+  uint32_t const k1 = dofSeg[t + 1];
+  for (uint32_t k = dofSeg[t]; k < k1; ++k)
+    acc += fe[dofContrib[k]];
+  b[t] = acc;
+}
+
+// --------------------------------------------------------------------------
+// closeMatrix: prune(1, eps) drops |v| <= eps
+template <typename T>
+__global__ void k_prune_count(int64_t nrows, int64_t const * __restrict__ rowptrS, T const * __restrict__ val,
+                              double eps, int64_t * __restrict__ cnt)
+{
+  int64_t const r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows)
+    return;
+  int64_t c = 0;
+  for (int64_t k = rowptrS[r]; k < rowptrS[r + 1]; ++k)
+    c += (fabs((double)val[k]) <= eps) ? 0 : 1;
+  cnt[r] = c;
+}
+
+template <typename T>
+__global__ void k_prune_compact(int64_t nrows, int64_t const * __restrict__ rowptrS, int32_t const * __restrict__ colS,
+                                T const * __restrict__ valS, double eps, int64_t const * __restrict__ rowptr,
+                                int32_t * __restrict__ col, T * __restrict__ val, T * __restrict__ dinv)
+{
+  int64_t const r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows)
+    return;
+  int64_t o = rowptr[r];
+  T d = T(0);
+  for (int64_t k = rowptrS[r]; k < rowptrS[r + 1]; ++k)
+  {
+    T const v = valS[k];
+    if (fabs((double)v) <= eps)
+      continue;
+    col[o] = colS[k];
+    val[o] = v;
+    if (colS[k] == (int32_t)r)
+      d = v;
+    ++o;
+  }
+  dinv[r] = d != T(0) ? T(1) / d : T(0);
+}
+
+template <typename Tout, typename Tin>
+__global__ void k_cast(int64_t m, Tin const * __restrict__ in, Tout * __restrict__ out)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m)
+    out[t] = (Tout)in[t];
+}
+
+// --------------------------------------------------------------------------
+// right-hand side: per (element, qpoint) x_q = origin + J xi_q (the affine
+// formula of include/curfe.hpp:80), JxW and f(x_q)
+__device__ inline double evalRhs(int kind, double const * p, double x, double y, double z)
+{
+  switch (kind)
+  {
+  case PXG_RHS_CONST: return p[0];
+  case PXG_RHS_SINSIN: return p[0] * sin(p[1] * x) * sin(p[2] * y);
+  case PXG_RHS_SINSINSIN: return p[0] * sin(p[1] * x) * sin(p[2] * y) * sin(p[3] * z);
+  case PXG_RHS_FEVAR1D: return sin(p[1] * x) - p[0] * cos(p[2] * x);
+  }
+  return 0.;
+}
+
+struct RhsParams
+{
+  double p[4];
+};
+
+template <int DIM, int NPE>
+__global__ void __launch_bounds__(kBlock)
+    k_qpoints(int64_t E, int nq, int geo, double const * __restrict__ xyz, uint32_t const * __restrict__ conn,
+              double const * __restrict__ geoGrad, double const * __restrict__ xi, double const * __restrict__ w,
+              int kind, RhsParams prm, double const * __restrict__ qvalues, double * __restrict__ qx,
+              double * __restrict__ wf)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= E * nq)
+    return;
+  int64_t const e = t / nq;
+  int const q = (int)(t - e * nq);
+  double J[3][DIM], org[3] = {0., 0., 0.}, x[NPE][3];
+#pragma unroll
+  for (int v = 0; v < NPE; ++v)
+  {
+    uint32_t const pid = conn[e * NPE + v];
+    x[v][0] = xyz[3ull * pid];
+    x[v][1] = xyz[3ull * pid + 1];
+    x[v][2] = xyz[3ull * pid + 2];
+  }
+  // element origin: midpoint for line/quad/hex, first vertex for tets
+  if (geo == kTet)
+  {
+    org[0] = x[0][0];
+    org[1] = x[0][1];
+    org[2] = x[0][2];
+  }
+  else
+  {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+      double s = x[0][k];
+#pragma unroll
+      for (int v = 1; v < NPE; ++v)
+        s += x[v][k];
+      org[k] = (1.0 / NPE) * s;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < DIM; ++c)
+    {
+      double s = 0.;
+#pragma unroll
+      for (int v = 0; v < NPE; ++v)
+        s += x[v][r] * geoGrad[(q * NPE + v) * DIM + c];
+      J[r][c] = s;
+    }
+  double A[DIM][DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a)
+#pragma unroll
+    for (int b = 0; b < DIM; ++b)
+      A[a][b] = J[0][a] * J[0][b] + J[1][a] * J[1][b] + J[2][a] * J[2][b];
+  double det;
+  if constexpr (DIM == 1)
+    det = A[0][0];
+  else if constexpr (DIM == 2)
+    det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+  else
+    det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) + A[0][1] * (A[1][2] * A[2][0] - A[1][0] * A[2][2]) +
+          A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+  double p[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+  {
+    double s = 0.;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c)
+      s += J[r][c] * xi[q * DIM + c];
+    p[r] = org[r] + s;
+  }
+  if (qx)
+  {
+    qx[3 * t] = p[0];
+    qx[3 * t + 1] = p[1];
+    qx[3 * t + 2] = p[2];
+  }
+  if (wf)
+  {
+    double const f = kind == PXG_RHS_QVALUES ? qvalues[t] : evalRhs(kind, prm.p, p[0], p[1], p[2]);
+    wf[2 * t] = sqrt(det) * w[q];
+    wf[2 * t + 1] = f;
+  }
+}
+
+// Fe_i = sum_q (JxW_q phi_i(q)) f_q, zeroed on constrained dofs
+__global__ void __launch_bounds__(kBlock)
+    k_rhs_rows(int64_t E, int n, int nq, double const * __restrict__ wf, double const * __restrict__ phi,
+               uint32_t const * __restrict__ emap, uint8_t const * __restrict__ bcFlag, int hasBC,
+               double * __restrict__ fe)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= E * n)
+    return;
+  int64_t const e = t / n;
+  int const i = (int)(t - e * n);
+  double s = 0.;
+  for (int q = 0; q < nq; ++q)
+    s += (wf[2 * (e * nq + q)] * phi[q * n + i]) * wf[2 * (e * nq + q) + 1];
+  if (hasBC && bcFlag[emap[t]])
+    s = 0.;
+  fe[t] = s;
+}
+
+__global__ void k_jxw(int64_t m, double const * __restrict__ geo, double * __restrict__ detJ,
+                      double * __restrict__ JxW, double const * __restrict__ w, int nq)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m)
+    return;
+  double const jw = geo[8 * t + 7];
+  JxW[t] = jw;
+  detJ[t] = jw / w[t % nq];
+}
+
+template <typename F>
+void dispatchGeo(int geo, F && f)
+{
+  switch (geo)
+  {
+  case kLine: f(std::integral_constant<int, 1>{}, std::integral_constant<int, 2>{}); break;
+  case kQuad: f(std::integral_constant<int, 2>{}, std::integral_constant<int, 4>{}); break;
+  case kHex: f(std::integral_constant<int, 3>{}, std::integral_constant<int, 8>{}); break;
+  case kTet: f(std::integral_constant<int, 3>{}, std::integral_constant<int, 4>{}); break;
+  default: throw Error(PXG_E_INVALID, "unknown geometry");
+  }
+}
+
+template <typename F>
+void dispatchFe(int feType, F && f)
+{
+  switch (feType)
+  {
+  case PXG_FE_LINE_P1: f(std::integral_constant<int, 2>{}, std::integral_constant<int, 1>{}); break;
+  case PXG_FE_QUAD_Q1: f(std::integral_constant<int, 4>{}, std::integral_constant<int, 2>{}); break;
+  case PXG_FE_QUAD_Q2: f(std::integral_constant<int, 9>{}, std::integral_constant<int, 2>{}); break;
+  case PXG_FE_HEX_Q1: f(std::integral_constant<int, 8>{}, std::integral_constant<int, 3>{}); break;
+  case PXG_FE_HEX_Q2: f(std::integral_constant<int, 27>{}, std::integral_constant<int, 3>{}); break;
+  case PXG_FE_TET_P1: f(std::integral_constant<int, 4>{}, std::integral_constant<int, 3>{}); break;
+  case PXG_FE_TET_P2: f(std::integral_constant<int, 10>{}, std::integral_constant<int, 3>{}); break;
+  default: throw Error(PXG_E_INVALID, "unknown FE type");
+  }
+}
+
+void checkStatus(pxg_ctx * c)
+{
+  int h = 0;
+  PXG_HIP(hipMemcpyAsync(&h, c->status.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  PXG_HIP(hipStreamSynchronize(c->stream));
+  PXG_REQUIRE(h == 0, PXG_E_NUMERIC, "non-positive Jacobian determinant in an element");
+}
+
+} // namespace
+
+// ===========================================================================
+void buildDofAdjacency(pxg_ctx * c)
+{
+  int const n = c->fe.n;
+  int64_t const E = c->nelem;
+  hipStream_t s = c->stream;
+  {
+    int64_t const mn = E * n;
+    DevBuf<uint32_t> keys, keysOut, ids;
+    keys.resize(mn);
+    keysOut.resize(mn);
+    ids.resize(mn);
+    c->dofContrib.resize(mn);
+    c->dofSeg.resize(c->ndofs + 1);
+    k_dof_keys<<<gridFor(mn), kBlock, 0, s>>>(mn, c->elemMap.p, keys.p, ids.p);
+    int eb = 1;
+    while (eb < 32 && ((uint64_t)1 << eb) < (uint64_t)c->ndofs)
+      ++eb;
+    size_t tb = 0;
+    PXG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys.p, keysOut.p, ids.p, c->dofContrib.p, (int)mn, 0,
+                                               eb, s));
+    c->sortTmp.resize(std::max(tb, c->sortTmp.n));
+    PXG_HIP(hipcub::DeviceRadixSort::SortPairs(c->sortTmp.p, tb, keys.p, keysOut.p, ids.p, c->dofContrib.p,
+                                               (int)mn, 0, eb, s));
+    k_dof_seg<<<gridFor(mn), kBlock, 0, s>>>(mn, keysOut.p, c->dofSeg.p);
+    PXG_HIP(hipGetLastError());
+    uint32_t const mm = (uint32_t)mn;
+    PXG_HIP(hipMemcpyAsync(c->dofSeg.p + c->ndofs, &mm, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    PXG_HIP(hipStreamSynchronize(s));
+  }
+  c->sortTmp.release();
+}
+
+void buildPattern(pxg_ctx * c)
+{
+  int const n = c->fe.n;
+  int64_t const E = c->nelem, nn = (int64_t)n * n, m = E * nn;
+  PXG_REQUIRE(m < (int64_t)0xffffffffLL, PXG_E_INVALID, "too many element-matrix entries for 32-bit maps");
+  PXG_REQUIRE(c->ndofs > 0 && (double)c->ndofs * (double)c->ndofs < 1.8e19, PXG_E_INVALID, "dof count");
+  hipStream_t s = c->stream;
+  int endBit = 1;
+  while (endBit < 64 && ((uint64_t)1 << endBit) < (uint64_t)c->ndofs * (uint64_t)c->ndofs)
+    ++endBit;
+  {
+    DevBuf<uint64_t> keys, keysOut;
+    DevBuf<uint32_t> ids, idsOut, flags;
+    keys.resize(m);
+    keysOut.resize(m);
+    ids.resize(m);
+    c->contrib.resize(m);
+    k_entry_keys<<<gridFor(m), kBlock, 0, s>>>(E, n, c->ndofs, c->elemMap.p, keys.p, ids.p);
+    PXG_HIP(hipGetLastError());
+    size_t tb = 0;
+    PXG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys.p, keysOut.p, ids.p, c->contrib.p, (int)m, 0,
+                                               endBit, s));
+    c->sortTmp.resize(std::max(tb, c->sortTmp.n));
+    PXG_HIP(hipcub::DeviceRadixSort::SortPairs(c->sortTmp.p, tb, keys.p, keysOut.p, ids.p, c->contrib.p, (int)m,
+                                               0, endBit, s));
+    keys.release();
+    ids.release();
+    flags.resize(m);
+    k_head_flags<<<gridFor(m), kBlock, 0, s>>>(m, keysOut.p, flags.p);
+    PXG_HIP(hipGetLastError());
+    idsOut.resize(m);
+    PXG_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, flags.p, idsOut.p, (int)m, s));
+    c->sortTmp.resize(std::max(tb, c->sortTmp.n));
+    PXG_HIP(hipcub::DeviceScan::InclusiveSum(c->sortTmp.p, tb, flags.p, idsOut.p, (int)m, s));
+    uint32_t nnzS = 0;
+    PXG_HIP(hipMemcpyAsync(&nnzS, idsOut.p + (m - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PXG_HIP(hipStreamSynchronize(s));
+    c->nnzS = nnzS;
+    c->segPtr.resize(nnzS + 1);
+    c->colS.resize(nnzS);
+    c->rowptrS.resize(c->ndofs + 1);
+    c->emapNnz.resize(m);
+    k_pattern_fill<<<gridFor(m), kBlock, 0, s>>>(m, c->ndofs, keysOut.p, idsOut.p, c->contrib.p, c->segPtr.p,
+                                                  c->colS.p, c->rowptrS.p, c->emapNnz.p);
+    PXG_HIP(hipGetLastError());
+    uint32_t const mm = (uint32_t)m;
+    int64_t const nz = nnzS;
+    PXG_HIP(hipMemcpyAsync(c->segPtr.p + nnzS, &mm, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    PXG_HIP(hipMemcpyAsync(c->rowptrS.p + c->ndofs, &nz, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    PXG_HIP(hipStreamSynchronize(s));
+  }
+  buildDofAdjacency(c);
+  c->sortTmp.release();
+  c->valS.resize(c->nnzS);
+  c->valS32.resize(c->nnzS);
+  PXG_HIP(hipMemsetAsync(c->valS.p, 0, c->nnzS * sizeof(double), s));
+  PXG_HIP(hipMemsetAsync(c->valS32.p, 0, c->nnzS * sizeof(float), s));
+  c->b.resize(c->ndofs);
+  PXG_HIP(hipMemsetAsync(c->b.p, 0, c->ndofs * sizeof(double), s));
+  c->closed = false;
+}
+
+void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
+{
+  PXG_REQUIRE(c->fe.feType && c->nelem && c->ndofs && c->nnzS, PXG_E_STATE,
+              "assemble needs set_fe, set_mesh, set_dofmap and build_pattern first");
+  PXG_REQUIRE(precision == PXG_FP64 || precision == PXG_FP32, PXG_E_INVALID, "precision");
+  int flags = 0;
+  double cS = 0., cSF = 0., cM = 0., cMF = 0.;
+  for (int k = 0; k < nterms; ++k)
+    switch (terms[k].kind)
+    {
+    case PXG_TERM_STIFFNESS: flags |= kHasStiff, cS += terms[k].coef; break;
+    case PXG_TERM_STIFFNESS_FE: flags |= kHasStiffFE, cSF += terms[k].coef; break;
+    case PXG_TERM_MASS: flags |= kHasMass, cM += terms[k].coef; break;
+    case PXG_TERM_MASS_FE: flags |= kHasMassFE, cMF += terms[k].coef; break;
+    default: throw Error(PXG_E_INVALID, "unknown term kind");
+    }
+  PXG_REQUIRE(!(flags & (kHasStiffFE | kHasMassFE)) || c->hasKappa, PXG_E_STATE,
+              "FE coefficient terms need pxg_set_coefficient");
+  if (c->valsDirty && precision != c->valPrecision)
+    throw Error(PXG_E_STATE, "cannot mix precisions before pxg_clear");
+  hipStream_t s = c->stream;
+  FeTables & T = c->fe;
+  int64_t const E = c->nelem;
+  c->geoBuf.resize(E * T.nq * 8);
+  PXG_HIP(hipMemsetAsync(c->status.p, 0, sizeof(int), s));
+  dispatchGeo(T.geo,
+              [&](auto dimC, auto npeC)
+              {
+                constexpr int D = decltype(dimC)::value, NPE = decltype(npeC)::value;
+                ProfScope ps(c, "geometry");
+                k_geometry<D, NPE><<<gridFor(E * T.nq), kBlock, 0, s>>>(
+                    E, T.nq, T.n, c->xyz.p, c->conn.p, T.d_geoGrad.p, T.d_w.p, T.d_phi.p, c->elemMap.p,
+                    c->kappa.p, cS, cSF, cM, cMF, flags, c->geoBuf.p, c->status.p);
+              });
+  PXG_HIP(hipGetLastError());
+  bool const hasMass = (flags & (kHasMass | kHasMassFE)) != 0;
+  c->feBuf.resize(E * T.n);
+  dispatchFe(T.feType,
+             [&](auto nC, auto dimC)
+             {
+               constexpr int N = decltype(nC)::value, D = decltype(dimC)::value;
+               if (precision == PXG_FP64)
+               {
+                 c->keBuf.resize(E * N * N);
+                 size_t const sh = (size_t)T.nq * N * (D + 1) * sizeof(double);
+                 {
+                 ProfScope ps(c, "element");
+                 k_element_rows<double, N, D><<<gridFor(E * N), kBlock, sh, s>>>(
+                     E, T.nq, c->geoBuf.p, T.d_grad.p, T.d_phi.p, hasMass, c->elemMap.p, c->bcFlag.p, c->bcVal.p,
+                     c->hasBC, c->keBuf.p, c->feBuf.p);
+                 }
+                 PXG_HIP(hipGetLastError());
+                 ProfScope ps2(c, "scatter");
+                 k_scatter<double><<<gridFor(c->nnzS), kBlock, 0, s>>>(c->nnzS, c->segPtr.p, c->contrib.p,
+                                                                      c->keBuf.p, c->valS.p);
+               }
+               else
+               {
+                 c->keBuf32.resize(E * N * N);
+                 size_t const sh = (size_t)T.nq * N * (D + 1) * sizeof(float);
+                 {
+                 ProfScope ps(c, "element32");
+                 k_element_rows<float, N, D><<<gridFor(E * N), kBlock, sh, s>>>(
+                     E, T.nq, c->geoBuf.p, T.d_grad32.p, T.d_phi32.p, hasMass, c->elemMap.p, c->bcFlag.p,
+                     c->bcVal.p, c->hasBC, c->keBuf32.p, c->feBuf.p);
+                 }
+                 PXG_HIP(hipGetLastError());
+                 ProfScope ps2(c, "scatter32");
+                 k_scatter<float><<<gridFor(c->nnzS), kBlock, 0, s>>>(c->nnzS, c->segPtr.p, c->contrib.p,
+                                                                     c->keBuf32.p, c->valS32.p);
+               }
+             });
+  PXG_HIP(hipGetLastError());
+  {
+    ProfScope ps(c, "rhs_scatter");
+    k_rhs_scatter<<<gridFor(c->ndofs), kBlock, 0, s>>>(c->ndofs, c->dofSeg.p, c->dofContrib.p, c->feBuf.p, c->b.p);
+  }
+  PXG_HIP(hipGetLastError());
+  c->valPrecision = precision;
+  c->valsDirty = true;
+  c->closed = false;
+  checkStatus(c);
+}
+
+void assembleRhs(pxg_ctx * c, int kind, double const * params, double const * qvaluesHost)
+{
+  PXG_REQUIRE(c->fe.feType && c->nelem && c->ndofs && c->nnzS, PXG_E_STATE,
+              "assemble_rhs needs set_fe, set_mesh, set_dofmap and build_pattern first");
+  PXG_REQUIRE(kind >= PXG_RHS_CONST && kind <= PXG_RHS_QVALUES, PXG_E_INVALID, "rhs kind");
+  hipStream_t s = c->stream;
+  FeTables & T = c->fe;
+  int64_t const E = c->nelem, m = E * T.nq;
+  RhsParams prm{{0., 0., 0., 0.}};
+  if (params)
+    for (int k = 0; k < 4; ++k)
+      prm.p[k] = params[k];
+  DevBuf<double> qv;
+  if (kind == PXG_RHS_QVALUES)
+  {
+    PXG_REQUIRE(qvaluesHost != nullptr, PXG_E_INVALID, "qvalues required");
+    qv.upload(qvaluesHost, m, s);
+  }
+  c->qwork.resize(2 * m);
+  dispatchGeo(T.geo,
+              [&](auto dimC, auto npeC)
+              {
+                constexpr int D = decltype(dimC)::value, NPE = decltype(npeC)::value;
+                k_qpoints<D, NPE><<<gridFor(m), kBlock, 0, s>>>(E, T.nq, T.geo, c->xyz.p, c->conn.p,
+                                                                 T.d_geoGrad.p, T.d_xi.p, T.d_w.p, kind, prm, qv.p,
+                                                                 nullptr, c->qwork.p);
+              });
+  PXG_HIP(hipGetLastError());
+  c->feBuf.resize(E * T.n);
+  k_rhs_rows<<<gridFor(E * T.n), kBlock, 0, s>>>(E, T.n, T.nq, c->qwork.p, T.d_phi.p, c->elemMap.p, c->bcFlag.p,
+                                                 c->hasBC, c->feBuf.p);
+  PXG_HIP(hipGetLastError());
+  k_rhs_scatter<<<gridFor(c->ndofs), kBlock, 0, s>>>(c->ndofs, c->dofSeg.p, c->dofContrib.p, c->feBuf.p, c->b.p);
+  PXG_HIP(hipGetLastError());
+  PXG_HIP(hipStreamSynchronize(s));
+}
+
+void qpoints(pxg_ctx * c, double * outHost)
+{
+  PXG_REQUIRE(c->fe.feType && c->nelem, PXG_E_STATE, "qpoints need set_fe and set_mesh");
+  hipStream_t s = c->stream;
+  FeTables & T = c->fe;
+  int64_t const E = c->nelem, m = E * T.nq;
+  DevBuf<double> qx;
+  qx.resize(3 * m);
+  RhsParams prm{{0., 0., 0., 0.}};
+  dispatchGeo(T.geo,
+              [&](auto dimC, auto npeC)
+              {
+                constexpr int D = decltype(dimC)::value, NPE = decltype(npeC)::value;
+                k_qpoints<D, NPE><<<gridFor(m), kBlock, 0, s>>>(E, T.nq, T.geo, c->xyz.p, c->conn.p,
+                                                                 T.d_geoGrad.p, T.d_xi.p, T.d_w.p, 0, prm, nullptr,
+                                                                 qx.p, nullptr);
+              });
+  PXG_HIP(hipGetLastError());
+  qx.download(outHost, 3 * m, s);
+  PXG_HIP(hipStreamSynchronize(s));
+}
+
+void geometryOut(pxg_ctx * c, double * detJ, double * JxW)
+{
+  PXG_REQUIRE(c->fe.feType && c->nelem, PXG_E_STATE, "geometry needs set_fe and set_mesh");
+  hipStream_t s = c->stream;
+  FeTables & T = c->fe;
+  int64_t const E = c->nelem, m = E * T.nq;
+  c->geoBuf.resize(m * 8);
+  PXG_HIP(hipMemsetAsync(c->status.p, 0, sizeof(int), s));
+  dispatchGeo(T.geo,
+              [&](auto dimC, auto npeC)
+              {
+                constexpr int D = decltype(dimC)::value, NPE = decltype(npeC)::value;
+                k_geometry<D, NPE><<<gridFor(m), kBlock, 0, s>>>(E, T.nq, T.n, c->xyz.p, c->conn.p, T.d_geoGrad.p,
+                                                                 T.d_w.p, T.d_phi.p, c->elemMap.p, c->kappa.p, 1.,
+                                                                 0., 0., 0., kHasStiff, c->geoBuf.p, c->status.p);
+              });
+  PXG_HIP(hipGetLastError());
+  DevBuf<double> dj, jw;
+  dj.resize(m);
+  jw.resize(m);
+  k_jxw<<<gridFor(m), kBlock, 0, s>>>(m, c->geoBuf.p, dj.p, jw.p, T.d_w.p, T.nq);
+  PXG_HIP(hipGetLastError());
+  if (detJ)
+    dj.download(detJ, m, s);
+  if (JxW)
+    jw.download(JxW, m, s);
+  PXG_HIP(hipStreamSynchronize(s));
+}
+
+void closeMatrix(pxg_ctx * c, double eps)
+{
+  PXG_REQUIRE(c->nnzS > 0, PXG_E_STATE, "close needs build_pattern");
+  hipStream_t s = c->stream;
+  int64_t const nr = c->ndofs;
+  DevBuf<int64_t> cnt;
+  cnt.resize(nr + 1);
+  bool const f32 = c->valPrecision == PXG_FP32;
+  ProfScope ps(c, "close");
+  if (f32)
+    k_prune_count<float><<<gridFor(nr), kBlock, 0, s>>>(nr, c->rowptrS.p, c->valS32.p, eps, cnt.p);
+  else
+    k_prune_count<double><<<gridFor(nr), kBlock, 0, s>>>(nr, c->rowptrS.p, c->valS.p, eps, cnt.p);
+  PXG_HIP(hipGetLastError());
+  PXG_HIP(hipMemsetAsync(cnt.p + nr, 0, sizeof(int64_t), s));
+  c->rowptr.resize(nr + 1);
+  size_t tb = 0;
+  PXG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.p, c->rowptr.p, (int)(nr + 1), s));
+  DevBuf<unsigned char> tmp;
+  tmp.resize(tb);
+  PXG_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, cnt.p, c->rowptr.p, (int)(nr + 1), s));
+  int64_t nnz = 0;
+  PXG_HIP(hipMemcpyAsync(&nnz, c->rowptr.p + nr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  PXG_HIP(hipStreamSynchronize(s));
+  c->nnz = nnz;
+  c->col.resize(nnz);
+  c->val.resize(nnz);
+  c->val32.resize(nnz);
+  c->dinv.resize(nr);
+  c->dinv32.resize(nr);
+  if (f32)
+  {
+    k_prune_compact<float><<<gridFor(nr), kBlock, 0, s>>>(nr, c->rowptrS.p, c->colS.p, c->valS32.p, eps,
+                                                          c->rowptr.p, c->col.p, c->val32.p, c->dinv32.p);
+    PXG_HIP(hipGetLastError());
+    k_cast<double, float><<<gridFor(nnz), kBlock, 0, s>>>(nnz, c->val32.p, c->val.p);
+    k_cast<double, float><<<gridFor(nr), kBlock, 0, s>>>(nr, c->dinv32.p, c->dinv.p);
+  }
+  else
+  {
+    k_prune_compact<double><<<gridFor(nr), kBlock, 0, s>>>(nr, c->rowptrS.p, c->colS.p, c->valS.p, eps,
+                                                           c->rowptr.p, c->col.p, c->val.p, c->dinv.p);
+    PXG_HIP(hipGetLastError());
+    k_cast<float, double><<<gridFor(nnz), kBlock, 0, s>>>(nnz, c->val.p, c->val32.p);
+    k_cast<float, double><<<gridFor(nr), kBlock, 0, s>>>(nr, c->dinv.p, c->dinv32.p);
+  }
+  PXG_HIP(hipGetLastError());
+  PXG_HIP(hipStreamSynchronize(s));
+  c->closed = true;
+}
+
+void clearValues(pxg_ctx * c)
+{
+  hipStream_t s = c->stream;
+  if (c->valS.p)
+    PXG_HIP(hipMemsetAsync(c->valS.p, 0, c->nnzS * sizeof(double), s));
+  if (c->valS32.p)
+    PXG_HIP(hipMemsetAsync(c->valS32.p, 0, c->nnzS * sizeof(float), s));
+  if (c->b.p)
+    PXG_HIP(hipMemsetAsync(c->b.p, 0, c->ndofs * sizeof(double), s));
+  PXG_HIP(hipStreamSynchronize(s));
+  c->valsDirty = false;
+  c->closed = false;
+}
+
+} // namespace pxg

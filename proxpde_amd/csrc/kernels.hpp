@@ -1,0 +1,72 @@
+// Host-side launchers of the device kernels (declared here, defined in the .hip files).
+#pragma once
+
+#include "ctx.hpp"
+#include "fe.hpp"
+
+namespace pxg
+{
+
+// assembly.hip
+void buildDofAdjacency(pxg_ctx * c);
+void buildPattern(pxg_ctx * c);
+void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision);
+void assembleRhs(pxg_ctx * c, int kind, double const * params, double const * qvaluesHost);
+void qpoints(pxg_ctx * c, double * outHost);
+void geometryOut(pxg_ctx * c, double * detJ, double * JxW);
+void closeMatrix(pxg_ctx * c, double eps);
+void clearValues(pxg_ctx * c);
+
+// sparse.hip: CSR kernels on raw device arrays
+struct CsrView
+{
+  int64_t nrows = 0, ncols = 0, nnz = 0;
+  int64_t const * rowptr = nullptr;
+  int32_t const * col = nullptr;
+  double const * val = nullptr;
+  float const * val32 = nullptr;
+  double const * dinv = nullptr;
+  float const * dinv32 = nullptr;
+};
+
+CsrView viewOf(pxg_ctx const * c);
+
+// y = A x
+template <typename T>
+void spmv(CsrView const & A, T const * x, T * y, hipStream_t s);
+// y = b - A x
+template <typename T>
+void residual(CsrView const & A, T const * b, T const * x, T * y, hipStream_t s);
+// y = x + omega D^-1 (b - A x)    (y != x)
+template <typename T>
+void jacobiStep(CsrView const & A, T const * b, T const * x, T * y, double omega, hipStream_t s);
+// y += A x
+template <typename T>
+void spmvAdd(CsrView const & A, T const * x, T * y, hipStream_t s);
+
+// deterministic FP64 dot product into a device scalar
+template <typename T>
+void dot(int64_t n, T const * a, T const * b, double * outDev, double * partial, hipStream_t s);
+
+// vector helpers
+template <typename T>
+void fill(int64_t n, T * x, T v, hipStream_t s);
+template <typename Tout, typename Tin>
+void convert(int64_t n, Tin const * x, Tout * y, hipStream_t s);
+template <typename T>
+void jacobiDiag(int64_t n, T const * dinv, T const * b, T * x, double omega, hipStream_t s);
+// x += alpha p ; r -= alpha q with alpha = sc[ia] / sc[ib]  (scalars on device)
+template <typename T>
+void cgUpdate(int64_t n, T * x, T const * p, T * r, T const * q, double const * sc, int ia, int ib, hipStream_t s);
+// p = z + beta p with beta = sc[ia] / sc[ib]
+template <typename T>
+void cgDirection(int64_t n, T * p, T const * z, double const * sc, int ia, int ib, hipStream_t s);
+// x += y (mixed precision)
+template <typename T, typename U>
+void addTo(int64_t n, T * x, U const * y, hipStream_t s);
+// y = a*x + b*y
+void axpby(int64_t n, double a, double const * x, double b, double * y, hipStream_t s);
+
+constexpr int kDotBlocks = 1024;
+
+} // namespace pxg

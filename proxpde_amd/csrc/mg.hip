@@ -1,0 +1,623 @@
+// Geometric multigrid-preconditioned CG on the assembled level matrices.
+//
+// The reference keeps only the transfer operators in the library
+// (include/multigrid.hpp:12-182: Prolongator / Restrictor from embedding
+// matrices) and its V-cycle in a test (test/multigrid.cpp:18-112, GMRES
+// smoother + LU coarse solve).  Here the cycle is the symmetric one a CG
+// preconditioner needs:
+//   pre-smooth  nu1 weighted-Jacobi sweeps from x = 0
+//   residual    r = b - A x
+//   restrict    b_c = P^T r          (R = P^T, not the row-normalised R)
+//   recurse     e_c = V(b_c), coarsest level: Jacobi-CG in FP64 to coarse_rtol
+//   prolongate  x += P e_c
+//   post-smooth nu2 sweeps
+// P(f, c) is the coarse basis function c evaluated at fine dof point f in the
+// fine element's parent (the values of the reference's embedding matrices),
+// with rows of constrained fine dofs and columns of constrained coarse dofs
+// removed so Dirichlet rows never receive a coarse correction.
+// FP64 mode runs everything in double.  FP32 mode runs the preconditioned CG
+// and the cycle in float (FP64 dot products, FP64 coarsest solve) inside an
+// FP64 iterative refinement whose residual b - A x is formed in FP64.
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace pxg
+{
+void mfApply(pxg_ctx * c, double const * kappa, int nb, double const * x, double * y, bool useBC);
+}
+
+using namespace pxg;
+
+struct pxg_mg
+{
+  std::string err;
+  int nlev = 0;
+  std::vector<pxg_ctx *> ctx;
+  std::vector<std::array<int, 3>> dims;
+  double omega = 2. / 3., coarseRtol = 1e-10;
+  int nu1 = 2, nu2 = 2;
+  double innerRtol = 1e-4;
+  hipStream_t s = nullptr;
+  struct Level
+  {
+    int64_t n = 0;
+    // prolongation from level+1 and its transpose
+    std::vector<int64_t> hPrp;
+    std::vector<int32_t> hPcol;
+    std::vector<double> hPval;
+    DevBuf<int64_t> Prp, Rrp;
+    DevBuf<int32_t> Pcol, Rcol;
+    DevBuf<double> Pval, Rval;
+    DevBuf<float> Pval32, Rval32;
+    CsrView P, R;
+    DevBuf<double> b, x, t, r;
+    DevBuf<float> b32, x32, t32, r32;
+  };
+  std::vector<Level> lv;
+  DevBuf<double> sc, partial;
+  DevBuf<double> r, z, p, q, d;
+  DevBuf<float> r32, z32, p32, q32, d32;
+  // coarsest-level CG vectors
+  DevBuf<double> cr, cz, cp, cq;
+};
+
+namespace
+{
+template <typename T>
+struct Buf;
+template <>
+struct Buf<double>
+{
+  static double * b(pxg_mg::Level & L) { return L.b.p; }
+  static double * x(pxg_mg::Level & L) { return L.x.p; }
+  static double * t(pxg_mg::Level & L) { return L.t.p; }
+  static double * r(pxg_mg::Level & L) { return L.r.p; }
+};
+template <>
+struct Buf<float>
+{
+  static float * b(pxg_mg::Level & L) { return L.b32.p; }
+  static float * x(pxg_mg::Level & L) { return L.x32.p; }
+  static float * t(pxg_mg::Level & L) { return L.t32.p; }
+  static float * r(pxg_mg::Level & L) { return L.r32.p; }
+};
+
+double readScalar(pxg_mg * m, int i)
+{
+  double h = 0.;
+  PXG_HIP(hipMemcpyAsync(&h, m->sc.p + i, sizeof(double), hipMemcpyDeviceToHost, m->s));
+  PXG_HIP(hipStreamSynchronize(m->s));
+  return h;
+}
+
+void copyScalar(pxg_mg * m, int dst, int src)
+{
+  PXG_HIP(hipMemcpyAsync(m->sc.p + dst, m->sc.p + src, sizeof(double), hipMemcpyDeviceToDevice, m->s));
+}
+
+// host construction of P for one level pair (structured hex meshes)
+void buildProlongation(pxg_mg * m, int l)
+{
+  pxg_ctx * F = m->ctx[l];
+  pxg_ctx * Cc = m->ctx[l + 1];
+  auto const & df = m->dims[l];
+  auto const & dc = m->dims[l + 1];
+  PXG_REQUIRE(F->fe.geo == kHex && Cc->fe.geo == kHex && F->fe.feType == Cc->fe.feType, PXG_E_INVALID,
+              "multigrid levels must be hex meshes of one FE type");
+  PXG_REQUIRE(df[0] == 2 * dc[0] && df[1] == 2 * dc[1] && df[2] == 2 * dc[2], PXG_E_INVALID,
+              "element counts must halve per level");
+  PXG_REQUIRE((int64_t)df[0] * df[1] * df[2] == F->nelem && (int64_t)dc[0] * dc[1] * dc[2] == Cc->nelem,
+              PXG_E_INVALID, "level dims do not match the meshes");
+  hipStream_t s = m->s;
+  FeInfo const & fe = feInfo(F->fe.feType);
+  int const n = fe.n;
+  // host copies of the setup data
+  std::vector<double> xf(3 * F->npts), xc(3 * Cc->npts);
+  std::vector<uint32_t> cf(F->nelem * 8), cc(Cc->nelem * 8), ef(F->nelem * n), ec(Cc->nelem * n);
+  std::vector<uint8_t> bf(F->ndofs), bcn(Cc->ndofs);
+  F->xyz.download(xf.data(), xf.size(), F->stream);
+  F->conn.download(cf.data(), cf.size(), F->stream);
+  F->elemMap.download(ef.data(), ef.size(), F->stream);
+  F->bcFlag.download(bf.data(), bf.size(), F->stream);
+  Cc->xyz.download(xc.data(), xc.size(), Cc->stream);
+  Cc->conn.download(cc.data(), cc.size(), Cc->stream);
+  Cc->elemMap.download(ec.data(), ec.size(), Cc->stream);
+  Cc->bcFlag.download(bcn.data(), bcn.size(), Cc->stream);
+  PXG_HIP(hipStreamSynchronize(F->stream));
+  PXG_HIP(hipStreamSynchronize(Cc->stream));
+  int64_t const nf = F->ndofs;
+  std::vector<int64_t> rowStart(nf + 1, 0);
+  std::vector<std::vector<std::pair<int32_t, double>>> rows(nf);
+  std::vector<uint8_t> seen(nf, 0);
+  double vf[24], vc[24], pf[27 * 3];
+  double const zero[3] = {0., 0., 0.};
+  for (int64_t e = 0; e < F->nelem; ++e)
+  {
+    int64_t const i = e % df[0], j = (e / df[0]) % df[1], k = e / ((int64_t)df[0] * df[1]);
+    int64_t const pe = (i / 2) + (int64_t)dc[0] * ((j / 2) + (int64_t)dc[1] * (k / 2));
+    for (int v = 0; v < 8; ++v)
+      for (int d = 0; d < 3; ++d)
+      {
+        vf[3 * v + d] = xf[3ull * cf[e * 8 + v] + d];
+        vc[3 * v + d] = xc[3ull * cc[pe * 8 + v] + d];
+      }
+    fe.dofPoints(vf, pf);
+    // affine map of the parent parallelepiped: x = mid + J xi
+    double mid[3] = {0., 0., 0.}, J[3][3] = {{0.}};
+    for (int v = 0; v < 8; ++v)
+    {
+      double g[3];
+      geoGrad(kHex, v, zero, g);
+      for (int r = 0; r < 3; ++r)
+      {
+        mid[r] += 0.125 * vc[3 * v + r];
+        for (int c = 0; c < 3; ++c)
+          J[r][c] += vc[3 * v + r] * g[c];
+      }
+    }
+    double const det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    double Ji[3][3];
+    Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+    Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+    Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    for (int a = 0; a < n; ++a)
+    {
+      uint32_t const dofF = ef[e * n + a];
+      if (seen[dofF])
+        continue;
+      seen[dofF] = 1;
+      if (bf[dofF])
+        continue; // constrained fine row: no coarse correction
+      double xi[3];
+      for (int r = 0; r < 3; ++r)
+      {
+        double t = 0.;
+        for (int c = 0; c < 3; ++c)
+          t += Ji[r][c] * (pf[3 * a + c] - mid[c]);
+        double const h = std::round(2. * t);
+        xi[r] = std::fabs(2. * t - h) < 1e-8 ? 0.5 * h : t;
+      }
+      auto & row = rows[dofF];
+      for (int b = 0; b < n; ++b)
+      {
+        uint32_t const dofC = ec[pe * n + b];
+        if (bcn[dofC])
+          continue;
+        double const val = fe.phi(b, xi);
+        if (std::fabs(val) > 1e-12)
+          row.push_back({(int32_t)dofC, val});
+      }
+      std::sort(row.begin(), row.end());
+    }
+  }
+  pxg_mg::Level & L = m->lv[l];
+  L.hPrp.assign(nf + 1, 0);
+  for (int64_t r = 0; r < nf; ++r)
+    L.hPrp[r + 1] = L.hPrp[r] + (int64_t)rows[r].size();
+  int64_t const nnz = L.hPrp[nf];
+  L.hPcol.resize(nnz);
+  L.hPval.resize(nnz);
+  for (int64_t r = 0; r < nf; ++r)
+    for (size_t k = 0; k < rows[r].size(); ++k)
+    {
+      L.hPcol[L.hPrp[r] + k] = rows[r][k].first;
+      L.hPval[L.hPrp[r] + k] = rows[r][k].second;
+    }
+  rows.clear();
+  // R = P^T (counting sort by column keeps rows of R in increasing fine index)
+  int64_t const ncr = Cc->ndofs;
+  std::vector<int64_t> rrp(ncr + 1, 0);
+  for (int64_t k = 0; k < nnz; ++k)
+    rrp[L.hPcol[k] + 1]++;
+  for (int64_t r = 0; r < ncr; ++r)
+    rrp[r + 1] += rrp[r];
+  std::vector<int32_t> rcol(nnz);
+  std::vector<double> rval(nnz);
+  {
+    std::vector<int64_t> pos(rrp.begin(), rrp.end() - 1);
+    for (int64_t r = 0; r < nf; ++r)
+      for (int64_t k = L.hPrp[r]; k < L.hPrp[r + 1]; ++k)
+      {
+        int64_t const o = pos[L.hPcol[k]]++;
+        rcol[o] = (int32_t)r;
+        rval[o] = L.hPval[k];
+      }
+  }
+  std::vector<float> p32(L.hPval.begin(), L.hPval.end()), r32(rval.begin(), rval.end());
+  L.Prp.upload(L.hPrp.data(), L.hPrp.size(), s);
+  L.Pcol.upload(L.hPcol.data(), nnz, s);
+  L.Pval.upload(L.hPval.data(), nnz, s);
+  L.Pval32.upload(p32.data(), nnz, s);
+  L.Rrp.upload(rrp.data(), rrp.size(), s);
+  L.Rcol.upload(rcol.data(), nnz, s);
+  L.Rval.upload(rval.data(), nnz, s);
+  L.Rval32.upload(r32.data(), nnz, s);
+  PXG_HIP(hipStreamSynchronize(s));
+  L.P = CsrView{nf, ncr, nnz, L.Prp.p, L.Pcol.p, L.Pval.p, L.Pval32.p, nullptr, nullptr};
+  L.R = CsrView{ncr, nf, nnz, L.Rrp.p, L.Rcol.p, L.Rval.p, L.Rval32.p, nullptr, nullptr};
+}
+
+// smoothing sweeps, ping-ponging between x and t, result left in x
+template <typename T>
+void smooth(CsrView const & A, T const * b, T * x, T * t, int sweeps, double omega, bool fromZero, hipStream_t s)
+{
+  if (sweeps <= 0)
+  {
+    if (fromZero)
+      fill<T>(A.nrows, x, T(0), s);
+    return;
+  }
+  T * cur = x;
+  int steps = sweeps;
+  if (fromZero)
+  {
+    // the first sweep from zero is x = omega D^-1 b; place it so that the
+    // remaining sweeps end in x
+    steps = sweeps - 1;
+    cur = (steps % 2 == 0) ? x : t;
+    jacobiDiag<T>(A.nrows, sizeof(T) == 8 ? (T const *)A.dinv : (T const *)A.dinv32, b, cur, omega, s);
+  }
+  for (int k = 0; k < steps; ++k)
+  {
+    T * oth = (cur == x) ? t : x;
+    jacobiStep<T>(A, b, cur, oth, omega, s);
+    cur = oth;
+  }
+  if (cur != x)
+    PXG_HIP(hipMemcpyAsync(x, cur, A.nrows * sizeof(T), hipMemcpyDeviceToDevice, s));
+}
+
+// coarsest level: Jacobi-preconditioned CG in FP64 from zero to coarseRtol
+void coarseSolve(pxg_mg * m, double const * b, double * x)
+{
+  pxg_ctx * c = m->ctx.back();
+  CsrView const A = viewOf(c);
+  int64_t const n = A.nrows;
+  hipStream_t s = m->s;
+  double * r = m->cr.p, * z = m->cz.p, * p = m->cp.p, * q = m->cq.p;
+  double * sc = m->sc.p;
+  fill<double>(n, x, 0., s);
+  PXG_HIP(hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  dot<double>(n, b, b, sc + 8, m->partial.p, s);
+  double const bn = std::sqrt(readScalar(m, 8));
+  if (bn == 0.)
+    return;
+  jacobiDiag<double>(n, A.dinv, r, z, 1.0, s);
+  PXG_HIP(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  dot<double>(n, r, z, sc + 9, m->partial.p, s);
+  int const maxit = 20000;
+  for (int it = 1; it <= maxit; ++it)
+  {
+    spmv<double>(A, p, q, s);
+    dot<double>(n, p, q, sc + 10, m->partial.p, s);
+    cgUpdate<double>(n, x, p, r, q, sc, 9, 10, s);
+    if (it % 8 == 0)
+    {
+      dot<double>(n, r, r, sc + 11, m->partial.p, s);
+      if (std::sqrt(readScalar(m, 11)) <= m->coarseRtol * bn)
+        return;
+    }
+    jacobiDiag<double>(n, A.dinv, r, z, 1.0, s);
+    dot<double>(n, r, z, sc + 12, m->partial.p, s);
+    cgDirection<double>(n, p, z, sc, 12, 9, s);
+    copyScalar(m, 9, 12);
+  }
+}
+
+template <typename T>
+void vcycle(pxg_mg * m, int l, T const * b, T * x)
+{
+  hipStream_t s = m->s;
+  if (l == m->nlev - 1)
+  {
+    pxg_mg::Level & L = m->lv[l];
+    if constexpr (sizeof(T) == 8)
+      coarseSolve(m, (double const *)b, (double *)x);
+    else
+    {
+      convert<double, float>(L.n, (float const *)b, L.b.p, s);
+      coarseSolve(m, L.b.p, L.x.p);
+      convert<float, double>(L.n, L.x.p, (float *)x, s);
+    }
+    return;
+  }
+  pxg_mg::Level & L = m->lv[l];
+  pxg_mg::Level & Lc = m->lv[l + 1];
+  CsrView const A = viewOf(m->ctx[l]);
+  T * t = Buf<T>::t(L);
+  T * r = Buf<T>::r(L);
+  smooth<T>(A, b, x, t, m->nu1, m->omega, true, s);
+  residual<T>(A, b, x, r, s);
+  T * bc = Buf<T>::b(Lc);
+  T * xc = Buf<T>::x(Lc);
+  spmv<T>(L.R, r, bc, s);
+  vcycle<T>(m, l + 1, bc, xc);
+  spmvAdd<T>(L.P, xc, x, s);
+  smooth<T>(A, b, x, t, m->nu2, m->omega, false, s);
+}
+
+// preconditioned CG in precision T on level 0; b, x in T; returns iterations
+template <typename T>
+int pcg(pxg_mg * m, T const * b, T * x, double rtol, int maxit, double & relres, bool zeroStart)
+{
+  hipStream_t s = m->s;
+  CsrView const A = viewOf(m->ctx[0]);
+  int64_t const n = A.nrows;
+  T * r = sizeof(T) == 8 ? (T *)m->r.p : (T *)m->r32.p;
+  T * z = sizeof(T) == 8 ? (T *)m->z.p : (T *)m->z32.p;
+  T * p = sizeof(T) == 8 ? (T *)m->p.p : (T *)m->p32.p;
+  T * q = sizeof(T) == 8 ? (T *)m->q.p : (T *)m->q32.p;
+  double * sc = m->sc.p;
+  if (zeroStart)
+  {
+    fill<T>(n, x, T(0), s);
+    PXG_HIP(hipMemcpyAsync(r, b, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+  }
+  else
+    residual<T>(A, b, x, r, s);
+  dot<T>(n, b, b, sc + 0, m->partial.p, s);
+  dot<T>(n, r, r, sc + 1, m->partial.p, s);
+  double const bn = std::sqrt(readScalar(m, 0));
+  double rn = std::sqrt(readScalar(m, 1));
+  relres = bn > 0. ? rn / bn : 0.;
+  if (relres <= rtol || bn == 0.)
+    return 0;
+  vcycle<T>(m, 0, r, z);
+  PXG_HIP(hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+  dot<T>(n, r, z, sc + 2, m->partial.p, s);
+  int it = 0;
+  while (it < maxit)
+  {
+    spmv<T>(A, p, q, s);
+    dot<T>(n, p, q, sc + 3, m->partial.p, s);
+    cgUpdate<T>(n, x, p, r, q, sc, 2, 3, s);
+    ++it;
+    dot<T>(n, r, r, sc + 1, m->partial.p, s);
+    rn = std::sqrt(readScalar(m, 1));
+    PXG_REQUIRE(std::isfinite(rn), PXG_E_NUMERIC, "CG breakdown (non-finite residual)");
+    relres = rn / bn;
+    if (relres <= rtol)
+      break;
+    vcycle<T>(m, 0, r, z);
+    dot<T>(n, r, z, sc + 4, m->partial.p, s);
+    cgDirection<T>(n, p, z, sc, 4, 2, s);
+    copyScalar(m, 2, 4);
+  }
+  return it;
+}
+
+// FP64 residual r = b - A x on level 0: assembled FP64 matrix, or the
+// matrix-free kappa-stiffness operator when level 0 holds FP32 values
+void residual64(pxg_mg * m, double const * b, double const * x, double * r)
+{
+  pxg_ctx * c = m->ctx[0];
+  if (c->valPrecision == PXG_FP64)
+    residual<double>(viewOf(c), b, x, r, m->s);
+  else
+  {
+    PXG_REQUIRE(c->hasKappa, PXG_E_STATE, "FP32 level 0 needs the kappa field for the FP64 residual");
+    PXG_HIP(hipStreamSynchronize(m->s));
+    mfApply(c, c->kappa.p, 1, x, r, true);
+    axpby(c->ndofs, 1.0, b, -1.0, r, m->s);
+  }
+}
+
+int solveIR(pxg_mg * m, double const * b, double * x, double rtol, int maxit, double & relres)
+{
+  hipStream_t s = m->s;
+  int64_t const n = m->lv[0].n;
+  double * r = m->r.p;
+  double * sc = m->sc.p;
+  dot<double>(n, b, b, sc + 5, m->partial.p, s);
+  double const bn = std::sqrt(readScalar(m, 5));
+  if (bn == 0.)
+  {
+    fill<double>(n, x, 0., s);
+    relres = 0.;
+    return 0;
+  }
+  int total = 0;
+  for (int outer = 0; outer < 50; ++outer)
+  {
+    residual64(m, b, x, r);
+    dot<double>(n, r, r, sc + 6, m->partial.p, s);
+    double const rn = std::sqrt(readScalar(m, 6));
+    PXG_REQUIRE(std::isfinite(rn), PXG_E_NUMERIC, "non-finite residual in refinement");
+    relres = rn / bn;
+    if (relres <= rtol || total >= maxit)
+      break;
+    convert<float, double>(n, r, m->r32.p, s);
+    // inner FP32 solve of A d = r to a relative tolerance that need not go
+    // below what the outer target still requires
+    double const inner = std::max(m->innerRtol, 0.5 * rtol / relres);
+    double innerRel = 0.;
+    // inner CG uses its own r32/z32/p32/q32; the right-hand side is copied to b32
+    PXG_HIP(hipMemcpyAsync(m->lv[0].b32.p, m->r32.p, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    total += pcg<float>(m, m->lv[0].b32.p, m->d32.p, inner, maxit - total, innerRel, true);
+    addTo<double, float>(n, x, m->d32.p, s);
+  }
+  return total;
+}
+} // namespace
+
+extern "C" {
+
+int pxg_mg_create(pxg_mg ** out, int nlev, pxg_ctx ** levels, const int * dims, double omega, int nu1, int nu2,
+                  double coarseRtol)
+{
+  if (!out || nlev < 1 || !levels || !dims)
+    return PXG_E_INVALID;
+  *out = nullptr;
+  auto * m = new (std::nothrow) pxg_mg;
+  if (!m)
+    return PXG_E_OOM;
+  try
+  {
+    m->nlev = nlev;
+    m->omega = omega;
+    m->nu1 = nu1;
+    m->nu2 = nu2;
+    m->coarseRtol = coarseRtol;
+    for (int l = 0; l < nlev; ++l)
+    {
+      PXG_REQUIRE(levels[l] && levels[l]->closed, PXG_E_STATE, "levels must be closed contexts");
+      m->ctx.push_back(levels[l]);
+      m->dims.push_back({dims[3 * l], dims[3 * l + 1], dims[3 * l + 2]});
+    }
+    PXG_HIP(hipSetDevice(levels[0]->device));
+    m->s = levels[0]->stream;
+    m->lv.resize(nlev);
+    for (int l = 0; l < nlev; ++l)
+    {
+      auto & L = m->lv[l];
+      L.n = levels[l]->ndofs;
+      L.b.resize(L.n);
+      L.x.resize(L.n);
+      L.t.resize(L.n);
+      L.r.resize(L.n);
+      L.b32.resize(L.n);
+      L.x32.resize(L.n);
+      L.t32.resize(L.n);
+      L.r32.resize(L.n);
+    }
+    for (int l = 0; l + 1 < nlev; ++l)
+      buildProlongation(m, l);
+    int64_t const n0 = m->lv[0].n, nc = m->lv.back().n;
+    m->sc.resize(16);
+    m->partial.resize(kDotBlocks);
+    m->r.resize(n0);
+    m->z.resize(n0);
+    m->p.resize(n0);
+    m->q.resize(n0);
+    m->d.resize(n0);
+    m->r32.resize(n0);
+    m->z32.resize(n0);
+    m->p32.resize(n0);
+    m->q32.resize(n0);
+    m->d32.resize(n0);
+    m->cr.resize(nc);
+    m->cz.resize(nc);
+    m->cp.resize(nc);
+    m->cq.resize(nc);
+  }
+  catch (Error const & e)
+  {
+    levels[0]->err = e.what();
+    delete m;
+    return e.code;
+  }
+  catch (std::exception const & e)
+  {
+    levels[0]->err = e.what();
+    delete m;
+    return PXG_E_INVALID;
+  }
+  *out = m;
+  return PXG_OK;
+}
+
+int pxg_mg_destroy(pxg_mg * m)
+{
+  delete m;
+  return PXG_OK;
+}
+
+const char * pxg_mg_last_error(const pxg_mg * m) { return m ? m->err.c_str() : "null multigrid"; }
+
+int pxg_mg_update(pxg_mg * m)
+{
+  if (!m)
+    return PXG_E_INVALID;
+  for (auto * c: m->ctx)
+    if (!c->closed)
+    {
+      m->err = "levels must be closed";
+      return PXG_E_STATE;
+    }
+  return PXG_OK;
+}
+
+int pxg_mg_solve(pxg_mg * m, int mode, const double * b, double * x, double rtol, int maxit, int * iters,
+                 double * relres)
+{
+  if (!m || !b || !x)
+    return PXG_E_INVALID;
+  try
+  {
+    PXG_HIP(hipSetDevice(m->ctx[0]->device));
+    for (auto * c: m->ctx)
+      PXG_REQUIRE(c->closed, PXG_E_STATE, "levels must be closed");
+    double rr = 0.;
+    int it = 0;
+    if (mode == PXG_FP32)
+      it = solveIR(m, b, x, rtol, maxit, rr);
+    else
+      it = pcg<double>(m, b, x, rtol, maxit, rr, false);
+    PXG_HIP(hipStreamSynchronize(m->s));
+    if (iters)
+      *iters = it;
+    if (relres)
+      *relres = rr;
+    m->err.clear();
+    if (rr > rtol)
+    {
+      m->err = "iteration limit reached before the tolerance";
+      return PXG_E_NOCONV;
+    }
+    return PXG_OK;
+  }
+  catch (Error const & e)
+  {
+    m->err = e.what();
+    return e.code;
+  }
+}
+
+int pxg_mg_vcycle(pxg_mg * m, const double * b, double * x)
+{
+  if (!m || !b || !x)
+    return PXG_E_INVALID;
+  try
+  {
+    PXG_HIP(hipSetDevice(m->ctx[0]->device));
+    vcycle<double>(m, 0, b, x);
+    PXG_HIP(hipStreamSynchronize(m->s));
+    return PXG_OK;
+  }
+  catch (Error const & e)
+  {
+    m->err = e.what();
+    return e.code;
+  }
+}
+
+int pxg_mg_prolongation(pxg_mg * m, int l, int64_t * rowptr, int32_t * col, double * val, int64_t * nnz)
+{
+  if (!m || l < 0 || l + 1 >= m->nlev)
+    return PXG_E_INVALID;
+  auto const & L = m->lv[l];
+  if (nnz)
+    *nnz = (int64_t)L.hPcol.size();
+  if (rowptr)
+    std::memcpy(rowptr, L.hPrp.data(), L.hPrp.size() * sizeof(int64_t));
+  if (col)
+    std::memcpy(col, L.hPcol.data(), L.hPcol.size() * sizeof(int32_t));
+  if (val)
+    std::memcpy(val, L.hPval.data(), L.hPval.size() * sizeof(double));
+  return PXG_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,335 @@
+// CSR operator kernels on gfx950: SpMV, residual, weighted Jacobi, transfer
+// operators (P x, P^T r as CSR products) and the vector kernels of CG.
+//
+// SpMV follows SparseMatrix<RowMajor> * Vec of the reference's multigrid test
+// (test/multigrid.cpp:78,81,97): one group of LPR lanes per row (LPR picked
+// from the mean row length so a 64-wide wavefront stays busy), lanes stride
+// the row, then a butterfly reduction inside the group; the summation order
+// is fixed by the launch shape, so results are bitwise repeatable.
+#include "kernels.hpp"
+
+#include <algorithm>
+
+namespace pxg
+{
+
+namespace
+{
+constexpr int kBlock = 256;
+
+inline unsigned blocksFor(int64_t work, int block = kBlock)
+{
+  int64_t g = (work + block - 1) / block;
+  return (unsigned)std::max<int64_t>(g, 1);
+}
+
+enum : int
+{
+  kModeApply = 0,    // y = A x
+  kModeResidual = 1, // y = b - A x
+  kModeJacobi = 2,   // y = x + omega dinv (b - A x)
+  kModeAdd = 3,      // y += A x
+};
+
+template <typename T, int LPR, int MODE>
+__global__ void __launch_bounds__(kBlock)
+    k_csr(int64_t nrows, int64_t const * __restrict__ rowptr, int32_t const * __restrict__ col,
+          T const * __restrict__ val, T const * __restrict__ x, T const * __restrict__ b,
+          T const * __restrict__ dinv, T omega, T * __restrict__ y)
+{
+  int64_t const gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int64_t const row = gid / LPR;
+  int const lane = (int)(gid % LPR);
+  if (row >= nrows)
+    return; // whole LPR groups leave together
+  int64_t const k0 = rowptr[row], k1 = rowptr[row + 1];
+  T s = T(0);
+  for (int64_t k = k0 + lane; k < k1; k += LPR)
+    s += val[k] * x[col[k]];
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1)
+    s += __shfl_xor(s, off, LPR);
+  if (lane == 0)
+  {
+    if constexpr (MODE == kModeApply)
+      y[row] = s;
+    else if constexpr (MODE == kModeResidual)
+      y[row] = b[row] - s;
+    else if constexpr (MODE == kModeJacobi)
+      y[row] = x[row] + omega * dinv[row] * (b[row] - s);
+    else
+      y[row] += s;
+  }
+}
+
+template <typename T, int MODE>
+void launchCsr(CsrView const & A, T const * val, T const * x, T const * b, T const * dinv, double omega, T * y,
+               hipStream_t s)
+{
+  if (A.nrows == 0)
+    return;
+  double const mean = A.nnz > 0 ? (double)A.nnz / (double)A.nrows : 1.0;
+  int lpr = 4;
+  if (mean > 12)
+    lpr = 8;
+  if (mean > 24)
+    lpr = 16;
+  if (mean > 56)
+    lpr = 32;
+  if (mean > 120)
+    lpr = 64;
+  unsigned const g = blocksFor(A.nrows * lpr);
+  T const om = (T)omega;
+  switch (lpr)
+  {
+  case 4: k_csr<T, 4, MODE><<<g, kBlock, 0, s>>>(A.nrows, A.rowptr, A.col, val, x, b, dinv, om, y); break;
+  case 8: k_csr<T, 8, MODE><<<g, kBlock, 0, s>>>(A.nrows, A.rowptr, A.col, val, x, b, dinv, om, y); break;
+  case 16: k_csr<T, 16, MODE><<<g, kBlock, 0, s>>>(A.nrows, A.rowptr, A.col, val, x, b, dinv, om, y); break;
+  case 32: k_csr<T, 32, MODE><<<g, kBlock, 0, s>>>(A.nrows, A.rowptr, A.col, val, x, b, dinv, om, y); break;
+  default: k_csr<T, 64, MODE><<<g, kBlock, 0, s>>>(A.nrows, A.rowptr, A.col, val, x, b, dinv, om, y); break;
+  }
+  PXG_HIP(hipGetLastError());
+}
+
+template <typename T>
+T const * valOf(CsrView const & A);
+template <>
+double const * valOf<double>(CsrView const & A)
+{
+  return A.val;
+}
+template <>
+float const * valOf<float>(CsrView const & A)
+{
+  return A.val32;
+}
+template <typename T>
+T const * dinvOf(CsrView const & A);
+template <>
+double const * dinvOf<double>(CsrView const & A)
+{
+  return A.dinv;
+}
+template <>
+float const * dinvOf<float>(CsrView const & A)
+{
+  return A.dinv32;
+}
+
+// deterministic dot product: fixed grid, per-block tree in FP64, then a
+// single-block pass over the partials
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_dot_partial(int64_t n, T const * __restrict__ a, T const * __restrict__ b, double * __restrict__ partial)
+{
+  __shared__ double sh[kBlock];
+  double s = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    s += (double)a[i] * (double)b[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1)
+  {
+    if ((int)threadIdx.x < off)
+      sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    partial[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_dot_final(int nparts, double const * __restrict__ partial, double * out)
+{
+  __shared__ double sh[kBlock];
+  double s = 0.;
+  for (int i = threadIdx.x; i < nparts; i += kBlock)
+    s += partial[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1)
+  {
+    if ((int)threadIdx.x < off)
+      sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    *out = sh[0];
+}
+
+template <typename T>
+__global__ void k_fill(int64_t n, T * x, T v)
+{
+  int64_t const i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n)
+    x[i] = v;
+}
+
+template <typename Tout, typename Tin>
+__global__ void k_convert(int64_t n, Tin const * __restrict__ x, Tout * __restrict__ y)
+{
+  int64_t const i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n)
+    y[i] = (Tout)x[i];
+}
+
+template <typename T>
+__global__ void k_jacobi_diag(int64_t n, T const * __restrict__ dinv, T const * __restrict__ b, T * __restrict__ x,
+                              T omega)
+{
+  int64_t const i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n)
+    x[i] = omega * dinv[i] * b[i];
+}
+
+template <typename T>
+__global__ void k_cg_update(int64_t n, T * __restrict__ x, T const * __restrict__ p, T * __restrict__ r,
+                            T const * __restrict__ q, double const * __restrict__ sc, int ia, int ib)
+{
+  int64_t const i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n)
+    return;
+  T const alpha = (T)(sc[ia] / sc[ib]);
+  x[i] += alpha * p[i];
+  r[i] -= alpha * q[i];
+}
+
+template <typename T>
+__global__ void k_cg_direction(int64_t n, T * __restrict__ p, T const * __restrict__ z, double const * __restrict__ sc,
+                               int ia, int ib)
+{
+  int64_t const i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n)
+    return;
+  T const beta = (T)(sc[ia] / sc[ib]);
+  p[i] = z[i] + beta * p[i];
+}
+
+template <typename T, typename U>
+__global__ void k_add_to(int64_t n, T * __restrict__ x, U const * __restrict__ y)
+{
+  int64_t const i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n)
+    x[i] += (T)y[i];
+}
+
+__global__ void k_axpby(int64_t n, double a, double const * __restrict__ x, double b, double * __restrict__ y)
+{
+  int64_t const i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n)
+    y[i] = a * x[i] + b * y[i];
+}
+
+} // namespace
+
+CsrView viewOf(pxg_ctx const * c)
+{
+  CsrView v;
+  v.nrows = c->ndofs;
+  v.ncols = c->ndofs;
+  v.nnz = c->nnz;
+  v.rowptr = c->rowptr.p;
+  v.col = c->col.p;
+  v.val = c->val.p;
+  v.val32 = c->val32.p;
+  v.dinv = c->dinv.p;
+  v.dinv32 = c->dinv32.p;
+  return v;
+}
+
+template <typename T>
+void spmv(CsrView const & A, T const * x, T * y, hipStream_t s)
+{
+  launchCsr<T, kModeApply>(A, valOf<T>(A), x, nullptr, nullptr, 0., y, s);
+}
+template <typename T>
+void residual(CsrView const & A, T const * b, T const * x, T * y, hipStream_t s)
+{
+  launchCsr<T, kModeResidual>(A, valOf<T>(A), x, b, nullptr, 0., y, s);
+}
+template <typename T>
+void jacobiStep(CsrView const & A, T const * b, T const * x, T * y, double omega, hipStream_t s)
+{
+  launchCsr<T, kModeJacobi>(A, valOf<T>(A), x, b, dinvOf<T>(A), omega, y, s);
+}
+template <typename T>
+void spmvAdd(CsrView const & A, T const * x, T * y, hipStream_t s)
+{
+  launchCsr<T, kModeAdd>(A, valOf<T>(A), x, nullptr, nullptr, 0., y, s);
+}
+
+template <typename T>
+void dot(int64_t n, T const * a, T const * b, double * outDev, double * partial, hipStream_t s)
+{
+  k_dot_partial<T><<<kDotBlocks, kBlock, 0, s>>>(n, a, b, partial);
+  k_dot_final<<<1, kBlock, 0, s>>>(kDotBlocks, partial, outDev);
+  PXG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void fill(int64_t n, T * x, T v, hipStream_t s)
+{
+  k_fill<T><<<blocksFor(n), kBlock, 0, s>>>(n, x, v);
+  PXG_HIP(hipGetLastError());
+}
+
+template <typename Tout, typename Tin>
+void convert(int64_t n, Tin const * x, Tout * y, hipStream_t s)
+{
+  k_convert<Tout, Tin><<<blocksFor(n), kBlock, 0, s>>>(n, x, y);
+  PXG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void jacobiDiag(int64_t n, T const * dinv, T const * b, T * x, double omega, hipStream_t s)
+{
+  k_jacobi_diag<T><<<blocksFor(n), kBlock, 0, s>>>(n, dinv, b, x, (T)omega);
+  PXG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void cgUpdate(int64_t n, T * x, T const * p, T * r, T const * q, double const * sc, int ia, int ib, hipStream_t s)
+{
+  k_cg_update<T><<<blocksFor(n), kBlock, 0, s>>>(n, x, p, r, q, sc, ia, ib);
+  PXG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void cgDirection(int64_t n, T * p, T const * z, double const * sc, int ia, int ib, hipStream_t s)
+{
+  k_cg_direction<T><<<blocksFor(n), kBlock, 0, s>>>(n, p, z, sc, ia, ib);
+  PXG_HIP(hipGetLastError());
+}
+
+template <typename T, typename U>
+void addTo(int64_t n, T * x, U const * y, hipStream_t s)
+{
+  k_add_to<T, U><<<blocksFor(n), kBlock, 0, s>>>(n, x, y);
+  PXG_HIP(hipGetLastError());
+}
+
+void axpby(int64_t n, double a, double const * x, double b, double * y, hipStream_t s)
+{
+  k_axpby<<<blocksFor(n), kBlock, 0, s>>>(n, a, x, b, y);
+  PXG_HIP(hipGetLastError());
+}
+
+#define PXG_INST(T)                                                                                \
+  template void spmv<T>(CsrView const &, T const *, T *, hipStream_t);                             \
+  template void residual<T>(CsrView const &, T const *, T const *, T *, hipStream_t);              \
+  template void jacobiStep<T>(CsrView const &, T const *, T const *, T *, double, hipStream_t);    \
+  template void spmvAdd<T>(CsrView const &, T const *, T *, hipStream_t);                          \
+  template void dot<T>(int64_t, T const *, T const *, double *, double *, hipStream_t);            \
+  template void fill<T>(int64_t, T *, T, hipStream_t);                                             \
+  template void jacobiDiag<T>(int64_t, T const *, T const *, T *, double, hipStream_t);            \
+  template void cgUpdate<T>(int64_t, T *, T const *, T *, T const *, double const *, int, int,     \
+                            hipStream_t);                                                          \
+  template void cgDirection<T>(int64_t, T *, T const *, double const *, int, int, hipStream_t);
+PXG_INST(double)
+PXG_INST(float)
+template void convert<float, double>(int64_t, double const *, float *, hipStream_t);
+template void convert<double, float>(int64_t, float const *, double *, hipStream_t);
+template void addTo<double, float>(int64_t, double *, float const *, hipStream_t);
+template void addTo<double, double>(int64_t, double *, double const *, hipStream_t);
+template void addTo<float, float>(int64_t, float *, float const *, hipStream_t);
+
+} // namespace pxg

@@ -1,0 +1,56 @@
+"""Host mesh / dof layer of the product versus the oracle: point and element
+numbering, boundary facets and markers, first-visit dof numbering, Dirichlet
+dof sets and dof points must agree bit for bit (CPU only, no device calls)."""
+import numpy as np
+import pytest
+
+CASES = [
+    ("line_p1", "line", (7,)),
+    ("quad_q1", "square", (5, 3)),
+    ("quad_q2", "square", (4, 3)),
+    ("hex_q1", "cube_hex", (3, 2, 4)),
+    ("hex_q2", "cube_hex", (3, 3, 2)),
+    ("tet_p1", "cube_tet5", (3, 2, 2)),
+    ("tet_p2", "cube_tet5", (2, 3, 2)),
+    ("tet_p1", "cube_tet6", (2, 2, 3)),
+    ("tet_p2", "cube_tet6", (3, 3, 3)),
+]
+LEN = {"line": (1.0, 0.0, 0.0), "square": (1.0, 1.0, 0.0)}
+SIDES = {"line": ("left", "right"), "square": ("bottom", "right", "top", "left")}
+
+
+@pytest.mark.parametrize("fe,mesh,n", CASES, ids=[f"{c[0]}-{c[1]}" for c in CASES])
+def test_mesh_and_dofmap_bit_exact(oracle, native, fe, mesh, n):
+    ln = LEN.get(mesh, (1.0, 1.0, 1.0))
+    o = oracle.Space(fe, mesh, n, length=ln)
+    m = native.HostMesh(mesh, n, (0.0, 0.0, 0.0), ln)
+    xo, co = o.mesh()
+    xp, cp = m.arrays()
+    assert np.array_equal(xo.view(np.uint64), xp.view(np.uint64))
+    assert np.array_equal(co, cp)
+    nd, emap = m.dofmap(fe)
+    assert nd == o.ndofs
+    assert np.array_equal(emap, o.elem_map())
+    sides = SIDES.get(mesh, ("bottom", "right", "top", "left", "back", "front"))
+    for sd in sides:
+        assert np.array_equal(m.bc_dofs(fe, emap, sd), o.bc_dofs(sd)), sd
+    # boundary facets: same (element, side, marker) set
+    bo = {tuple(r) for r in o.bfacets()}
+    bp = {tuple(r) for r in m.bfacets()}
+    assert bo == bp
+    xdo = o.dof_coords()
+    xdp = m.dof_coords(fe, emap, nd)
+    assert np.array_equal(xdo.view(np.uint64), xdp.view(np.uint64))
+
+
+def test_large_q1_hex_numbering_matches_closed_form(native):
+    """Q1 on a hex grid numbers points in first-visit order; check the map
+    is a bijection onto the points and element 0 gets dofs 0..7."""
+    m = native.HostMesh("cube_hex", (16, 16, 16))
+    nd, emap = m.dofmap("hex_q1")
+    assert nd == 17 ** 3
+    assert np.array_equal(emap[0], np.arange(8))
+    _, conn = m.arrays()
+    perm = np.full(m.npts, -1)
+    perm[conn.ravel()] = emap.ravel()
+    assert (perm >= 0).all() and len(np.unique(perm)) == m.npts

@@ -1,0 +1,158 @@
+"""CPU checks that pin the oracle: the reference's own known answers, closed
+forms of the quadrature rules and bases, and exactness (patch) tests.
+
+Known answers: ||u_h - I_h u||_2 from the reference tests, compared at the
+reference's absolute tolerance 1e-12 (include/def.hpp:357-384).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+PI = np.pi
+
+
+def _solve(rp, col, val, b):
+    n = len(b)
+    A = sp.csr_matrix((val, col.astype(np.int64), rp), shape=(n, n))
+    return spl.spsolve(A.tocsc(), b)
+
+
+def _poisson_left_bottom(O, fe, mesh, n, length, qr=None):
+    s = O.Space(fe, mesh, n, length=length, qr=qr)
+    flag = np.zeros(s.ndofs, np.uint8)
+    for side in ("left", "bottom"):
+        flag[s.bc_dofs(side)] = 1
+    rp, col, v, b = s.assemble([("stiffness", 1.0)], bc=(flag, np.zeros(s.ndofs)), rhs="sinsin",
+                               rhs_params=(2.5 * PI * PI, 0.5 * PI, 1.5 * PI))
+    u = _solve(rp, col, v, b)
+    x = s.dof_coords()
+    return np.linalg.norm(u - np.sin(0.5 * PI * x[:, 0]) * np.sin(1.5 * PI * x[:, 1]))
+
+
+TESTQR = (np.array([[0.0, -2 / 3], [2 / 3, 0.0], [0.0, 2 / 3], [-2 / 3, 0.0]]), np.ones(4))
+
+KNOWN = [
+    # (fe, mesh, n, length, qr, expected, reference file:line)
+    ("quad_q1", "square", (10, 10), (1, 1, 0), None, 2.049777877938e-02, "test/poisson2dquad.cpp:131"),
+    ("quad_q1", "square", (20, 20), (1, 1, 0), None, 9.732187610621e-03, "test/poisson2dquad.cpp:141"),
+    ("quad_q1", "square", (10, 10), (1, 1, 0), TESTQR, 7.925676692262e-03, "test/poisson2dquad.cpp:151"),
+    ("quad_q1", "square", (20, 20), (1, 1, 0), TESTQR, 3.914800457522e-03, "test/poisson2dquad.cpp:161"),
+    ("quad_q2", "square", (10, 10), (1, 1, 0), None, 1.435848348336241e-04, "test/poisson2dquad_q2.cpp:69"),
+    ("hex_q1", "cube_hex", (4, 4, 4), (1, 1, 1), None, 0.1352139631934435, "test/poisson3dhex.cpp:118"),
+    ("hex_q2", "cube_hex", (4, 4, 4), (1, 1, 1), None, 0.0070042836287997, "test/poisson3dhex_q2.cpp:260"),
+    ("hex_q2", "cube_hex", (8, 8, 8), (1, 1, 1), None, 0.001164556821288369, "test/poisson3dhex_q2.cpp:266"),
+]
+
+
+@pytest.mark.parametrize("fe,mesh,n,length,qr,expected,ref", KNOWN, ids=[k[-1] for k in KNOWN])
+def test_known_answer_poisson(oracle, fe, mesh, n, length, qr, expected, ref):
+    err = _poisson_left_bottom(oracle, fe, mesh, n, length, qr)
+    assert abs(err - expected) < 1e-12, (err, expected, ref)
+
+
+def test_known_answer_fevar(oracle):
+    """Variable-coefficient stiffness + mass, 1D (test/fevar.cpp:58-104)."""
+    s = oracle.Space("line_p1", "line", (20,), length=(1, 0, 0))
+    flag = np.zeros(s.ndofs, np.uint8)
+    flag[s.bc_dofs("left")] = 1
+    x = s.dof_coords()
+    nu = np.sin(0.5 * PI * x[:, 0])
+    rp, col, v, b = s.assemble([("mass", 1.0), ("stiffness_fe", 1.0)], kappa=nu, bc=(flag, np.zeros(s.ndofs)),
+                               rhs="fevar1d", rhs_params=(0.25 * PI * PI, 0.5 * PI, PI))
+    u = _solve(rp, col, v, b)
+    err = np.linalg.norm(u - np.sin(0.5 * PI * x[:, 0]))
+    assert abs(err - 0.00111592884191975) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# quadrature exactness against closed-form monomial integrals (test/qr.cpp)
+def _mono_cube(p, dim):
+    # int_{[-1,1]^dim} prod x_k^{p_k}
+    out = 1.0
+    for k in range(dim):
+        out *= 0.0 if p[k] % 2 else 2.0 / (p[k] + 1)
+    return out
+
+
+def _mono_tet(p):
+    from math import factorial
+    a, b, c = p
+    return factorial(a) * factorial(b) * factorial(c) / factorial(a + b + c + 3)
+
+
+@pytest.mark.parametrize("fe,deg", [("quad_q1", 3), ("quad_q2", 5), ("hex_q1", 3), ("hex_q2", 5),
+                                    ("tet_p1", 2), ("tet_p2", 4)])
+def test_quadrature_exactness(oracle, fe, deg):
+    mesh = {"quad_q1": "square", "quad_q2": "square", "hex_q1": "cube_hex", "hex_q2": "cube_hex",
+            "tet_p1": "cube_tet5", "tet_p2": "cube_tet5"}[fe]
+    # reference element: one element spanning the reference coordinates
+    if fe.startswith("tet"):
+        s = oracle.Space(fe, mesh, (1, 1, 1))
+        xyz, conn = s.mesh()
+        # move the vertices of element 0 onto the reference tetrahedron
+        xyz[conn[0]] = [[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]]
+        s.set_coords(xyz)
+        g = s.geometry(0, 1)
+        q, w = g["qpt"][0], g["JxW"][0]
+        tol = 1e-13
+        for a in range(deg + 1):
+            for b in range(deg + 1 - a):
+                for c in range(deg + 1 - a - b):
+                    num = np.sum(w * q[:, 0] ** a * q[:, 1] ** b * q[:, 2] ** c)
+                    assert abs(num - _mono_tet((a, b, c))) < tol * 10, (a, b, c)
+    else:
+        dim = 2 if fe.startswith("quad") else 3
+        ln = (2.0, 2.0, 2.0 if dim == 3 else 0.0)
+        s = oracle.Space(fe, mesh, (1, 1, 1)[:dim], origin=(-1.0, -1.0, -1.0 if dim == 3 else 0.0), length=ln)
+        g = s.geometry()
+        q, w = g["qpt"][0], g["JxW"][0]
+        for p in np.ndindex(*([deg + 1] * dim)):
+            if sum(p) > deg and fe.endswith("q1"):
+                continue
+            num = np.sum(w * np.prod([q[:, k] ** p[k] for k in range(dim)], axis=0))
+            assert abs(num - _mono_cube(p, dim)) < 1e-13, p
+
+
+@pytest.mark.parametrize("fe,mesh", [("quad_q1", "square"), ("quad_q2", "square"), ("hex_q1", "cube_hex"),
+                                     ("hex_q2", "cube_hex"), ("tet_p1", "cube_tet6"), ("tet_p2", "cube_tet6")])
+def test_basis_interpolates_and_partition_of_unity(oracle, fe, mesh):
+    """phi_j(x_i) = delta_ij and sum_j grad phi_j = 0 (test/reffe_test.cpp:27-56),
+    checked through interpolation: the FE interpolant of a polynomial in the
+    space reproduces it, so mass-matrix row sums give the element volume."""
+    n = (2, 2, 2)[: (2 if mesh == "square" else 3)]
+    s = oracle.Space(fe, mesh, n, length=(1, 1, 1) if mesh != "square" else (1, 1, 0))
+    rp, col, v, b = s.assemble([("mass", 1.0)])
+    assert abs(v.sum() - 1.0) < 1e-12  # sum of all mass entries = |Omega|
+    rp, col, v, b = s.assemble([("stiffness", 1.0)])
+    A = sp.csr_matrix((v, col.astype(np.int64), rp), shape=(s.ndofs, s.ndofs))
+    assert np.abs(A @ np.ones(s.ndofs)).max() < 1e-12  # stiffness annihilates constants
+    x = s.dof_coords()
+    lin = x[:, 0] + 2 * x[:, 1] - 0.5 * (x[:, 2] if mesh != "square" else 0)
+    # energy of a linear function: |grad|^2 * |Omega|
+    g2 = 1 + 4 + (0.25 if mesh != "square" else 0)
+    assert abs(lin @ (A @ lin) - g2) < 1e-11
+
+
+@pytest.mark.parametrize("fe,mesh,pert", [("tet_p2", "cube_tet6", 0.2), ("tet_p2", "cube_tet5", 0.2),
+                                          ("hex_q2", "cube_hex", 0.0), ("quad_q2", "square", 0.0)])
+def test_patch_quadratic_exact(oracle, fe, mesh, pert):
+    """Quadratic elements reproduce u = x^2 + y^2 (+ z^2) exactly (-lap u = -2 dim),
+    also on perturbed (affine) tets: pins CurFE, the P2/Q2 bases and the rhs."""
+    dim = 2 if mesh == "square" else 3
+    n = (3, 3, 3)[:dim]
+    ln = (1.0, 1.0, 1.0) if dim == 3 else (1.0, 1.0, 0.0)
+    s = oracle.Space(fe, mesh, n, length=ln)
+    if pert:
+        from proxpde_amd.workloads import perturb_interior
+        xyz, _ = s.mesh()
+        s.set_coords(perturb_interior(xyz, 1.0 / 3, seed=5, amount=pert))
+    x = s.dof_coords()
+    u = (x[:, :dim] ** 2).sum(axis=1)
+    flag = np.zeros(s.ndofs, np.uint8)
+    sides = ("bottom", "right", "top", "left") + (("back", "front") if dim == 3 else ())
+    for sd in sides:
+        flag[s.bc_dofs(sd)] = 1
+    rp, col, v, b = s.assemble([("stiffness", 1.0)], bc=(flag, u), rhs="const", rhs_params=(-2.0 * dim,))
+    uh = _solve(rp, col, v, b)
+    assert np.abs(uh - u).max() < 1e-11
