@@ -105,8 +105,9 @@ def main():
     a.profile(False)
     nnz = a.nnz
 
-    # --- SpMV on the assembled matrix (torch stream so torch events time it)
-    stream = torch.cuda.current_stream()
+    # --- SpMV on the assembled matrix, on a torch stream so torch events time it
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
     a.set_stream(stream.cuda_stream)
     gen = torch.Generator(device=dev)
     gen.manual_seed(0)
@@ -134,7 +135,6 @@ def main():
     torch.cuda.synchronize()
     spmv32_ms = ev0.elapsed_time(ev1) / args.spmv
     spmv32_gbs = (nnz * 8 + 8 * (prob.ndofs + 1) + 8 * prob.ndofs) / (spmv32_ms * 1e-3) / 1e9
-    a.set_stream(None)
 
     # --- roofline of the dominant kernel
     per = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in prof.items()}

@@ -77,8 +77,10 @@ typedef struct pxg_term
 /* context ----------------------------------------------------------------- */
 int pxg_create(pxg_ctx ** ctx, int device);
 int pxg_destroy(pxg_ctx * ctx);
+/* message of the last failure on ctx (ctx NULL: the last failed pxg_create) */
 const char * pxg_last_error(const pxg_ctx * ctx);
-/* run the context's work on an existing HIP stream (NULL: own stream) */
+/* run the context's work on an existing HIP stream (NULL: the default
+ * stream); the context creates a non-blocking stream of its own at create  */
 int pxg_set_stream(pxg_ctx * ctx, void * hip_stream);
 int pxg_synchronize(pxg_ctx * ctx);
 /* per-kernel timing with HIP events on the context stream: enable resets the

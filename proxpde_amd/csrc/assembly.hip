@@ -341,42 +341,72 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // --------------------------------------------------------------------------
-// closeMatrix: prune(1, eps) drops |v| <= eps
+// closeMatrix: prune(1, eps) drops |v| <= eps.  Keep flags over the
+// structural nonzeros, an exclusive scan gives every kept entry its slot and
+// every row its start; the diagonal is read through its structural position.
 template <typename T>
-__global__ void k_prune_count(int64_t nrows, int64_t const * __restrict__ rowptrS, T const * __restrict__ val,
-                              double eps, int64_t * __restrict__ cnt)
+__global__ void __launch_bounds__(kBlock)
+    k_keep_flags(int64_t nnzS, T const * __restrict__ val, double eps, uint32_t * __restrict__ keep)
+{
+  int64_t const k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nnzS)
+    keep[k] = (fabs((double)val[k]) <= eps) ? 0u : 1u;
+  else if (k == nnzS)
+    keep[k] = 0u;
+}
+
+__global__ void k_rowptr_gather(int64_t nrows, int64_t const * __restrict__ rowptrS,
+                                uint32_t const * __restrict__ pos, int64_t * __restrict__ rowptr)
 {
   int64_t const r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrows)
-    return;
-  int64_t c = 0;
-  for (int64_t k = rowptrS[r]; k < rowptrS[r + 1]; ++k)
-    c += (fabs((double)val[k]) <= eps) ? 0 : 1;
-  cnt[r] = c;
+  if (r <= nrows)
+    rowptr[r] = pos[rowptrS[r]];
 }
 
 template <typename T>
-__global__ void k_prune_compact(int64_t nrows, int64_t const * __restrict__ rowptrS, int32_t const * __restrict__ colS,
-                                T const * __restrict__ valS, double eps, int64_t const * __restrict__ rowptr,
-                                int32_t * __restrict__ col, T * __restrict__ val, T * __restrict__ dinv)
+__global__ void __launch_bounds__(kBlock)
+    k_compact(int64_t nnzS, T const * __restrict__ valS, int32_t const * __restrict__ colS, double eps,
+              uint32_t const * __restrict__ pos, int32_t * __restrict__ col, T * __restrict__ val)
+{
+  int64_t const k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnzS)
+    return;
+  T const v = valS[k];
+  if (fabs((double)v) <= eps)
+    return;
+  uint32_t const o = pos[k];
+  col[o] = colS[k];
+  val[o] = v;
+}
+
+template <typename T>
+__global__ void k_diag_inverse(int64_t nrows, uint32_t const * __restrict__ diagPos, T const * __restrict__ valS,
+                               double eps, T * __restrict__ dinv)
 {
   int64_t const r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows)
     return;
-  int64_t o = rowptr[r];
-  T d = T(0);
-  for (int64_t k = rowptrS[r]; k < rowptrS[r + 1]; ++k)
+  T const d = valS[diagPos[r]];
+  dinv[r] = (fabs((double)d) <= eps || d == T(0)) ? T(0) : T(1) / d;
+}
+
+// structural position of every diagonal entry
+__global__ void k_diag_pos(int64_t nrows, int64_t const * __restrict__ rowptrS, int32_t const * __restrict__ colS,
+                           uint32_t * __restrict__ diagPos)
+{
+  int64_t const r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows)
+    return;
+  int64_t lo = rowptrS[r], hi = rowptrS[r + 1];
+  while (lo + 1 < hi)
   {
-    T const v = valS[k];
-    if (fabs((double)v) <= eps)
-      continue;
-    col[o] = colS[k];
-    val[o] = v;
-    if (colS[k] == (int32_t)r)
-      d = v;
-    ++o;
+    int64_t const mid = (lo + hi) / 2;
+    if (colS[mid] <= (int32_t)r)
+      lo = mid;
+    else
+      hi = mid;
   }
-  dinv[r] = d != T(0) ? T(1) / d : T(0);
+  diagPos[r] = (uint32_t)lo;
 }
 
 template <typename Tout, typename Tin>
@@ -651,6 +681,9 @@ void buildPattern(pxg_ctx * c)
   }
   buildDofAdjacency(c);
   c->sortTmp.release();
+  c->diagPos.resize(c->ndofs);
+  k_diag_pos<<<gridFor(c->ndofs), kBlock, 0, s>>>(c->ndofs, c->rowptrS.p, c->colS.p, c->diagPos.p);
+  PXG_HIP(hipGetLastError());
   c->valS.resize(c->nnzS);
   c->valS32.resize(c->nnzS);
   PXG_HIP(hipMemsetAsync(c->valS.p, 0, c->nnzS * sizeof(double), s));
@@ -836,44 +869,45 @@ void closeMatrix(pxg_ctx * c, double eps)
 {
   PXG_REQUIRE(c->nnzS > 0, PXG_E_STATE, "close needs build_pattern");
   hipStream_t s = c->stream;
-  int64_t const nr = c->ndofs;
-  DevBuf<int64_t> cnt;
-  cnt.resize(nr + 1);
+  int64_t const nr = c->ndofs, ns = c->nnzS;
   bool const f32 = c->valPrecision == PXG_FP32;
   ProfScope ps(c, "close");
+  c->keep.resize(ns + 1);
+  c->pos.resize(ns + 1);
   if (f32)
-    k_prune_count<float><<<gridFor(nr), kBlock, 0, s>>>(nr, c->rowptrS.p, c->valS32.p, eps, cnt.p);
+    k_keep_flags<float><<<gridFor(ns + 1), kBlock, 0, s>>>(ns, c->valS32.p, eps, c->keep.p);
   else
-    k_prune_count<double><<<gridFor(nr), kBlock, 0, s>>>(nr, c->rowptrS.p, c->valS.p, eps, cnt.p);
+    k_keep_flags<double><<<gridFor(ns + 1), kBlock, 0, s>>>(ns, c->valS.p, eps, c->keep.p);
   PXG_HIP(hipGetLastError());
-  PXG_HIP(hipMemsetAsync(cnt.p + nr, 0, sizeof(int64_t), s));
-  c->rowptr.resize(nr + 1);
   size_t tb = 0;
-  PXG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.p, c->rowptr.p, (int)(nr + 1), s));
-  DevBuf<unsigned char> tmp;
-  tmp.resize(tb);
-  PXG_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, cnt.p, c->rowptr.p, (int)(nr + 1), s));
-  int64_t nnz = 0;
-  PXG_HIP(hipMemcpyAsync(&nnz, c->rowptr.p + nr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  PXG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, c->keep.p, c->pos.p, (int)(ns + 1), s));
+  c->scanTmp.resize(std::max(tb, c->scanTmp.n));
+  PXG_HIP(hipcub::DeviceScan::ExclusiveSum(c->scanTmp.p, tb, c->keep.p, c->pos.p, (int)(ns + 1), s));
+  uint32_t nnz = 0;
+  PXG_HIP(hipMemcpyAsync(&nnz, c->pos.p + ns, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   PXG_HIP(hipStreamSynchronize(s));
   c->nnz = nnz;
-  c->col.resize(nnz);
-  c->val.resize(nnz);
-  c->val32.resize(nnz);
+  c->rowptr.resize(nr + 1);
+  c->col.resize(std::max<int64_t>(nnz, 1));
+  c->val.resize(std::max<int64_t>(nnz, 1));
+  c->val32.resize(std::max<int64_t>(nnz, 1));
   c->dinv.resize(nr);
   c->dinv32.resize(nr);
+  k_rowptr_gather<<<gridFor(nr + 1), kBlock, 0, s>>>(nr, c->rowptrS.p, c->pos.p, c->rowptr.p);
   if (f32)
   {
-    k_prune_compact<float><<<gridFor(nr), kBlock, 0, s>>>(nr, c->rowptrS.p, c->colS.p, c->valS32.p, eps,
-                                                          c->rowptr.p, c->col.p, c->val32.p, c->dinv32.p);
+    k_compact<float><<<gridFor(ns), kBlock, 0, s>>>(ns, c->valS32.p, c->colS.p, eps, c->pos.p, c->col.p,
+                                                    c->val32.p);
+    k_diag_inverse<float><<<gridFor(nr), kBlock, 0, s>>>(nr, c->diagPos.p, c->valS32.p, eps, c->dinv32.p);
     PXG_HIP(hipGetLastError());
     k_cast<double, float><<<gridFor(nnz), kBlock, 0, s>>>(nnz, c->val32.p, c->val.p);
     k_cast<double, float><<<gridFor(nr), kBlock, 0, s>>>(nr, c->dinv32.p, c->dinv.p);
   }
   else
   {
-    k_prune_compact<double><<<gridFor(nr), kBlock, 0, s>>>(nr, c->rowptrS.p, c->colS.p, c->valS.p, eps,
-                                                           c->rowptr.p, c->col.p, c->val.p, c->dinv.p);
+    k_compact<double><<<gridFor(ns), kBlock, 0, s>>>(ns, c->valS.p, c->colS.p, eps, c->pos.p, c->col.p,
+                                                     c->val.p);
+    k_diag_inverse<double><<<gridFor(nr), kBlock, 0, s>>>(nr, c->diagPos.p, c->valS.p, eps, c->dinv.p);
     PXG_HIP(hipGetLastError());
     k_cast<float, double><<<gridFor(nnz), kBlock, 0, s>>>(nnz, c->val.p, c->val32.p);
     k_cast<float, double><<<gridFor(nr), kBlock, 0, s>>>(nr, c->dinv.p, c->dinv32.p);

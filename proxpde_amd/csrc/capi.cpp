@@ -10,6 +10,9 @@ using namespace pxg;
 
 namespace
 {
+// message of the last failure that had no context to hold it (pxg_create)
+thread_local std::string g_lastError;
+
 template <typename F>
 int guarded(pxg_ctx * c, F && f)
 {
@@ -116,6 +119,7 @@ int pxg_create(pxg_ctx ** out, int device)
                    });
   if (rc != PXG_OK)
   {
+    g_lastError = c->err;
     delete c;
     return rc;
   }
@@ -138,7 +142,7 @@ int pxg_destroy(pxg_ctx * c)
   return PXG_OK;
 }
 
-const char * pxg_last_error(const pxg_ctx * c) { return c ? c->err.c_str() : "null context"; }
+const char * pxg_last_error(const pxg_ctx * c) { return c ? c->err.c_str() : g_lastError.c_str(); }
 
 int pxg_set_stream(pxg_ctx * c, void * stream)
 {
@@ -148,16 +152,8 @@ int pxg_set_stream(pxg_ctx * c, void * stream)
                    PXG_HIP(hipStreamSynchronize(c->stream));
                    if (c->ownStream)
                      PXG_HIP(hipStreamDestroy(c->stream));
-                   if (stream)
-                   {
-                     c->stream = (hipStream_t)stream;
-                     c->ownStream = false;
-                   }
-                   else
-                   {
-                     PXG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-                     c->ownStream = true;
-                   }
+                   c->stream = (hipStream_t)stream; // NULL: the default stream
+                   c->ownStream = false;
                  });
 }
 

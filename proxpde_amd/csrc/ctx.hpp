@@ -138,6 +138,9 @@ struct pxg_ctx
   pxg::DevBuf<uint32_t> segPtr;  // nnzS + 1 offsets into contrib
   pxg::DevBuf<uint32_t> contrib; // E*n*n contribution ids sorted by (row, col, element)
   pxg::DevBuf<uint32_t> emapNnz; // E*n*n -> structural nnz id
+  pxg::DevBuf<uint32_t> diagPos; // structural position of each diagonal entry
+  pxg::DevBuf<uint32_t> keep, pos; // close(): keep flags and their exclusive scan
+  pxg::DevBuf<unsigned char> scanTmp;
   pxg::DevBuf<uint32_t> dofSeg;  // ndofs + 1 offsets into dofContrib
   pxg::DevBuf<uint32_t> dofContrib; // E*n (element, local) ids sorted by (dof, element)
   // accumulated values (structural) and rhs

@@ -188,7 +188,8 @@ __global__ void k_cg_update(int64_t n, T * __restrict__ x, T const * __restrict_
   int64_t const i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n)
     return;
-  T const alpha = (T)(sc[ia] / sc[ib]);
+  // a zero denominator means the search direction vanished (exact convergence)
+  T const alpha = sc[ib] != 0. ? (T)(sc[ia] / sc[ib]) : T(0);
   x[i] += alpha * p[i];
   r[i] -= alpha * q[i];
 }
@@ -200,7 +201,7 @@ __global__ void k_cg_direction(int64_t n, T * __restrict__ p, T const * __restri
   int64_t const i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n)
     return;
-  T const beta = (T)(sc[ia] / sc[ib]);
+  T const beta = sc[ib] != 0. ? (T)(sc[ia] / sc[ib]) : T(0);
   p[i] = z[i] + beta * p[i];
 }
 

@@ -42,6 +42,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() or python -m proxpde_amd.build")
+    # One HIP runtime per process: when PyTorch is present its runtime is
+    # loaded first, so the library binds to it (same soname) and device
+    # pointers / streams can be shared with torch tensors.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, i32, i64 = C.c_void_p, C.c_int, C.c_int64
     dp, fp = C.POINTER(C.c_double), C.POINTER(C.c_float)
@@ -190,7 +197,7 @@ class Assembler:
         h = C.c_void_p()
         rc = L.pxg_create(C.byref(h), device)
         if rc != 0:
-            raise PxgError(rc, "pxg_create failed")
+            raise PxgError(rc, "pxg_create failed: " + L.pxg_last_error(None).decode())
         self.h = h
         self.ndofs = 0
         self.nnz = 0
