@@ -47,6 +47,60 @@ def element_kernel_flops(nq=27, n=27, dim=3):
     return nq * n * (2 * dim * dim + 2 * dim * n)
 
 
+def solve_and_adjoint(args, dev, N, W):
+    """Q2 hex n^3: u = K(kappa)^{-1} f (MG-PCG to rel. residual 1e-8, levels
+    n, n/2, ...), J = 1/2 |u - u_t|^2 and dJ/dkappa by the adjoint solve.
+    kappa = exp(GRF seed 2); u_t = K(kappa_t)^{-1} f with kappa_t = exp(GRF seed 4).
+    Timed: assembly of all levels + solve + adjoint (forward and backward)."""
+    import torch
+
+    from proxpde_amd import autograd as A
+
+    n = args.n
+    ns = [n >> l for l in range(args.solve_levels)]
+    t0 = time.time()
+    solver = A.KappaSolver("hex_q2", ns)
+    t_setup = time.time() - t0
+    x = solver.probs[0].dof_coords()
+    kap = torch.from_numpy(np.exp(W.grf(x, seed=2, ell=0.1, features=1024, device=dev))).to(dev)
+    kap_t = torch.from_numpy(np.exp(W.grf(x, seed=4, ell=0.1, features=1024, device=dev))).to(dev)
+    b = torch.from_numpy(solver.rhs(1.0)).to(dev)
+    with torch.no_grad():
+        ut = solver(kap_t, b, rtol=1e-10)
+    res = {}
+    sol = {}
+    for name, mode in (("fp64", N.FP64), ("fp32_ir", N.FP32)):
+        times = []
+        for rep in range(args.solve_reps + 1):
+            k = kap.clone().requires_grad_()
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            u = solver(k, b, rtol=1e-8, mode=mode)
+            J = 0.5 * ((u - ut) ** 2).sum()
+            J.backward()
+            torch.cuda.synchronize()
+            if rep > 0:
+                times.append(time.perf_counter() - ts)
+        sol[name] = (u.detach(), k.grad.detach(), float(J))
+        res[name] = {"ms": 1e3 * min(times)}
+    # FP64 residual of each solution with the matrix-free FP64 operator
+    op = A.KappaOperator(solver.ctx[0], use_bc=True)
+    for name in sol:
+        u = sol[name][0]
+        with torch.no_grad():
+            r = b - op(kap, u.unsqueeze(1)).squeeze(1)
+        res[name]["relres_fp64"] = float(r.norm() / b.norm())
+    u64, g64, J64 = sol["fp64"]
+    u32, g32, J32 = sol["fp32_ir"]
+    res["fp32_ir"]["u_rel_err_vs_fp64"] = float((u32 - u64).norm() / u64.norm())
+    res["fp32_ir"]["grad_rel_err_vs_fp64"] = float((g32 - g64).norm() / g64.norm())
+    res["speedup_fp32_ir_over_fp64"] = res["fp64"]["ms"] / res["fp32_ir"]["ms"]
+    res["levels"] = ns
+    res["setup_s"] = t_setup
+    res["J"] = J64
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,6 +110,9 @@ def main():
     ap.add_argument("--spmv", type=int, default=100, help="SpMV applications timed")
     ap.add_argument("--cpu-sample", type=int, default=12, help="elements per axis of the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--solve-levels", type=int, default=4)
+    ap.add_argument("--solve-reps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -154,6 +211,11 @@ def main():
         roof = {"kernel": dom, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": None, "traffic": None}
 
+    # --- solve + adjoint gradient, plain FP64 vs FP32 + FP64 refinement
+    solve = None
+    if not args.no_solve:
+        solve = solve_and_adjoint(args, dev, N, W)
+
     # --- CPU baseline: the oracle's serial Builder on a bounded sample
     cpu = None
     if not args.no_cpu:
@@ -200,6 +262,7 @@ def main():
                  "fp32_ms": spmv32_ms, "fp32_GB/s": spmv32_gbs, "bytes": spmv_bytes},
         "reference_flops_per_elem": q2hex_flops_per_elem(),
         "reference_gflops_per_s": q2hex_flops_per_elem() * value / 1e9,
+        "solve_adjoint": solve,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -108,6 +108,8 @@ int pxg_set_dirichlet(pxg_ctx * ctx, int64_t n_c, const uint32_t * dofs, const d
 
 /* nodal coefficient field kappa (FEVar on the same space) for FE terms    */
 int pxg_set_coefficient(pxg_ctx * ctx, const double * kappa_host);
+/* same from device memory (copied on the context stream)                  */
+int pxg_set_coefficient_device(pxg_ctx * ctx, const double * kappa_dev);
 
 /* CSR sparsity pattern and element->nnz map from the dof map (once per mesh):
  * sorted segmented reduction of the E*n*n (row, col) keys.                */

@@ -69,6 +69,34 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     return LIB
 
 
+TORCH_SRC = os.path.join(HERE, "torch_ext", "ops.cpp")
+TORCH_LIB = os.path.join(HERE, "libproxpde_torch.so")
+
+
+def build_torch_ext(verbose: bool = False) -> str:
+    """LibTorch autograd operators (host C++ only: g++ against the torch headers,
+    linked to libproxpde_amd.so)."""
+    import torch.utils.cpp_extension as ce
+    import torch
+
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    inc = sum([["-I", p] for p in ce.include_paths()], [])
+    libdirs = ce.library_paths()
+    tmp = TORCH_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-D_GLIBCXX_USE_CXX11_ABI={abi}", *inc,
+           TORCH_SRC, "-o", tmp, *[f"-L{d}" for d in libdirs], "-ltorch", "-ltorch_cpu", "-lc10",
+           f"-L{HERE}", "-lproxpde_amd", "-Wl,-rpath,$ORIGIN", *[f"-Wl,-rpath,{d}" for d in libdirs]]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"torch extension build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, TORCH_LIB)
+    if verbose:
+        print(f"built {TORCH_LIB}")
+    return TORCH_LIB
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    if "--torch" in sys.argv:
+        build_torch_ext(verbose=True)
     sys.exit(0)

@@ -281,6 +281,20 @@ int pxg_set_coefficient(pxg_ctx * c, const double * kappa)
                  });
 }
 
+int pxg_set_coefficient_device(pxg_ctx * c, const double * kappa)
+{
+  return guarded(c,
+                 [&]
+                 {
+                   PXG_REQUIRE(c->ndofs > 0, PXG_E_STATE, "set_dofmap before set_coefficient");
+                   PXG_REQUIRE(kappa, PXG_E_INVALID, "null coefficient");
+                   c->kappa.resize(c->ndofs);
+                   PXG_HIP(hipMemcpyAsync(c->kappa.p, kappa, c->ndofs * sizeof(double), hipMemcpyDeviceToDevice,
+                                          c->stream));
+                   c->hasKappa = true;
+                 });
+}
+
 int pxg_build_pattern(pxg_ctx * c, int64_t * nnz)
 {
   return guarded(c,

@@ -558,6 +558,7 @@ int pxg_mg_solve(pxg_mg * m, int mode, const double * b, double * x, double rtol
   try
   {
     PXG_HIP(hipSetDevice(m->ctx[0]->device));
+    m->s = m->ctx[0]->stream; // follow the finest context (its stream may have been replaced)
     for (auto * c: m->ctx)
       PXG_REQUIRE(c->closed, PXG_E_STATE, "levels must be closed");
     double rr = 0.;
@@ -593,6 +594,7 @@ int pxg_mg_vcycle(pxg_mg * m, const double * b, double * x)
   try
   {
     PXG_HIP(hipSetDevice(m->ctx[0]->device));
+    m->s = m->ctx[0]->stream; // follow the finest context (its stream may have been replaced)
     vcycle<double>(m, 0, b, x);
     PXG_HIP(hipStreamSynchronize(m->s));
     return PXG_OK;

@@ -287,6 +287,11 @@ class Assembler:
             self._ck(lib().pxg_get_csr(self.h, _p(rowptr, C.c_int64), _p(col, C.c_int32), _p(val), None, _p(b)))
         return rowptr, col, val, b
 
+    def rhs_host(self):
+        b = np.empty(self.ndofs)
+        self._ck(lib().pxg_get_csr(self.h, None, None, None, None, _p(b)))
+        return b
+
     def pattern(self):
         rowptr = np.empty(self.ndofs + 1, dtype=np.int64)
         col = np.empty(self.nnz_structural, dtype=np.int32)
