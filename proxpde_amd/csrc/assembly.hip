@@ -22,6 +22,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <string>
 
 namespace pxg
 {
@@ -716,33 +718,50 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
   hipStream_t s = c->stream;
   FeTables & T = c->fe;
   int64_t const E = c->nelem;
-  c->geoBuf.resize(E * T.nq * 8);
   PXG_HIP(hipMemsetAsync(c->status.p, 0, sizeof(int), s));
-  dispatchGeo(T.geo,
-              [&](auto dimC, auto npeC)
-              {
-                constexpr int D = decltype(dimC)::value, NPE = decltype(npeC)::value;
-                ProfScope ps(c, "geometry");
-                k_geometry<D, NPE><<<gridFor(E * T.nq), kBlock, 0, s>>>(
-                    E, T.nq, T.n, c->xyz.p, c->conn.p, T.d_geoGrad.p, T.d_w.p, T.d_phi.p, c->elemMap.p,
-                    c->kappa.p, cS, cSF, cM, cMF, flags, c->geoBuf.p, c->status.p);
-              });
-  PXG_HIP(hipGetLastError());
   bool const hasMass = (flags & (kHasMass | kHasMassFE)) != 0;
   c->feBuf.resize(E * T.n);
+  // fused geometry + matrix-core element kernel for the quadratic elements;
+  // geometry kernel + row kernel otherwise (PXG_ELEMENT_PATH=rows forces the latter)
+  static bool const rowsOnly = []
+  {
+    char const * v = std::getenv("PXG_ELEMENT_PATH");
+    return v && std::string(v) == "rows";
+  }();
+  bool fused = false;
+  if (!rowsOnly)
+  {
+    ProfScope ps(c, precision == PXG_FP64 ? "element" : "element32");
+    fused = elementMfma(c, flags, cS, cSF, cM, cMF, precision);
+  }
+  if (!fused)
+  {
+    c->geoBuf.resize(E * T.nq * 8);
+    dispatchGeo(T.geo,
+                [&](auto dimC, auto npeC)
+                {
+                  constexpr int D = decltype(dimC)::value, NPE = decltype(npeC)::value;
+                  ProfScope ps(c, "geometry");
+                  k_geometry<D, NPE><<<gridFor(E * T.nq), kBlock, 0, s>>>(
+                      E, T.nq, T.n, c->xyz.p, c->conn.p, T.d_geoGrad.p, T.d_w.p, T.d_phi.p, c->elemMap.p,
+                      c->kappa.p, cS, cSF, cM, cMF, flags, c->geoBuf.p, c->status.p);
+                });
+    PXG_HIP(hipGetLastError());
+  }
   dispatchFe(T.feType,
              [&](auto nC, auto dimC)
              {
                constexpr int N = decltype(nC)::value, D = decltype(dimC)::value;
                if (precision == PXG_FP64)
                {
-                 c->keBuf.resize(E * N * N);
-                 size_t const sh = (size_t)T.nq * N * (D + 1) * sizeof(double);
+                 if (!fused)
                  {
-                 ProfScope ps(c, "element");
-                 k_element_rows<double, N, D><<<gridFor(E * N), kBlock, sh, s>>>(
-                     E, T.nq, c->geoBuf.p, T.d_grad.p, T.d_phi.p, hasMass, c->elemMap.p, c->bcFlag.p, c->bcVal.p,
-                     c->hasBC, c->keBuf.p, c->feBuf.p);
+                   c->keBuf.resize(E * N * N);
+                   size_t const sh = (size_t)T.nq * N * (D + 1) * sizeof(double);
+                   ProfScope ps(c, "element");
+                   k_element_rows<double, N, D><<<gridFor(E * N), kBlock, sh, s>>>(
+                       E, T.nq, c->geoBuf.p, T.d_grad.p, T.d_phi.p, hasMass, c->elemMap.p, c->bcFlag.p,
+                       c->bcVal.p, c->hasBC, c->keBuf.p, c->feBuf.p);
                  }
                  PXG_HIP(hipGetLastError());
                  ProfScope ps2(c, "scatter");
@@ -751,13 +770,14 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                }
                else
                {
-                 c->keBuf32.resize(E * N * N);
-                 size_t const sh = (size_t)T.nq * N * (D + 1) * sizeof(float);
+                 if (!fused)
                  {
-                 ProfScope ps(c, "element32");
-                 k_element_rows<float, N, D><<<gridFor(E * N), kBlock, sh, s>>>(
-                     E, T.nq, c->geoBuf.p, T.d_grad32.p, T.d_phi32.p, hasMass, c->elemMap.p, c->bcFlag.p,
-                     c->bcVal.p, c->hasBC, c->keBuf32.p, c->feBuf.p);
+                   c->keBuf32.resize(E * N * N);
+                   size_t const sh = (size_t)T.nq * N * (D + 1) * sizeof(float);
+                   ProfScope ps(c, "element32");
+                   k_element_rows<float, N, D><<<gridFor(E * N), kBlock, sh, s>>>(
+                       E, T.nq, c->geoBuf.p, T.d_grad32.p, T.d_phi32.p, hasMass, c->elemMap.p, c->bcFlag.p,
+                       c->bcVal.p, c->hasBC, c->keBuf32.p, c->feBuf.p);
                  }
                  PXG_HIP(hipGetLastError());
                  ProfScope ps2(c, "scatter32");

@@ -1,0 +1,422 @@
+// Element matrices on the matrix cores (gfx950 MFMA), geometry fused in.
+//
+// For an element with local basis gradients g_j(q) (reference coordinates),
+// per-point metric M_q = s_q (J_q^T J_q)^{-1} and mass weight m_q:
+//     Ke = A * C,   A[i][k] = g_i(q,a) | phi_i(q)                     (N x K, constant)
+//                   C[k][j] = sum_b M_q^{ab} g_j(q,b) | m_q phi_j(q)  (K x N, per element)
+// with k running over the (q, a) pairs, then over q for the mass rows.  A is
+// the same for every element, so its MFMA fragments stay in registers for the
+// whole kernel; each C operand value is formed in registers right before its
+// matrix instruction from the per-point metric (LDS) and the gradient table
+// (LDS), so no K x N staging buffer is needed and more waves fit per CU.
+// The N x N product runs as v_mfma_f64_16x16x4f64 (FP64) or
+// v_mfma_f32_16x16x4f32 (FP32) tiles; only the upper tiles are multiplied
+// (Ke is symmetric) and the lower one is mirrored.  The epilogue applies the
+// per-element Dirichlet elimination of Builder::buildLhs (reference
+// include/builder.hpp:83-124) and writes Ke rows contiguously plus the lifting Fe.
+//
+// One wavefront per element; kWaves wavefronts per block share the constant
+// tables in LDS; blocks stride over the elements.
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace pxg
+{
+
+namespace
+{
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double>
+{
+  using Acc = __attribute__((__vector_size__(4 * sizeof(double)))) double;
+  static __device__ inline Acc mma(double a, double b, Acc c)
+  {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float>
+{
+  using Acc = __attribute__((__vector_size__(4 * sizeof(float)))) float;
+  static __device__ inline Acc mma(float a, float b, Acc c)
+  {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int kWaves = 4;
+
+template <int N, int DIM, int NQ, bool MASS>
+struct Shape
+{
+  static constexpr int NT = (N + 15) / 16;                // output tiles per side
+  static constexpr int KS = NQ * DIM;                     // stiffness rows of C
+  static constexpr int K = KS + (MASS ? NQ : 0);          // all rows of C
+  static constexpr int STEPS = (K + 3) / 4;               // MFMA k steps of 4
+};
+
+// per-wave LDS
+template <typename T, int N, int NPE, int NQ>
+struct WaveLds
+{
+  double x[NPE * 3];
+  double kap[N];
+  double bcv[N];
+  double M[NQ][6];
+  double mq[NQ];
+  int flag[N];
+  T Ke[N][N + 1];
+};
+
+template <typename T, int N, int DIM, int NPE, int NQ, bool MASS>
+__global__ void __launch_bounds__(64 * kWaves)
+    k_element_mfma(int64_t E, double const * __restrict__ xyz, uint32_t const * __restrict__ conn,
+                   uint32_t const * __restrict__ emap, double const * __restrict__ kappa,
+                   uint8_t const * __restrict__ bcFlag, double const * __restrict__ bcVal, int hasBC,
+                   double const * __restrict__ geoGradG, double const * __restrict__ wG,
+                   double const * __restrict__ phiG, double const * __restrict__ gradG,
+                   T const * __restrict__ afrag, double cS, double cSF, double cM, double cMF, int flags,
+                   T * __restrict__ ke, double * __restrict__ fe, int * __restrict__ status)
+{
+  using S = Shape<N, DIM, NQ, MASS>;
+  using L = WaveLds<T, N, NPE, NQ>;
+  using Acc = typename Mfma<T>::Acc;
+  constexpr int NT = S::NT, KS = S::KS, STEPS = S::STEPS;
+  bool const needK = (flags & 10) != 0;
+
+  __shared__ double sGeo[NQ * NPE * DIM];
+  __shared__ double sPhi[NQ * N];
+  __shared__ double sGrad[NQ * N * DIM];
+  __shared__ double sW[NQ];
+  __shared__ L lds[kWaves];
+
+  int const tid = threadIdx.x;
+  for (int k = tid; k < NQ * NPE * DIM; k += blockDim.x)
+    sGeo[k] = geoGradG[k];
+  for (int k = tid; k < NQ * N; k += blockDim.x)
+    sPhi[k] = phiG[k];
+  for (int k = tid; k < NQ * N * DIM; k += blockDim.x)
+    sGrad[k] = gradG[k];
+  for (int k = tid; k < NQ; k += blockDim.x)
+    sW[k] = wG[k];
+
+  int const wave = tid >> 6, lane = tid & 63;
+  L & W = lds[wave];
+
+  // constant operand fragments: A[16 ti + lane%16][4 s + lane/16]
+  T a[NT][STEPS];
+#pragma unroll
+  for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+    for (int s = 0; s < STEPS; ++s)
+      a[ti][s] = afrag[(ti * STEPS + s) * 64 + lane];
+  __syncthreads();
+
+  int const kr = lane >> 4, jc = lane & 15;
+  int64_t const perIter = (int64_t)gridDim.x * kWaves;
+  int64_t const iters = (E + perIter - 1) / perIter;
+  for (int64_t it = 0; it < iters; ++it)
+  {
+    int64_t const e = (it * gridDim.x + blockIdx.x) * kWaves + wave;
+    bool const active = e < E;
+    // ---- 1. gather element data
+    if (active)
+    {
+      if (lane < NPE * 3)
+        W.x[lane] = xyz[3ull * conn[e * NPE + lane / 3] + lane % 3];
+      if (lane < N)
+      {
+        uint32_t const id = emap[e * N + lane];
+        W.kap[lane] = needK ? kappa[id] : 0.;
+        int const f = hasBC ? (int)bcFlag[id] : 0;
+        W.flag[lane] = f;
+        W.bcv[lane] = f ? bcVal[id] : 0.;
+      }
+    }
+    __syncthreads();
+    // ---- 2. geometry, one quadrature point per lane
+    if (active && lane < NQ)
+    {
+      int const q = lane;
+      double J[3][DIM];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c)
+        {
+          double s = 0.;
+#pragma unroll
+          for (int v = 0; v < NPE; ++v)
+            s += W.x[3 * v + r] * sGeo[(q * NPE + v) * DIM + c];
+          J[r][c] = s;
+        }
+      double A[DIM][DIM];
+#pragma unroll
+      for (int i = 0; i < DIM; ++i)
+#pragma unroll
+        for (int j = 0; j < DIM; ++j)
+          A[i][j] = J[0][i] * J[0][j] + J[1][i] * J[1][j] + J[2][i] * J[2][j];
+      double det, I00 = 0., I01 = 0., I02 = 0., I11 = 0., I12 = 0., I22 = 0.;
+      if constexpr (DIM == 2)
+      {
+        det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+        I00 = A[1][1] / det;
+        I01 = -A[0][1] / det;
+        I11 = A[0][0] / det;
+      }
+      else
+      {
+        double const c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+        double const c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+        double const c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+        det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+        I00 = c00 / det;
+        I01 = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+        I02 = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+        I11 = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+        I12 = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;
+        I22 = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
+      }
+      if (!(det > 0.))
+        atomicOr(status, 1);
+      double const JxW = sqrt(det) * sW[q];
+      double kq = 0.;
+      if (needK)
+        for (int i = 0; i < N; ++i)
+          kq += sPhi[q * N + i] * W.kap[i];
+      double s = 0., m = 0.;
+      if (flags & 1)
+        s += cS * JxW;
+      if (flags & 2)
+        s += (kq * JxW) * cSF;
+      if (flags & 4)
+        m += cM * JxW;
+      if (flags & 8)
+        m += (cMF * kq) * JxW;
+      W.M[q][0] = s * I00;
+      W.M[q][1] = s * I01;
+      W.M[q][2] = s * I02;
+      W.M[q][3] = s * I11;
+      W.M[q][4] = s * I12;
+      W.M[q][5] = s * I22;
+      W.mq[q] = m;
+    }
+    __syncthreads();
+    // ---- 3. Ke tiles on the matrix cores (upper tiles only); the C operand
+    //         C[4s + kr][16 tj + jc] is formed in registers per step
+    Acc acc[NT][NT];
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < NT; ++tj)
+        acc[ti][tj] = Acc{0, 0, 0, 0};
+    if (active)
+    {
+#pragma unroll
+      for (int s = 0; s < STEPS; ++s)
+      {
+        int const k = 4 * s + kr;
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj)
+        {
+          int const j = 16 * tj + jc;
+          double v = 0.;
+          if (j < N)
+          {
+            if (k < KS)
+            {
+              int const q = k / DIM, a0 = k - (k / DIM) * DIM;
+              double const * g = sGrad + (q * N + j) * DIM;
+              double const * Mq = W.M[q];
+              if constexpr (DIM == 2)
+                v = (a0 == 0) ? (Mq[0] * g[0] + Mq[1] * g[1]) : (Mq[1] * g[0] + Mq[3] * g[1]);
+              else
+              {
+                // row a0 of the symmetric metric stored as (00 01 02 11 12 22)
+                int const i0 = a0 == 0 ? 0 : (a0 == 1 ? 1 : 2);
+                int const i1 = a0 == 0 ? 1 : (a0 == 1 ? 3 : 4);
+                int const i2 = a0 == 0 ? 2 : (a0 == 1 ? 4 : 5);
+                v = Mq[i0] * g[0] + Mq[i1] * g[1] + Mq[i2] * g[2];
+              }
+            }
+            else if (MASS && k < KS + NQ)
+            {
+              int const q = k - KS;
+              v = W.mq[q] * sPhi[q * N + j];
+            }
+          }
+          T const b = (T)v;
+#pragma unroll
+          for (int ti = 0; ti <= tj; ++ti)
+            acc[ti][tj] = Mfma<T>::mma(a[ti][s], b, acc[ti][tj]);
+        }
+      }
+      // tiles -> Ke in LDS: D[i][j], i = 16 ti + 4 (lane/16) + r, j = 16 tj + lane%16
+      int const ib = 4 * kr;
+#pragma unroll
+      for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+        for (int ti = 0; ti <= tj; ++ti)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+          {
+            int const i = 16 * ti + ib + r, j = 16 * tj + jc;
+            if (i < N && j < N)
+            {
+              T const v = acc[ti][tj][r];
+              W.Ke[i][j] = v;
+              if (ti != tj)
+                W.Ke[j][i] = v;
+            }
+          }
+    }
+    __syncthreads();
+    // ---- 4. Dirichlet elimination, lifting, coalesced store
+    if (active)
+    {
+      if (lane < N)
+      {
+        int const i = lane;
+        double f = 0.;
+        if (W.flag[i])
+          f = W.bcv[i];
+        else if (hasBC)
+        {
+          double s = 0.;
+          for (int j = 0; j < N; ++j)
+            if (W.flag[j])
+              s += (double)W.Ke[i][j] * W.bcv[j];
+          f = 0. - s;
+        }
+        fe[e * N + i] = f;
+      }
+      T * out = ke + e * (N * N);
+      for (int t = lane; t < N * N; t += 64)
+      {
+        int const i = t / N, j = t - (t / N) * N;
+        T v = W.Ke[i][j];
+        if (W.flag[i] | W.flag[j])
+          v = (i == j) ? T(1) : T(0);
+        out[t] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// host: MFMA fragments of the constant operand A (N x K)
+template <typename T, int N, int DIM, int NQ, bool MASS>
+std::vector<T> buildFragments(FeTables const & tab)
+{
+  using S = Shape<N, DIM, NQ, MASS>;
+  std::vector<T> out((size_t)S::NT * S::STEPS * 64, T(0));
+  for (int ti = 0; ti < S::NT; ++ti)
+    for (int s = 0; s < S::STEPS; ++s)
+      for (int l = 0; l < 64; ++l)
+      {
+        int const i = 16 * ti + (l & 15), k = 4 * s + (l >> 4);
+        double v = 0.;
+        if (i < N)
+        {
+          if (k < S::KS)
+            v = tab.grad[((size_t)(k / DIM) * N + i) * DIM + k % DIM];
+          else if (MASS && k < S::KS + NQ)
+            v = tab.phi[(size_t)(k - S::KS) * N + i];
+        }
+        out[((size_t)ti * S::STEPS + s) * 64 + l] = (T)v;
+      }
+  return out;
+}
+
+template <typename T, int N, int DIM, int NPE, int NQ, bool MASS>
+void launchMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, double cMF, T * ke)
+{
+  FeTables & tab = c->fe;
+  // cache key: precision and mass rows
+  int const key = (sizeof(T) == 8 ? 1 : 2) | (MASS ? 4 : 0);
+  T const * dfrag = nullptr;
+  if constexpr (sizeof(T) == 8)
+  {
+    if (tab.afragKey64 != key)
+    {
+      auto frag = buildFragments<T, N, DIM, NQ, MASS>(tab);
+      tab.d_afrag.upload(frag.data(), frag.size(), c->stream);
+      PXG_HIP(hipStreamSynchronize(c->stream));
+      tab.afragKey64 = key;
+    }
+    dfrag = tab.d_afrag.p;
+  }
+  else
+  {
+    if (tab.afragKey32 != key)
+    {
+      auto frag = buildFragments<T, N, DIM, NQ, MASS>(tab);
+      tab.d_afrag32.upload(frag.data(), frag.size(), c->stream);
+      PXG_HIP(hipStreamSynchronize(c->stream));
+      tab.afragKey32 = key;
+    }
+    dfrag = tab.d_afrag32.p;
+  }
+  int dev = 0, ncu = 0;
+  PXG_HIP(hipGetDevice(&dev));
+  PXG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  int64_t const blocksNeeded = (c->nelem + kWaves - 1) / kWaves;
+  int const grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocksNeeded, (int64_t)ncu * 8));
+  k_element_mfma<T, N, DIM, NPE, NQ, MASS><<<grid, 64 * kWaves, 0, c->stream>>>(
+      c->nelem, c->xyz.p, c->conn.p, c->elemMap.p, c->kappa.p, c->bcFlag.p, c->bcVal.p, c->hasBC ? 1 : 0,
+      tab.d_geoGrad.p, tab.d_w.p, tab.d_phi.p, tab.d_grad.p, dfrag, cS, cSF, cM, cMF, flags, ke, c->feBuf.p,
+      c->status.p);
+  PXG_HIP(hipGetLastError());
+}
+
+template <typename T, int N, int DIM, int NPE, int NQ>
+void launchMfmaMass(pxg_ctx * c, int flags, double cS, double cSF, double cM, double cMF, T * ke)
+{
+  if (flags & 12)
+    launchMfma<T, N, DIM, NPE, NQ, true>(c, flags, cS, cSF, cM, cMF, ke);
+  else
+    launchMfma<T, N, DIM, NPE, NQ, false>(c, flags, cS, cSF, cM, cMF, ke);
+}
+
+} // namespace
+
+// true when the fused MFMA kernel handles this FE with its recommended
+// quadrature size; the caller falls back to the geometry + row kernels otherwise
+bool elementMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, double cMF, int precision)
+{
+  FeTables const & T = c->fe;
+  auto run = [&](auto tag)
+  {
+    using TT = decltype(tag);
+    TT * ke = nullptr;
+    if constexpr (sizeof(TT) == 8)
+    {
+      c->keBuf.resize((size_t)c->nelem * T.n * T.n);
+      ke = c->keBuf.p;
+    }
+    else
+    {
+      c->keBuf32.resize((size_t)c->nelem * T.n * T.n);
+      ke = c->keBuf32.p;
+    }
+    if (T.feType == PXG_FE_HEX_Q2 && T.nq == 27)
+      launchMfmaMass<TT, 27, 3, 8, 27>(c, flags, cS, cSF, cM, cMF, ke);
+    else if (T.feType == PXG_FE_TET_P2 && T.nq == 11)
+      launchMfmaMass<TT, 10, 3, 4, 11>(c, flags, cS, cSF, cM, cMF, ke);
+    else if (T.feType == PXG_FE_QUAD_Q2 && T.nq == 9)
+      launchMfmaMass<TT, 9, 2, 4, 9>(c, flags, cS, cSF, cM, cMF, ke);
+    else
+      return false;
+    return true;
+  };
+  if (precision == PXG_FP32)
+    return run(float{});
+  return run(double{});
+}
+
+} // namespace pxg

@@ -9,6 +9,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace pxg
 {
@@ -62,6 +63,29 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// lanes per row from the mean row length; PXG_SPMV_LPR (4..64) overrides the
+// choice for launch-shape tuning
+int lanesPerRow(double mean)
+{
+  static int const forced = []
+  {
+    char const * e = std::getenv("PXG_SPMV_LPR");
+    int v = e ? std::atoi(e) : 0;
+    return (v == 4 || v == 8 || v == 16 || v == 32 || v == 64) ? v : 0;
+  }();
+  if (forced)
+    return forced;
+  if (mean > 120)
+    return 64;
+  if (mean > 56)
+    return 32;
+  if (mean > 24)
+    return 16;
+  if (mean > 12)
+    return 8;
+  return 4;
+}
+
 template <typename T, int MODE>
 void launchCsr(CsrView const & A, T const * val, T const * x, T const * b, T const * dinv, double omega, T * y,
                hipStream_t s)
@@ -69,15 +93,7 @@ void launchCsr(CsrView const & A, T const * val, T const * x, T const * b, T con
   if (A.nrows == 0)
     return;
   double const mean = A.nnz > 0 ? (double)A.nnz / (double)A.nrows : 1.0;
-  int lpr = 4;
-  if (mean > 12)
-    lpr = 8;
-  if (mean > 24)
-    lpr = 16;
-  if (mean > 56)
-    lpr = 32;
-  if (mean > 120)
-    lpr = 64;
+  int lpr = lanesPerRow(mean);
   unsigned const g = blocksFor(A.nrows * lpr);
   T const om = (T)omega;
   switch (lpr)
