@@ -70,6 +70,7 @@ def solve_and_adjoint(args, dev, N, W):
     res = {}
     sol = {}
     for name, mode in (("fp64", N.FP64), ("fp32_ir", N.FP32)):
+        progress(f"solve+adjoint {name}")
         times = []
         for rep in range(args.solve_reps + 1):
             k = kap.clone().requires_grad_()
@@ -99,6 +100,10 @@ def solve_and_adjoint(args, dev, N, W):
     res["setup_s"] = t_setup
     res["J"] = J64
     return res
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -148,6 +153,7 @@ def main():
         a.assemble(terms)
         a.close(1e-16)
 
+    progress(f"setup done (host {t_setup_host:.1f} s, pattern {t_pattern:.2f} s); assembling")
     for _ in range(args.warmup):
         step()
     a.profile(True)
@@ -162,6 +168,7 @@ def main():
     a.profile(False)
     nnz = a.nnz
 
+    progress(f"assembly {ms_step:.2f} ms/step; SpMV")
     # --- SpMV on the assembled matrix, on a torch stream so torch events time it
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
@@ -213,11 +220,13 @@ def main():
 
     # --- solve + adjoint gradient, plain FP64 vs FP32 + FP64 refinement
     solve = None
+    progress("solve + adjoint")
     if not args.no_solve:
         solve = solve_and_adjoint(args, dev, N, W)
 
     # --- CPU baseline: the oracle's serial Builder on a bounded sample
     cpu = None
+    progress("CPU baseline")
     if not args.no_cpu:
         from oracle import oracle as O
         m = args.cpu_sample

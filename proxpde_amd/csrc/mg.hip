@@ -298,7 +298,7 @@ void coarseSolve(pxg_mg * m, double const * b, double * x)
   jacobiDiag<double>(n, A.dinv, r, z, 1.0, s);
   PXG_HIP(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   dot<double>(n, r, z, sc + 9, m->partial.p, s);
-  int const maxit = 20000;
+  int const maxit = 4000;
   for (int it = 1; it <= maxit; ++it)
   {
     spmv<double>(A, p, q, s);
