@@ -54,7 +54,16 @@ struct Mfma<float>
   static __device__ inline int row(int r, int g) { return 4 * g + r; }
 };
 
-constexpr int kWaves = 4;
+constexpr int kWaves = 8;
+
+// ordering point for the lanes of one wavefront: LDS accesses of a wave
+// execute in order, so a compiler fence plus a wave barrier suffices
+__device__ inline void waveSync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <int N, int DIM, int NQ, bool MASS>
 struct Shape
@@ -98,6 +107,7 @@ __global__ void __launch_bounds__(64 * kWaves)
   __shared__ double sPhi[NQ * N];
   __shared__ double sGrad[NQ * N * DIM];
   __shared__ double sW[NQ];
+  __shared__ T sA[NT * STEPS * 64]; // constant operand fragments
   __shared__ L lds[kWaves];
 
   int const tid = threadIdx.x;
@@ -113,39 +123,64 @@ __global__ void __launch_bounds__(64 * kWaves)
   int const wave = tid >> 6, lane = tid & 63;
   L & W = lds[wave];
 
-  // constant operand fragments: A[16 ti + lane%16][4 s + lane/16]
-  T a[NT][STEPS];
-#pragma unroll
-  for (int ti = 0; ti < NT; ++ti)
-#pragma unroll
-    for (int s = 0; s < STEPS; ++s)
-      a[ti][s] = afrag[(ti * STEPS + s) * 64 + lane];
+  // constant operand fragments: sA[(ti STEPS + s) 64 + lane] = A[16 ti + lane%16][4 s + lane/16]
+  for (int k = tid; k < NT * STEPS * 64; k += blockDim.x)
+    sA[k] = afrag[k];
   __syncthreads();
 
   int const kr = lane >> 4, jc = lane & 15;
-  int64_t const perIter = (int64_t)gridDim.x * kWaves;
-  int64_t const iters = (E + perIter - 1) / perIter;
-  for (int64_t it = 0; it < iters; ++it)
+  // waves own disjoint element streams and LDS regions: after the tables are
+  // loaded they synchronise only with themselves
+  int64_t const stride = (int64_t)gridDim.x * kWaves;
+  int64_t e = (int64_t)blockIdx.x * kWaves + wave;
+  // gathered data of the element being processed, loaded one element ahead
+  double rx = 0., rk = 0., rb = 0.;
+  int rf = 0;
+  auto gatherIds = [&](int64_t el, uint32_t & pid, uint32_t & id)
   {
-    int64_t const e = (it * gridDim.x + blockIdx.x) * kWaves + wave;
-    bool const active = e < E;
-    // ---- 1. gather element data
-    if (active)
+    if (el < E)
     {
       if (lane < NPE * 3)
-        W.x[lane] = xyz[3ull * conn[e * NPE + lane / 3] + lane % 3];
+        pid = conn[el * NPE + lane / 3];
+      if (lane < N)
+        id = emap[el * N + lane];
+    }
+  };
+  auto gatherVals = [&](int64_t el, uint32_t pid, uint32_t id)
+  {
+    if (el < E)
+    {
+      if (lane < NPE * 3)
+        rx = xyz[3ull * pid + lane % 3];
       if (lane < N)
       {
-        uint32_t const id = emap[e * N + lane];
-        W.kap[lane] = needK ? kappa[id] : 0.;
-        int const f = hasBC ? (int)bcFlag[id] : 0;
-        W.flag[lane] = f;
-        W.bcv[lane] = f ? bcVal[id] : 0.;
+        rk = needK ? kappa[id] : 0.;
+        rf = hasBC ? (int)bcFlag[id] : 0;
+        rb = rf ? bcVal[id] : 0.;
       }
     }
-    __syncthreads();
+  };
+  {
+    uint32_t pid = 0, id = 0;
+    gatherIds(e, pid, id);
+    gatherVals(e, pid, id);
+  }
+  for (; e < E; e += stride)
+  {
+    // ---- 1. element data -> LDS
+    if (lane < NPE * 3)
+      W.x[lane] = rx;
+    if (lane < N)
+    {
+      W.kap[lane] = rk;
+      W.flag[lane] = rf;
+      W.bcv[lane] = rb;
+    }
+    waveSync();
+    uint32_t npid = 0, nid = 0;
+    gatherIds(e + stride, npid, nid);
     // ---- 2. geometry, one quadrature point per lane
-    if (active && lane < NQ)
+    if (lane < NQ)
     {
       int const q = lane;
       double J[3][DIM];
@@ -170,9 +205,10 @@ __global__ void __launch_bounds__(64 * kWaves)
       if constexpr (DIM == 2)
       {
         det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
-        I00 = A[1][1] / det;
-        I01 = -A[0][1] / det;
-        I11 = A[0][0] / det;
+        double const r = 1. / det;
+        I00 = A[1][1] * r;
+        I01 = -A[0][1] * r;
+        I11 = A[0][0] * r;
       }
       else
       {
@@ -180,12 +216,13 @@ __global__ void __launch_bounds__(64 * kWaves)
         double const c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
         double const c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
         det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
-        I00 = c00 / det;
-        I01 = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
-        I02 = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
-        I11 = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
-        I12 = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;
-        I22 = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
+        double const r = 1. / det;
+        I00 = c00 * r;
+        I01 = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) * r;
+        I02 = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * r;
+        I11 = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) * r;
+        I12 = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * r;
+        I22 = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * r;
       }
       if (!(det > 0.))
         atomicOr(status, 1);
@@ -211,7 +248,9 @@ __global__ void __launch_bounds__(64 * kWaves)
       W.M[q][5] = s * I22;
       W.mq[q] = m;
     }
-    __syncthreads();
+    waveSync();
+    // next element's values: their latency hides behind the matrix products
+    gatherVals(e + stride, npid, nid);
     // ---- 3. Ke tiles on the matrix cores (upper tiles only); the C operand
     //         C[4s + kr][16 tj + jc] is formed in registers per step
     Acc acc[NT][NT];
@@ -220,96 +259,90 @@ __global__ void __launch_bounds__(64 * kWaves)
 #pragma unroll
       for (int tj = 0; tj < NT; ++tj)
         acc[ti][tj] = Acc{0, 0, 0, 0};
-    if (active)
+#pragma unroll 1
+    for (int s = 0; s < STEPS; ++s)
     {
-#pragma unroll
-      for (int s = 0; s < STEPS; ++s)
-      {
-        int const k = 4 * s + kr;
-#pragma unroll
-        for (int tj = 0; tj < NT; ++tj)
-        {
-          int const j = 16 * tj + jc;
-          double v = 0.;
-          if (j < N)
-          {
-            if (k < KS)
-            {
-              int const q = k / DIM, a0 = k - (k / DIM) * DIM;
-              double const * g = sGrad + (q * N + j) * DIM;
-              double const * Mq = W.M[q];
-              if constexpr (DIM == 2)
-                v = (a0 == 0) ? (Mq[0] * g[0] + Mq[1] * g[1]) : (Mq[1] * g[0] + Mq[3] * g[1]);
-              else
-              {
-                // row a0 of the symmetric metric stored as (00 01 02 11 12 22)
-                int const i0 = a0 == 0 ? 0 : (a0 == 1 ? 1 : 2);
-                int const i1 = a0 == 0 ? 1 : (a0 == 1 ? 3 : 4);
-                int const i2 = a0 == 0 ? 2 : (a0 == 1 ? 4 : 5);
-                v = Mq[i0] * g[0] + Mq[i1] * g[1] + Mq[i2] * g[2];
-              }
-            }
-            else if (MASS && k < KS + NQ)
-            {
-              int const q = k - KS;
-              v = W.mq[q] * sPhi[q * N + j];
-            }
-          }
-          T const b = (T)v;
-#pragma unroll
-          for (int ti = 0; ti <= tj; ++ti)
-            acc[ti][tj] = Mfma<T>::mma(a[ti][s], b, acc[ti][tj]);
-        }
-      }
-      // tiles -> Ke in LDS: D[i][j], i = 16 ti + row(r, lane/16), j = 16 tj + lane%16
+      int const k = 4 * s + kr;
 #pragma unroll
       for (int tj = 0; tj < NT; ++tj)
-#pragma unroll
-        for (int ti = 0; ti <= tj; ++ti)
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
+      {
+        int const j = 16 * tj + jc;
+        double v = 0.;
+        if (j < N)
+        {
+          if (k < KS)
           {
-            int const i = 16 * ti + Mfma<T>::row(r, kr), j = 16 * tj + jc;
-            if (i < N && j < N)
+            int const q = k / DIM, a0 = k - (k / DIM) * DIM;
+            double const * g = sGrad + (q * N + j) * DIM;
+            double const * Mq = W.M[q];
+            if constexpr (DIM == 2)
+              v = (a0 == 0) ? (Mq[0] * g[0] + Mq[1] * g[1]) : (Mq[1] * g[0] + Mq[3] * g[1]);
+            else
             {
-              T const v = acc[ti][tj][r];
-              W.Ke[i][j] = v;
-              if (ti != tj)
-                W.Ke[j][i] = v;
+              // row a0 of the symmetric metric stored as (00 01 02 11 12 22)
+              int const i0 = a0 == 0 ? 0 : (a0 == 1 ? 1 : 2);
+              int const i1 = a0 == 0 ? 1 : (a0 == 1 ? 3 : 4);
+              int const i2 = a0 == 0 ? 2 : (a0 == 1 ? 4 : 5);
+              v = Mq[i0] * g[0] + Mq[i1] * g[1] + Mq[i2] * g[2];
             }
           }
-    }
-    __syncthreads();
-    // ---- 4. Dirichlet elimination, lifting, coalesced store
-    if (active)
-    {
-      if (lane < N)
-      {
-        int const i = lane;
-        double f = 0.;
-        if (W.flag[i])
-          f = W.bcv[i];
-        else if (hasBC)
-        {
-          double s = 0.;
-          for (int j = 0; j < N; ++j)
-            if (W.flag[j])
-              s += (double)W.Ke[i][j] * W.bcv[j];
-          f = 0. - s;
+          else if (MASS && k < KS + NQ)
+          {
+            int const q = k - KS;
+            v = W.mq[q] * sPhi[q * N + j];
+          }
         }
-        fe[e * N + i] = f;
-      }
-      T * out = ke + e * (N * N);
-      for (int t = lane; t < N * N; t += 64)
-      {
-        int const i = t / N, j = t - (t / N) * N;
-        T v = W.Ke[i][j];
-        if (W.flag[i] | W.flag[j])
-          v = (i == j) ? T(1) : T(0);
-        out[t] = v;
+        T const b = (T)v;
+#pragma unroll
+        for (int ti = 0; ti <= tj; ++ti)
+          acc[ti][tj] = Mfma<T>::mma(sA[(ti * STEPS + s) * 64 + lane], b, acc[ti][tj]);
       }
     }
-    __syncthreads();
+    // tiles -> Ke in LDS: D[i][j], i = 16 ti + row(r, lane/16), j = 16 tj + lane%16
+#pragma unroll
+    for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+      for (int ti = 0; ti <= tj; ++ti)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+          int const i = 16 * ti + Mfma<T>::row(r, kr), j = 16 * tj + jc;
+          if (i < N && j < N)
+          {
+            T const v = acc[ti][tj][r];
+            W.Ke[i][j] = v;
+            if (ti != tj)
+              W.Ke[j][i] = v;
+          }
+        }
+    waveSync();
+    // ---- 4. Dirichlet elimination, lifting, coalesced store
+    if (lane < N)
+    {
+      int const i = lane;
+      double f = 0.;
+      if (W.flag[i])
+        f = W.bcv[i];
+      else if (hasBC)
+      {
+        double s = 0.;
+        for (int j = 0; j < N; ++j)
+          if (W.flag[j])
+            s += (double)W.Ke[i][j] * W.bcv[j];
+        f = 0. - s;
+      }
+      fe[e * N + i] = f;
+    }
+    T * out = ke + e * (N * N);
+    for (int t = lane; t < N * N; t += 64)
+    {
+      int const i = t / N, j = t - (t / N) * N;
+      T v = W.Ke[i][j];
+      if (W.flag[i] | W.flag[j])
+        v = (i == j) ? T(1) : T(0);
+      out[t] = v;
+    }
+    waveSync(); // LDS of this element is reused by the next one
   }
 }
 
@@ -370,7 +403,7 @@ void launchMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, double
   PXG_HIP(hipGetDevice(&dev));
   PXG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   int64_t const blocksNeeded = (c->nelem + kWaves - 1) / kWaves;
-  int const grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocksNeeded, (int64_t)ncu * 8));
+  int const grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocksNeeded, (int64_t)ncu * 4));
   k_element_mfma<T, N, DIM, NPE, NQ, MASS><<<grid, 64 * kWaves, 0, c->stream>>>(
       c->nelem, c->xyz.p, c->conn.p, c->elemMap.p, c->kappa.p, c->bcFlag.p, c->bcVal.p, c->hasBC ? 1 : 0,
       tab.d_geoGrad.p, tab.d_w.p, tab.d_phi.p, tab.d_grad.p, dfrag, cS, cSF, cM, cMF, flags, ke, c->feBuf.p,
