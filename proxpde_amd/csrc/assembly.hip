@@ -31,6 +31,7 @@ namespace pxg
 namespace
 {
 constexpr int kBlock = 256;
+constexpr double kDefaultPrune = 1e-16; // Builder::closeMatrix, include/builder.hpp:369
 
 inline unsigned gridFor(int64_t work, int block = kBlock)
 {
@@ -313,19 +314,28 @@ __global__ void k_dof_seg(int64_t m, uint32_t const * __restrict__ keys, uint32_
 // deterministic scatter: each structural nonzero sums its contributions in
 // element order onto the value it already holds (successive buildLhs calls
 // accumulate like the reference's triplet list; no float atomics)
-template <typename T>
+// OVERWRITE: the values are logically zero (first build after clear), so they
+// are not read; the keep flag of closeMatrix's default prune(1, 1e-16) is
+// written alongside so close does not re-read the values.
+template <typename T, bool OVERWRITE>
 __global__ void __launch_bounds__(kBlock)
     k_scatter(int64_t nnzS, uint32_t const * __restrict__ segPtr, uint32_t const * __restrict__ contrib,
-              T const * __restrict__ ke, T * __restrict__ val)
+              T const * __restrict__ ke, T * __restrict__ val, uint32_t * __restrict__ keep)
 {
   int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nnzS)
+  if (t > nnzS)
     return;
-  T acc = val[t];
+  if (t == nnzS)
+  {
+    keep[t] = 0u; // scan sentinel
+    return;
+  }
+  T acc = OVERWRITE ? T(0) : val[t];
   uint32_t const k1 = segPtr[t + 1];
   for (uint32_t k = segPtr[t]; k < k1; ++k)
     acc += ke[contrib[k]];
   val[t] = acc;
+  keep[t] = (fabs((double)acc) <= kDefaultPrune) ? 0u : 1u;
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -688,8 +698,8 @@ void buildPattern(pxg_ctx * c)
   PXG_HIP(hipGetLastError());
   c->valS.resize(c->nnzS);
   c->valS32.resize(c->nnzS);
-  PXG_HIP(hipMemsetAsync(c->valS.p, 0, c->nnzS * sizeof(double), s));
-  PXG_HIP(hipMemsetAsync(c->valS32.p, 0, c->nnzS * sizeof(float), s));
+  c->valsZero = true;
+  c->keepFresh = false;
   c->b.resize(c->ndofs);
   PXG_HIP(hipMemsetAsync(c->b.p, 0, c->ndofs * sizeof(double), s));
   c->closed = false;
@@ -748,6 +758,9 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                 });
     PXG_HIP(hipGetLastError());
   }
+  // values are logically zero after clear / pattern build: overwrite them
+  bool const overwrite = c->valsZero;
+  c->keep.resize(c->nnzS + 1);
   dispatchFe(T.feType,
              [&](auto nC, auto dimC)
              {
@@ -765,8 +778,12 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                  }
                  PXG_HIP(hipGetLastError());
                  ProfScope ps2(c, "scatter");
-                 k_scatter<double><<<gridFor(c->nnzS), kBlock, 0, s>>>(c->nnzS, c->segPtr.p, c->contrib.p,
-                                                                      c->keBuf.p, c->valS.p);
+                 if (overwrite)
+                   k_scatter<double, true><<<gridFor(c->nnzS + 1), kBlock, 0, s>>>(
+                       c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf.p, c->valS.p, c->keep.p);
+                 else
+                   k_scatter<double, false><<<gridFor(c->nnzS + 1), kBlock, 0, s>>>(
+                       c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf.p, c->valS.p, c->keep.p);
                }
                else
                {
@@ -781,8 +798,12 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                  }
                  PXG_HIP(hipGetLastError());
                  ProfScope ps2(c, "scatter32");
-                 k_scatter<float><<<gridFor(c->nnzS), kBlock, 0, s>>>(c->nnzS, c->segPtr.p, c->contrib.p,
-                                                                     c->keBuf32.p, c->valS32.p);
+                 if (overwrite)
+                   k_scatter<float, true><<<gridFor(c->nnzS + 1), kBlock, 0, s>>>(
+                       c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf32.p, c->valS32.p, c->keep.p);
+                 else
+                   k_scatter<float, false><<<gridFor(c->nnzS + 1), kBlock, 0, s>>>(
+                       c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf32.p, c->valS32.p, c->keep.p);
                }
              });
   PXG_HIP(hipGetLastError());
@@ -793,6 +814,8 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
   PXG_HIP(hipGetLastError());
   c->valPrecision = precision;
   c->valsDirty = true;
+  c->valsZero = false;
+  c->keepFresh = true; // flags of the default prune match the values just written
   c->closed = false;
   checkStatus(c);
 }
@@ -894,11 +917,22 @@ void closeMatrix(pxg_ctx * c, double eps)
   ProfScope ps(c, "close");
   c->keep.resize(ns + 1);
   c->pos.resize(ns + 1);
-  if (f32)
-    k_keep_flags<float><<<gridFor(ns + 1), kBlock, 0, s>>>(ns, c->valS32.p, eps, c->keep.p);
-  else
-    k_keep_flags<double><<<gridFor(ns + 1), kBlock, 0, s>>>(ns, c->valS.p, eps, c->keep.p);
-  PXG_HIP(hipGetLastError());
+  if (c->valsZero)
+  {
+    // nothing built since clear: the values are zero
+    PXG_HIP(hipMemsetAsync(c->valS.p, 0, ns * sizeof(double), s));
+    PXG_HIP(hipMemsetAsync(c->valS32.p, 0, ns * sizeof(float), s));
+    c->valsZero = false;
+    c->keepFresh = false;
+  }
+  if (!(c->keepFresh && eps == kDefaultPrune))
+  {
+    if (f32)
+      k_keep_flags<float><<<gridFor(ns + 1), kBlock, 0, s>>>(ns, c->valS32.p, eps, c->keep.p);
+    else
+      k_keep_flags<double><<<gridFor(ns + 1), kBlock, 0, s>>>(ns, c->valS.p, eps, c->keep.p);
+    PXG_HIP(hipGetLastError());
+  }
   size_t tb = 0;
   PXG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, c->keep.p, c->pos.p, (int)(ns + 1), s));
   c->scanTmp.resize(std::max(tb, c->scanTmp.n));
@@ -929,21 +963,29 @@ void closeMatrix(pxg_ctx * c, double eps)
                                                      c->val.p);
     k_diag_inverse<double><<<gridFor(nr), kBlock, 0, s>>>(nr, c->diagPos.p, c->valS.p, eps, c->dinv.p);
     PXG_HIP(hipGetLastError());
-    k_cast<float, double><<<gridFor(nnz), kBlock, 0, s>>>(nnz, c->val.p, c->val32.p);
     k_cast<float, double><<<gridFor(nr), kBlock, 0, s>>>(nr, c->dinv.p, c->dinv32.p);
   }
   PXG_HIP(hipGetLastError());
   PXG_HIP(hipStreamSynchronize(s));
+  c->val32Valid = f32; // the FP32 copy of FP64 values is made on first use
   c->closed = true;
+}
+
+void ensureVal32(pxg_ctx * c)
+{
+  if (c->val32Valid || !c->closed)
+    return;
+  k_cast<float, double><<<gridFor(c->nnz), kBlock, 0, c->stream>>>(c->nnz, c->val.p, c->val32.p);
+  PXG_HIP(hipGetLastError());
+  c->val32Valid = true;
 }
 
 void clearValues(pxg_ctx * c)
 {
   hipStream_t s = c->stream;
-  if (c->valS.p)
-    PXG_HIP(hipMemsetAsync(c->valS.p, 0, c->nnzS * sizeof(double), s));
-  if (c->valS32.p)
-    PXG_HIP(hipMemsetAsync(c->valS32.p, 0, c->nnzS * sizeof(float), s));
+  // structural values are overwritten by the next build (valsZero); only b is zeroed
+  c->valsZero = true;
+  c->keepFresh = false;
   if (c->b.p)
     PXG_HIP(hipMemsetAsync(c->b.p, 0, c->ndofs * sizeof(double), s));
   PXG_HIP(hipStreamSynchronize(s));

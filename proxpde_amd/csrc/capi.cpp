@@ -364,7 +364,10 @@ int pxg_get_csr(pxg_ctx * c, int64_t * rowptr, int32_t * col, double * val, floa
                    if (val)
                      c->val.download(val, c->nnz, s);
                    if (val32)
+                   {
+                     ensureVal32(c);
                      c->val32.download(val32, c->nnz, s);
+                   }
                    if (b)
                      c->b.download(b, c->ndofs, s);
                    PXG_HIP(hipStreamSynchronize(s));
@@ -400,6 +403,8 @@ int pxg_spmv(pxg_ctx * c, int precision, const void * x, void * y)
                  {
                    PXG_REQUIRE(c->closed, PXG_E_STATE, "close before spmv");
                    PXG_REQUIRE(x && y && x != y, PXG_E_INVALID, "x and y must be distinct device buffers");
+                   if (precision == PXG_FP32)
+                     ensureVal32(c);
                    CsrView const A = viewOf(c);
                    if (precision == PXG_FP32)
                      spmv<float>(A, (float const *)x, (float *)y, c->stream);
@@ -415,6 +420,8 @@ int pxg_jacobi(pxg_ctx * c, int precision, double omega, int sweeps, const void 
                  {
                    PXG_REQUIRE(c->closed, PXG_E_STATE, "close before jacobi");
                    PXG_REQUIRE(b && x && sweeps >= 0, PXG_E_INVALID, "jacobi arguments");
+                   if (precision == PXG_FP32)
+                     ensureVal32(c);
                    CsrView const A = viewOf(c);
                    int64_t const n = c->ndofs;
                    if (precision == PXG_FP32)
@@ -459,7 +466,10 @@ int pxg_csr_device(pxg_ctx * c, const int64_t ** rowptr, const int32_t ** col, c
                    if (val)
                      *val = c->val.p;
                    if (val32)
+                   {
+                     ensureVal32(c);
                      *val32 = c->val32.p;
+                   }
                    if (nnz)
                      *nnz = c->nnz;
                  });

@@ -150,6 +150,9 @@ struct pxg_ctx
   // accumulated values (structural) and rhs
   int valPrecision = PXG_FP64;
   bool valsDirty = false;
+  bool valsZero = false;  // structural values are logically zero (not yet written)
+  bool keepFresh = false; // keep flags of prune(1e-16) match the structural values
+  bool val32Valid = false; // FP32 copy of the closed values is current
   pxg::DevBuf<double> valS;
   pxg::DevBuf<float> valS32;
   pxg::DevBuf<double> b;

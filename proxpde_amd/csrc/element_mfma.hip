@@ -259,7 +259,7 @@ __global__ void __launch_bounds__(64 * kWaves)
 #pragma unroll
       for (int tj = 0; tj < NT; ++tj)
         acc[ti][tj] = Acc{0, 0, 0, 0};
-#pragma unroll 1
+#pragma unroll 3
     for (int s = 0; s < STEPS; ++s)
     {
       int const k = 4 * s + kr;
@@ -317,31 +317,43 @@ __global__ void __launch_bounds__(64 * kWaves)
         }
     waveSync();
     // ---- 4. Dirichlet elimination, lifting, coalesced store
+    // wave vote: does the element touch a constrained dof (rf already holds the next element)
+    bool const anyBC = __any(lane < N && W.flag[lane] != 0);
     if (lane < N)
     {
       int const i = lane;
       double f = 0.;
-      if (W.flag[i])
-        f = W.bcv[i];
-      else if (hasBC)
+      if (anyBC)
       {
-        double s = 0.;
-        for (int j = 0; j < N; ++j)
-          if (W.flag[j])
-            s += (double)W.Ke[i][j] * W.bcv[j];
-        f = 0. - s;
+        if (W.flag[i])
+          f = W.bcv[i];
+        else
+        {
+          double s = 0.;
+#pragma unroll
+          for (int j = 0; j < N; ++j)
+            s += (double)W.Ke[i][j] * W.bcv[j]; // bcv is zero on free dofs
+          f = 0. - s;
+        }
       }
       fe[e * N + i] = f;
     }
     T * out = ke + e * (N * N);
-    for (int t = lane; t < N * N; t += 64)
-    {
-      int const i = t / N, j = t - (t / N) * N;
-      T v = W.Ke[i][j];
-      if (W.flag[i] | W.flag[j])
-        v = (i == j) ? T(1) : T(0);
-      out[t] = v;
-    }
+    if (anyBC)
+      for (int t = lane; t < N * N; t += 64)
+      {
+        int const i = t / N, j = t - (t / N) * N;
+        T v = W.Ke[i][j];
+        if (W.flag[i] | W.flag[j])
+          v = (i == j) ? T(1) : T(0);
+        out[t] = v;
+      }
+    else
+      for (int t = lane; t < N * N; t += 64)
+      {
+        int const i = t / N, j = t - (t / N) * N;
+        out[t] = W.Ke[i][j];
+      }
     waveSync(); // LDS of this element is reused by the next one
   }
 }

@@ -16,6 +16,8 @@ void qpoints(pxg_ctx * c, double * outHost);
 void geometryOut(pxg_ctx * c, double * detJ, double * JxW);
 void closeMatrix(pxg_ctx * c, double eps);
 void clearValues(pxg_ctx * c);
+// FP32 copy of the closed FP64 values, made on first use
+void ensureVal32(pxg_ctx * c);
 // element_mfma.hip: fused geometry + MFMA element kernel; false if not applicable
 bool elementMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, double cMF, int precision);
 

@@ -564,7 +564,11 @@ int pxg_mg_solve(pxg_mg * m, int mode, const double * b, double * x, double rtol
     double rr = 0.;
     int it = 0;
     if (mode == PXG_FP32)
+    {
+      for (auto * c: m->ctx)
+        ensureVal32(c);
       it = solveIR(m, b, x, rtol, maxit, rr);
+    }
     else
       it = pcg<double>(m, b, x, rtol, maxit, rr, false);
     PXG_HIP(hipStreamSynchronize(m->s));
