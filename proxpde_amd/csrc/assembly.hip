@@ -47,6 +47,8 @@ enum : int
   kHasMassFE = 8,
 };
 
+__host__ __device__ inline uint32_t packedPos(int n, int i, int j);
+
 // --------------------------------------------------------------------------
 // geometry: one thread per (element, quadrature point)
 // J = sum_v x_v (x) grad psi_v(xi_q); A = J^T J; detJ = sqrt(det A);
@@ -238,10 +240,12 @@ __global__ void __launch_bounds__(kBlock)
       f = 0. - s;
     }
   }
-  T * out = ke + t * N;
+  // symmetric: store the upper triangle, row i from its diagonal on
+  T * out = ke + e * (N * (N + 1) / 2) + packedPos(N, i, i) - i;
 #pragma unroll
   for (int j = 0; j < N; ++j)
-    out[j] = acc[j];
+    if (j >= i)
+      out[j] = acc[j];
   fe[t] = f;
 }
 
@@ -269,9 +273,19 @@ __global__ void k_head_flags(int64_t m, uint64_t const * __restrict__ keys, uint
   flags[t] = (t == 0 || keys[t] != keys[t - 1]) ? 1u : 0u;
 }
 
-// after an inclusive scan of the head flags: nnz id of every sorted entry
-__global__ void k_pattern_fill(int64_t m, int64_t ndofs, uint64_t const * __restrict__ keys,
-                               uint32_t const * __restrict__ scan, uint32_t const * __restrict__ sortedIds,
+// position of entry (i, j) of a symmetric n x n element matrix stored as its
+// upper triangle, row by row
+__host__ __device__ inline uint32_t packedPos(int n, int i, int j)
+{
+  int const a = i < j ? i : j, b = i < j ? j : i;
+  return (uint32_t)(a * n - (a * (a - 1)) / 2 + (b - a));
+}
+
+// after an inclusive scan of the head flags: nnz id of every sorted entry; the
+// contribution list is rewritten to point into the packed (upper-triangle)
+// element matrices
+__global__ void k_pattern_fill(int64_t m, int64_t ndofs, int n, uint64_t const * __restrict__ keys,
+                               uint32_t const * __restrict__ scan, uint32_t * __restrict__ sortedIds,
                                uint32_t * __restrict__ segPtr, int32_t * __restrict__ colS,
                                int64_t * __restrict__ rowptrS, uint32_t * __restrict__ emapNnz)
 {
@@ -279,7 +293,13 @@ __global__ void k_pattern_fill(int64_t m, int64_t ndofs, uint64_t const * __rest
   if (t >= m)
     return;
   uint32_t const id = scan[t] - 1u;
-  emapNnz[sortedIds[t]] = id;
+  uint32_t const orig = sortedIds[t];
+  emapNnz[orig] = id;
+  {
+    uint32_t const nn = (uint32_t)(n * n), np = (uint32_t)(n * (n + 1) / 2);
+    uint32_t const e = orig / nn, r = orig - e * nn;
+    sortedIds[t] = e * np + packedPos(n, (int)(r / n), (int)(r % n));
+  }
   uint64_t const k = keys[t];
   if (t == 0 || k != keys[t - 1])
   {
@@ -682,7 +702,7 @@ void buildPattern(pxg_ctx * c)
     c->colS.resize(nnzS);
     c->rowptrS.resize(c->ndofs + 1);
     c->emapNnz.resize(m);
-    k_pattern_fill<<<gridFor(m), kBlock, 0, s>>>(m, c->ndofs, keysOut.p, idsOut.p, c->contrib.p, c->segPtr.p,
+    k_pattern_fill<<<gridFor(m), kBlock, 0, s>>>(m, c->ndofs, n, keysOut.p, idsOut.p, c->contrib.p, c->segPtr.p,
                                                   c->colS.p, c->rowptrS.p, c->emapNnz.p);
     PXG_HIP(hipGetLastError());
     uint32_t const mm = (uint32_t)m;
@@ -769,7 +789,7 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                {
                  if (!fused)
                  {
-                   c->keBuf.resize(E * N * N);
+                   c->keBuf.resize(E * (N * (N + 1) / 2));
                    size_t const sh = (size_t)T.nq * N * (D + 1) * sizeof(double);
                    ProfScope ps(c, "element");
                    k_element_rows<double, N, D><<<gridFor(E * N), kBlock, sh, s>>>(
@@ -789,7 +809,7 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                {
                  if (!fused)
                  {
-                   c->keBuf32.resize(E * N * N);
+                   c->keBuf32.resize(E * (N * (N + 1) / 2));
                    size_t const sh = (size_t)T.nq * N * (D + 1) * sizeof(float);
                    ProfScope ps(c, "element32");
                    k_element_rows<float, N, D><<<gridFor(E * N), kBlock, sh, s>>>(

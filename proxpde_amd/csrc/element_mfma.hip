@@ -108,7 +108,19 @@ __global__ void __launch_bounds__(64 * kWaves)
   __shared__ double sGrad[NQ * N * DIM];
   __shared__ double sW[NQ];
   __shared__ T sA[NT * STEPS * 64]; // constant operand fragments
+  constexpr int NP = N * (N + 1) / 2;
+  __shared__ unsigned short sPair[NP]; // (row << 8 | col) of each packed upper-triangle slot
   __shared__ L lds[kWaves];
+  for (int t = threadIdx.x; t < NP; t += blockDim.x)
+  {
+    int i = 0, start = 0;
+    while (start + (N - i) <= t)
+    {
+      start += N - i;
+      ++i;
+    }
+    sPair[t] = (unsigned short)((i << 8) | (i + t - start));
+  }
 
   int const tid = threadIdx.x;
   for (int k = tid; k < NQ * NPE * DIM; k += blockDim.x)
@@ -338,22 +350,20 @@ __global__ void __launch_bounds__(64 * kWaves)
       }
       fe[e * N + i] = f;
     }
-    T * out = ke + e * (N * N);
+    // symmetric: the upper triangle is stored, row by row
+    T * out = ke + e * NP;
     if (anyBC)
-      for (int t = lane; t < N * N; t += 64)
+      for (int t = lane; t < NP; t += 64)
       {
-        int const i = t / N, j = t - (t / N) * N;
+        int const i = sPair[t] >> 8, j = sPair[t] & 255;
         T v = W.Ke[i][j];
         if (W.flag[i] | W.flag[j])
           v = (i == j) ? T(1) : T(0);
         out[t] = v;
       }
     else
-      for (int t = lane; t < N * N; t += 64)
-      {
-        int const i = t / N, j = t - (t / N) * N;
-        out[t] = W.Ke[i][j];
-      }
+      for (int t = lane; t < NP; t += 64)
+        out[t] = W.Ke[sPair[t] >> 8][sPair[t] & 255];
     waveSync(); // LDS of this element is reused by the next one
   }
 }
@@ -443,14 +453,15 @@ bool elementMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, doubl
   {
     using TT = decltype(tag);
     TT * ke = nullptr;
+    size_t const np = (size_t)T.n * (T.n + 1) / 2; // packed symmetric element matrix
     if constexpr (sizeof(TT) == 8)
     {
-      c->keBuf.resize((size_t)c->nelem * T.n * T.n);
+      c->keBuf.resize((size_t)c->nelem * np);
       ke = c->keBuf.p;
     }
     else
     {
-      c->keBuf32.resize((size_t)c->nelem * T.n * T.n);
+      c->keBuf32.resize((size_t)c->nelem * np);
       ke = c->keBuf32.p;
     }
     if (T.feType == PXG_FE_HEX_Q2 && T.nq == 27)

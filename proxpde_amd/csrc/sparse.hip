@@ -44,8 +44,27 @@ __global__ void __launch_bounds__(kBlock)
   if (row >= nrows)
     return; // whole LPR groups leave together
   int64_t const k0 = rowptr[row], k1 = rowptr[row + 1];
+  // up to kUnroll entries per lane are loaded before any is used, so the
+  // column/value stream and the x gathers of a lane are all in flight at once;
+  // the matrix streams bypass the caches (non-temporal), keeping x resident
+  constexpr int kUnroll = 8;
+  int const len = (int)(k1 - k0);
+  int32_t cc[kUnroll];
+  T vv[kUnroll];
+#pragma unroll
+  for (int m = 0; m < kUnroll; ++m)
+  {
+    int64_t const k = k0 + lane + (int64_t)m * LPR;
+    bool const in = lane + m * LPR < len;
+    cc[m] = in ? __builtin_nontemporal_load(col + k) : 0;
+    vv[m] = in ? __builtin_nontemporal_load(val + k) : T(0);
+  }
   T s = T(0);
-  for (int64_t k = k0 + lane; k < k1; k += LPR)
+#pragma unroll
+  for (int m = 0; m < kUnroll; ++m)
+    if (lane + m * LPR < len)
+      s += vv[m] * x[cc[m]];
+  for (int64_t k = k0 + lane + (int64_t)kUnroll * LPR; k < k1; k += LPR)
     s += val[k] * x[col[k]];
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1)
@@ -75,11 +94,11 @@ int lanesPerRow(double mean)
   }();
   if (forced)
     return forced;
-  if (mean > 120)
+  if (mean > 192)
     return 64;
-  if (mean > 56)
+  if (mean > 96)
     return 32;
-  if (mean > 24)
+  if (mean > 40)
     return 16;
   if (mean > 12)
     return 8;
