@@ -84,6 +84,15 @@ def solve_and_adjoint(args, dev, N, W):
                 times.append(time.perf_counter() - ts)
         sol[name] = (u.detach(), k.grad.detach(), float(J))
         res[name] = {"ms": 1e3 * min(times)}
+        # the bare solve on the matrices of this mode (still assembled): time and
+        # CG iterations (FP32 mode: inner iterations summed over refinements)
+        xs = torch.zeros_like(b)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        its, _ = solver.mg.solve_dev(b.data_ptr(), xs.data_ptr(), 1e-8, 1000, mode)
+        torch.cuda.synchronize()
+        res[name]["solve_only_ms"] = 1e3 * (time.perf_counter() - ts)
+        res[name]["cg_iterations"] = its
     # FP64 residual of each solution with the matrix-free FP64 operator
     op = A.KappaOperator(solver.ctx[0], use_bc=True)
     for name in sol:
@@ -116,8 +125,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=12, help="elements per axis of the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
-    ap.add_argument("--solve-levels", type=int, default=4)
+    ap.add_argument("--solve-levels", type=int, default=5)
     ap.add_argument("--solve-reps", type=int, default=2)
+    ap.add_argument("--copy-gib", type=float, default=1.0,
+                    help="size of a plain device copy timed for reference (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -177,28 +188,49 @@ def main():
     gen.manual_seed(0)
     x = torch.rand(prob.ndofs, dtype=torch.float64, device=dev, generator=gen) * 2 - 1
     y = torch.empty_like(x)
-    for _ in range(5):
-        a.spmv_dev(x.data_ptr(), y.data_ptr())
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    for _ in range(args.spmv):
-        a.spmv_dev(x.data_ptr(), y.data_ptr())
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    spmv_ms = ev0.elapsed_time(ev1) / args.spmv
-    spmv_bytes = nnz * 12 + 8 * (prob.ndofs + 1) + 16 * prob.ndofs
-    spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     x32 = x.float()
     y32 = torch.empty_like(x32)
-    for _ in range(5):
-        a.spmv_dev(x32.data_ptr(), y32.data_ptr(), precision=N.FP32)
-    ev0.record(stream)
-    for _ in range(args.spmv):
-        a.spmv_dev(x32.data_ptr(), y32.data_ptr(), precision=N.FP32)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    spmv32_ms = ev0.elapsed_time(ev1) / args.spmv
-    spmv32_gbs = (nnz * 8 + 8 * (prob.ndofs + 1) + 8 * prob.ndofs) / (spmv32_ms * 1e-3) / 1e9
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def time_spmv(xv, yv, prec):
+        for _ in range(5):
+            a.spmv_dev(xv.data_ptr(), yv.data_ptr(), precision=prec)
+        ev0.record(stream)
+        for _ in range(args.spmv):
+            a.spmv_dev(xv.data_ptr(), yv.data_ptr(), precision=prec)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / args.spmv
+
+    # algorithmic bytes: values + column indices, row pointers, x and y once
+    spmv_bytes = nnz * 12 + 8 * (prob.ndofs + 1) + 16 * prob.ndofs
+    spmv32_bytes = nnz * 8 + 8 * (prob.ndofs + 1) + 8 * prob.ndofs
+    spmv_ms = time_spmv(x, y, N.FP64)
+    spmv32_ms = time_spmv(x32, y32, N.FP32)
+    spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    spmv32_gbs = spmv32_bytes / (spmv32_ms * 1e-3) / 1e9
+    # the row-group kernel on the same matrix, for comparison
+    os.environ["PXG_SPMV_KERNEL"] = "rowgroup"
+    rg_ms = time_spmv(x, y, N.FP64)
+    rg32_ms = time_spmv(x32, y32, N.FP32)
+    del os.environ["PXG_SPMV_KERNEL"]
+
+    # --- a plain device-to-device copy of known size: the achievable HBM rate,
+    # and the calibration of the FETCH_SIZE / WRITE_SIZE counters
+    copy_cal = None
+    if args.copy_gib > 0:
+        nel = int(args.copy_gib * 2 ** 30) // 8
+        src = torch.ones(nel, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        ev0.record(stream)
+        for _ in range(3):
+            dst.copy_(src)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        cms = ev0.elapsed_time(ev1) / 3
+        copy_cal = {"bytes_each_way": nel * 8, "ms": cms, "GB/s_read_plus_write": 2 * nel * 8 / (cms * 1e-3) / 1e9}
+        del src, dst
 
     # --- roofline of the dominant kernel
     per = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in prof.items()}
@@ -210,7 +242,10 @@ def main():
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                 "traffic": None}
     elif dom == "scatter":
-        by = E * 27 * 27 * (8 + 4) + nnz_s * (4 + 8 + 8)
+        # contribution ids (4 B per element-matrix entry), the packed upper
+        # triangle of each element matrix (27*28/2 FP64), segment offsets (4 B),
+        # the summed value (8 B) and the keep flag (4 B) per structural entry
+        by = E * 27 * 27 * 4 + E * (27 * 28 // 2) * 8 + (nnz_s + 1) * 4 + nnz_s * (8 + 4)
         achieved = by / (per["scatter"] * 1e-3) / 1e9
         roof = {"kernel": "k_scatter<double>", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
@@ -268,7 +303,9 @@ def main():
         "pattern_build_s": t_pattern,
         "host_setup_s": t_setup_host,
         "spmv": {"ms": spmv_ms, "GB/s": spmv_gbs, "frac_hbm": spmv_gbs / HBM_PEAK_GBS,
-                 "fp32_ms": spmv32_ms, "fp32_GB/s": spmv32_gbs, "bytes": spmv_bytes},
+                 "fp32_ms": spmv32_ms, "fp32_GB/s": spmv32_gbs, "bytes": spmv_bytes,
+                 "rowgroup_kernel_ms": rg_ms, "rowgroup_kernel_fp32_ms": rg32_ms},
+        "device_copy": copy_cal,
         "reference_flops_per_elem": q2hex_flops_per_elem(),
         "reference_gflops_per_s": q2hex_flops_per_elem() * value / 1e9,
         "solve_adjoint": solve,

@@ -156,6 +156,11 @@ int pxg_csr_device(pxg_ctx * ctx, const int64_t ** rowptr, const int32_t ** col,
                    const float ** val32, int64_t * nnz);
 /* device pointer to b (n_dofs doubles)                                     */
 int pxg_rhs_device(pxg_ctx * ctx, double ** b_dev);
+/* number of closes of this context so far: state derived from the closed
+ * matrix (a factorisation, a saved solve) is current while it is unchanged.
+ * No counterpart in the reference, whose matrices are plain values; it is
+ * for callers that cache per-matrix state (autograd solve, multigrid).     */
+int pxg_matrix_stamp(pxg_ctx * ctx, uint64_t * stamp);
 
 /* ---- matrix-free y = K(kappa) x and its gradients ------------------------
  * nb vectors interleaved as [n_dofs][nb]; use_bc applies the Dirichlet form
@@ -171,11 +176,14 @@ typedef struct pxg_mg pxg_mg;
 /* levels: closed contexts (finest first) on structured hex meshes whose
  * element counts halve per level; dims: n_lev x 3 element counts.
  * Weighted Jacobi (omega, nu1 pre / nu2 post sweeps), R = P^T, coarsest
- * level solved by Jacobi-CG to coarse_rtol in FP64.                       */
+ * level solved directly in FP64 (dense inverse of its free-dof block, up to
+ * 4096 free dofs; larger: Jacobi-CG to coarse_rtol).                      */
 int pxg_mg_create(pxg_mg ** mg, int n_lev, pxg_ctx ** levels, const int * dims, double omega, int nu1, int nu2,
                   double coarse_rtol);
 int pxg_mg_destroy(pxg_mg * mg);
-/* refresh level matrices after a re-assembly (same patterns)               */
+/* refresh level matrices after a re-assembly (same patterns): refactors
+ * the coarsest level now (solve and vcycle also do it when they find the
+ * coarsest matrix closed again since the last factorisation)               */
 int pxg_mg_update(pxg_mg * mg);
 /* solve A x = b (device pointers, FP64); mode PXG_FP64: MG-PCG in FP64;
  * PXG_FP32: FP32 MG-PCG inner solves inside FP64 iterative refinement

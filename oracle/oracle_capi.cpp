@@ -365,6 +365,7 @@ void * orc_mg_new(int nlev, void ** ctxs, double omega, int nu1, int nu2, double
         L.R = transpose(Q, cc->dofs.ndofs);
       }
     }
+    m->mg.factorCoarse();
     return m;
   }
   catch (...)

@@ -1,6 +1,8 @@
 // Oracle (test infrastructure only): sparse kernels, MG-PCG, matrix-free operator.
 #include "oracle_solve.hpp"
 
+#include <stdexcept>
+
 #include <algorithm>
 #include <cmath>
 
@@ -179,10 +181,62 @@ Csr rowNormalised(Csr const & A)
   return B;
 }
 
+void Multigrid::factorCoarse()
+{
+  coarseL.clear();
+  Csr const & A = lv.back().A;
+  i64 const n = A.nrows;
+  if (n > kDenseCoarse)
+    return;
+  std::vector<double> L(n * n, 0.);
+  for (i64 r = 0; r < n; ++r)
+    for (i64 k = A.rowptr[r]; k < A.rowptr[r + 1]; ++k)
+      if ((i64)A.col[k] <= r)
+        L[r * n + A.col[k]] = A.val[k];
+  // right-looking Cholesky, A = L L^T
+  for (i64 j = 0; j < n; ++j)
+  {
+    double d = L[j * n + j];
+    if (!(d > 0.))
+      throw std::runtime_error("coarsest matrix is not SPD");
+    d = std::sqrt(d);
+    L[j * n + j] = d;
+    for (i64 i = j + 1; i < n; ++i)
+      L[i * n + j] /= d;
+    for (i64 k = j + 1; k < n; ++k)
+    {
+      double const lkj = L[k * n + j];
+      if (lkj != 0.)
+        for (i64 i = k; i < n; ++i)
+          L[i * n + k] -= L[i * n + j] * lkj;
+    }
+  }
+  coarseL = std::move(L);
+}
+
 void Multigrid::vcycle(int l, double const * b, double * x) const
 {
   MgLevel const & L = lv[l];
   i64 const n = L.A.nrows;
+  if (l == (int)lv.size() - 1 && !coarseL.empty())
+  {
+    // forward and back substitution with the Cholesky factor
+    for (i64 i = 0; i < n; ++i)
+    {
+      double t = b[i];
+      for (i64 k = 0; k < i; ++k)
+        t -= coarseL[i * n + k] * x[k];
+      x[i] = t / coarseL[i * n + i];
+    }
+    for (i64 i = n - 1; i >= 0; --i)
+    {
+      double t = x[i];
+      for (i64 k = i + 1; k < n; ++k)
+        t -= coarseL[k * n + i] * x[k];
+      x[i] = t / coarseL[i * n + i];
+    }
+    return;
+  }
   if (l == (int)lv.size() - 1)
   {
     // coarsest level: Jacobi-preconditioned CG from zero to coarseRtol

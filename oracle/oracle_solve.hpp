@@ -42,6 +42,12 @@ struct Multigrid
   int nu1 = 2, nu2 = 2;
   double coarseRtol = 1e-10;
   int coarseMaxit = 10000;
+  // coarsest level: dense Cholesky factor (row-major lower triangle) when it
+  // has at most kDenseCoarse rows, the direct coarse solve of the reference
+  // test (test/multigrid.cpp: SparseLU); larger: Jacobi-CG to coarseRtol
+  static constexpr i64 kDenseCoarse = 2048;
+  std::vector<double> coarseL;
+  void factorCoarse();
   void vcycle(int l, double const * b, double * x) const;
   // preconditioned CG; returns iterations, final relative residual in relres
   int pcg(double const * b, double * x, double rtol, int maxit, double & relres) const;

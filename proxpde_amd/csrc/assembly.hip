@@ -337,7 +337,7 @@ __global__ void k_dof_seg(int64_t m, uint32_t const * __restrict__ keys, uint32_
 // OVERWRITE: the values are logically zero (first build after clear), so they
 // are not read; the keep flag of closeMatrix's default prune(1, 1e-16) is
 // written alongside so close does not re-read the values.
-template <typename T, bool OVERWRITE>
+template <typename T, bool OVERWRITE, int kMaxC>
 __global__ void __launch_bounds__(kBlock)
     k_scatter(int64_t nnzS, uint32_t const * __restrict__ segPtr, uint32_t const * __restrict__ contrib,
               T const * __restrict__ ke, T * __restrict__ val, uint32_t * __restrict__ keep)
@@ -351,11 +351,60 @@ __global__ void __launch_bounds__(kBlock)
     return;
   }
   T acc = OVERWRITE ? T(0) : val[t];
-  uint32_t const k1 = segPtr[t + 1];
-  for (uint32_t k = segPtr[t]; k < k1; ++k)
+  uint32_t const k0 = segPtr[t], k1 = segPtr[t + 1];
+  // the first kMaxC contributions: ids and element-matrix values are loaded
+  // unconditionally (index clamped into the segment, which is never empty)
+  // so all gathers of a thread are in flight together; summed in order
+  uint32_t id[kMaxC];
+#pragma unroll
+  for (int j = 0; j < kMaxC; ++j)
+    id[j] = contrib[min(k0 + j, k1 - 1)];
+  T v[kMaxC];
+#pragma unroll
+  for (int j = 0; j < kMaxC; ++j)
+    v[j] = ke[id[j]];
+#pragma unroll
+  for (int j = 0; j < kMaxC; ++j)
+    if (k0 + j < k1)
+      acc += v[j];
+  for (uint32_t k = k0 + kMaxC; k < k1; ++k)
     acc += ke[contrib[k]];
   val[t] = acc;
   keep[t] = (fabs((double)acc) <= kDefaultPrune) ? 0u : 1u;
+}
+
+// contributions loaded ahead per structural entry (PXG_SCATTER_UNROLL = 1, 2,
+// 4 or 8 overrides the default for launch tuning)
+int scatterUnroll()
+{
+  char const * e = std::getenv("PXG_SCATTER_UNROLL");
+  int const v = e ? std::atoi(e) : 0;
+  return (v == 1 || v == 2 || v == 4 || v == 8) ? v : 4;
+}
+
+template <typename T, bool OVERWRITE>
+void launchScatterU(int u, int64_t nnzS, uint32_t const * segPtr, uint32_t const * contrib, T const * ke, T * val,
+                    uint32_t * keep, hipStream_t s)
+{
+  unsigned const g = gridFor(nnzS + 1);
+  switch (u)
+  {
+  case 1: k_scatter<T, OVERWRITE, 1><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, val, keep); break;
+  case 2: k_scatter<T, OVERWRITE, 2><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, val, keep); break;
+  case 8: k_scatter<T, OVERWRITE, 8><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, val, keep); break;
+  default: k_scatter<T, OVERWRITE, 4><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, val, keep); break;
+  }
+}
+
+template <typename T>
+void launchScatter(bool overwrite, int64_t nnzS, uint32_t const * segPtr, uint32_t const * contrib, T const * ke,
+                   T * val, uint32_t * keep, hipStream_t s)
+{
+  int const u = scatterUnroll();
+  if (overwrite)
+    launchScatterU<T, true>(u, nnzS, segPtr, contrib, ke, val, keep, s);
+  else
+    launchScatterU<T, false>(u, nnzS, segPtr, contrib, ke, val, keep, s);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -723,6 +772,7 @@ void buildPattern(pxg_ctx * c)
   c->b.resize(c->ndofs);
   PXG_HIP(hipMemsetAsync(c->b.p, 0, c->ndofs * sizeof(double), s));
   c->closed = false;
+  ++c->stamp;
 }
 
 void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
@@ -798,12 +848,8 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                  }
                  PXG_HIP(hipGetLastError());
                  ProfScope ps2(c, "scatter");
-                 if (overwrite)
-                   k_scatter<double, true><<<gridFor(c->nnzS + 1), kBlock, 0, s>>>(
-                       c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf.p, c->valS.p, c->keep.p);
-                 else
-                   k_scatter<double, false><<<gridFor(c->nnzS + 1), kBlock, 0, s>>>(
-                       c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf.p, c->valS.p, c->keep.p);
+                 launchScatter<double>(overwrite, c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf.p, c->valS.p,
+                                       c->keep.p, s);
                }
                else
                {
@@ -818,12 +864,8 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                  }
                  PXG_HIP(hipGetLastError());
                  ProfScope ps2(c, "scatter32");
-                 if (overwrite)
-                   k_scatter<float, true><<<gridFor(c->nnzS + 1), kBlock, 0, s>>>(
-                       c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf32.p, c->valS32.p, c->keep.p);
-                 else
-                   k_scatter<float, false><<<gridFor(c->nnzS + 1), kBlock, 0, s>>>(
-                       c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf32.p, c->valS32.p, c->keep.p);
+                 launchScatter<float>(overwrite, c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf32.p, c->valS32.p,
+                                      c->keep.p, s);
                }
              });
   PXG_HIP(hipGetLastError());
@@ -837,6 +879,7 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
   c->valsZero = false;
   c->keepFresh = true; // flags of the default prune match the values just written
   c->closed = false;
+  ++c->stamp;
   checkStatus(c);
 }
 
@@ -974,7 +1017,6 @@ void closeMatrix(pxg_ctx * c, double eps)
                                                     c->val32.p);
     k_diag_inverse<float><<<gridFor(nr), kBlock, 0, s>>>(nr, c->diagPos.p, c->valS32.p, eps, c->dinv32.p);
     PXG_HIP(hipGetLastError());
-    k_cast<double, float><<<gridFor(nnz), kBlock, 0, s>>>(nnz, c->val32.p, c->val.p);
     k_cast<double, float><<<gridFor(nr), kBlock, 0, s>>>(nr, c->dinv32.p, c->dinv.p);
   }
   else
@@ -987,8 +1029,11 @@ void closeMatrix(pxg_ctx * c, double eps)
   }
   PXG_HIP(hipGetLastError());
   PXG_HIP(hipStreamSynchronize(s));
-  c->val32Valid = f32; // the FP32 copy of FP64 values is made on first use
+  // the copy in the other precision is made on first use
+  c->val32Valid = f32;
+  c->val64Valid = !f32;
   c->closed = true;
+  ++c->stamp;
 }
 
 void ensureVal32(pxg_ctx * c)
@@ -998,6 +1043,15 @@ void ensureVal32(pxg_ctx * c)
   k_cast<float, double><<<gridFor(c->nnz), kBlock, 0, c->stream>>>(c->nnz, c->val.p, c->val32.p);
   PXG_HIP(hipGetLastError());
   c->val32Valid = true;
+}
+
+void ensureVal64(pxg_ctx * c)
+{
+  if (c->val64Valid || !c->closed)
+    return;
+  k_cast<double, float><<<gridFor(c->nnz), kBlock, 0, c->stream>>>(c->nnz, c->val32.p, c->val.p);
+  PXG_HIP(hipGetLastError());
+  c->val64Valid = true;
 }
 
 void clearValues(pxg_ctx * c)
@@ -1011,6 +1065,7 @@ void clearValues(pxg_ctx * c)
   PXG_HIP(hipStreamSynchronize(s));
   c->valsDirty = false;
   c->closed = false;
+  ++c->stamp;
 }
 
 } // namespace pxg

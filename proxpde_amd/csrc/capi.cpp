@@ -243,6 +243,7 @@ int pxg_set_dofmap(pxg_ctx * c, int64_t ndofs, const uint32_t * emap)
                    c->hasKappa = false;
                    c->nnzS = 0;
                    c->closed = false;
+                   ++c->stamp;
                    PXG_HIP(hipStreamSynchronize(c->stream));
                  });
 }
@@ -362,7 +363,10 @@ int pxg_get_csr(pxg_ctx * c, int64_t * rowptr, int32_t * col, double * val, floa
                    if (col)
                      c->col.download(col, c->nnz, s);
                    if (val)
+                   {
+                     ensureVal64(c);
                      c->val.download(val, c->nnz, s);
+                   }
                    if (val32)
                    {
                      ensureVal32(c);
@@ -405,6 +409,9 @@ int pxg_spmv(pxg_ctx * c, int precision, const void * x, void * y)
                    PXG_REQUIRE(x && y && x != y, PXG_E_INVALID, "x and y must be distinct device buffers");
                    if (precision == PXG_FP32)
                      ensureVal32(c);
+                   else
+                     ensureVal64(c);
+                   ensureRowBlocks(c);
                    CsrView const A = viewOf(c);
                    if (precision == PXG_FP32)
                      spmv<float>(A, (float const *)x, (float *)y, c->stream);
@@ -422,6 +429,9 @@ int pxg_jacobi(pxg_ctx * c, int precision, double omega, int sweeps, const void 
                    PXG_REQUIRE(b && x && sweeps >= 0, PXG_E_INVALID, "jacobi arguments");
                    if (precision == PXG_FP32)
                      ensureVal32(c);
+                   else
+                     ensureVal64(c);
+                   ensureRowBlocks(c);
                    CsrView const A = viewOf(c);
                    int64_t const n = c->ndofs;
                    if (precision == PXG_FP32)
@@ -464,7 +474,10 @@ int pxg_csr_device(pxg_ctx * c, const int64_t ** rowptr, const int32_t ** col, c
                    if (col)
                      *col = c->col.p;
                    if (val)
+                   {
+                     ensureVal64(c);
                      *val = c->val.p;
+                   }
                    if (val32)
                    {
                      ensureVal32(c);
@@ -472,6 +485,16 @@ int pxg_csr_device(pxg_ctx * c, const int64_t ** rowptr, const int32_t ** col, c
                    }
                    if (nnz)
                      *nnz = c->nnz;
+                 });
+}
+
+int pxg_matrix_stamp(pxg_ctx * c, uint64_t * stamp)
+{
+  return guarded(c,
+                 [&]
+                 {
+                   PXG_REQUIRE(stamp, PXG_E_INVALID, "null stamp");
+                   *stamp = c->stamp;
                  });
 }
 

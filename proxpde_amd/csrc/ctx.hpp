@@ -153,6 +153,8 @@ struct pxg_ctx
   bool valsZero = false;  // structural values are logically zero (not yet written)
   bool keepFresh = false; // keep flags of prune(1e-16) match the structural values
   bool val32Valid = false; // FP32 copy of the closed values is current
+  bool val64Valid = false; // FP64 copy of the closed values is current
+  uint64_t stamp = 0;      // counts closes: changes whenever the closed matrix may have changed
   pxg::DevBuf<double> valS;
   pxg::DevBuf<float> valS32;
   pxg::DevBuf<double> b;
@@ -165,6 +167,9 @@ struct pxg_ctx
   pxg::DevBuf<float> val32;
   pxg::DevBuf<double> dinv;
   pxg::DevBuf<float> dinv32;
+  pxg::DevBuf<int32_t> blockRow; // row blocks of the stream SpMV kernel
+  int64_t nStreamBlocks = 0;
+  uint64_t blockStamp = 0; // matrix stamp the row blocks were built for (0: none)
   // work buffers
   pxg::DevBuf<double> geoBuf; // E*nq*8
   pxg::DevBuf<double> keBuf;  // E*n*n (fp64)

@@ -16,8 +16,9 @@ void qpoints(pxg_ctx * c, double * outHost);
 void geometryOut(pxg_ctx * c, double * detJ, double * JxW);
 void closeMatrix(pxg_ctx * c, double eps);
 void clearValues(pxg_ctx * c);
-// FP32 copy of the closed FP64 values, made on first use
+// FP32 copy of closed FP64 values / FP64 copy of closed FP32 values, made on first use
 void ensureVal32(pxg_ctx * c);
+void ensureVal64(pxg_ctx * c);
 // element_mfma.hip: fused geometry + MFMA element kernel; false if not applicable
 bool elementMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, double cMF, int precision);
 
@@ -31,7 +32,20 @@ struct CsrView
   float const * val32 = nullptr;
   double const * dinv = nullptr;
   float const * dinv32 = nullptr;
+  // row blocks of the stream kernel (block q: rows blockRow[q] .. blockRow[q+1]-1,
+  // at most kStreamSlots entries); null: the row-group kernel is used
+  int32_t const * blockRow = nullptr;
+  int64_t nblocks = 0;
 };
+
+// entries staged in LDS by one block of the stream kernel
+constexpr int kStreamSlots = 2048;
+// builds the row-block table of a CSR matrix (device rowptr); nblocks = 0
+// when some row is too long for the stream kernel.  Synchronises the stream.
+void buildRowBlocks(int64_t nrows, int64_t nnz, int64_t const * rowptr, DevBuf<int32_t> & blockRow,
+                    int64_t & nblocks, hipStream_t s);
+// row blocks of the closed matrix of a context, built on first use after a close
+void ensureRowBlocks(pxg_ctx * c);
 
 CsrView viewOf(pxg_ctx const * c);
 

@@ -8,7 +8,11 @@
 //   pre-smooth  nu1 weighted-Jacobi sweeps from x = 0
 //   residual    r = b - A x
 //   restrict    b_c = P^T r          (R = P^T, not the row-normalised R)
-//   recurse     e_c = V(b_c), coarsest level: Jacobi-CG in FP64 to coarse_rtol
+//   recurse     e_c = V(b_c), coarsest level: direct solve in FP64 (dense
+//               inverse of its free-dof block, made on the GPU by Gauss-Jordan
+//               elimination whenever the coarse matrix changes and applied as
+//               one matrix-vector product, like the reference test's LU coarse
+//               solve); above kDenseMax free dofs Jacobi-CG to coarse_rtol
 //   prolongate  x += P e_c
 //   post-smooth nu2 sweeps
 // P(f, c) is the coarse basis function c evaluated at fine dof point f in the
@@ -54,6 +58,7 @@ struct pxg_mg
     DevBuf<int32_t> Pcol, Rcol;
     DevBuf<double> Pval, Rval;
     DevBuf<float> Pval32, Rval32;
+    DevBuf<int32_t> Pblk, Rblk; // row blocks of the stream SpMV kernel
     CsrView P, R;
     DevBuf<double> b, x, t, r;
     DevBuf<float> b32, x32, t32, r32;
@@ -64,6 +69,14 @@ struct pxg_mg
   DevBuf<float> r32, z32, p32, q32, d32;
   // coarsest-level CG vectors
   DevBuf<double> cr, cz, cp, cq;
+  // coarsest-level direct solve: inverse of the free-dof block
+  bool dense = false;
+  uint64_t denseStamp = 0;
+  int64_t nfree = 0;
+  DevBuf<int32_t> freeIdx;  // coarse dof -> free index, -1 for constrained dofs
+  DevBuf<int32_t> freeDofs; // free index -> coarse dof
+  DevBuf<double> inv, invTmp;
+  DevBuf<int> flag;
 };
 
 namespace
@@ -86,6 +99,91 @@ struct Buf<float>
   static float * t(pxg_mg::Level & L) { return L.t32.p; }
   static float * r(pxg_mg::Level & L) { return L.r32.p; }
 };
+
+// largest free-dof block inverted densely on the coarsest level (8 n^2 bytes,
+// n Gauss-Jordan steps of n^2 work each)
+constexpr int64_t kDenseMax = 4096;
+
+__global__ void k_dense_scatter(int64_t n, int32_t const * __restrict__ freeIdx, int64_t const * __restrict__ rowptr,
+                                int32_t const * __restrict__ col, double const * __restrict__ val, int64_t nf,
+                                double * __restrict__ D)
+{
+  int64_t const r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n)
+    return;
+  int64_t const i = freeIdx[r];
+  if (i < 0)
+    return;
+  for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k)
+  {
+    int64_t const j = freeIdx[col[k]];
+    if (j >= 0)
+      D[i * nf + j] = val[k];
+  }
+}
+
+// one Gauss-Jordan step on pivot k, A -> B (out of place, so every thread
+// reads the pivot row and column as they were before the step).  After steps
+// 0..n-1 an SPD matrix has become its inverse; the pivots of an SPD matrix
+// stay positive, so no pivoting is needed.
+__global__ void k_gauss_jordan(int64_t nf, int64_t k, double const * __restrict__ A, double * __restrict__ B)
+{
+  int64_t const t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nf * nf)
+    return;
+  int64_t const i = t / nf, j = t - i * nf;
+  double const piv = A[k * nf + k];
+  double v;
+  if (i == k)
+    v = (j == k) ? 1. / piv : A[k * nf + j] / piv;
+  else if (j == k)
+    v = -A[i * nf + k] / piv;
+  else
+    v = A[t] - A[i * nf + k] * (A[k * nf + j] / piv);
+  B[t] = v;
+}
+
+// one block: flag = 1 if a diagonal entry of the inverse is not positive and
+// finite (the matrix was not SPD)
+__global__ void k_inverse_check(int64_t nf, double const * __restrict__ inv, int * __restrict__ flag)
+{
+  int bad = 0;
+  for (int64_t i = threadIdx.x; i < nf; i += blockDim.x)
+  {
+    double const d = inv[i * nf + i];
+    bad |= !(d > 0. && isfinite(d));
+  }
+  bad = __syncthreads_or(bad);
+  if (threadIdx.x == 0)
+    *flag = bad;
+}
+
+// x = A^-1 b on the coarsest level: one wave per row; constrained rows are
+// diagonal (x = b / a_rr), free rows take a dot with their inverse row
+__global__ void k_dense_apply(int64_t n, int64_t nf, int32_t const * __restrict__ freeIdx,
+                              int32_t const * __restrict__ freeDofs, double const * __restrict__ inv,
+                              double const * __restrict__ dinv, double const * __restrict__ b, double * __restrict__ x)
+{
+  int const lane = threadIdx.x & 63;
+  int64_t const r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  if (r >= n)
+    return;
+  int64_t const i = freeIdx[r];
+  if (i < 0)
+  {
+    if (lane == 0)
+      x[r] = dinv[r] * b[r];
+    return;
+  }
+  double const * row = inv + i * nf;
+  double acc = 0.;
+  for (int64_t j = lane; j < nf; j += 64)
+    acc += row[j] * b[freeDofs[j]];
+  for (int o = 32; o > 0; o >>= 1)
+    acc += __shfl_xor(acc, o);
+  if (lane == 0)
+    x[r] = acc;
+}
 
 double readScalar(pxg_mg * m, int i)
 {
@@ -123,11 +221,13 @@ void buildProlongation(pxg_mg * m, int l)
   F->xyz.download(xf.data(), xf.size(), F->stream);
   F->conn.download(cf.data(), cf.size(), F->stream);
   F->elemMap.download(ef.data(), ef.size(), F->stream);
-  F->bcFlag.download(bf.data(), bf.size(), F->stream);
+  if (F->hasBC)
+    F->bcFlag.download(bf.data(), bf.size(), F->stream);
   Cc->xyz.download(xc.data(), xc.size(), Cc->stream);
   Cc->conn.download(cc.data(), cc.size(), Cc->stream);
   Cc->elemMap.download(ec.data(), ec.size(), Cc->stream);
-  Cc->bcFlag.download(bcn.data(), bcn.size(), Cc->stream);
+  if (Cc->hasBC)
+    Cc->bcFlag.download(bcn.data(), bcn.size(), Cc->stream);
   PXG_HIP(hipStreamSynchronize(F->stream));
   PXG_HIP(hipStreamSynchronize(Cc->stream));
   int64_t const nf = F->ndofs;
@@ -248,6 +348,10 @@ void buildProlongation(pxg_mg * m, int l)
   PXG_HIP(hipStreamSynchronize(s));
   L.P = CsrView{nf, ncr, nnz, L.Prp.p, L.Pcol.p, L.Pval.p, L.Pval32.p, nullptr, nullptr};
   L.R = CsrView{ncr, nf, nnz, L.Rrp.p, L.Rcol.p, L.Rval.p, L.Rval32.p, nullptr, nullptr};
+  buildRowBlocks(nf, nnz, L.Prp.p, L.Pblk, L.P.nblocks, s);
+  buildRowBlocks(ncr, nnz, L.Rrp.p, L.Rblk, L.R.nblocks, s);
+  L.P.blockRow = L.P.nblocks > 0 ? L.Pblk.p : nullptr;
+  L.R.blockRow = L.R.nblocks > 0 ? L.Rblk.p : nullptr;
 }
 
 // smoothing sweeps, ping-ponging between x and t, result left in x
@@ -280,13 +384,85 @@ void smooth(CsrView const & A, T const * b, T * x, T * t, int sweeps, double ome
     PXG_HIP(hipMemcpyAsync(x, cur, A.nrows * sizeof(T), hipMemcpyDeviceToDevice, s));
 }
 
-// coarsest level: Jacobi-preconditioned CG in FP64 from zero to coarseRtol
+// (re)builds the dense inverse of the coarsest free-dof block when the coarse
+// matrix has changed since the last factorisation
+void factorCoarse(pxg_mg * m)
+{
+  pxg_ctx * c = m->ctx.back();
+  hipStream_t s = m->s;
+  ensureVal64(c);
+  if (m->denseStamp == c->stamp && c->stamp != 0)
+    return;
+  int64_t const n = c->ndofs;
+  std::vector<uint8_t> bc(n, 0);
+  if (c->hasBC)
+  {
+    c->bcFlag.download(bc.data(), n, s);
+    PXG_HIP(hipStreamSynchronize(s));
+  }
+  std::vector<int32_t> idx(n, -1), dofs;
+  for (int64_t r = 0; r < n; ++r)
+    if (!bc[r])
+    {
+      idx[r] = (int32_t)dofs.size();
+      dofs.push_back((int32_t)r);
+    }
+  int64_t const nf = (int64_t)dofs.size();
+  m->dense = false;
+  m->denseStamp = c->stamp;
+  if (nf > kDenseMax)
+    return; // iterative coarse solve
+  m->nfree = nf;
+  m->freeIdx.upload(idx.data(), n, s);
+  if (nf == 0)
+  {
+    m->dense = true;
+    m->denseStamp = c->stamp;
+    return;
+  }
+  m->freeDofs.upload(dofs.data(), nf, s);
+  m->inv.resize(nf * nf);
+  m->invTmp.resize(nf * nf);
+  m->flag.resize(1);
+  CsrView const A = viewOf(c);
+  int const bs = 256;
+  unsigned const g2 = (unsigned)((nf * nf + bs - 1) / bs);
+  fill<double>(nf * nf, m->inv.p, 0., s);
+  k_dense_scatter<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(n, m->freeIdx.p, A.rowptr, A.col, A.val, nf, m->inv.p);
+  double * a = m->inv.p, * b = m->invTmp.p;
+  for (int64_t k = 0; k < nf; ++k)
+  {
+    k_gauss_jordan<<<g2, bs, 0, s>>>(nf, k, a, b);
+    std::swap(a, b);
+  }
+  if (a != m->inv.p)
+    PXG_HIP(hipMemcpyAsync(m->inv.p, a, nf * nf * sizeof(double), hipMemcpyDeviceToDevice, s));
+  k_inverse_check<<<1, bs, 0, s>>>(nf, m->inv.p, m->flag.p);
+  PXG_HIP(hipGetLastError());
+  int bad = 0;
+  PXG_HIP(hipMemcpyAsync(&bad, m->flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  PXG_HIP(hipStreamSynchronize(s));
+  // a singular or indefinite coarse block (no Dirichlet dofs, pure Neumann)
+  // keeps the iterative coarse solve
+  m->dense = !bad;
+  m->denseStamp = c->stamp;
+}
+
+// coarsest level: dense direct solve, or Jacobi-preconditioned CG in FP64
+// from zero to coarseRtol when the free block is too large to invert
 void coarseSolve(pxg_mg * m, double const * b, double * x)
 {
   pxg_ctx * c = m->ctx.back();
   CsrView const A = viewOf(c);
   int64_t const n = A.nrows;
   hipStream_t s = m->s;
+  if (m->dense)
+  {
+    k_dense_apply<<<(unsigned)((n + 3) / 4), 256, 0, s>>>(n, m->nfree, m->freeIdx.p, m->freeDofs.p, m->inv.p,
+                                                        A.dinv, b, x);
+    PXG_HIP(hipGetLastError());
+    return;
+  }
   double * r = m->cr.p, * z = m->cz.p, * p = m->cp.p, * q = m->cq.p;
   double * sc = m->sc.p;
   fill<double>(n, x, 0., s);
@@ -495,6 +671,7 @@ int pxg_mg_create(pxg_mg ** out, int nlev, pxg_ctx ** levels, const int * dims, 
     }
     for (int l = 0; l + 1 < nlev; ++l)
       buildProlongation(m, l);
+    factorCoarse(m);
     int64_t const n0 = m->lv[0].n, nc = m->lv.back().n;
     m->sc.resize(16);
     m->partial.resize(kDotBlocks);
@@ -541,13 +718,21 @@ int pxg_mg_update(pxg_mg * m)
 {
   if (!m)
     return PXG_E_INVALID;
-  for (auto * c: m->ctx)
-    if (!c->closed)
-    {
-      m->err = "levels must be closed";
-      return PXG_E_STATE;
-    }
-  return PXG_OK;
+  try
+  {
+    PXG_HIP(hipSetDevice(m->ctx[0]->device));
+    m->s = m->ctx[0]->stream;
+    for (auto * c: m->ctx)
+      PXG_REQUIRE(c->closed, PXG_E_STATE, "levels must be closed");
+    factorCoarse(m);
+    m->err.clear();
+    return PXG_OK;
+  }
+  catch (Error const & e)
+  {
+    m->err = e.what();
+    return e.code;
+  }
 }
 
 int pxg_mg_solve(pxg_mg * m, int mode, const double * b, double * x, double rtol, int maxit, int * iters,
@@ -561,6 +746,9 @@ int pxg_mg_solve(pxg_mg * m, int mode, const double * b, double * x, double rtol
     m->s = m->ctx[0]->stream; // follow the finest context (its stream may have been replaced)
     for (auto * c: m->ctx)
       PXG_REQUIRE(c->closed, PXG_E_STATE, "levels must be closed");
+    factorCoarse(m);
+    for (auto * c: m->ctx)
+      ensureRowBlocks(c);
     double rr = 0.;
     int it = 0;
     if (mode == PXG_FP32)
@@ -570,7 +758,11 @@ int pxg_mg_solve(pxg_mg * m, int mode, const double * b, double * x, double rtol
       it = solveIR(m, b, x, rtol, maxit, rr);
     }
     else
+    {
+      for (auto * c: m->ctx)
+        ensureVal64(c);
       it = pcg<double>(m, b, x, rtol, maxit, rr, false);
+    }
     PXG_HIP(hipStreamSynchronize(m->s));
     if (iters)
       *iters = it;
@@ -599,6 +791,13 @@ int pxg_mg_vcycle(pxg_mg * m, const double * b, double * x)
   {
     PXG_HIP(hipSetDevice(m->ctx[0]->device));
     m->s = m->ctx[0]->stream; // follow the finest context (its stream may have been replaced)
+    for (auto * c: m->ctx)
+    {
+      PXG_REQUIRE(c->closed, PXG_E_STATE, "levels must be closed");
+      ensureVal64(c);
+      ensureRowBlocks(c);
+    }
+    factorCoarse(m);
     vcycle<double>(m, 0, b, x);
     PXG_HIP(hipStreamSynchronize(m->s));
     return PXG_OK;

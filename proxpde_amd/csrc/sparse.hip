@@ -2,10 +2,15 @@
 // operators (P x, P^T r as CSR products) and the vector kernels of CG.
 //
 // SpMV follows SparseMatrix<RowMajor> * Vec of the reference's multigrid test
-// (test/multigrid.cpp:78,81,97): one group of LPR lanes per row (LPR picked
-// from the mean row length so a 64-wide wavefront stays busy), lanes stride
-// the row, then a butterfly reduction inside the group; the summation order
-// is fixed by the launch shape, so results are bitwise repeatable.
+// (test/multigrid.cpp:78,81,97).  Two kernels, both with a summation order
+// fixed by the matrix and the launch shape (bitwise repeatable):
+//  * stream kernel (the CSR-stream scheme of CSR-Adaptive, Greathouse and
+//    Daga, SC'14): a block takes the rows whose first entry falls in one
+//    window of `cap` entries (at most kStreamSlots entries in all), streams
+//    their column/value arrays fully coalesced, multiplies by the gathered x
+//    into LDS, then reduces each row from LDS with a group of 8 lanes;
+//  * row-group kernel (no row-block table, or rows too long): one group of
+//    LPR lanes per row, lanes stride the row, butterfly reduction.
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -49,21 +54,28 @@ __global__ void __launch_bounds__(kBlock)
   // the matrix streams bypass the caches (non-temporal), keeping x resident
   constexpr int kUnroll = 8;
   int const len = (int)(k1 - k0);
-  int32_t cc[kUnroll];
-  T vv[kUnroll];
-#pragma unroll
-  for (int m = 0; m < kUnroll; ++m)
-  {
-    int64_t const k = k0 + lane + (int64_t)m * LPR;
-    bool const in = lane + m * LPR < len;
-    cc[m] = in ? __builtin_nontemporal_load(col + k) : 0;
-    vv[m] = in ? __builtin_nontemporal_load(val + k) : T(0);
-  }
   T s = T(0);
+  if (len > 0)
+  {
+    // indices clamped into the row: the loads carry no condition, so they
+    // are all issued before the first use
+    int32_t cc[kUnroll];
+    T vv[kUnroll];
 #pragma unroll
-  for (int m = 0; m < kUnroll; ++m)
-    if (lane + m * LPR < len)
-      s += vv[m] * x[cc[m]];
+    for (int m = 0; m < kUnroll; ++m)
+    {
+      int64_t const k = k0 + min(lane + m * LPR, len - 1);
+      cc[m] = __builtin_nontemporal_load(col + k);
+      vv[m] = __builtin_nontemporal_load(val + k);
+    }
+    T xv[kUnroll];
+#pragma unroll
+    for (int m = 0; m < kUnroll; ++m)
+      xv[m] = x[cc[m]];
+#pragma unroll
+    for (int m = 0; m < kUnroll; ++m)
+      s += lane + m * LPR < len ? vv[m] * xv[m] : T(0);
+  }
   for (int64_t k = k0 + lane + (int64_t)kUnroll * LPR; k < k1; k += LPR)
     s += val[k] * x[col[k]];
 #pragma unroll
@@ -80,6 +92,114 @@ __global__ void __launch_bounds__(kBlock)
     else
       y[row] += s;
   }
+}
+
+template <typename T, int MODE>
+__device__ inline void csrOut(int64_t row, T s, T const * __restrict__ x, T const * __restrict__ b,
+                              T const * __restrict__ dinv, T omega, T * __restrict__ y)
+{
+  if constexpr (MODE == kModeApply)
+    y[row] = s;
+  else if constexpr (MODE == kModeResidual)
+    y[row] = b[row] - s;
+  else if constexpr (MODE == kModeJacobi)
+    y[row] = x[row] + omega * dinv[row] * (b[row] - s);
+  else
+    y[row] += s;
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kBlock)
+    k_csr_stream(int32_t const * __restrict__ blockRow, int64_t const * __restrict__ rowptr,
+                 int32_t const * __restrict__ col, T const * __restrict__ val, T const * __restrict__ x,
+                 T const * __restrict__ b, T const * __restrict__ dinv, T omega, T * __restrict__ y)
+{
+  constexpr int kSlots = kStreamSlots / kBlock; // entries per thread
+  constexpr int G = 8;                          // lanes per row in the reduction
+  __shared__ T prod[kStreamSlots];
+  int64_t const r0 = blockRow[blockIdx.x], r1 = blockRow[blockIdx.x + 1];
+  if (r0 == r1)
+    return; // a window covered by a row that started earlier
+  int64_t const base = rowptr[r0], end = rowptr[r1];
+  int const t = threadIdx.x;
+  if (end > base) // block-uniform
+  {
+    // every load is issued unconditionally (indices clamped into the block,
+    // out-of-range slots masked to zero) so all of them are in flight at once
+    int32_t cc[kSlots];
+    T vv[kSlots];
+#pragma unroll
+    for (int m = 0; m < kSlots; ++m)
+    {
+      int64_t const k = min(base + t + m * kBlock, end - 1);
+      cc[m] = __builtin_nontemporal_load(col + k);
+      vv[m] = __builtin_nontemporal_load(val + k);
+    }
+    T xv[kSlots];
+#pragma unroll
+    for (int m = 0; m < kSlots; ++m)
+      xv[m] = x[cc[m]];
+#pragma unroll
+    for (int m = 0; m < kSlots; ++m)
+      prod[t + m * kBlock] = base + t + m * kBlock < end ? vv[m] * xv[m] : T(0);
+  }
+  __syncthreads();
+  int const lane = t % G;
+  // the lanes of one group run the same rows, so the group stays converged
+  for (int64_t row = r0 + t / G; row < r1; row += kBlock / G)
+  {
+    int const k0 = (int)(rowptr[row] - base), k1 = (int)(rowptr[row + 1] - base);
+    // rows up to kRed * G entries in one unrolled pass of independent reads
+    constexpr int kRed = 16;
+    T s = T(0);
+#pragma unroll
+    for (int j = 0; j < kRed; ++j)
+    {
+      int const k = k0 + lane + j * G;
+      T const v = prod[min(k, kStreamSlots - 1)];
+      s += k < k1 ? v : T(0);
+    }
+    for (int k = k0 + lane + kRed * G; k < k1; k += G)
+      s += prod[k];
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+      s += __shfl_xor(s, off, G);
+    if (lane == 0)
+      csrOut<T, MODE>(row, s, x, b, dinv, omega, y);
+  }
+}
+
+__global__ void k_max_row_len(int64_t nrows, int64_t const * __restrict__ rowptr, int * __restrict__ out)
+{
+  __shared__ int sh[kBlock];
+  int m = 0;
+  for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * kBlock)
+    m = max(m, (int)min<int64_t>(rowptr[r + 1] - rowptr[r], 1 << 30));
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1)
+  {
+    if ((int)threadIdx.x < off)
+      sh[threadIdx.x] = max(sh[threadIdx.x], sh[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    out[blockIdx.x] = sh[0];
+}
+
+// blockRow[q] = first row whose first entry is >= q * cap (q = 0 .. nb);
+// row r takes the q with rowptr[r-1] < q * cap <= rowptr[r] (rowptr[-1] = -inf,
+// rowptr[nrows] = +inf), so every q is written exactly once
+__global__ void k_block_rows(int64_t nrows, int64_t const * __restrict__ rowptr, int64_t cap, int64_t nb,
+                             int32_t * __restrict__ blockRow)
+{
+  int64_t const r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r > nrows)
+    return;
+  int64_t const lo = r == 0 ? 0 : rowptr[r - 1] / cap + 1;
+  int64_t const hi = r == nrows ? nb : min(rowptr[r] / cap, nb);
+  for (int64_t q = lo; q <= hi; ++q)
+    blockRow[q] = (int32_t)r;
 }
 
 // lanes per row from the mean row length; PXG_SPMV_LPR (4..64) overrides the
@@ -105,16 +225,30 @@ int lanesPerRow(double mean)
   return 4;
 }
 
+// PXG_SPMV_KERNEL=rowgroup selects the row-group kernel (launch-shape tuning)
+bool rowGroupForced()
+{
+  char const * e = std::getenv("PXG_SPMV_KERNEL");
+  return e && e[0] == 'r';
+}
+
 template <typename T, int MODE>
 void launchCsr(CsrView const & A, T const * val, T const * x, T const * b, T const * dinv, double omega, T * y,
                hipStream_t s)
 {
   if (A.nrows == 0)
     return;
+  T const om = (T)omega;
+  if (A.blockRow && A.nblocks > 0 && !rowGroupForced())
+  {
+    k_csr_stream<T, MODE><<<(unsigned)A.nblocks, kBlock, 0, s>>>(A.blockRow, A.rowptr, A.col, val, x, b, dinv, om,
+                                                                   y);
+    PXG_HIP(hipGetLastError());
+    return;
+  }
   double const mean = A.nnz > 0 ? (double)A.nnz / (double)A.nrows : 1.0;
   int lpr = lanesPerRow(mean);
   unsigned const g = blocksFor(A.nrows * lpr);
-  T const om = (T)omega;
   switch (lpr)
   {
   case 4: k_csr<T, 4, MODE><<<g, kBlock, 0, s>>>(A.nrows, A.rowptr, A.col, val, x, b, dinv, om, y); break;
@@ -257,6 +391,43 @@ __global__ void k_axpby(int64_t n, double a, double const * __restrict__ x, doub
 
 } // namespace
 
+void buildRowBlocks(int64_t nrows, int64_t nnz, int64_t const * rowptr, DevBuf<int32_t> & blockRow,
+                    int64_t & nblocks, hipStream_t s)
+{
+  nblocks = 0;
+  if (nrows <= 0 || nnz <= 0 || nrows >= (int64_t)1 << 31)
+    return;
+  constexpr int kParts = 256;
+  DevBuf<int> part;
+  part.resize(kParts);
+  k_max_row_len<<<kParts, kBlock, 0, s>>>(nrows, rowptr, part.p);
+  PXG_HIP(hipGetLastError());
+  int h[kParts];
+  PXG_HIP(hipMemcpyAsync(h, part.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  PXG_HIP(hipStreamSynchronize(s));
+  int maxLen = 0;
+  for (int v: h)
+    maxLen = std::max(maxLen, v);
+  if (maxLen > kStreamSlots / 2)
+    return; // rows too long for one block's staging: row-group kernel
+  // a window of cap entries plus the tail of its last row fits the staging
+  int64_t const cap = kStreamSlots - std::max(maxLen, 1);
+  int64_t const nb = nnz / cap + 1;
+  blockRow.resize(nb + 1);
+  k_block_rows<<<blocksFor(nrows + 1), kBlock, 0, s>>>(nrows, rowptr, cap, nb, blockRow.p);
+  PXG_HIP(hipGetLastError());
+  PXG_HIP(hipStreamSynchronize(s));
+  nblocks = nb;
+}
+
+void ensureRowBlocks(pxg_ctx * c)
+{
+  if (!c->closed || c->blockStamp == c->stamp)
+    return;
+  buildRowBlocks(c->ndofs, c->nnz, c->rowptr.p, c->blockRow, c->nStreamBlocks, c->stream);
+  c->blockStamp = c->stamp;
+}
+
 CsrView viewOf(pxg_ctx const * c)
 {
   CsrView v;
@@ -269,6 +440,9 @@ CsrView viewOf(pxg_ctx const * c)
   v.val32 = c->val32.p;
   v.dinv = c->dinv.p;
   v.dinv32 = c->dinv32.p;
+  bool const blocks = c->nStreamBlocks > 0 && c->blockStamp == c->stamp;
+  v.blockRow = blocks ? c->blockRow.p : nullptr;
+  v.nblocks = blocks ? c->nStreamBlocks : 0;
   return v;
 }
 

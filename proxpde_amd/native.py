@@ -88,6 +88,7 @@ def lib():
         "pxg_jacobi": [vp, i32, C.c_double, i32, vp, vp],
         "pxg_csr_device": [vp, C.POINTER(i64p), C.POINTER(i32p), C.POINTER(dp), C.POINTER(fp), i64p],
         "pxg_rhs_device": [vp, C.POINTER(dp)],
+        "pxg_matrix_stamp": [vp, C.POINTER(C.c_uint64)],
         "pxg_mf_apply": [vp, vp, i32, vp, vp, i32],
         "pxg_mf_grad_kappa": [vp, i32, vp, vp, vp, i32],
         "pxg_mg_create": [pp, i32, pp, C.POINTER(C.c_int), C.c_double, i32, i32, C.c_double],
@@ -274,6 +275,12 @@ class Assembler:
 
     def clear(self):
         self._ck(lib().pxg_clear(self.h))
+
+    def stamp(self):
+        """Changes whenever the closed matrix is replaced or invalidated."""
+        st = C.c_uint64()
+        self._ck(lib().pxg_matrix_stamp(self.h, C.byref(st)))
+        return st.value
 
     def csr(self, fp32=False):
         rowptr = np.empty(self.ndofs + 1, dtype=np.int64)

@@ -12,6 +12,8 @@
 //                 MG-PCG (mode 0: FP64; mode 1: FP32 inner solves in FP64 refinement)
 //       backward: adjoint solve K lambda = du;  db = lambda;
 //                 dkappa = -d/dkappa (lambda . K(kappa) u)  (Dirichlet rows masked)
+//                 The level matrices are re-assembled only if some level was
+//                 closed or cleared since the forward pass (pxg_matrix_stamp).
 //
 // Handles (pxg_ctx*, pxg_mg*) and the HIP stream are passed as integers.
 #include <torch/library.h>
@@ -107,6 +109,18 @@ struct SolveSetup
   bool fp32Levels;
 };
 
+std::vector<int64_t> stampsOf(SolveSetup const & S)
+{
+  std::vector<int64_t> out;
+  for (auto * c: S.levels)
+  {
+    uint64_t st = 0;
+    check(pxg_matrix_stamp(c, &st), c, "pxg_matrix_stamp");
+    out.push_back((int64_t)st);
+  }
+  return out;
+}
+
 // assemble K(kappa) on all levels (coarse kappa by injection), close them
 void assembleLevels(SolveSetup const & S, Tensor const & kappa)
 {
@@ -185,6 +199,7 @@ public:
     ctx->saved_data["mode"] = mode;
     ctx->saved_data["maxit"] = maxit;
     ctx->saved_data["fp32"] = fp32Levels;
+    ctx->saved_data["stamps"] = stampsOf(S);
     ctx->save_for_backward({kc, u});
     return u;
   }
@@ -194,8 +209,10 @@ public:
     auto saved = ctx->get_saved_variables();
     auto kappa = saved[0], u = saved[1];
     SolveSetup S = setupFrom(ctx);
-    // the level matrices still hold K(kappa) unless another solve ran since
-    assembleLevels(S, kappa);
+    // the level matrices still hold K(kappa) unless a level was closed or
+    // cleared since the forward pass (another solve, a right-hand side)
+    if (stampsOf(S) != ctx->saved_data["stamps"].toIntVector())
+      assembleLevels(S, kappa);
     Tensor lambda = solveWith(S, grads[0]);
     Tensor gk;
     if (ctx->needs_input_grad(0))

@@ -102,3 +102,25 @@ def test_kappa_solve_fp32_refined_matches_fp64(ag, gpu):
     err = float((u32 - u64).norm() / u64.norm())
     print(f"  FP32+IR vs FP64 relative difference {err:.2e}")
     assert err < 1e-6
+
+
+def test_kappa_solve_backward_reassembles_only_when_needed(ag, gpu):
+    """The backward pass reuses the level matrices of the forward pass unless a
+    level was cleared or closed in between; both routes give one gradient."""
+    solver = ag.KappaSolver("hex_q2", (4, 2))
+    x = solver.probs[0].dof_coords()
+    kap = torch.from_numpy(np.exp(np.sin(3 * x[:, 0]) * np.cos(2 * x[:, 1]))).to(gpu)
+    b = torch.from_numpy(solver.rhs(1.0)).to(gpu)
+    grads = []
+    for intervene in (False, True):
+        k = kap.clone().requires_grad_()
+        u = solver(k, b, rtol=1e-12)
+        stamps = [c.stamp() for c in solver.ctx]
+        if intervene:
+            solver.rhs(1.0)  # clears the finest level: the matrices must be rebuilt
+            assert solver.ctx[0].stamp() != stamps[0]
+        (0.5 * (u ** 2).sum()).backward()
+        if not intervene:
+            assert [c.stamp() for c in solver.ctx] == stamps
+        grads.append(k.grad.clone())
+    assert float((grads[0] - grads[1]).abs().max()) <= 1e-12 * float(grads[0].abs().max())

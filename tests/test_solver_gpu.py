@@ -93,7 +93,53 @@ def test_prolongation_and_vcycle(oracle, native, gpu, fe, ns):
     xt = torch.zeros_like(bt)
     mg.vcycle_dev(bt.data_ptr(), xt.data_ptr())
     x_ref = omg.vcycle(b)
-    assert np.abs(xt.cpu().numpy() - x_ref).max() <= 1e-8 * np.abs(x_ref).max()
+    # both coarsest solves are direct (dense inverse / dense Cholesky)
+    assert np.abs(xt.cpu().numpy() - x_ref).max() <= 1e-11 * np.abs(x_ref).max()
+
+
+def test_coarse_direct_solve_and_stamp(native, gpu):
+    """A one-level hierarchy is the coarsest solve itself: the exact solution;
+    re-assembly changes the matrix stamp and the next cycle refactors."""
+    import torch
+    prob, a, kap, dofs = _hex_level("hex_q2", 3, _kappa)
+    mg = native.Multigrid([a], [(3, 3, 3)])
+    for scale in (1.0, 3.0):
+        rp, col, val, b = a.csr()
+        A = sp.csr_matrix((val, col, rp), shape=(prob.ndofs, prob.ndofs))
+        u = spl.spsolve(A.tocsc(), b)
+        bt = torch.from_numpy(b).to(gpu)
+        xt = torch.zeros_like(bt)
+        mg.vcycle_dev(bt.data_ptr(), xt.data_ptr())
+        assert np.abs(xt.cpu().numpy() - u).max() <= 1e-12 * np.abs(u).max()
+        s0 = a.stamp()
+        a.set_coefficient(scale * 3.0 * kap)
+        a.clear()
+        assert a.stamp() != s0
+        a.assemble([("stiffness_fe", 1.0)])
+        a.assemble_rhs("const", (1.0,))
+        a.close()
+
+
+def test_prolongation_known_answer(native, gpu):
+    """Reference test/prolongation.cpp:209 (Q1, 3x2 -> 6x4, u = (1-x)^2, |P u_c| =
+    3.009757793463), extruded to hex 3x2x1 -> 6x4x2 where each of the 3 fine
+    z-layers repeats the 2D values: |P u_c| = sqrt(3) * 3.009757793463.  No
+    Dirichlet dofs, so no rows or columns of P are removed."""
+    ctxs = []
+    for n in ((6, 4, 2), (3, 2, 1)):
+        prob = W.Problem("hex_q1", n)
+        a = prob.assembler()
+        a.set_dirichlet(np.zeros(0, np.uint32), np.zeros(0))
+        a.set_coefficient(np.ones(prob.ndofs))
+        a.build_pattern()
+        a.assemble([("stiffness_fe", 1.0), ("mass", 1.0)])
+        a.close()
+        ctxs.append((prob, a))
+    mg = native.Multigrid([a for _, a in ctxs], [(6, 4, 2), (3, 2, 1)])
+    rp, col, val = mg.prolongation(0)
+    P = sp.csr_matrix((val, col, rp), shape=(ctxs[0][0].ndofs, ctxs[1][0].ndofs))
+    uc = (1.0 - ctxs[1][0].dof_coords()[:, 0]) ** 2
+    assert abs(np.linalg.norm(P @ uc) - np.sqrt(3.0) * 3.009757793463) <= 1e-11
 
 
 @pytest.mark.parametrize("fe,ns", [("hex_q1", (16, 8, 4, 2)), ("hex_q2", (8, 4, 2))])
