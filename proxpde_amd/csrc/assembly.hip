@@ -373,6 +373,72 @@ __global__ void __launch_bounds__(kBlock)
   keep[t] = (fabs((double)acc) <= kDefaultPrune) ? 0u : 1u;
 }
 
+// Rebuild after clear into the pruned layout of the previous close: entry t
+// is summed as in k_scatter; kept entries of the layout go to valC[pos[t]],
+// pruned ones to valS[t]; a prune flag that differs from the layout's sets
+// *changed, and close then rebuilds the layout from the split values.
+template <typename T, int kMaxC>
+__global__ void __launch_bounds__(kBlock)
+    k_scatter_direct(int64_t nnzS, uint32_t const * __restrict__ segPtr, uint32_t const * __restrict__ contrib,
+                     T const * __restrict__ ke, uint32_t const * __restrict__ keep, uint32_t const * __restrict__ pos,
+                     T * __restrict__ valC, T * __restrict__ valS, int * __restrict__ changed)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnzS)
+    return;
+  uint32_t const k0 = segPtr[t], k1 = segPtr[t + 1];
+  uint32_t const kept = keep[t], o = pos[t];
+  uint32_t id[kMaxC];
+#pragma unroll
+  for (int j = 0; j < kMaxC; ++j)
+    id[j] = contrib[min(k0 + j, k1 - 1)];
+  T v[kMaxC];
+#pragma unroll
+  for (int j = 0; j < kMaxC; ++j)
+    v[j] = ke[id[j]];
+  T acc = T(0);
+#pragma unroll
+  for (int j = 0; j < kMaxC; ++j)
+    if (k0 + j < k1)
+      acc += v[j];
+  for (uint32_t k = k0 + kMaxC; k < k1; ++k)
+    acc += ke[contrib[k]];
+  uint32_t const now = (fabs((double)acc) <= kDefaultPrune) ? 0u : 1u;
+  if (kept)
+    valC[o] = acc;
+  else
+    valS[t] = acc;
+  if (now != kept)
+    *changed = 1;
+}
+
+// structural values back from the split storage
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_unsplit(int64_t nnzS, uint32_t const * __restrict__ keep, uint32_t const * __restrict__ pos,
+              T const * __restrict__ valC, T * __restrict__ valS)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nnzS && keep[t])
+    valS[t] = valC[pos[t]];
+}
+
+template <typename T>
+__global__ void k_diag_inverse_split(int64_t nrows, uint32_t const * __restrict__ diagPos,
+                                     uint32_t const * __restrict__ keep, uint32_t const * __restrict__ pos,
+                                     T const * __restrict__ valC, T const * __restrict__ valS, double eps,
+                                     T * __restrict__ dinv)
+{
+  int64_t const r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows)
+    return;
+  uint32_t const dp = diagPos[r];
+  T const d = keep[dp] ? valC[pos[dp]] : valS[dp];
+  dinv[r] = (fabs((double)d) <= eps || d == T(0)) ? T(0) : T(1) / d;
+}
+
+__global__ void k_set_int(int * p, int v) { *p = v; }
+
 // contributions loaded ahead per structural entry (PXG_SCATTER_UNROLL = 1, 2,
 // 4 or 8 overrides the default for launch tuning)
 int scatterUnroll()
@@ -393,6 +459,23 @@ void launchScatterU(int u, int64_t nnzS, uint32_t const * segPtr, uint32_t const
   case 2: k_scatter<T, OVERWRITE, 2><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, val, keep); break;
   case 8: k_scatter<T, OVERWRITE, 8><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, val, keep); break;
   default: k_scatter<T, OVERWRITE, 4><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, val, keep); break;
+  }
+}
+
+template <typename T>
+void launchScatterDirect(int64_t nnzS, uint32_t const * segPtr, uint32_t const * contrib, T const * ke,
+                         uint32_t const * keep, uint32_t const * pos, T * valC, T * valS, int * changed,
+                         hipStream_t s)
+{
+  unsigned const g = gridFor(nnzS);
+  switch (scatterUnroll())
+  {
+  case 1: k_scatter_direct<T, 1><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, keep, pos, valC, valS, changed); break;
+  case 2: k_scatter_direct<T, 2><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, keep, pos, valC, valS, changed); break;
+  case 8: k_scatter_direct<T, 8><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, keep, pos, valC, valS, changed); break;
+  default:
+    k_scatter_direct<T, 4><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, keep, pos, valC, valS, changed);
+    break;
   }
 }
 
@@ -769,6 +852,8 @@ void buildPattern(pxg_ctx * c)
   c->valS32.resize(c->nnzS);
   c->valsZero = true;
   c->keepFresh = false;
+  c->valsSplit = false;
+  c->layoutValid = false;
   c->b.resize(c->ndofs);
   PXG_HIP(hipMemsetAsync(c->b.p, 0, c->ndofs * sizeof(double), s));
   c->closed = false;
@@ -830,7 +915,19 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
   }
   // values are logically zero after clear / pattern build: overwrite them
   bool const overwrite = c->valsZero;
+  // a rebuild after clear writes straight into the pruned layout of the last close
+  bool const direct = overwrite && c->layoutValid && c->layoutPrecision == precision;
+  if (!direct && !overwrite)
+    unsplitValues(c); // accumulating: structural values first
+  c->valsSplit = false;
   c->keep.resize(c->nnzS + 1);
+  if (direct)
+  {
+    c->changed.resize(1);
+    k_set_int<<<1, 1, 0, s>>>(c->changed.p, 0);
+  }
+  else
+    c->layoutValid = false; // the scatter below rewrites the keep flags
   dispatchFe(T.feType,
              [&](auto nC, auto dimC)
              {
@@ -848,8 +945,12 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                  }
                  PXG_HIP(hipGetLastError());
                  ProfScope ps2(c, "scatter");
-                 launchScatter<double>(overwrite, c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf.p, c->valS.p,
-                                       c->keep.p, s);
+                 if (direct)
+                   launchScatterDirect<double>(c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf.p, c->keep.p, c->pos.p,
+                                               c->val.p, c->valS.p, c->changed.p, s);
+                 else
+                   launchScatter<double>(overwrite, c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf.p, c->valS.p,
+                                         c->keep.p, s);
                }
                else
                {
@@ -864,8 +965,12 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                  }
                  PXG_HIP(hipGetLastError());
                  ProfScope ps2(c, "scatter32");
-                 launchScatter<float>(overwrite, c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf32.p, c->valS32.p,
-                                      c->keep.p, s);
+                 if (direct)
+                   launchScatterDirect<float>(c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf32.p, c->keep.p,
+                                              c->pos.p, c->val32.p, c->valS32.p, c->changed.p, s);
+                 else
+                   launchScatter<float>(overwrite, c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf32.p, c->valS32.p,
+                                        c->keep.p, s);
                }
              });
   PXG_HIP(hipGetLastError());
@@ -877,7 +982,11 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
   c->valPrecision = precision;
   c->valsDirty = true;
   c->valsZero = false;
-  c->keepFresh = true; // flags of the default prune match the values just written
+  // normal scatter: the flags of the default prune match the values just
+  // written; direct scatter: the values are split and the keep flags are the
+  // layout's
+  c->keepFresh = !direct;
+  c->valsSplit = direct;
   c->closed = false;
   ++c->stamp;
   checkStatus(c);
@@ -978,6 +1087,37 @@ void closeMatrix(pxg_ctx * c, double eps)
   int64_t const nr = c->ndofs, ns = c->nnzS;
   bool const f32 = c->valPrecision == PXG_FP32;
   ProfScope ps(c, "close");
+  if (c->valsSplit)
+  {
+    int ch = 1;
+    PXG_HIP(hipMemcpyAsync(&ch, c->changed.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    PXG_HIP(hipStreamSynchronize(s));
+    if (!ch && eps == kDefaultPrune && c->layoutValid && c->layoutPrecision == c->valPrecision)
+    {
+      // every prune flag matches the layout: rowptr and col stand, the kept
+      // values are in place; only the inverse diagonal is new
+      if (f32)
+      {
+        k_diag_inverse_split<float><<<gridFor(nr), kBlock, 0, s>>>(nr, c->diagPos.p, c->keep.p, c->pos.p,
+                                                                    c->val32.p, c->valS32.p, eps, c->dinv32.p);
+        k_cast<double, float><<<gridFor(nr), kBlock, 0, s>>>(nr, c->dinv32.p, c->dinv.p);
+      }
+      else
+      {
+        k_diag_inverse_split<double><<<gridFor(nr), kBlock, 0, s>>>(nr, c->diagPos.p, c->keep.p, c->pos.p,
+                                                                     c->val.p, c->valS.p, eps, c->dinv.p);
+        k_cast<float, double><<<gridFor(nr), kBlock, 0, s>>>(nr, c->dinv.p, c->dinv32.p);
+      }
+      PXG_HIP(hipGetLastError());
+      PXG_HIP(hipStreamSynchronize(s));
+      c->val32Valid = f32;
+      c->val64Valid = !f32;
+      c->closed = true;
+      ++c->stamp;
+      return;
+    }
+    unsplitValues(c);
+  }
   c->keep.resize(ns + 1);
   c->pos.resize(ns + 1);
   if (c->valsZero)
@@ -1034,6 +1174,22 @@ void closeMatrix(pxg_ctx * c, double eps)
   c->val64Valid = !f32;
   c->closed = true;
   ++c->stamp;
+  c->layoutValid = eps == kDefaultPrune;
+  c->layoutPrecision = c->valPrecision;
+}
+
+void unsplitValues(pxg_ctx * c)
+{
+  if (!c->valsSplit)
+    return;
+  int64_t const ns = c->nnzS;
+  if (c->valPrecision == PXG_FP32)
+    k_unsplit<float><<<gridFor(ns), kBlock, 0, c->stream>>>(ns, c->keep.p, c->pos.p, c->val32.p, c->valS32.p);
+  else
+    k_unsplit<double><<<gridFor(ns), kBlock, 0, c->stream>>>(ns, c->keep.p, c->pos.p, c->val.p, c->valS.p);
+  PXG_HIP(hipGetLastError());
+  c->valsSplit = false;
+  c->keepFresh = false;
 }
 
 void ensureVal32(pxg_ctx * c)
@@ -1060,6 +1216,7 @@ void clearValues(pxg_ctx * c)
   // structural values are overwritten by the next build (valsZero); only b is zeroed
   c->valsZero = true;
   c->keepFresh = false;
+  c->valsSplit = false;
   if (c->b.p)
     PXG_HIP(hipMemsetAsync(c->b.p, 0, c->ndofs * sizeof(double), s));
   PXG_HIP(hipStreamSynchronize(s));

@@ -243,6 +243,8 @@ int pxg_set_dofmap(pxg_ctx * c, int64_t ndofs, const uint32_t * emap)
                    c->hasKappa = false;
                    c->nnzS = 0;
                    c->closed = false;
+                   c->layoutValid = false;
+                   c->valsSplit = false;
                    ++c->stamp;
                    PXG_HIP(hipStreamSynchronize(c->stream));
                  });

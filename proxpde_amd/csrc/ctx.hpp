@@ -154,6 +154,14 @@ struct pxg_ctx
   bool keepFresh = false; // keep flags of prune(1e-16) match the structural values
   bool val32Valid = false; // FP32 copy of the closed values is current
   bool val64Valid = false; // FP64 copy of the closed values is current
+  // pruned layout (keep, pos, rowptr, col) of the last close with the default
+  // threshold: a rebuild after clear writes kept entries straight into it
+  bool layoutValid = false;
+  int layoutPrecision = PXG_FP64;
+  // structural values stored split: kept entries in val / val32 at pos[], the
+  // pruned ones in valS / valS32
+  bool valsSplit = false;
+  pxg::DevBuf<int> changed; // direct scatter: some prune flag differs from the layout
   uint64_t stamp = 0;      // counts closes: changes whenever the closed matrix may have changed
   pxg::DevBuf<double> valS;
   pxg::DevBuf<float> valS32;

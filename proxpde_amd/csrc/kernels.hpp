@@ -15,6 +15,8 @@ void assembleRhs(pxg_ctx * c, int kind, double const * params, double const * qv
 void qpoints(pxg_ctx * c, double * outHost);
 void geometryOut(pxg_ctx * c, double * detJ, double * JxW);
 void closeMatrix(pxg_ctx * c, double eps);
+// structural values back from the split storage of a direct rebuild
+void unsplitValues(pxg_ctx * c);
 void clearValues(pxg_ctx * c);
 // FP32 copy of closed FP64 values / FP64 copy of closed FP32 values, made on first use
 void ensureVal32(pxg_ctx * c);

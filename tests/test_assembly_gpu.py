@@ -141,6 +141,65 @@ def test_accumulating_builds_and_clear(oracle, gpu):
     assert _rel_ok(v, v_o)[0] and _rel_ok(b, b_o)[0]
 
 
+def _bitwise_equal(x, y):
+    return all(np.array_equal(np.asarray(p).view(np.uint8), np.asarray(q).view(np.uint8)) for p, q in zip(x, y))
+
+
+def test_rebuild_into_pruned_layout(oracle, gpu):
+    """A rebuild after clear writes into the pruned layout of the last close;
+    every outcome equals a fresh build bit for bit: same prune set (fast close),
+    changed prune set (all-zero stiffness, then back), accumulation onto a
+    direct rebuild, and FP32 rebuilds."""
+    prob, o, kappa, dofs, vals, bc = build_pair(oracle, "hex_q2", "cube_hex", (3, 3, 2))
+    kappa2 = kappa[::-1].copy() * 1.5
+
+    def fresh(steps, kap, precision=0):
+        f = prob.assembler()
+        f.set_dirichlet(dofs, vals)
+        f.set_coefficient(kap)
+        f.build_pattern()
+        for t in steps:
+            f.assemble(t, precision=precision)
+        f.assemble_rhs("const", (1.0,))
+        f.close()
+        return f.csr(fp32=precision == 1)
+
+    a = prob.assembler()
+    a.set_dirichlet(dofs, vals)
+    a.set_coefficient(kappa)
+    a.build_pattern()
+
+    def rebuild(steps, kap, precision=0):
+        a.clear()
+        a.set_coefficient(kap)
+        for t in steps:
+            a.assemble(t, precision=precision)
+        a.assemble_rhs("const", (1.0,))
+        a.close()
+        return a.csr(fp32=precision == 1)
+
+    stiff = [("stiffness_fe", 1.0)]
+    r1 = rebuild([stiff], kappa)            # first close: full path
+    assert _bitwise_equal(r1, fresh([stiff], kappa))
+    r2 = rebuild([stiff], kappa2)           # same prune set: fast close
+    assert _bitwise_equal(r2, fresh([stiff], kappa2))
+    rp_o, col_o, v_o, b_o = o.assemble(stiff, kappa=kappa2, bc=bc, rhs="const", rhs_params=(1.0,))
+    assert np.array_equal(r2[1], col_o.astype(np.int32)) and _rel_ok(r2[2], v_o)[0]
+    zero = [("stiffness_fe", 0.0)]
+    r3 = rebuild([zero], kappa2)            # prune set changes: rebuilt layout
+    assert _bitwise_equal(r3, fresh([zero], kappa2))
+    assert len(r3[1]) < len(r2[1])
+    r4 = rebuild([stiff], kappa2)           # and back
+    assert _bitwise_equal(r4, r2)
+    both = [stiff, [("mass", 1.0)]]
+    r5 = rebuild(both, kappa2)              # accumulate onto a direct rebuild
+    assert _bitwise_equal(r5, fresh(both, kappa2))
+    r6 = rebuild([stiff], kappa, precision=1)   # FP32: full path, then direct
+    r7 = rebuild([stiff], kappa2, precision=1)
+    assert _bitwise_equal(r6, fresh([stiff], kappa, precision=1))
+    assert _bitwise_equal(r7, fresh([stiff], kappa2, precision=1))
+
+
 KNOWN = [
     ("quad_q1", (10, 10), None, 2.049777877938e-02),
     ("quad_q1", (20, 20), None, 9.732187610621e-03),
