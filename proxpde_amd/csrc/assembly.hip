@@ -47,8 +47,6 @@ enum : int
   kHasMassFE = 8,
 };
 
-__host__ __device__ inline uint32_t packedPos(int n, int i, int j);
-
 // --------------------------------------------------------------------------
 // geometry: one thread per (element, quadrature point)
 // J = sum_v x_v (x) grad psi_v(xi_q); A = J^T J; detJ = sqrt(det A);
@@ -240,12 +238,11 @@ __global__ void __launch_bounds__(kBlock)
       f = 0. - s;
     }
   }
-  // symmetric: store the upper triangle, row i from its diagonal on
-  T * out = ke + e * (N * (N + 1) / 2) + packedPos(N, i, i) - i;
+  // row i of the element matrix, row-major
+  T * out = ke + (e * N + i) * N;
 #pragma unroll
   for (int j = 0; j < N; ++j)
-    if (j >= i)
-      out[j] = acc[j];
+    out[j] = acc[j];
   fe[t] = f;
 }
 
@@ -273,17 +270,9 @@ __global__ void k_head_flags(int64_t m, uint64_t const * __restrict__ keys, uint
   flags[t] = (t == 0 || keys[t] != keys[t - 1]) ? 1u : 0u;
 }
 
-// position of entry (i, j) of a symmetric n x n element matrix stored as its
-// upper triangle, row by row
-__host__ __device__ inline uint32_t packedPos(int n, int i, int j)
-{
-  int const a = i < j ? i : j, b = i < j ? j : i;
-  return (uint32_t)(a * n - (a * (a - 1)) / 2 + (b - a));
-}
-
-// after an inclusive scan of the head flags: nnz id of every sorted entry; the
-// contribution list is rewritten to point into the packed (upper-triangle)
-// element matrices
+// after an inclusive scan of the head flags: nnz id of every sorted entry;
+// the contribution list holds the element-matrix entry ids (e n^2 + i n + j)
+// sorted by (row, col, element)
 __global__ void k_pattern_fill(int64_t m, int64_t ndofs, int n, uint64_t const * __restrict__ keys,
                                uint32_t const * __restrict__ scan, uint32_t * __restrict__ sortedIds,
                                uint32_t * __restrict__ segPtr, int32_t * __restrict__ colS,
@@ -295,11 +284,6 @@ __global__ void k_pattern_fill(int64_t m, int64_t ndofs, int n, uint64_t const *
   uint32_t const id = scan[t] - 1u;
   uint32_t const orig = sortedIds[t];
   emapNnz[orig] = id;
-  {
-    uint32_t const nn = (uint32_t)(n * n), np = (uint32_t)(n * (n + 1) / 2);
-    uint32_t const e = orig / nn, r = orig - e * nn;
-    sortedIds[t] = e * np + packedPos(n, (int)(r / n), (int)(r % n));
-  }
   uint64_t const k = keys[t];
   if (t == 0 || k != keys[t - 1])
   {
@@ -380,14 +364,15 @@ __global__ void __launch_bounds__(kBlock)
 template <typename T, int kMaxC>
 __global__ void __launch_bounds__(kBlock)
     k_scatter_direct(int64_t nnzS, uint32_t const * __restrict__ segPtr, uint32_t const * __restrict__ contrib,
-                     T const * __restrict__ ke, uint32_t const * __restrict__ keep, uint32_t const * __restrict__ pos,
-                     T * __restrict__ valC, T * __restrict__ valS, int * __restrict__ changed)
+                     T const * __restrict__ ke, uint32_t const * __restrict__ pos, T * __restrict__ valC,
+                     T * __restrict__ valS, int * __restrict__ changed)
 {
   int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nnzS)
     return;
   uint32_t const k0 = segPtr[t], k1 = segPtr[t + 1];
-  uint32_t const kept = keep[t], o = pos[t];
+  // pos is the exclusive scan of the layout's keep flags (sentinel included)
+  uint32_t const o = pos[t], kept = pos[t + 1] - o;
   uint32_t id[kMaxC];
 #pragma unroll
   for (int j = 0; j < kMaxC; ++j)
@@ -464,18 +449,15 @@ void launchScatterU(int u, int64_t nnzS, uint32_t const * segPtr, uint32_t const
 
 template <typename T>
 void launchScatterDirect(int64_t nnzS, uint32_t const * segPtr, uint32_t const * contrib, T const * ke,
-                         uint32_t const * keep, uint32_t const * pos, T * valC, T * valS, int * changed,
-                         hipStream_t s)
+                         uint32_t const * pos, T * valC, T * valS, int * changed, hipStream_t s)
 {
   unsigned const g = gridFor(nnzS);
   switch (scatterUnroll())
   {
-  case 1: k_scatter_direct<T, 1><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, keep, pos, valC, valS, changed); break;
-  case 2: k_scatter_direct<T, 2><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, keep, pos, valC, valS, changed); break;
-  case 8: k_scatter_direct<T, 8><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, keep, pos, valC, valS, changed); break;
-  default:
-    k_scatter_direct<T, 4><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, keep, pos, valC, valS, changed);
-    break;
+  case 1: k_scatter_direct<T, 1><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, pos, valC, valS, changed); break;
+  case 2: k_scatter_direct<T, 2><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, pos, valC, valS, changed); break;
+  case 8: k_scatter_direct<T, 8><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, pos, valC, valS, changed); break;
+  default: k_scatter_direct<T, 4><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, pos, valC, valS, changed); break;
   }
 }
 
@@ -936,7 +918,7 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                {
                  if (!fused)
                  {
-                   c->keBuf.resize(E * (N * (N + 1) / 2));
+                   c->keBuf.resize(E * N * N);
                    size_t const sh = (size_t)T.nq * N * (D + 1) * sizeof(double);
                    ProfScope ps(c, "element");
                    k_element_rows<double, N, D><<<gridFor(E * N), kBlock, sh, s>>>(
@@ -946,8 +928,8 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                  PXG_HIP(hipGetLastError());
                  ProfScope ps2(c, "scatter");
                  if (direct)
-                   launchScatterDirect<double>(c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf.p, c->keep.p, c->pos.p,
-                                               c->val.p, c->valS.p, c->changed.p, s);
+                   launchScatterDirect<double>(c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf.p, c->pos.p, c->val.p,
+                                               c->valS.p, c->changed.p, s);
                  else
                    launchScatter<double>(overwrite, c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf.p, c->valS.p,
                                          c->keep.p, s);
@@ -956,7 +938,7 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                {
                  if (!fused)
                  {
-                   c->keBuf32.resize(E * (N * (N + 1) / 2));
+                   c->keBuf32.resize(E * N * N);
                    size_t const sh = (size_t)T.nq * N * (D + 1) * sizeof(float);
                    ProfScope ps(c, "element32");
                    k_element_rows<float, N, D><<<gridFor(E * N), kBlock, sh, s>>>(
@@ -966,8 +948,8 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                  PXG_HIP(hipGetLastError());
                  ProfScope ps2(c, "scatter32");
                  if (direct)
-                   launchScatterDirect<float>(c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf32.p, c->keep.p,
-                                              c->pos.p, c->val32.p, c->valS32.p, c->changed.p, s);
+                   launchScatterDirect<float>(c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf32.p, c->pos.p,
+                                              c->val32.p, c->valS32.p, c->changed.p, s);
                  else
                    launchScatter<float>(overwrite, c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf32.p, c->valS32.p,
                                         c->keep.p, s);

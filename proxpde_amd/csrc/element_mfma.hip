@@ -5,15 +5,17 @@
 //     Ke = A * C,   A[i][k] = g_i(q,a) | phi_i(q)                     (N x K, constant)
 //                   C[k][j] = sum_b M_q^{ab} g_j(q,b) | m_q phi_j(q)  (K x N, per element)
 // with k running over the (q, a) pairs, then over q for the mass rows.  A is
-// the same for every element, so its MFMA fragments stay in registers for the
-// whole kernel; each C operand value is formed in registers right before its
+// the same for every element, so its MFMA fragments are built once and kept
+// in LDS for the whole kernel; each C operand value is formed in registers right before its
 // matrix instruction from the per-point metric (LDS) and the gradient table
 // (LDS), so no K x N staging buffer is needed and more waves fit per CU.
 // The N x N product runs as v_mfma_f64_16x16x4f64 (FP64) or
 // v_mfma_f32_16x16x4f32 (FP32) tiles; only the upper tiles are multiplied
-// (Ke is symmetric) and the lower one is mirrored.  The epilogue applies the
-// per-element Dirichlet elimination of Builder::buildLhs (reference
-// include/builder.hpp:83-124) and writes Ke rows contiguously plus the lifting Fe.
+// (Ke is symmetric) and stored to both triangles of the row-major Ke, so the
+// scatter reads whole element rows.  The epilogue applies the per-element
+// Dirichlet elimination of Builder::buildLhs (reference
+// include/builder.hpp:83-124) on the accumulators and writes Ke plus the
+// lifting Fe.
 //
 // One wavefront per element; kWaves wavefronts per block share the constant
 // tables in LDS; blocks stride over the elements.
@@ -83,8 +85,8 @@ struct WaveLds
   double bcv[N];
   double M[NQ][6];
   double mq[NQ];
+  double f[N]; // lifting accumulator of the Dirichlet elimination
   int flag[N];
-  T Ke[N][N + 1];
 };
 
 template <typename T, int N, int DIM, int NPE, int NQ, bool MASS>
@@ -108,19 +110,7 @@ __global__ void __launch_bounds__(64 * kWaves)
   __shared__ double sGrad[NQ * N * DIM];
   __shared__ double sW[NQ];
   __shared__ T sA[NT * STEPS * 64]; // constant operand fragments
-  constexpr int NP = N * (N + 1) / 2;
-  __shared__ unsigned short sPair[NP]; // (row << 8 | col) of each packed upper-triangle slot
   __shared__ L lds[kWaves];
-  for (int t = threadIdx.x; t < NP; t += blockDim.x)
-  {
-    int i = 0, start = 0;
-    while (start + (N - i) <= t)
-    {
-      start += N - i;
-      ++i;
-    }
-    sPair[t] = (unsigned short)((i << 8) | (i + t - start));
-  }
 
   int const tid = threadIdx.x;
   for (int k = tid; k < NQ * NPE * DIM; k += blockDim.x)
@@ -187,6 +177,7 @@ __global__ void __launch_bounds__(64 * kWaves)
       W.kap[lane] = rk;
       W.flag[lane] = rf;
       W.bcv[lane] = rb;
+      W.f[lane] = 0.;
     }
     waveSync();
     uint32_t npid = 0, nid = 0;
@@ -195,18 +186,30 @@ __global__ void __launch_bounds__(64 * kWaves)
     if (lane < NQ)
     {
       int const q = lane;
+      // vertex by vertex, so only one vertex's coordinates and mapping
+      // gradients are live at a time (register pressure sets the occupancy)
       double J[3][DIM];
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int c = 0; c < DIM; ++c)
-        {
-          double s = 0.;
+          J[r][c] = 0.;
+#pragma unroll 1
+      for (int v = 0; v < NPE; ++v)
+      {
+        double gv[DIM];
 #pragma unroll
-          for (int v = 0; v < NPE; ++v)
-            s += W.x[3 * v + r] * sGeo[(q * NPE + v) * DIM + c];
-          J[r][c] = s;
+        for (int c = 0; c < DIM; ++c)
+          gv[c] = sGeo[(q * NPE + v) * DIM + c];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+        {
+          double const xr = W.x[3 * v + r];
+#pragma unroll
+          for (int c = 0; c < DIM; ++c)
+            J[r][c] += xr * gv[c];
         }
+      }
       double A[DIM][DIM];
 #pragma unroll
       for (int i = 0; i < DIM; ++i)
@@ -241,6 +244,7 @@ __global__ void __launch_bounds__(64 * kWaves)
       double const JxW = sqrt(det) * sW[q];
       double kq = 0.;
       if (needK)
+#pragma unroll 3
         for (int i = 0; i < N; ++i)
           kq += sPhi[q * N + i] * W.kap[i];
       double s = 0., m = 0.;
@@ -271,7 +275,9 @@ __global__ void __launch_bounds__(64 * kWaves)
 #pragma unroll
       for (int tj = 0; tj < NT; ++tj)
         acc[ti][tj] = Acc{0, 0, 0, 0};
-#pragma unroll 3
+    // not unrolled: it keeps the kernel at or below 128 VGPRs, i.e. two blocks
+    // (16 waves) per CU; the other waves hide the LDS and MFMA latencies
+#pragma unroll 1
     for (int s = 0; s < STEPS; ++s)
     {
       int const k = 4 * s + kr;
@@ -310,60 +316,72 @@ __global__ void __launch_bounds__(64 * kWaves)
           acc[ti][tj] = Mfma<T>::mma(sA[(ti * STEPS + s) * 64 + lane], b, acc[ti][tj]);
       }
     }
-    // tiles -> Ke in LDS: D[i][j], i = 16 ti + row(r, lane/16), j = 16 tj + lane%16
+    // ---- 4. the upper triangle goes from the accumulators straight to the
+    //         row-major store, mirrored into the lower one: tile (ti, tj),
+    //         value r of a lane is D[i][j], i = 16 ti + row(r, lane/16),
+    //         j = 16 tj + lane%16 (16 consecutive j per row are contiguous).
+    //         Dirichlet elimination: rows/columns of constrained dofs become
+    //         unit rows; the lifting f = -Ke bcv uses the unmodified Ke, its
+    //         lower part through symmetry.
+    bool const anyBC = __any(lane < N && W.flag[lane] != 0);
+    T * out = ke + e * (N * N);
+    double colPart[NT];
 #pragma unroll
     for (int tj = 0; tj < NT; ++tj)
+    {
+      colPart[tj] = 0.;
+      int const j = 16 * tj + jc;
+      int const jj = j < N ? j : N - 1;
+      int const fj = W.flag[jj];
+      double const bj = W.bcv[jj];
 #pragma unroll
       for (int ti = 0; ti <= tj; ++ti)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
         {
-          int const i = 16 * ti + Mfma<T>::row(r, kr), j = 16 * tj + jc;
-          if (i < N && j < N)
+          int const i = 16 * ti + Mfma<T>::row(r, kr);
+          bool const valid = i < N && j < N && j >= i;
+          T v = acc[ti][tj][r];
+          if (anyBC)
           {
-            T const v = acc[ti][tj][r];
-            W.Ke[i][j] = v;
-            if (ti != tj)
-              W.Ke[j][i] = v;
+            int const ii = i < N ? i : N - 1;
+            double const dv = valid ? (double)v : 0.;
+            // row i collects Ke_ij bcv_j (j >= i), row j collects Ke_ij bcv_i (j > i)
+            double rp = dv * bj;
+#pragma unroll
+            for (int off = 8; off > 0; off >>= 1)
+              rp += __shfl_xor(rp, off);
+            if (jc == 0 && i < N)
+              W.f[i] += rp;
+            if (j > i)
+              colPart[tj] += dv * W.bcv[ii];
+            if (W.flag[ii] | fj)
+              v = (i == j) ? T(1) : T(0);
+          }
+          if (valid)
+          {
+            out[i * N + j] = v;
+            if (j > i)
+              out[j * N + i] = v; // the lower triangle repeats the upper one exactly
           }
         }
-    waveSync();
-    // ---- 4. Dirichlet elimination, lifting, coalesced store
-    // wave vote: does the element touch a constrained dof (rf already holds the next element)
-    bool const anyBC = __any(lane < N && W.flag[lane] != 0);
-    if (lane < N)
-    {
-      int const i = lane;
-      double f = 0.;
-      if (anyBC)
-      {
-        if (W.flag[i])
-          f = W.bcv[i];
-        else
-        {
-          double s = 0.;
-#pragma unroll
-          for (int j = 0; j < N; ++j)
-            s += (double)W.Ke[i][j] * W.bcv[j]; // bcv is zero on free dofs
-          f = 0. - s;
-        }
-      }
-      fe[e * N + i] = f;
     }
-    // symmetric: the upper triangle is stored, row by row
-    T * out = ke + e * NP;
     if (anyBC)
-      for (int t = lane; t < NP; t += 64)
+    {
+#pragma unroll
+      for (int tj = 0; tj < NT; ++tj)
       {
-        int const i = sPair[t] >> 8, j = sPair[t] & 255;
-        T v = W.Ke[i][j];
-        if (W.flag[i] | W.flag[j])
-          v = (i == j) ? T(1) : T(0);
-        out[t] = v;
+        double cp = colPart[tj];
+        cp += __shfl_xor(cp, 16);
+        cp += __shfl_xor(cp, 32);
+        int const j = 16 * tj + jc;
+        if (kr == 0 && j < N)
+          W.f[j] += cp;
       }
-    else
-      for (int t = lane; t < NP; t += 64)
-        out[t] = W.Ke[sPair[t] >> 8][sPair[t] & 255];
+      waveSync();
+    }
+    if (lane < N)
+      fe[e * N + lane] = anyBC ? (W.flag[lane] ? W.bcv[lane] : 0. - W.f[lane]) : 0.;
     waveSync(); // LDS of this element is reused by the next one
   }
 }
@@ -424,8 +442,13 @@ void launchMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, double
   int dev = 0, ncu = 0;
   PXG_HIP(hipGetDevice(&dev));
   PXG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  // one resident wave of blocks, each striding over the elements
+  int perCu = 0;
+  PXG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_element_mfma<T, N, DIM, NPE, NQ, MASS>,
+                                                       64 * kWaves, 0));
   int64_t const blocksNeeded = (c->nelem + kWaves - 1) / kWaves;
-  int const grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocksNeeded, (int64_t)ncu * 4));
+  int const grid =
+      (int)std::max<int64_t>(1, std::min<int64_t>(blocksNeeded, (int64_t)ncu * std::max(perCu, 1)));
   k_element_mfma<T, N, DIM, NPE, NQ, MASS><<<grid, 64 * kWaves, 0, c->stream>>>(
       c->nelem, c->xyz.p, c->conn.p, c->elemMap.p, c->kappa.p, c->bcFlag.p, c->bcVal.p, c->hasBC ? 1 : 0,
       tab.d_geoGrad.p, tab.d_w.p, tab.d_phi.p, tab.d_grad.p, dfrag, cS, cSF, cM, cMF, flags, ke, c->feBuf.p,
@@ -453,7 +476,7 @@ bool elementMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, doubl
   {
     using TT = decltype(tag);
     TT * ke = nullptr;
-    size_t const np = (size_t)T.n * (T.n + 1) / 2; // packed symmetric element matrix
+    size_t const np = (size_t)T.n * T.n; // full row-major element matrix
     if constexpr (sizeof(TT) == 8)
     {
       c->keBuf.resize((size_t)c->nelem * np);
