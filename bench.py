@@ -40,13 +40,6 @@ def q2hex_flops_per_elem(nq=27, n=27, dim=3, g=27, kappa=True):
     return nq * per_q
 
 
-def element_kernel_flops(nq=27, n=27, dim=3):
-    """Flops of the element-matrix contraction alone (what the element kernel
-    computes from the per-point metric): per q, u = M g_i (2*dim^2 per row) and
-    n dot products of length dim per row, accumulated (2*dim flops each)."""
-    return nq * n * (2 * dim * dim + 2 * dim * n)
-
-
 def solve_and_adjoint(args, dev, N, W):
     """Q2 hex n^3: u = K(kappa)^{-1} f (MG-PCG to rel. residual 1e-8, levels
     n, n/2, ...), J = 1/2 |u - u_t|^2 and dJ/dkappa by the adjoint solve.
@@ -241,26 +234,38 @@ def main():
         copy_cal = {"bytes_each_way": nel * 8, "ms": cms, "GB/s_read_plus_write": 2 * nel * 8 / (cms * 1e-3) / 1e9}
         del src, dst
 
-    # --- roofline of the dominant kernel
+    # --- rooflines: the dominant kernel of the build step (the JSON "roofline"),
+    # and the scatter and SpMV beside it.  achieved = algorithmic work per launch
+    # / the launch's average duration (HIP events on the launch stream);
+    # traffic = HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter
+    # runs recorded in profiles/ (calibrated against a plain device copy).
     per = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in prof.items()}
-    dom = max(per, key=per.get)
-    if dom == "element":
-        fl = element_kernel_flops() * E
-        achieved = fl / (per["element"] * 1e-3) / 1e12
-        roof = {"kernel": "k_element_rows<double,27,3>", "bound": "mfma", "achieved": achieved,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                "traffic": None}
-    elif dom == "scatter":
-        # contribution ids (4 B per element-matrix entry), the packed upper
-        # triangle of each element matrix (27*28/2 FP64), segment offsets (4 B),
-        # the summed value (8 B) and the keep flag (4 B) per structural entry
-        by = E * 27 * 27 * 4 + E * (27 * 28 // 2) * 8 + (nnz_s + 1) * 4 + nnz_s * (8 + 4)
-        achieved = by / (per["scatter"] * 1e-3) / 1e9
-        roof = {"kernel": "k_scatter<double>", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
-    else:
-        roof = {"kernel": dom, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": None, "traffic": None}
+    traffic = {}
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_q2hex64.json")
+    if n == 64 and os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("kernels", {})
+
+    def tr(key):
+        return traffic[key]["bytes"] if key in traffic else None
+
+    # element kernel: reference-formulation flops (q2hex_flops_per_elem) per element
+    el_fl = q2hex_flops_per_elem() * E
+    el_ach = el_fl / (per["element"] * 1e-3) / 1e12 if per["element"] else None
+    rf_element = {"kernel": "k_element_mfma<double,27,3,8,27,false>", "bound": "mfma", "achieved": el_ach,
+                  "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                  "frac": el_ach / FP64_PEAK_TFLOPS if el_ach else None, "traffic": tr("element")}
+    # scatter into the pruned layout: per structural entry the segment offset,
+    # the layout position (4 B each) and the value written (8 B); per
+    # element-matrix entry its id (4 B) and value (8 B)
+    sc_by = (nnz_s + 1) * 4 * 2 + nnz * 8 + E * 27 * 27 * (4 + 8)
+    sc_ach = sc_by / (per["scatter"] * 1e-3) / 1e9 if per["scatter"] else None
+    rf_scatter = {"kernel": "k_scatter_direct<double,4>", "bound": "hbm", "achieved": sc_ach, "peak": HBM_PEAK_GBS,
+                  "unit": "GB/s", "frac": sc_ach / HBM_PEAK_GBS if sc_ach else None, "traffic": tr("scatter")}
+    rf_spmv = {"kernel": "k_csr_stream<double,0>", "bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS,
+               "unit": "GB/s", "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": tr("spmv_fp64")}
+    roof = rf_element if per["element"] >= per["scatter"] else rf_scatter
+    rooflines = {"element": rf_element, "scatter": rf_scatter, "spmv_fp64": rf_spmv}
 
     # --- solve + adjoint gradient, plain FP64 vs FP32 + FP64 refinement
     solve = None
@@ -320,6 +325,7 @@ def main():
         "reference_gflops_per_s": q2hex_flops_per_elem() * value / 1e9,
         "solve_adjoint": solve,
         "roofline": roof,
+        "rooflines": rooflines,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

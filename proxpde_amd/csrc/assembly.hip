@@ -408,18 +408,30 @@ __global__ void __launch_bounds__(kBlock)
     valS[t] = valC[pos[t]];
 }
 
+// inverse diagonal from split values: the compact position of every kept
+// diagonal is recorded with the layout (diagC), a pruned one is read from valS
 template <typename T>
-__global__ void k_diag_inverse_split(int64_t nrows, uint32_t const * __restrict__ diagPos,
-                                     uint32_t const * __restrict__ keep, uint32_t const * __restrict__ pos,
-                                     T const * __restrict__ valC, T const * __restrict__ valS, double eps,
-                                     T * __restrict__ dinv)
+__global__ void k_diag_inverse_split(int64_t nrows, uint32_t const * __restrict__ diagC,
+                                     uint32_t const * __restrict__ diagPos, T const * __restrict__ valC,
+                                     T const * __restrict__ valS, double eps, T * __restrict__ dinv)
+{
+  int64_t const r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows)
+    return;
+  uint32_t const dc = diagC[r];
+  T const d = dc != ~0u ? valC[dc] : valS[diagPos[r]];
+  dinv[r] = (fabs((double)d) <= eps || d == T(0)) ? T(0) : T(1) / d;
+}
+
+__global__ void k_diag_compact(int64_t nrows, uint32_t const * __restrict__ diagPos,
+                               uint32_t const * __restrict__ keep, uint32_t const * __restrict__ pos,
+                               uint32_t * __restrict__ diagC)
 {
   int64_t const r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows)
     return;
   uint32_t const dp = diagPos[r];
-  T const d = keep[dp] ? valC[pos[dp]] : valS[dp];
-  dinv[r] = (fabs((double)d) <= eps || d == T(0)) ? T(0) : T(1) / d;
+  diagC[r] = keep[dp] ? pos[dp] : ~0u;
 }
 
 __global__ void k_set_int(int * p, int v) { *p = v; }
@@ -1069,37 +1081,37 @@ void closeMatrix(pxg_ctx * c, double eps)
   int64_t const nr = c->ndofs, ns = c->nnzS;
   bool const f32 = c->valPrecision == PXG_FP32;
   ProfScope ps(c, "close");
-  if (c->valsSplit)
+  if (c->valsSplit && eps == kDefaultPrune && c->layoutValid && c->layoutPrecision == c->valPrecision)
   {
+    // if every prune flag matches the layout, rowptr and col stand and the
+    // kept values are in place: only the inverse diagonal is new.  It is
+    // computed before the flag is read back, so one synchronisation suffices.
+    if (f32)
+    {
+      k_diag_inverse_split<float><<<gridFor(nr), kBlock, 0, s>>>(nr, c->diagC.p, c->diagPos.p, c->val32.p,
+                                                                  c->valS32.p, eps, c->dinv32.p);
+      k_cast<double, float><<<gridFor(nr), kBlock, 0, s>>>(nr, c->dinv32.p, c->dinv.p);
+    }
+    else
+    {
+      k_diag_inverse_split<double><<<gridFor(nr), kBlock, 0, s>>>(nr, c->diagC.p, c->diagPos.p, c->val.p,
+                                                                   c->valS.p, eps, c->dinv.p);
+      k_cast<float, double><<<gridFor(nr), kBlock, 0, s>>>(nr, c->dinv.p, c->dinv32.p);
+    }
+    PXG_HIP(hipGetLastError());
     int ch = 1;
     PXG_HIP(hipMemcpyAsync(&ch, c->changed.p, sizeof(int), hipMemcpyDeviceToHost, s));
     PXG_HIP(hipStreamSynchronize(s));
-    if (!ch && eps == kDefaultPrune && c->layoutValid && c->layoutPrecision == c->valPrecision)
+    if (!ch)
     {
-      // every prune flag matches the layout: rowptr and col stand, the kept
-      // values are in place; only the inverse diagonal is new
-      if (f32)
-      {
-        k_diag_inverse_split<float><<<gridFor(nr), kBlock, 0, s>>>(nr, c->diagPos.p, c->keep.p, c->pos.p,
-                                                                    c->val32.p, c->valS32.p, eps, c->dinv32.p);
-        k_cast<double, float><<<gridFor(nr), kBlock, 0, s>>>(nr, c->dinv32.p, c->dinv.p);
-      }
-      else
-      {
-        k_diag_inverse_split<double><<<gridFor(nr), kBlock, 0, s>>>(nr, c->diagPos.p, c->keep.p, c->pos.p,
-                                                                     c->val.p, c->valS.p, eps, c->dinv.p);
-        k_cast<float, double><<<gridFor(nr), kBlock, 0, s>>>(nr, c->dinv.p, c->dinv32.p);
-      }
-      PXG_HIP(hipGetLastError());
-      PXG_HIP(hipStreamSynchronize(s));
       c->val32Valid = f32;
       c->val64Valid = !f32;
       c->closed = true;
       ++c->stamp;
       return;
     }
-    unsplitValues(c);
   }
+  unsplitValues(c); // the prune set changed (or another threshold): structural values back
   c->keep.resize(ns + 1);
   c->pos.resize(ns + 1);
   if (c->valsZero)
@@ -1158,6 +1170,12 @@ void closeMatrix(pxg_ctx * c, double eps)
   ++c->stamp;
   c->layoutValid = eps == kDefaultPrune;
   c->layoutPrecision = c->valPrecision;
+  if (c->layoutValid)
+  {
+    c->diagC.resize(nr);
+    k_diag_compact<<<gridFor(nr), kBlock, 0, s>>>(nr, c->diagPos.p, c->keep.p, c->pos.p, c->diagC.p);
+    PXG_HIP(hipGetLastError());
+  }
 }
 
 void unsplitValues(pxg_ctx * c)

@@ -143,6 +143,7 @@ struct pxg_ctx
   pxg::DevBuf<uint32_t> contrib; // E*n*n contribution ids sorted by (row, col, element)
   pxg::DevBuf<uint32_t> emapNnz; // E*n*n -> structural nnz id
   pxg::DevBuf<uint32_t> diagPos; // structural position of each diagonal entry
+  pxg::DevBuf<uint32_t> diagC;   // compact position of each diagonal in the pruned layout (~0u: pruned)
   pxg::DevBuf<uint32_t> keep, pos; // close(): keep flags and their exclusive scan
   pxg::DevBuf<unsigned char> scanTmp;
   pxg::DevBuf<uint32_t> dofSeg;  // ndofs + 1 offsets into dofContrib
