@@ -40,6 +40,47 @@ def q2hex_flops_per_elem(nq=27, n=27, dim=3, g=27, kappa=True):
     return nq * per_q
 
 
+def kappa_apply_batch(args, dev, a, kappa_host, nb=8):
+    """y = K(kappa) x for a batch of nb vectors (x ~ N(0,1), seed 3, layout
+    [ndofs][nb]) through the autograd operator, forward and backward (dx = K dy,
+    dkappa by the element contraction), on the assembler's mesh."""
+    import torch
+
+    from proxpde_amd import autograd as A
+
+    op = A.KappaOperator(a)
+    g = torch.Generator(device=dev)
+    g.manual_seed(3)
+    kap = torch.from_numpy(kappa_host).to(dev)
+    x = torch.randn(a.ndofs, nb, dtype=torch.float64, device=dev, generator=g)
+    w = torch.randn(a.ndofs, nb, dtype=torch.float64, device=dev, generator=g)
+    k = kap.clone().requires_grad_()
+    xr = x.clone().requires_grad_()
+    for _ in range(2):
+        y = op(k, xr)
+        (y * w).sum().backward()
+    reps = 5
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y = op(kap, x)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        k.grad = None
+        xr.grad = None
+        y = op(k, xr)
+        (y * w).sum().backward()
+    torch.cuda.synchronize()
+    t_fb = (time.perf_counter() - t0) / reps
+    E = a.nelem
+    # SURVEY-style algorithmic work of the forward apply: Q [1140 geometry + 2n kappa + nb (12 n + 5)]
+    fl = E * 27 * (1140 + 2 * 27 + nb * (12 * 27 + 5))
+    return {"batch": nb, "forward_ms": 1e3 * t_fwd, "forward_backward_ms": 1e3 * t_fb,
+            "forward_gflops": fl / t_fwd / 1e9}
+
+
 def solve_and_adjoint(args, dev, N, W):
     """Q2 hex n^3: u = K(kappa)^{-1} f (MG-PCG to rel. residual 1e-8, levels
     n, n/2, ...), J = 1/2 |u - u_t|^2 and dJ/dkappa by the adjoint solve.
@@ -268,6 +309,9 @@ def main():
     rooflines = {"element": rf_element, "scatter": rf_scatter, "spmv_fp64": rf_spmv}
 
     # --- solve + adjoint gradient, plain FP64 vs FP32 + FP64 refinement
+    progress("matrix-free K(kappa) x, batch 8")
+    mf = kappa_apply_batch(args, dev, a, np.asarray(kappa))
+
     solve = None
     progress("solve + adjoint")
     if not args.no_solve:
@@ -323,6 +367,7 @@ def main():
         "device_copy": copy_cal,
         "reference_flops_per_elem": q2hex_flops_per_elem(),
         "reference_gflops_per_s": q2hex_flops_per_elem() * value / 1e9,
+        "kappa_apply": mf,
         "solve_adjoint": solve,
         "roofline": roof,
         "rooflines": rooflines,

@@ -185,6 +185,7 @@ struct pxg_ctx
   pxg::DevBuf<float> keBuf32; // E*n*n (fp32)
   pxg::DevBuf<double> feBuf;  // E*n
   pxg::DevBuf<double> qwork;  // E*nq scratch
+  pxg::DevBuf<double> mfBuf;  // matrix-free element results [E][n][nb]
   pxg::DevBuf<int> status;    // device error flag
   pxg::DevBuf<unsigned char> sortTmp;
   // kernel timing with HIP events on the context stream (off by default)

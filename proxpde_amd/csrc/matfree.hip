@@ -19,218 +19,267 @@ namespace pxg
 
 namespace
 {
-constexpr int kMfBlock = 128;
 constexpr int kMaxNb = 16;
+constexpr int kMfWaves = 8; // wavefronts per block (they share the tables)
+constexpr int kMaxQ = 64;
 
-template <int DIM, int NPE>
-__device__ inline double geomAt(int64_t e, int q, double const * __restrict__ xyz,
-                                uint32_t const * __restrict__ conn, double const * __restrict__ geoGrad,
-                                double (&I)[3][3])
+__device__ inline void waveSync()
 {
-  double J[3][DIM];
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int c = 0; c < DIM; ++c)
-      J[r][c] = 0.;
-#pragma unroll
-  for (int v = 0; v < NPE; ++v)
-  {
-    uint32_t const pid = conn[e * NPE + v];
-    double const x0 = xyz[3ull * pid], x1 = xyz[3ull * pid + 1], x2 = xyz[3ull * pid + 2];
-#pragma unroll
-    for (int c = 0; c < DIM; ++c)
-    {
-      double const g = geoGrad[(q * NPE + v) * DIM + c];
-      J[0][c] += x0 * g;
-      J[1][c] += x1 * g;
-      J[2][c] += x2 * g;
-    }
-  }
-  double A[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
-#pragma unroll
-  for (int a = 0; a < DIM; ++a)
-#pragma unroll
-    for (int b = 0; b < DIM; ++b)
-      A[a][b] = J[0][a] * J[0][b] + J[1][a] * J[1][b] + J[2][a] * J[2][b];
-  double det;
-  if constexpr (DIM == 1)
-  {
-    det = A[0][0];
-    I[0][0] = 1. / det;
-  }
-  else if constexpr (DIM == 2)
-  {
-    det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
-    I[0][0] = A[1][1] / det;
-    I[0][1] = I[1][0] = -A[0][1] / det;
-    I[1][1] = A[0][0] / det;
-  }
-  else
-  {
-    double const c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
-    double const c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
-    double const c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
-    det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
-    I[0][0] = c00 / det;
-    I[0][1] = I[1][0] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
-    I[0][2] = I[2][0] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
-    I[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
-    I[1][2] = I[2][1] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;
-    I[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
-  }
-  return det;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// phase 1: per (element, q) the flux w_bq = kappa_q JxW_q A^{-1} v_bq into LDS;
-// phase 2: per (element, i) y_{b,i} = sum_q g_i(q) . w_bq into the element buffer
-template <int DIM, int NPE>
-__global__ void __launch_bounds__(kMfBlock)
-    k_mf_apply(int64_t E, int n, int nq, int nb, int epb, double const * __restrict__ xyz,
-               uint32_t const * __restrict__ conn, double const * __restrict__ geoGrad,
-               double const * __restrict__ w, double const * __restrict__ phi, double const * __restrict__ grad,
-               uint32_t const * __restrict__ emap, double const * __restrict__ kappa,
-               uint8_t const * __restrict__ cons, double const * __restrict__ x, double * __restrict__ ybuf,
-               int * __restrict__ status)
+// LDS of one element being processed by a group of SUB lanes
+// (NBC vectors of the batch are staged at a time)
+template <int DIM, int NPE, int N, int SUB, int NBC, bool GRAD>
+struct ElemLds
 {
-  extern __shared__ double sm[]; // epb * nq * nb * DIM
-  int64_t const e0 = (int64_t)blockIdx.x * epb;
+  double xyz[NPE * 3];
+  double kap[N];
+  int flag[N];
+  double X[N][NBC];
+  double Y[GRAD ? N : 1][NBC];
+  double Wf[GRAD ? 1 : SUB][NBC][DIM];
+  double C[GRAD ? SUB : 1];
+};
+
+// One group of SUB lanes per element (64 / SUB elements per wavefront), the
+// constant tables of the block in LDS.  Per element:
+//   gather   lane l < N: dof id, kappa, constraint flag; lane l < 3 NPE: one
+//            vertex coordinate; then per chunk of the batch the x (and ybar)
+//            values, zero on constrained dofs
+//   point    lane q < nq: J, A = J^T J, A^{-1}, JxW, kappa_q; reference
+//            gradients v_b = sum_j g_j(q) x_{b,j} read from LDS;
+//            apply: w_bq = kappa_q JxW_q A^{-1} v_bq -> LDS
+//            grad : c_q += sum_b v_bq(x)^T A^{-1} v_bq(ybar)
+//   row      lane i < N: apply: y_{b,i} = sum_q g_i(q) . w_bq (constrained rows:
+//            x_{b,i}, the unit diagonal of each element) -> element buffer;
+//            grad: g_i = sum_q phi_i(q) JxW_q c_q -> element buffer
+// All sums run in a fixed order; the element buffer is reduced per dof in
+// element order by k_gather, so the result is bitwise repeatable.
+template <int DIM, int NPE, int N, int SUB, int NBC, bool GRAD>
+__global__ void __launch_bounds__(64 * kMfWaves)
+    k_mf_elem(int64_t E, int nq, int nb, double const * __restrict__ xyz, uint32_t const * __restrict__ conn,
+              double const * __restrict__ geoGradG, double const * __restrict__ wG, double const * __restrict__ phiG,
+              double const * __restrict__ gradG, uint32_t const * __restrict__ emap,
+              double const * __restrict__ kappa, uint8_t const * __restrict__ cons, double const * __restrict__ x,
+              double const * __restrict__ ybar, double * __restrict__ out, int * __restrict__ status)
+{
+  constexpr int EPW = 64 / SUB;
+  using L = ElemLds<DIM, NPE, N, SUB, NBC, GRAD>;
+  // tables for nq <= SUB quadrature points
+  __shared__ double sGrad[SUB * N * DIM];
+  __shared__ double sPhi[SUB * N];
+  __shared__ double sGeo[SUB * NPE * DIM];
+  __shared__ double sW[SUB];
+  __shared__ L lds[kMfWaves * EPW];
   int const tid = threadIdx.x;
-  if (tid < epb * nq)
+  for (int k = tid; k < nq * N * DIM; k += blockDim.x)
+    sGrad[k] = gradG[k];
+  for (int k = tid; k < nq * N; k += blockDim.x)
+    sPhi[k] = phiG[k];
+  for (int k = tid; k < nq * NPE * DIM; k += blockDim.x)
+    sGeo[k] = geoGradG[k];
+  for (int k = tid; k < nq; k += blockDim.x)
+    sW[k] = wG[k];
+  __syncthreads();
+
+  int const wave = tid >> 6, lane = tid & 63;
+  int const sub = lane / SUB, l = lane - sub * SUB;
+  L & M = lds[wave * EPW + sub];
+  int64_t const stride = (int64_t)gridDim.x * kMfWaves * EPW;
+  // the wave loops uniformly; each group checks its own element
+  for (int64_t base = ((int64_t)blockIdx.x * kMfWaves + wave) * EPW; base < E; base += stride)
   {
-    int const le = tid / nq, q = tid - le * nq;
-    int64_t const e = e0 + le;
-    if (e < E)
+    int64_t const e = base + sub;
+    bool const live = e < E;
+    uint32_t id = 0;
+    if (live)
     {
-      double I[3][3];
-      double const det = geomAt<DIM, NPE>(e, q, xyz, conn, geoGrad, I);
+      if (l < NPE * 3)
+        M.xyz[l] = xyz[3ull * conn[e * NPE + l / 3] + l % 3];
+      if (l < N)
+      {
+        id = emap[e * N + l];
+        M.kap[l] = GRAD ? 0. : kappa[id];
+        M.flag[l] = cons ? (int)cons[id] : 0;
+      }
+    }
+    waveSync();
+    // geometry of point q = l, kept in registers across the batch
+    double I00 = 0., I01 = 0., I02 = 0., I11 = 0., I12 = 0., I22 = 0., sq = 0.;
+    if (live && l < nq)
+    {
+      int const q = l;
+      double J[3][DIM];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c)
+          J[r][c] = 0.;
+#pragma unroll
+      for (int v = 0; v < NPE; ++v)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c)
+        {
+          double const g = sGeo[(q * NPE + v) * DIM + c];
+          J[0][c] += M.xyz[3 * v] * g;
+          J[1][c] += M.xyz[3 * v + 1] * g;
+          J[2][c] += M.xyz[3 * v + 2] * g;
+        }
+      double A[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+#pragma unroll
+      for (int a = 0; a < DIM; ++a)
+#pragma unroll
+        for (int b = 0; b < DIM; ++b)
+          A[a][b] = J[0][a] * J[0][b] + J[1][a] * J[1][b] + J[2][a] * J[2][b];
+      double det;
+      if constexpr (DIM == 1)
+      {
+        det = A[0][0];
+        I00 = 1. / det;
+      }
+      else if constexpr (DIM == 2)
+      {
+        det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+        I00 = A[1][1] / det;
+        I01 = -A[0][1] / det;
+        I11 = A[0][0] / det;
+      }
+      else
+      {
+        double const c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+        double const c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+        double const c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+        det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+        I00 = c00 / det;
+        I01 = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+        I02 = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+        I11 = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+        I12 = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;
+        I22 = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
+      }
       if (!(det > 0.))
         atomicOr(status, 1);
-      double kq = 0.;
-      for (int i = 0; i < n; ++i)
-        kq += phi[q * n + i] * kappa[emap[e * n + i]];
-      double const s = kq * (sqrt(det) * w[q]);
-      for (int b = 0; b < nb; ++b)
+      double const jxw = sqrt(det) * sW[q];
+      if constexpr (GRAD)
+        sq = jxw;
+      else
       {
-        double v[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d)
-          v[d] = 0.;
-        for (int j = 0; j < n; ++j)
-        {
-          uint32_t const id = emap[e * n + j];
-          double const xv = (cons && cons[id]) ? 0. : x[(size_t)id * nb + b];
-#pragma unroll
-          for (int d = 0; d < DIM; ++d)
-            v[d] += grad[(q * n + j) * DIM + d] * xv;
-        }
-        double * o = sm + ((size_t)(le * nq + q) * nb + b) * DIM;
-#pragma unroll
-        for (int a = 0; a < DIM; ++a)
-        {
-          double t = 0.;
-#pragma unroll
-          for (int d = 0; d < DIM; ++d)
-            t += I[a][d] * v[d];
-          o[a] = s * t;
-        }
+        double kq = 0.;
+        for (int i = 0; i < N; ++i)
+          kq += sPhi[q * N + i] * M.kap[i];
+        sq = kq * jxw;
       }
     }
-  }
-  __syncthreads();
-  if (tid < epb * n)
-  {
-    int const le = tid / n, i = tid - le * n;
-    int64_t const e = e0 + le;
-    if (e < E)
+    double cq = 0.; // gradient: sum over the batch at point q
+    for (int b0 = 0; b0 < nb; b0 += NBC)
     {
-      uint32_t const id = emap[e * n + i];
-      bool const ci = cons && cons[id];
-      for (int b = 0; b < nb; ++b)
+      if (live && l < N)
       {
-        double y = 0.;
-        if (ci)
-          y = x[(size_t)id * nb + b]; // unit diagonal per element on constrained rows
-        else
-          for (int q = 0; q < nq; ++q)
+        bool const fl = M.flag[l] != 0;
+#pragma unroll
+        for (int bb = 0; bb < NBC; ++bb)
+        {
+          int const b = b0 + bb;
+          M.X[l][bb] = (b < nb && !fl) ? x[(size_t)id * nb + b] : 0.;
+          if constexpr (GRAD)
+            M.Y[l][bb] = (b < nb && !fl) ? ybar[(size_t)id * nb + b] : 0.;
+        }
+      }
+      waveSync();
+      if (live && l < nq)
+      {
+        int const q = l;
+#pragma unroll 1
+        for (int bb = 0; bb < NBC; ++bb)
+        {
+          double v[3] = {0., 0., 0.}, u[3] = {0., 0., 0.};
+#pragma unroll 3
+          for (int j = 0; j < N; ++j)
           {
-            double const * f = sm + ((size_t)(le * nq + q) * nb + b) * DIM;
+            double const xv = M.X[j][bb];
+            double yv = 0.;
+            if constexpr (GRAD)
+              yv = M.Y[j][bb];
 #pragma unroll
             for (int d = 0; d < DIM; ++d)
-              y += grad[(q * n + i) * DIM + d] * f[d];
+            {
+              double const g = sGrad[(q * N + j) * DIM + d];
+              v[d] += g * xv;
+              if constexpr (GRAD)
+                u[d] += g * yv;
+            }
           }
-        ybuf[((size_t)e * n + i) * nb + b] = y;
-      }
-    }
-  }
-}
-
-template <int DIM, int NPE>
-__global__ void __launch_bounds__(kMfBlock)
-    k_mf_gradk(int64_t E, int n, int nq, int nb, int epb, double const * __restrict__ xyz,
-               uint32_t const * __restrict__ conn, double const * __restrict__ geoGrad,
-               double const * __restrict__ w, double const * __restrict__ phi, double const * __restrict__ grad,
-               uint32_t const * __restrict__ emap, uint8_t const * __restrict__ cons, double const * __restrict__ x,
-               double const * __restrict__ ybar, double * __restrict__ gbuf)
-{
-  extern __shared__ double sm[]; // epb * nq
-  int64_t const e0 = (int64_t)blockIdx.x * epb;
-  int const tid = threadIdx.x;
-  if (tid < epb * nq)
-  {
-    int const le = tid / nq, q = tid - le * nq;
-    int64_t const e = e0 + le;
-    double c = 0.;
-    if (e < E)
-    {
-      double I[3][3];
-      double const det = geomAt<DIM, NPE>(e, q, xyz, conn, geoGrad, I);
-      for (int b = 0; b < nb; ++b)
-      {
-        double vx[DIM], vy[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d)
-          vx[d] = vy[d] = 0.;
-        for (int j = 0; j < n; ++j)
-        {
-          uint32_t const id = emap[e * n + j];
-          if (cons && cons[id])
-            continue;
-          double const xv = x[(size_t)id * nb + b], yv = ybar[(size_t)id * nb + b];
-#pragma unroll
-          for (int d = 0; d < DIM; ++d)
+          // I v with I symmetric (00 01 02 11 12 22)
+          double t[3];
+          t[0] = I00 * v[0] + (DIM > 1 ? I01 * v[1] : 0.) + (DIM > 2 ? I02 * v[2] : 0.);
+          t[1] = DIM > 1 ? I01 * v[0] + I11 * v[1] + (DIM > 2 ? I12 * v[2] : 0.) : 0.;
+          t[2] = DIM > 2 ? I02 * v[0] + I12 * v[1] + I22 * v[2] : 0.;
+          if constexpr (GRAD)
           {
-            double const g = grad[(q * n + j) * DIM + d];
-            vx[d] += g * xv;
-            vy[d] += g * yv;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              cq += t[d] * u[d];
+          }
+          else
+          {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              M.Wf[q][bb][d] = sq * t[d];
           }
         }
-#pragma unroll
-        for (int a = 0; a < DIM; ++a)
-#pragma unroll
-          for (int d = 0; d < DIM; ++d)
-            c += vx[a] * I[a][d] * vy[d];
       }
-      c *= sqrt(det) * w[q];
+      waveSync();
+      if constexpr (!GRAD)
+      {
+        if (live && l < N)
+        {
+          int const i = l;
+          bool const fl = M.flag[i] != 0;
+#pragma unroll
+          for (int bb = 0; bb < NBC; ++bb)
+          {
+            int const b = b0 + bb;
+            if (b >= nb)
+              break;
+            double y = 0.;
+            if (fl)
+              y = x[(size_t)id * nb + b]; // unit diagonal per element on constrained rows
+            else
+#pragma unroll 3
+              for (int q = 0; q < nq; ++q)
+              {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                  y += sGrad[(q * N + i) * DIM + d] * M.Wf[q][bb][d];
+              }
+            out[((size_t)e * N + i) * nb + b] = y;
+          }
+        }
+        waveSync(); // X and Wf are rewritten by the next chunk
+      }
     }
-    sm[le * nq + q] = c;
-  }
-  __syncthreads();
-  if (tid < epb * n)
-  {
-    int const le = tid / n, i = tid - le * n;
-    int64_t const e = e0 + le;
-    if (e < E)
+    if constexpr (GRAD)
     {
-      double g = 0.;
-      for (int q = 0; q < nq; ++q)
-        g += phi[q * n + i] * sm[le * nq + q];
-      gbuf[(size_t)e * n + i] = g;
+      if (live && l < nq)
+        M.C[l] = cq * sq;
+      waveSync();
+      if (live && l < N)
+      {
+        double g = 0.;
+        for (int q = 0; q < nq; ++q)
+          g += sPhi[q * N + l] * M.C[q];
+        out[(size_t)e * N + l] = g;
+      }
+      waveSync();
     }
   }
 }
 
-// y[d][b] = sum over (element, local) entries of dof d, in element order
+// y[d][b] = sum over the (element, local) entries of dof d, in element order;
+// the first kMaxC entries are loaded ahead (indices clamped into the segment)
 __global__ void k_gather(int64_t ndofs, int nb, uint32_t const * __restrict__ seg,
                          uint32_t const * __restrict__ ids, double const * __restrict__ buf, double * __restrict__ y)
 {
@@ -239,22 +288,44 @@ __global__ void k_gather(int64_t ndofs, int nb, uint32_t const * __restrict__ se
     return;
   int64_t const d = t / nb;
   int const b = (int)(t - d * nb);
+  uint32_t const k0 = seg[d], k1 = seg[d + 1];
   double acc = 0.;
-  for (uint32_t k = seg[d]; k < seg[d + 1]; ++k)
-    acc += buf[(size_t)ids[k] * nb + b];
+  if (k1 > k0)
+  {
+    constexpr int kMaxC = 8;
+    uint32_t id[kMaxC];
+#pragma unroll
+    for (int j = 0; j < kMaxC; ++j)
+      id[j] = ids[min(k0 + j, k1 - 1)];
+    double v[kMaxC];
+#pragma unroll
+    for (int j = 0; j < kMaxC; ++j)
+      v[j] = buf[(size_t)id[j] * nb + b];
+#pragma unroll
+    for (int j = 0; j < kMaxC; ++j)
+      if (k0 + j < k1)
+        acc += v[j];
+    for (uint32_t k = k0 + kMaxC; k < k1; ++k)
+      acc += buf[(size_t)ids[k] * nb + b];
+  }
   y[t] = acc;
 }
 
+// compile-time element shape per FE type: (DIM, NPE, N)
 template <typename F>
-void dispatchGeoMf(int geo, F && f)
+void dispatchFeMf(int feType, F && f)
 {
-  switch (geo)
+  using std::integral_constant;
+  switch (feType)
   {
-  case kLine: f(std::integral_constant<int, 1>{}, std::integral_constant<int, 2>{}); break;
-  case kQuad: f(std::integral_constant<int, 2>{}, std::integral_constant<int, 4>{}); break;
-  case kHex: f(std::integral_constant<int, 3>{}, std::integral_constant<int, 8>{}); break;
-  case kTet: f(std::integral_constant<int, 3>{}, std::integral_constant<int, 4>{}); break;
-  default: throw Error(PXG_E_INVALID, "unknown geometry");
+  case PXG_FE_LINE_P1: f(integral_constant<int, 1>{}, integral_constant<int, 2>{}, integral_constant<int, 2>{}); break;
+  case PXG_FE_QUAD_Q1: f(integral_constant<int, 2>{}, integral_constant<int, 4>{}, integral_constant<int, 4>{}); break;
+  case PXG_FE_QUAD_Q2: f(integral_constant<int, 2>{}, integral_constant<int, 4>{}, integral_constant<int, 9>{}); break;
+  case PXG_FE_HEX_Q1: f(integral_constant<int, 3>{}, integral_constant<int, 8>{}, integral_constant<int, 8>{}); break;
+  case PXG_FE_HEX_Q2: f(integral_constant<int, 3>{}, integral_constant<int, 8>{}, integral_constant<int, 27>{}); break;
+  case PXG_FE_TET_P1: f(integral_constant<int, 3>{}, integral_constant<int, 4>{}, integral_constant<int, 4>{}); break;
+  case PXG_FE_TET_P2: f(integral_constant<int, 3>{}, integral_constant<int, 4>{}, integral_constant<int, 10>{}); break;
+  default: throw Error(PXG_E_INVALID, "unknown FE type");
   }
 }
 
@@ -262,9 +333,54 @@ void prepare(pxg_ctx * c, int nb)
 {
   PXG_REQUIRE(c->fe.feType && c->nelem && c->ndofs, PXG_E_STATE, "matrix-free needs set_fe, set_mesh, set_dofmap");
   PXG_REQUIRE(nb >= 1 && nb <= kMaxNb, PXG_E_INVALID, "batch size must be 1..16");
-  PXG_REQUIRE(c->fe.nq <= kMfBlock && c->fe.n <= kMfBlock, PXG_E_INVALID, "element too large");
+  PXG_REQUIRE(c->fe.nq <= kMaxQ, PXG_E_INVALID, "at most 64 quadrature points");
   if (!c->dofSeg.p)
     buildDofAdjacency(c);
+}
+
+// one launch of the element kernel: lanes per element = the smallest power of
+// two holding n, nq and 3 npe; one resident wave of blocks striding over E
+template <bool GRAD>
+void launchElem(pxg_ctx * c, double const * kappa, uint8_t const * cons, int nb, double const * x,
+                double const * ybar, double * outBuf)
+{
+  FeTables & T = c->fe;
+  int const need = std::max({T.n, T.nq, 3 * T.npe});
+  int dev = 0, ncu = 0;
+  PXG_HIP(hipGetDevice(&dev));
+  PXG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  dispatchFeMf(T.feType,
+               [&](auto dimC, auto npeC, auto nC)
+               {
+                 constexpr int D = decltype(dimC)::value, NPE = decltype(npeC)::value, NN = decltype(nC)::value;
+                 auto go = [&](auto subC, auto nbcC)
+                 {
+                   constexpr int SUB = decltype(subC)::value, NBC = decltype(nbcC)::value;
+                   auto kern = k_mf_elem<D, NPE, NN, SUB, NBC, GRAD>;
+                   int perCu = 0;
+                   PXG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kern, 64 * kMfWaves, 0));
+                   int64_t const per = (int64_t)kMfWaves * (64 / SUB);
+                   int64_t const need = (c->nelem + per - 1) / per;
+                   int const grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)ncu * std::max(perCu, 1)));
+                   kern<<<grid, 64 * kMfWaves, 0, c->stream>>>(c->nelem, T.nq, nb, c->xyz.p, c->conn.p, T.d_geoGrad.p,
+                                                               T.d_w.p, T.d_phi.p, T.d_grad.p, c->elemMap.p, kappa,
+                                                               cons, x, ybar, outBuf, c->status.p);
+                 };
+                 auto pick = [&](auto subC)
+                 {
+                   if (nb == 1)
+                     go(subC, std::integral_constant<int, 1>{});
+                   else
+                     go(subC, std::integral_constant<int, 4>{});
+                 };
+                 if (need <= 16)
+                   pick(std::integral_constant<int, 16>{});
+                 else if (need <= 32)
+                   pick(std::integral_constant<int, 32>{});
+                 else
+                   pick(std::integral_constant<int, 64>{});
+               });
+  PXG_HIP(hipGetLastError());
 }
 } // namespace
 
@@ -272,27 +388,15 @@ void mfApply(pxg_ctx * c, double const * kappa, int nb, double const * x, double
 {
   prepare(c, nb);
   FeTables & T = c->fe;
-  int const epb = std::max(1, kMfBlock / std::max(T.nq, T.n));
   int64_t const E = c->nelem;
-  unsigned const grid = (unsigned)((E + epb - 1) / epb);
-  size_t const sh = (size_t)epb * T.nq * nb * T.dim * sizeof(double);
-  DevBuf<double> ybuf;
-  ybuf.resize((size_t)E * T.n * nb);
+  c->mfBuf.resize(std::max<size_t>(c->mfBuf.n, (size_t)E * T.n * nb));
   hipStream_t s = c->stream;
-  PXG_HIP(hipMemsetAsync(c->status.p, 0, sizeof(int), s));
   uint8_t const * cons = (useBC && c->hasBC) ? c->bcFlag.p : nullptr;
   ProfScope ps(c, "mf_apply");
-  dispatchGeoMf(T.geo,
-                [&](auto dimC, auto npeC)
-                {
-                  constexpr int D = decltype(dimC)::value, NPE = decltype(npeC)::value;
-                  k_mf_apply<D, NPE><<<grid, kMfBlock, sh, s>>>(E, T.n, T.nq, nb, epb, c->xyz.p, c->conn.p,
-                                                               T.d_geoGrad.p, T.d_w.p, T.d_phi.p, T.d_grad.p,
-                                                               c->elemMap.p, kappa, cons, x, ybuf.p, c->status.p);
-                });
-  PXG_HIP(hipGetLastError());
+  PXG_HIP(hipMemsetAsync(c->status.p, 0, sizeof(int), s));
+  launchElem<false>(c, kappa, cons, nb, x, nullptr, c->mfBuf.p);
   int64_t const m = c->ndofs * nb;
-  k_gather<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(c->ndofs, nb, c->dofSeg.p, c->dofContrib.p, ybuf.p, y);
+  k_gather<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(c->ndofs, nb, c->dofSeg.p, c->dofContrib.p, c->mfBuf.p, y);
   PXG_HIP(hipGetLastError());
   int h = 0;
   PXG_HIP(hipMemcpyAsync(&h, c->status.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -304,25 +408,14 @@ void mfGradKappa(pxg_ctx * c, int nb, double const * x, double const * ybar, dou
 {
   prepare(c, nb);
   FeTables & T = c->fe;
-  int const epb = std::max(1, kMfBlock / std::max(T.nq, T.n));
   int64_t const E = c->nelem;
-  unsigned const grid = (unsigned)((E + epb - 1) / epb);
-  size_t const sh = (size_t)epb * T.nq * sizeof(double);
-  DevBuf<double> gbuf;
-  gbuf.resize((size_t)E * T.n);
+  c->mfBuf.resize(std::max<size_t>(c->mfBuf.n, (size_t)E * T.n));
   hipStream_t s = c->stream;
   uint8_t const * cons = (useBC && c->hasBC) ? c->bcFlag.p : nullptr;
   ProfScope ps(c, "mf_gradk");
-  dispatchGeoMf(T.geo,
-                [&](auto dimC, auto npeC)
-                {
-                  constexpr int D = decltype(dimC)::value, NPE = decltype(npeC)::value;
-                  k_mf_gradk<D, NPE><<<grid, kMfBlock, sh, s>>>(E, T.n, T.nq, nb, epb, c->xyz.p, c->conn.p,
-                                                               T.d_geoGrad.p, T.d_w.p, T.d_phi.p, T.d_grad.p,
-                                                               c->elemMap.p, cons, x, ybar, gbuf.p);
-                });
-  PXG_HIP(hipGetLastError());
-  k_gather<<<(unsigned)((c->ndofs + 255) / 256), 256, 0, s>>>(c->ndofs, 1, c->dofSeg.p, c->dofContrib.p, gbuf.p,
+  PXG_HIP(hipMemsetAsync(c->status.p, 0, sizeof(int), s));
+  launchElem<true>(c, nullptr, cons, nb, x, ybar, c->mfBuf.p);
+  k_gather<<<(unsigned)((c->ndofs + 255) / 256), 256, 0, s>>>(c->ndofs, 1, c->dofSeg.p, c->dofContrib.p, c->mfBuf.p,
                                                               gk);
   PXG_HIP(hipGetLastError());
   PXG_HIP(hipStreamSynchronize(s));
