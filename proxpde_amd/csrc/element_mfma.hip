@@ -83,8 +83,9 @@ struct WaveLds
   double x[NPE * 3];
   double kap[N];
   double bcv[N];
-  double M[NQ][6];
-  double mq[NQ];
+  // rows of the per-point C operand factor: a < DIM the metric row
+  // s_q (J^T J)^{-1}[a][:], a = DIM the mass row (m_q, 0, 0)
+  double M[NQ][4][3];
   double f[N]; // lifting accumulator of the Dirichlet elimination
   int flag[N];
 };
@@ -110,6 +111,7 @@ __global__ void __launch_bounds__(64 * kWaves)
   __shared__ double sGrad[NQ * N * DIM];
   __shared__ double sW[NQ];
   __shared__ T sA[NT * STEPS * 64]; // constant operand fragments
+  __shared__ int sKA[STEPS * 4];      // row k = 4 s + kr of C: (q << 2 | a), -1 for padding
   __shared__ L lds[kWaves];
 
   int const tid = threadIdx.x;
@@ -128,6 +130,8 @@ __global__ void __launch_bounds__(64 * kWaves)
   // constant operand fragments: sA[(ti STEPS + s) 64 + lane] = A[16 ti + lane%16][4 s + lane/16]
   for (int k = tid; k < NT * STEPS * 64; k += blockDim.x)
     sA[k] = afrag[k];
+  for (int k = tid; k < STEPS * 4; k += blockDim.x)
+    sKA[k] = k < KS ? (((k / DIM) << 2) | (k % DIM)) : ((MASS && k < KS + NQ) ? (((k - KS) << 2) | DIM) : -1);
   __syncthreads();
 
   int const kr = lane >> 4, jc = lane & 15;
@@ -256,13 +260,21 @@ __global__ void __launch_bounds__(64 * kWaves)
         m += cM * JxW;
       if (flags & 8)
         m += (cMF * kq) * JxW;
-      W.M[q][0] = s * I00;
-      W.M[q][1] = s * I01;
-      W.M[q][2] = s * I02;
-      W.M[q][3] = s * I11;
-      W.M[q][4] = s * I12;
-      W.M[q][5] = s * I22;
-      W.mq[q] = m;
+      W.M[q][0][0] = s * I00;
+      W.M[q][0][1] = s * I01;
+      W.M[q][0][2] = s * I02;
+      W.M[q][1][0] = s * I01;
+      W.M[q][1][1] = s * I11;
+      W.M[q][1][2] = s * I12;
+      if constexpr (DIM == 3)
+      {
+        W.M[q][2][0] = s * I02;
+        W.M[q][2][1] = s * I12;
+        W.M[q][2][2] = s * I22;
+      }
+      W.M[q][DIM][0] = m;
+      W.M[q][DIM][1] = 0.;
+      W.M[q][DIM][2] = 0.;
     }
     waveSync();
     // next element's values: their latency hides behind the matrix products
@@ -275,45 +287,58 @@ __global__ void __launch_bounds__(64 * kWaves)
 #pragma unroll
       for (int tj = 0; tj < NT; ++tj)
         acc[ti][tj] = Acc{0, 0, 0, 0};
-    // not unrolled: it keeps the kernel at or below 128 VGPRs, i.e. two blocks
-    // (16 waves) per CU; the other waves hide the LDS and MFMA latencies
+    // C[4s + kr][16 tj + jc] for row code (q, a) of step s: the metric row
+    // (or the mass row) times the reference gradient (or phi) of dof j at q;
+    // indices are clamped and the value masked, so there are no branches
+    auto cval = [&](int s, int tj) -> T
+    {
+      int const code = sKA[4 * s + kr];
+      int const j = 16 * tj + jc;
+      bool const valid = code >= 0 && j < N;
+      int const c = code < 0 ? 0 : code;
+      int const q = c >> 2, a = c & 3;
+      int const jj = j < N ? j : N - 1;
+      double const * m = W.M[q][a];
+      double const * g = sGrad + (q * N + jj) * DIM;
+      double g0 = g[0];
+      if constexpr (MASS)
+        g0 = a == DIM ? sPhi[q * N + jj] : g0;
+      double v = m[0] * g0 + m[1] * g[1];
+      if constexpr (DIM == 3)
+        v += m[2] * g[2];
+      return valid ? (T)v : T(0);
+    };
+    // software-pipelined: the operands of step s+1 are read from LDS while the
+    // matrix instructions of step s run.  Not unrolled: it keeps the kernel
+    // at or below 128 VGPRs, i.e. two blocks (16 waves) per CU
+    T aC[NT], bC[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+    {
+      aC[t] = sA[(t * STEPS) * 64 + lane];
+      bC[t] = cval(0, t);
+    }
 #pragma unroll 1
     for (int s = 0; s < STEPS; ++s)
     {
-      int const k = 4 * s + kr;
+      int const sn = s + 1 < STEPS ? s + 1 : s;
+      T aN[NT], bN[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+      {
+        aN[t] = sA[(t * STEPS + sn) * 64 + lane];
+        bN[t] = cval(sn, t);
+      }
 #pragma unroll
       for (int tj = 0; tj < NT; ++tj)
-      {
-        int const j = 16 * tj + jc;
-        double v = 0.;
-        if (j < N)
-        {
-          if (k < KS)
-          {
-            int const q = k / DIM, a0 = k - (k / DIM) * DIM;
-            double const * g = sGrad + (q * N + j) * DIM;
-            double const * Mq = W.M[q];
-            if constexpr (DIM == 2)
-              v = (a0 == 0) ? (Mq[0] * g[0] + Mq[1] * g[1]) : (Mq[1] * g[0] + Mq[3] * g[1]);
-            else
-            {
-              // row a0 of the symmetric metric stored as (00 01 02 11 12 22)
-              int const i0 = a0 == 0 ? 0 : (a0 == 1 ? 1 : 2);
-              int const i1 = a0 == 0 ? 1 : (a0 == 1 ? 3 : 4);
-              int const i2 = a0 == 0 ? 2 : (a0 == 1 ? 4 : 5);
-              v = Mq[i0] * g[0] + Mq[i1] * g[1] + Mq[i2] * g[2];
-            }
-          }
-          else if (MASS && k < KS + NQ)
-          {
-            int const q = k - KS;
-            v = W.mq[q] * sPhi[q * N + j];
-          }
-        }
-        T const b = (T)v;
 #pragma unroll
         for (int ti = 0; ti <= tj; ++ti)
-          acc[ti][tj] = Mfma<T>::mma(sA[(ti * STEPS + s) * 64 + lane], b, acc[ti][tj]);
+          acc[ti][tj] = Mfma<T>::mma(aC[ti], bC[tj], acc[ti][tj]);
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+      {
+        aC[t] = aN[t];
+        bC[t] = bN[t];
       }
     }
     // ---- 4. the upper triangle goes from the accumulators straight to the
