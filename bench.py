@@ -145,6 +145,154 @@ def solve_and_adjoint(args, dev, N, W):
     return res
 
 
+def _sync_time(fn, reps=1):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps, out
+
+
+def config_c2(args, dev, N, W):
+    """Q1 hex 128^3, 2x2x2 Gauss, stiffness (kappa = 1), no Dirichlet; FP64 and
+    FP32 element kernels; then 100 SpMV on x ~ U(-1, 1) seed 0, FP64 and FP32."""
+    import torch
+    n = 128
+    prob = W.Problem("hex_q1", (n, n, n))
+    a = prob.assembler()
+    a.set_dirichlet(np.zeros(0, np.uint32), np.zeros(0))
+    a.build_pattern()
+    res = {}
+    for name, prec in (("fp64", N.FP64), ("fp32", N.FP32)):
+        def build():
+            a.clear()
+            a.assemble([("stiffness", 1.0)], precision=prec)
+            a.close(1e-16)
+        build()
+        t, _ = _sync_time(build, 3)
+        res[f"build_{name}_ms"] = 1e3 * t
+        res[f"elements_per_s_{name}"] = prob.mesh.nelem / t
+    # FP32 element kernels against the FP64 matrix (the prune of cancellation
+    # zeros may differ between the precisions: compared on the union pattern)
+    import scipy.sparse as sp
+    rp32, col32, v32, _ = a.csr(fp32=True)
+    a.clear()
+    a.assemble([("stiffness", 1.0)], precision=N.FP64)
+    a.close(1e-16)
+    rp, col, v, _ = a.csr()
+    A64 = sp.csr_matrix((v, col, rp), shape=(prob.ndofs, prob.ndofs))
+    A32 = sp.csr_matrix((v32.astype(np.float64), col32, rp32), shape=(prob.ndofs, prob.ndofs))
+    res["nnz"] = int(len(v))
+    res["nnz_fp32"] = int(len(v32))
+    res["fp32_values_max_rel_err"] = float(abs(A32 - A64).max() / np.abs(v).max())
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    x = torch.rand(prob.ndofs, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    y = torch.empty_like(x)
+    a.set_stream(torch.cuda.current_stream().cuda_stream)
+    for name, prec, xv, yv, s in (("fp64", N.FP64, x, y, 8), ("fp32", N.FP32, x.float(), y.float(), 4)):
+        a.spmv_dev(xv.data_ptr(), yv.data_ptr(), precision=prec)
+        t, _ = _sync_time(lambda: a.spmv_dev(xv.data_ptr(), yv.data_ptr(), precision=prec), 100)
+        by = len(v) * (s + 4) + 8 * (prob.ndofs + 1) + 2 * s * prob.ndofs
+        res[f"spmv_{name}_ms"] = 1e3 * t
+        res[f"spmv_{name}_GB/s"] = by / t / 1e9
+    return {"metric": "C2: Q1 hex 128^3 stiffness build + 100 SpMV", "config": {"workload": "Q1 hex 128^3, kappa=1, "
+            "no Dirichlet", "elements": prob.mesh.nelem, "dofs": prob.ndofs}, "results": res}
+
+
+def config_c3(args, dev, N, W):
+    """P2 tets, 48^3 cubes x 6 (Kuhn), interior vertices perturbed U(-0.2h, 0.2h)
+    seed 1, Gauss-11, kappa = exp(GRF seed 2) at the P2 nodes; stiffness(kappa) +
+    mass(1); FP64 and FP32 element kernels, FP32 error against FP64."""
+    n = 48
+    prob = W.Problem("tet_p2", (n, n, n), mesh="cube_tet6")
+    prob.set_coords(W.perturb_interior(prob.xyz, 1.0 / n, seed=1, amount=0.2))
+    kap = np.exp(W.grf(prob.dof_coords(), seed=2, std=1.0, ell=0.1, features=4096))
+    a = prob.assembler()
+    d = prob.all_sides()
+    a.set_dirichlet(d, np.zeros(len(d)))
+    a.set_coefficient(kap)
+    a.build_pattern()
+    terms = [("stiffness_fe", 1.0), ("mass", 1.0)]
+    res = {}
+    out = {}
+    for name, prec in (("fp64", N.FP64), ("fp32", N.FP32)):
+        def build():
+            a.clear()
+            a.assemble(terms, precision=prec)
+            a.close(1e-16)
+        build()
+        t, _ = _sync_time(build, 3)
+        res[f"build_{name}_ms"] = 1e3 * t
+        res[f"elements_per_s_{name}"] = prob.mesh.nelem / t
+        out[name] = a.csr(fp32=prec == N.FP32)
+    import scipy.sparse as sp
+    nd = prob.ndofs
+    A64 = sp.csr_matrix((out["fp64"][2], out["fp64"][1], out["fp64"][0]), shape=(nd, nd))
+    A32 = sp.csr_matrix((out["fp32"][2].astype(np.float64), out["fp32"][1], out["fp32"][0]), shape=(nd, nd))
+    res["nnz"] = int(A64.nnz)
+    res["fp32_values_max_rel_err"] = float(abs(A32 - A64).max() / abs(A64).max())
+    return {"metric": "C3: P2 tet (6-tet Kuhn) 48^3 stiffness(kappa)+mass build", "config": {
+            "workload": "P2 tet 48^3 x 6, perturbed 0.2h, Gauss-11, lognormal kappa", "elements": prob.mesh.nelem,
+            "dofs": prob.ndofs}, "results": res}
+
+
+def config_c5(args, dev, N, W):
+    """Q1 hex 256^3, MG levels 256/128/64/32/16/8 (Jacobi 2/3, 2+2), PCG to 1e-8,
+    f = 1, Dirichlet 0 on all sides; J = 1/2 |u - u_t|^2 with u_t = K(kappa_t)^{-1} f,
+    kappa_t lognormal seed 4; J and dJ/dkappa at kappa = 1; FP64 vs FP32-refined."""
+    import torch
+
+    from proxpde_amd import autograd as A
+    n = 256
+    ns = [n >> l for l in range(6)]
+    t0 = time.time()
+    solver = A.KappaSolver("hex_q1", ns)
+    t_setup = time.time() - t0
+    x = solver.probs[0].dof_coords()
+    kap_t = torch.from_numpy(np.exp(W.grf(x, seed=4, ell=0.1, features=1024, device=dev))).to(dev)
+    b = torch.from_numpy(solver.rhs(1.0)).to(dev)
+    with torch.no_grad():
+        ut = solver(kap_t, b, rtol=1e-10)
+    kap = torch.ones(solver.ndofs, dtype=torch.float64, device=dev)
+    res = {"levels": ns, "setup_s": t_setup}
+    sol = {}
+    for name, mode in (("fp64", N.FP64), ("fp32_ir", N.FP32)):
+        progress(f"C5 solve+adjoint {name}")
+        times = []
+        for rep in range(3):
+            k = kap.clone().requires_grad_()
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            u = solver(k, b, rtol=1e-8, mode=mode)
+            J = 0.5 * ((u - ut) ** 2).sum()
+            J.backward()
+            torch.cuda.synchronize()
+            if rep > 0:
+                times.append(time.perf_counter() - ts)
+        sol[name] = (u.detach(), k.grad.detach())
+        xs = torch.zeros_like(b)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        its, _ = solver.mg.solve_dev(b.data_ptr(), xs.data_ptr(), 1e-8, 1000, mode)
+        torch.cuda.synchronize()
+        res[name] = {"ms": 1e3 * min(times), "solve_only_ms": 1e3 * (time.perf_counter() - ts), "cg_iterations": its}
+    op = A.KappaOperator(solver.ctx[0], use_bc=True)
+    for name in sol:
+        with torch.no_grad():
+            r = b - op(kap, sol[name][0].unsqueeze(1)).squeeze(1)
+        res[name]["relres_fp64"] = float(r.norm() / b.norm())
+    res["fp32_ir"]["u_rel_err_vs_fp64"] = float((sol["fp32_ir"][0] - sol["fp64"][0]).norm() / sol["fp64"][0].norm())
+    res["fp32_ir"]["grad_rel_err_vs_fp64"] = float((sol["fp32_ir"][1] - sol["fp64"][1]).norm() /
+                                                   sol["fp64"][1].norm())
+    res["speedup_fp32_ir_over_fp64"] = res["fp64"]["ms"] / res["fp32_ir"]["ms"]
+    return {"metric": "C5: Q1 hex 256^3 solve + adjoint gradient, FP64 vs FP32-refined", "config": {
+            "workload": "Q1 hex 256^3, f=1, Dirichlet 0, MG 6 levels, PCG 1e-8, kappa=1, u_t from kappa_t seed 4",
+            "dofs": solver.ndofs}, "results": res}
+
+
 def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -161,6 +309,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--solve-levels", type=int, default=5)
     ap.add_argument("--solve-reps", type=int, default=2)
+    ap.add_argument("--config", choices=["c2", "c3", "c5"], default=None,
+                    help="run one of the other workloads instead of the headline one (one JSON line)")
     ap.add_argument("--copy-gib", type=float, default=1.0,
                     help="size of a plain device copy timed for reference (0: none)")
     args = ap.parse_args()
@@ -176,6 +326,13 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.init()
+    if args.config:
+        fn = {"c2": config_c2, "c3": config_c3, "c5": config_c5}[args.config]
+        progress(f"config {args.config}")
+        out = fn(args, dev, N, W)
+        out.update({"n_gpus": 1, "data": "synthetic", "dtype": "f64 / f32"})
+        print(json.dumps(out))
+        return
     n = args.n
     t0 = time.time()
     prob = W.Problem("hex_q2", (n, n, n))

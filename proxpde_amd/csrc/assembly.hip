@@ -213,6 +213,26 @@ __global__ void __launch_bounds__(kBlock)
         acc[j] += pm * sPhi[q * N + j];
     }
   }
+  if constexpr (sizeof(T) == 4 && 64 % N == 0)
+  {
+    // FP32: flush rounding noise of exactly-zero entries (below 2^-19 of
+    // sqrt(Ke_ii Ke_jj)) so the FP32 matrix keeps the FP64 sparsity; the N
+    // rows of an element are consecutive lanes of one wavefront
+    T di = T(0);
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      if (j == i)
+        di = acc[j];
+    float const dabs = fabsf((float)di);
+    int const lane0 = (int)(threadIdx.x & 63) - i;
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+    {
+      float const dj = __shfl(dabs, lane0 + j);
+      if (fabsf((float)acc[j]) <= 0x1p-19f * sqrtf(dabs * dj))
+        acc[j] = T(0);
+    }
+  }
   double f = 0.;
   if (hasBC)
   {

@@ -86,7 +86,8 @@ struct WaveLds
   // rows of the per-point C operand factor: a < DIM the metric row
   // s_q (J^T J)^{-1}[a][:], a = DIM the mass row (m_q, 0, 0)
   double M[NQ][4][3];
-  double f[N]; // lifting accumulator of the Dirichlet elimination
+  double f[N];  // lifting accumulator of the Dirichlet elimination
+  double dg[N]; // diagonal of Ke (FP32: the flush threshold of noise entries)
   int flag[N];
 };
 
@@ -350,6 +351,39 @@ __global__ void __launch_bounds__(64 * kWaves)
     //         lower part through symmetry.
     bool const anyBC = __any(lane < N && W.flag[lane] != 0);
     T * out = ke + e * (N * N);
+    if constexpr (sizeof(T) == 4)
+    {
+      // FP32: entries below 2^-19 sqrt(Ke_ii Ke_jj) are rounding noise of
+      // entries that are zero in exact arithmetic (FP64 gives exact zeros or
+      // |v| <= 1e-16 there, which close prunes); they are flushed to zero so
+      // the FP32 matrix keeps the FP64 sparsity
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+          int const i = 16 * t + Mfma<T>::row(r, kr), j = 16 * t + jc;
+          if (i == j && i < N)
+            W.dg[i] = fabs((double)acc[t][t][r]);
+        }
+      waveSync();
+#pragma unroll
+      for (int tj = 0; tj < NT; ++tj)
+      {
+        int const j = 16 * tj + jc;
+        double const dj = W.dg[j < N ? j : N - 1];
+#pragma unroll
+        for (int ti = 0; ti <= tj; ++ti)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+          {
+            int const i = 16 * ti + Mfma<T>::row(r, kr);
+            double const di = W.dg[i < N ? i : N - 1];
+            if (fabs((double)acc[ti][tj][r]) <= 0x1p-19 * sqrt(di * dj))
+              acc[ti][tj][r] = T(0);
+          }
+      }
+    }
     double colPart[NT];
 #pragma unroll
     for (int tj = 0; tj < NT; ++tj)

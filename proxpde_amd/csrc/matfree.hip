@@ -89,24 +89,80 @@ __global__ void __launch_bounds__(64 * kMfWaves)
   int const sub = lane / SUB, l = lane - sub * SUB;
   L & M = lds[wave * EPW + sub];
   int64_t const stride = (int64_t)gridDim.x * kMfWaves * EPW;
+  // gathered data of the next element, loaded one element ahead so the
+  // global-memory latency hides behind the current element's work:
+  // ids first (conn, emap), values (coordinates, kappa, flag, the first
+  // chunk of x / ybar) after the geometry phase
+  constexpr int CPL = (NPE * 3 + SUB - 1) / SUB; // coordinates per lane
+  uint32_t pPid[CPL], pId = 0;
+  double pX[CPL], pK = 0., pXv[NBC], pYv[NBC];
+  int pF = 0;
+  auto gatherIds = [&](int64_t el)
+  {
+    if (el < E)
+    {
+#pragma unroll
+      for (int c = 0; c < CPL; ++c)
+        if (l + c * SUB < NPE * 3)
+          pPid[c] = conn[el * NPE + (l + c * SUB) / 3];
+      if (l < N)
+        pId = emap[el * N + l];
+    }
+  };
+  auto gatherVals = [&](int64_t el)
+  {
+    if (el < E)
+    {
+#pragma unroll
+      for (int c = 0; c < CPL; ++c)
+        if (l + c * SUB < NPE * 3)
+          pX[c] = xyz[3ull * pPid[c] + (l + c * SUB) % 3];
+      if (l < N)
+      {
+        if constexpr (!GRAD)
+          pK = kappa[pId];
+        pF = cons ? (int)cons[pId] : 0;
+#pragma unroll
+        for (int bb = 0; bb < NBC; ++bb)
+        {
+          pXv[bb] = bb < nb ? x[(size_t)pId * nb + bb] : 0.;
+          if constexpr (GRAD)
+            pYv[bb] = bb < nb ? ybar[(size_t)pId * nb + bb] : 0.;
+        }
+      }
+    }
+  };
+  int64_t const first = ((int64_t)blockIdx.x * kMfWaves + wave) * EPW;
+  gatherIds(first + sub);
+  gatherVals(first + sub);
   // the wave loops uniformly; each group checks its own element
-  for (int64_t base = ((int64_t)blockIdx.x * kMfWaves + wave) * EPW; base < E; base += stride)
+  for (int64_t base = first; base < E; base += stride)
   {
     int64_t const e = base + sub;
     bool const live = e < E;
     uint32_t id = 0;
     if (live)
     {
-      if (l < NPE * 3)
-        M.xyz[l] = xyz[3ull * conn[e * NPE + l / 3] + l % 3];
+#pragma unroll
+      for (int c = 0; c < CPL; ++c)
+        if (l + c * SUB < NPE * 3)
+          M.xyz[l + c * SUB] = pX[c];
       if (l < N)
       {
-        id = emap[e * N + l];
-        M.kap[l] = GRAD ? 0. : kappa[id];
-        M.flag[l] = cons ? (int)cons[id] : 0;
+        id = pId;
+        M.kap[l] = pK;
+        M.flag[l] = pF;
+#pragma unroll
+        for (int bb = 0; bb < NBC; ++bb)
+        {
+          M.X[l][bb] = pF ? 0. : pXv[bb];
+          if constexpr (GRAD)
+            M.Y[l][bb] = pF ? 0. : pYv[bb];
+        }
       }
     }
     waveSync();
+    gatherIds(e + stride);
     // geometry of point q = l, kept in registers across the batch
     double I00 = 0., I01 = 0., I02 = 0., I11 = 0., I12 = 0., I22 = 0., sq = 0.;
     if (live && l < nq)
@@ -173,10 +229,11 @@ __global__ void __launch_bounds__(64 * kMfWaves)
         sq = kq * jxw;
       }
     }
+    gatherVals(e + stride);
     double cq = 0.; // gradient: sum over the batch at point q
     for (int b0 = 0; b0 < nb; b0 += NBC)
     {
-      if (live && l < N)
+      if (b0 > 0 && live && l < N) // the first chunk came with the prefetch
       {
         bool const fl = M.flag[l] != 0;
 #pragma unroll
@@ -339,13 +396,13 @@ void prepare(pxg_ctx * c, int nb)
 }
 
 // one launch of the element kernel: lanes per element = the smallest power of
-// two holding n, nq and 3 npe; one resident wave of blocks striding over E
+// two (at least 8) holding n and nq; one resident wave of blocks striding over E
 template <bool GRAD>
 void launchElem(pxg_ctx * c, double const * kappa, uint8_t const * cons, int nb, double const * x,
                 double const * ybar, double * outBuf)
 {
   FeTables & T = c->fe;
-  int const need = std::max({T.n, T.nq, 3 * T.npe});
+  int const need = std::max(T.n, T.nq);
   int dev = 0, ncu = 0;
   PXG_HIP(hipGetDevice(&dev));
   PXG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -356,6 +413,8 @@ void launchElem(pxg_ctx * c, double const * kappa, uint8_t const * cons, int nb,
                  auto go = [&](auto subC, auto nbcC)
                  {
                    constexpr int SUB = decltype(subC)::value, NBC = decltype(nbcC)::value;
+                   if constexpr (SUB >= NN) // (the dispatch below never picks a smaller group)
+                   {
                    auto kern = k_mf_elem<D, NPE, NN, SUB, NBC, GRAD>;
                    int perCu = 0;
                    PXG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kern, 64 * kMfWaves, 0));
@@ -365,6 +424,7 @@ void launchElem(pxg_ctx * c, double const * kappa, uint8_t const * cons, int nb,
                    kern<<<grid, 64 * kMfWaves, 0, c->stream>>>(c->nelem, T.nq, nb, c->xyz.p, c->conn.p, T.d_geoGrad.p,
                                                                T.d_w.p, T.d_phi.p, T.d_grad.p, c->elemMap.p, kappa,
                                                                cons, x, ybar, outBuf, c->status.p);
+                   }
                  };
                  auto pick = [&](auto subC)
                  {
@@ -373,7 +433,9 @@ void launchElem(pxg_ctx * c, double const * kappa, uint8_t const * cons, int nb,
                    else
                      go(subC, std::integral_constant<int, 4>{});
                  };
-                 if (need <= 16)
+                 if (need <= 8)
+                   pick(std::integral_constant<int, 8>{});
+                 else if (need <= 16)
                    pick(std::integral_constant<int, 16>{});
                  else if (need <= 32)
                    pick(std::integral_constant<int, 32>{});

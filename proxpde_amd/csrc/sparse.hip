@@ -108,14 +108,14 @@ __device__ inline void csrOut(int64_t row, T s, T const * __restrict__ x, T cons
     y[row] += s;
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, int G>
 __global__ void __launch_bounds__(kBlock)
     k_csr_stream(int32_t const * __restrict__ blockRow, int64_t const * __restrict__ rowptr,
                  int32_t const * __restrict__ col, T const * __restrict__ val, T const * __restrict__ x,
                  T const * __restrict__ b, T const * __restrict__ dinv, T omega, T * __restrict__ y)
 {
   constexpr int kSlots = kStreamSlots / kBlock; // entries per thread
-  constexpr int G = 8;                          // lanes per row in the reduction
+  // G lanes per row in the reduction (chosen from the mean row length)
   __shared__ T prod[kStreamSlots];
   int64_t const r0 = blockRow[blockIdx.x], r1 = blockRow[blockIdx.x + 1];
   if (r0 == r1)
@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(kBlock)
   {
     int const k0 = (int)(rowptr[row] - base), k1 = (int)(rowptr[row + 1] - base);
     // rows up to kRed * G entries in one unrolled pass of independent reads
-    constexpr int kRed = 16;
+    constexpr int kRed = G >= 8 ? 16 : 8;
     T s = T(0);
 #pragma unroll
     for (int j = 0; j < kRed; ++j)
@@ -161,9 +161,12 @@ __global__ void __launch_bounds__(kBlock)
     }
     for (int k = k0 + lane + kRed * G; k < k1; k += G)
       s += prod[k];
+    if constexpr (G > 1)
+    {
 #pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1)
-      s += __shfl_xor(s, off, G);
+      for (int off = G / 2; off > 0; off >>= 1)
+        s += __shfl_xor(s, off, G);
+    }
     if (lane == 0)
       csrOut<T, MODE>(row, s, x, b, dinv, omega, y);
   }
@@ -239,14 +242,22 @@ void launchCsr(CsrView const & A, T const * val, T const * x, T const * b, T con
   if (A.nrows == 0)
     return;
   T const om = (T)omega;
+  double const mean = A.nnz > 0 ? (double)A.nnz / (double)A.nrows : 1.0;
   if (A.blockRow && A.nblocks > 0 && !rowGroupForced())
   {
-    k_csr_stream<T, MODE><<<(unsigned)A.nblocks, kBlock, 0, s>>>(A.blockRow, A.rowptr, A.col, val, x, b, dinv, om,
-                                                                   y);
+    unsigned const g = (unsigned)A.nblocks;
+    // lanes per row in the reduction: about 4-8 entries per lane
+    if (mean > 48)
+      k_csr_stream<T, MODE, 8><<<g, kBlock, 0, s>>>(A.blockRow, A.rowptr, A.col, val, x, b, dinv, om, y);
+    else if (mean > 20)
+      k_csr_stream<T, MODE, 4><<<g, kBlock, 0, s>>>(A.blockRow, A.rowptr, A.col, val, x, b, dinv, om, y);
+    else if (mean > 8)
+      k_csr_stream<T, MODE, 2><<<g, kBlock, 0, s>>>(A.blockRow, A.rowptr, A.col, val, x, b, dinv, om, y);
+    else
+      k_csr_stream<T, MODE, 1><<<g, kBlock, 0, s>>>(A.blockRow, A.rowptr, A.col, val, x, b, dinv, om, y);
     PXG_HIP(hipGetLastError());
     return;
   }
-  double const mean = A.nnz > 0 ? (double)A.nnz / (double)A.nrows : 1.0;
   int lpr = lanesPerRow(mean);
   unsigned const g = blocksFor(A.nrows * lpr);
   switch (lpr)

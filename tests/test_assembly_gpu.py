@@ -304,3 +304,26 @@ def test_fp32_element_kernels_vs_fp64_oracle(oracle, gpu, fe, mesh, n, pert):
     err = np.abs(v32.astype(np.float64) - v_o).max() / np.abs(v_o).max()
     print(f"  {fe}: FP32 element kernel max error / max|A| = {err:.2e}")
     assert err < 5e-6
+
+
+@pytest.mark.parametrize("fe,n,sides", [("hex_q1", (6, 5, 4), False), ("hex_q2", (3, 3, 3), True),
+                                        ("quad_q1", (7, 6), True)])
+def test_fp32_build_keeps_fp64_sparsity(gpu, fe, n, sides):
+    """kappa = 1 on a uniform mesh: FP64 element matrices have exact zeros that
+    close prunes; the FP32 kernels flush their rounding noise there, so both
+    precisions give the same pruned pattern."""
+    prob = W.Problem(fe, n)
+    a = prob.assembler()
+    d = prob.all_sides() if sides else np.zeros(0, np.uint32)
+    a.set_dirichlet(d, np.zeros(len(d)))
+    a.set_coefficient(np.ones(prob.ndofs))
+    a.build_pattern()
+    out = {}
+    for prec in (0, 1):
+        a.clear()
+        a.assemble([("stiffness_fe", 1.0)], precision=prec)
+        a.close()
+        out[prec] = a.csr(fp32=prec == 1)
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    err = np.abs(out[1][2].astype(np.float64) - out[0][2]).max() / np.abs(out[0][2]).max()
+    assert err < 1e-6
