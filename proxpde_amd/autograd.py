@@ -99,8 +99,13 @@ class KappaSolver:
         a.assemble_rhs("const", (float(f),))
         return a.rhs_host()
 
-    def __call__(self, kappa, b, rtol=1e-8, mode=N.FP64, maxit=1000, fp32_levels=None):
+    def __call__(self, kappa, b, rtol=1e-8, mode=N.FP64, maxit=1000, fp32_levels=None, fine_fp64=False):
+        """mode FP32: FP32 element matrices on all but the coarsest level (fp32_levels)
+        and the FP64 refinement residual through the matrix-free operator; with
+        fine_fp64 the finest level is assembled in FP64, its FP32 copy rounded from
+        it, and the residual uses that matrix (cheaper for low-order elements)."""
         if fp32_levels is None:
             fp32_levels = mode == N.FP32
         return torch.ops.proxpde.kappa_solve(kappa, b, [c.h.value for c in self.ctx], self.inject,
-                                             self.mg.h.value, _stream(), rtol, int(mode), maxit, bool(fp32_levels))
+                                             self.mg.h.value, _stream(), rtol, int(mode), maxit, bool(fp32_levels),
+                                             bool(fine_fp64))

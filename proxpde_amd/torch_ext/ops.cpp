@@ -6,10 +6,14 @@
 //       backward: dx = K^T dy = K dy;  dkappa_n = sum_e sum_q phi_n JxW grad(dy).grad(x)
 //                 (geometry recomputed, nothing stored but kappa and x)
 //
-//   proxpde::kappa_solve(kappa, b, ctxs, inject, mg, stream, rtol, mode, maxit, fp32_levels)
+//   proxpde::kappa_solve(kappa, b, ctxs, inject, mg, stream, rtol, mode, maxit, fp32_levels, fine_fp64)
 //       forward : re-assembles K(kappa) on every multigrid level (kappa injected to
 //                 the coarse dofs through `inject`), closes, and solves K u = b with
-//                 MG-PCG (mode 0: FP64; mode 1: FP32 inner solves in FP64 refinement)
+//                 MG-PCG (mode 0: FP64; mode 1: FP32 inner solves in FP64 refinement).
+//                 fp32_levels: element matrices of all but the coarsest level in FP32;
+//                 the refinement residual then uses the FP64 matrix-free operator.
+//                 fine_fp64: the finest level is still assembled in FP64 (its FP32
+//                 copy is rounded from it) and the residual uses that matrix.
 //       backward: adjoint solve K lambda = du;  db = lambda;
 //                 dkappa = -d/dkappa (lambda . K(kappa) u)  (Dirichlet rows masked)
 //                 The level matrices are re-assembled only if some level was
@@ -106,7 +110,7 @@ struct SolveSetup
   void * stream;
   double rtol;
   int64_t mode, maxit;
-  bool fp32Levels;
+  bool fp32Levels, fineFp64;
 };
 
 std::vector<int64_t> stampsOf(SolveSetup const & S)
@@ -134,7 +138,7 @@ void assembleLevels(SolveSetup const & S, Tensor const & kappa)
     check(pxg_set_coefficient_device(c, kl.data_ptr<double>()), c, "pxg_set_coefficient_device");
     check(pxg_clear(c), c, "pxg_clear");
     // the coarsest level stays FP64 (FP64 coarse solve)
-    int const prec = (S.fp32Levels && l + 1 < nl) ? PXG_FP32 : PXG_FP64;
+    int const prec = (S.fp32Levels && l + 1 < nl && !(l == 0 && S.fineFp64)) ? PXG_FP32 : PXG_FP64;
     check(pxg_assemble(c, 1, &term, prec), c, "pxg_assemble");
     int64_t nnz = 0;
     check(pxg_close(c, 1e-16, &nnz), c, "pxg_close");
@@ -166,6 +170,7 @@ SolveSetup setupFrom(AutogradContext * ctx)
   S.mode = ctx->saved_data["mode"].toInt();
   S.maxit = ctx->saved_data["maxit"].toInt();
   S.fp32Levels = ctx->saved_data["fp32"].toBool();
+  S.fineFp64 = ctx->saved_data["fine64"].toBool();
   return S;
 }
 
@@ -174,7 +179,7 @@ class KappaSolve: public torch::autograd::Function<KappaSolve>
 public:
   static Tensor forward(AutogradContext * ctx, Tensor kappa, Tensor b, std::vector<int64_t> levels,
                         std::vector<Tensor> inject, int64_t mg, int64_t stream, double rtol, int64_t mode,
-                        int64_t maxit, bool fp32Levels)
+                        int64_t maxit, bool fp32Levels, bool fineFp64)
   {
     TORCH_CHECK(kappa.scalar_type() == torch::kFloat64 && b.scalar_type() == torch::kFloat64, "float64 tensors");
     TORCH_CHECK(!levels.empty() && inject.size() + 1 == levels.size(), "one injection map per coarse level");
@@ -188,6 +193,7 @@ public:
     S.mode = mode;
     S.maxit = maxit;
     S.fp32Levels = fp32Levels;
+    S.fineFp64 = fineFp64;
     auto kc = kappa.contiguous();
     assembleLevels(S, kc);
     Tensor u = solveWith(S, b);
@@ -199,6 +205,7 @@ public:
     ctx->saved_data["mode"] = mode;
     ctx->saved_data["maxit"] = maxit;
     ctx->saved_data["fp32"] = fp32Levels;
+    ctx->saved_data["fine64"] = fineFp64;
     ctx->saved_data["stamps"] = stampsOf(S);
     ctx->save_for_backward({kc, u});
     return u;
@@ -217,7 +224,7 @@ public:
     Tensor gk;
     if (ctx->needs_input_grad(0))
       gk = -gradKappa(S.levels[0], S.stream, u, lambda, true);
-    return {gk, lambda, Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor()};
+    return {gk, lambda, Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor()};
   }
 };
 
@@ -227,9 +234,9 @@ Tensor kappaApply(Tensor kappa, Tensor x, int64_t handle, int64_t stream, bool u
 }
 
 Tensor kappaSolve(Tensor kappa, Tensor b, std::vector<int64_t> levels, std::vector<Tensor> inject, int64_t mg,
-                  int64_t stream, double rtol, int64_t mode, int64_t maxit, bool fp32Levels)
+                  int64_t stream, double rtol, int64_t mode, int64_t maxit, bool fp32Levels, bool fineFp64)
 {
-  return KappaSolve::apply(kappa, b, levels, inject, mg, stream, rtol, mode, maxit, fp32Levels);
+  return KappaSolve::apply(kappa, b, levels, inject, mg, stream, rtol, mode, maxit, fp32Levels, fineFp64);
 }
 } // namespace
 
@@ -237,7 +244,7 @@ TORCH_LIBRARY(proxpde, m)
 {
   m.def("kappa_apply(Tensor kappa, Tensor x, int ctx, int stream, bool use_bc) -> Tensor");
   m.def("kappa_solve(Tensor kappa, Tensor b, int[] levels, Tensor[] inject, int mg, int stream, float rtol, "
-        "int mode, int maxit, bool fp32_levels) -> Tensor");
+        "int mode, int maxit, bool fp32_levels, bool fine_fp64) -> Tensor");
 }
 
 TORCH_LIBRARY_IMPL(proxpde, CompositeImplicitAutograd, m)

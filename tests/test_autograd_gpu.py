@@ -92,13 +92,14 @@ def test_kappa_solve_adjoint_gradient(ag, oracle, gpu, fe, ns):
     assert np.abs(g - g_ref).max() <= 1e-7 * np.abs(g_ref).max()
 
 
-def test_kappa_solve_fp32_refined_matches_fp64(ag, gpu):
+@pytest.mark.parametrize("fine_fp64", [False, True])
+def test_kappa_solve_fp32_refined_matches_fp64(ag, gpu, fine_fp64):
     solver = ag.KappaSolver("hex_q1", (16, 8, 4, 2))
     x = solver.probs[0].dof_coords()
     kt = torch.from_numpy(np.exp(np.sin(4 * x[:, 0]) * np.cos(3 * x[:, 1]))).to(gpu)
     b = torch.from_numpy(solver.rhs(1.0)).to(gpu)
     u64 = solver(kt, b, rtol=1e-8, mode=N.FP64)
-    u32 = solver(kt, b, rtol=1e-8, mode=N.FP32)
+    u32 = solver(kt, b, rtol=1e-8, mode=N.FP32, fine_fp64=fine_fp64)
     err = float((u32 - u64).norm() / u64.norm())
     print(f"  FP32+IR vs FP64 relative difference {err:.2e}")
     assert err < 1e-6
