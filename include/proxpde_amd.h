@@ -62,6 +62,10 @@ extern "C" {
 #define PXG_RHS_SINSINSIN 3  /* p0 sin(p1 x) sin(p2 y) sin(p3 z)        */
 #define PXG_RHS_FEVAR1D 4    /* sin(p1 x) - p0 cos(p2 x)                */
 #define PXG_RHS_QVALUES 5    /* host values per (element, qpoint)       */
+/* AssemblyProjection(p0, f_h) (assembly_rhs.hpp:94-148, 285-311): qvalues
+ * holds the n_dofs nodal values f_h in the same FE space;
+ * Fe += p0 JxW_q phi_q (phi_q . f_h,local)                               */
+#define PXG_RHS_PROJECTION 6
 
 #define PXG_FP64 0
 #define PXG_FP32 1
@@ -122,7 +126,8 @@ int pxg_build_pattern(pxg_ctx * ctx, int64_t * nnz_structural);
  * PXG_FP32 (element kernel and values in float).                          */
 int pxg_assemble(pxg_ctx * ctx, int n_terms, const pxg_term * terms, int precision);
 
-/* Builder::buildRhs with AssemblyRhsAnalytic (include/builder.hpp:279-364) */
+/* Builder::buildRhs with AssemblyRhsAnalytic or AssemblyProjection
+ * (include/builder.hpp:279-364)                                           */
 int pxg_assemble_rhs(pxg_ctx * ctx, int rhs_kind, const double * params, const double * qvalues);
 
 /* quadrature points x_q of every element (n_elem x nq x 3), for host

@@ -6,6 +6,7 @@
  *   DOF::setupElemMap (first-visit numbering)             include/dof.hpp:112-329
  *   BCEss::setupDofMap (dofs of marked facets)            include/bc.hpp:253-295
  *   dof points used by interpolateAnalyticFunction        include/fespace.hpp:195-238
+ *   readGMSH (GMSH 2.2 ASCII)                             include/mesh.hpp:785-1062
  * Pure host code, no device memory.  Status codes as in proxpde_amd.h.
  */
 #ifndef PROXPDE_AMD_HOST_H
@@ -36,6 +37,20 @@ typedef struct pxh_mesh pxh_mesh;
 
 int pxh_mesh_structured(pxh_mesh ** mesh, int kind, const double * origin, const double * length, int nx, int ny,
                         int nz);
+
+/* element geometries of an unstructured mesh */
+#define PXH_GEO_LINE 1
+#define PXH_GEO_QUAD 2
+#define PXH_GEO_HEX 3
+#define PXH_GEO_TET 4
+/* GMSH 2.2 ASCII mesh file, readGMSH (include/mesh.hpp:785-1062): the volume
+ * elements of geometry `geo` (quadratic ones read through their vertices),
+ * the points they use renumbered in file order, every boundary facet marked
+ * with the first tag of the facet element of the file on the same vertices
+ * (0 when the file lists none).  Errors: PXG_E_INVALID + pxh_last_error.  */
+int pxh_mesh_gmsh(pxh_mesh ** mesh, const char * path, int geo);
+/* message of the last failed host-layer call                              */
+const char * pxh_last_error(void);
 int pxh_mesh_destroy(pxh_mesh * mesh);
 /* sizes: points, elements, points per element, boundary facets */
 int pxh_mesh_info(const pxh_mesh * mesh, int64_t * n_pts, int64_t * n_elem, int * npe, int64_t * n_bfacets);

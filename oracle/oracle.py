@@ -23,7 +23,7 @@ FE = {"line_p1": 1, "quad_q1": 2, "quad_q2": 3, "hex_q1": 4, "hex_q2": 5, "tet_p
 MESH = {"line": 1, "square": 2, "cube_hex": 3, "cube_tet5": 4, "cube_tet6": 5}
 SIDE = {"bottom": 1, "right": 2, "top": 3, "left": 4, "back": 5, "front": 6}
 TERM = {"stiffness": 1, "stiffness_fe": 2, "mass": 3, "mass_fe": 4}
-RHS = {"none": 0, "const": 1, "sinsin": 2, "sinsinsin": 3, "fevar1d": 4, "qvalues": 5}
+RHS = {"none": 0, "const": 1, "sinsin": 2, "sinsinsin": 3, "fevar1d": 4, "qvalues": 5, "projection": 6}
 
 _lib = None
 
@@ -50,6 +50,7 @@ def lib():
         L.orc_error.restype = C.c_char_p
         L.orc_error.argtypes = [vp]
         L.orc_mesh.argtypes = [vp, i32, dp, dp, i32, i32, i32]
+        L.orc_mesh_gmsh.argtypes = [vp, C.c_char_p, i32]
         L.orc_set_coords.argtypes = [vp, dp]
         L.orc_set_qr.argtypes = [vp, i32, dp, dp]
         L.orc_info.restype = i64
@@ -95,7 +96,9 @@ def _f64(a):
 class Space:
     """Mesh + FE space + Builder state of the oracle for one problem."""
 
-    def __init__(self, fe: str, mesh: str, n, origin=(0.0, 0.0, 0.0), length=(1.0, 1.0, 1.0), qr=None):
+    def __init__(self, fe: str, mesh: str, n, origin=(0.0, 0.0, 0.0), length=(1.0, 1.0, 1.0), qr=None,
+                 gmsh=None):
+        """mesh: a structured kind with n elements per axis, or "gmsh" with gmsh = path."""
         self.L = lib()
         self.fe = fe
         self.h = self.L.orc_new(FE[fe])
@@ -105,7 +108,11 @@ class Space:
         self.n = n
         o = _f64(origin)
         ln = _f64(length)
-        rc = self.L.orc_mesh(self.h, MESH[mesh], _p(o), _p(ln), int(n[0]), int(n[1]), int(n[2]))
+        if mesh == "gmsh":
+            geo = {"line_p1": 1, "quad_q1": 2, "quad_q2": 2, "hex_q1": 3, "hex_q2": 3, "tet_p1": 4, "tet_p2": 4}[fe]
+            rc = self.L.orc_mesh_gmsh(self.h, os.fsencode(gmsh), geo)
+        else:
+            rc = self.L.orc_mesh(self.h, MESH[mesh], _p(o), _p(ln), int(n[0]), int(n[1]), int(n[2]))
         if rc != 0:
             raise RuntimeError(self.L.orc_error(self.h).decode())
         if qr is not None:

@@ -90,6 +90,22 @@ int orc_mesh(void * p, int kind, double const * origin, double const * length, i
       });
 }
 
+// mesh of a GMSH 2.2 file (geo: 1 line, 2 quad, 3 hex, 4 tet)
+int orc_mesh_gmsh(void * p, char const * path, int geo)
+{
+  auto * c = static_cast<Ctx *>(p);
+  return guard(
+      c,
+      [&]
+      {
+        readGmsh(c->mesh, path, geo);
+        if (c->mesh.geo != refFE(c->feType).geo)
+          throw std::runtime_error("mesh element does not match the FE");
+        c->dofs = buildDofMap(c->mesh, refFE(c->feType));
+        return 0;
+      });
+}
+
 // overwrite point coordinates (e.g. perturbed interior vertices)
 int orc_set_coords(void * p, double const * xyz)
 {

@@ -2,6 +2,10 @@
 // dof numbering, CurFE and Builder restated on the CPU.  See oracle_fe.hpp.
 #include "oracle_fe.hpp"
 
+#include <fstream>
+#include <map>
+#include <stdexcept>
+#include <string>
 #include <algorithm>
 #include <numeric>
 
@@ -663,6 +667,130 @@ void buildBoundaryFacets(Mesh & m, double const * origin, double const * length)
   }
 }
 
+void readGmsh(Mesh & m, std::string const & path, int geo)
+{
+  std::ifstream in(path);
+  if (!in)
+    throw std::runtime_error("cannot open " + path);
+  std::vector<std::string> tok;
+  for (std::string w; in >> w;)
+    tok.push_back(w);
+  size_t k = 0;
+  auto next = [&]() -> std::string const &
+  {
+    if (k >= tok.size())
+      throw std::runtime_error("gmsh: unexpected end of file");
+    return tok[k++];
+  };
+  if (next() != "$MeshFormat" || next() != "2.2" || next() != "0" || next() != "8" || next() != "$EndMeshFormat")
+    throw std::runtime_error("gmsh: not a 2.2 ASCII file");
+  GeoTopo const & t = geoTopo(geo);
+  // node counts of the element types of the format
+  std::map<int, int> nodes{{15, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 4}, {5, 8}, {8, 3}, {9, 6}, {10, 9}, {11, 10}, {12, 27}};
+  std::map<int, std::pair<int, int>> volType{{1, {1, 8}}, {2, {3, 10}}, {3, {5, 12}}, {4, {4, 11}}};
+  std::map<int, std::pair<int, int>> facetType{{1, {15, 15}}, {2, {1, 8}}, {3, {3, 10}}, {4, {2, 9}}};
+  auto const vt = volType.at(geo), ft = facetType.at(geo);
+  std::vector<double> pts;
+  std::vector<u32> vconn;
+  std::vector<std::pair<std::vector<u32>, int>> ffac;
+  while (k < tok.size())
+  {
+    std::string const sec = next();
+    if (sec == "$Nodes")
+    {
+      long const nn = std::stol(next());
+      for (long i = 0; i < nn; ++i)
+      {
+        if (std::stol(next()) != i + 1)
+          throw std::runtime_error("gmsh: node ids not consecutive");
+        for (int d = 0; d < 3; ++d)
+          pts.push_back(std::stod(next()));
+      }
+      if (next() != "$EndNodes")
+        throw std::runtime_error("gmsh: $EndNodes");
+    }
+    else if (sec == "$Elements")
+    {
+      long const ne = std::stol(next());
+      for (long i = 0; i < ne; ++i)
+      {
+        next(); // id
+        int const type = std::stoi(next());
+        int const ntags = std::stoi(next());
+        int tag0 = 0;
+        for (int j = 0; j < ntags; ++j)
+        {
+          int const tg = std::stoi(next());
+          if (j == 0)
+            tag0 = tg;
+        }
+        std::vector<u32> c(nodes.at(type));
+        for (auto & v: c)
+          v = (u32)(std::stol(next()) - 1);
+        if (type == vt.first || type == vt.second)
+          vconn.insert(vconn.end(), c.begin(), c.begin() + t.npts);
+        else if (type == ft.first || type == ft.second)
+          ffac.push_back({std::vector<u32>(c.begin(), c.begin() + t.facet_npts), tag0});
+      }
+      if (next() != "$EndElements")
+        throw std::runtime_error("gmsh: $EndElements");
+    }
+    else
+    {
+      std::string const end = "$End" + sec.substr(1);
+      while (next() != end)
+      {
+      }
+    }
+  }
+  // renumber the points used by volume elements, in file order
+  std::vector<char> used(pts.size() / 3, 0);
+  for (u32 v: vconn)
+    used.at(v) = 1;
+  std::vector<u32> id(used.size(), 0xffffffffu);
+  m = Mesh{};
+  m.geo = geo;
+  m.npe = t.npts;
+  for (size_t i = 0; i < used.size(); ++i)
+    if (used[i])
+    {
+      id[i] = (u32)(m.coords.size() / 3);
+      m.coords.insert(m.coords.end(), {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]});
+    }
+  for (u32 v: vconn)
+    m.conn.push_back(id[v]);
+  // boundary facets (seen once), marked from the file's facets
+  std::map<std::vector<u32>, int> mark;
+  for (auto & f: ffac)
+  {
+    std::vector<u32> key;
+    for (u32 v: f.first)
+      key.push_back(id.at(v));
+    std::sort(key.begin(), key.end());
+    mark[key] = f.second;
+  }
+  std::map<std::vector<u32>, std::pair<Facet, int>> seen;
+  for (i64 e = 0; e < m.nelem(); ++e)
+    for (int s = 0; s < t.nfacets; ++s)
+    {
+      std::vector<u32> p(t.facet_npts);
+      for (int j = 0; j < t.facet_npts; ++j)
+        p[j] = m.conn[e * m.npe + t.facets[s][j]];
+      std::vector<u32> key = p;
+      std::sort(key.begin(), key.end());
+      auto [it, ins] = seen.insert({key, {Facet{p, (u32)e, s, true, SIDE_NONE}, 0}});
+      it->second.second++;
+    }
+  for (auto & [key, fc]: seen)
+    if (fc.second == 1)
+    {
+      Facet f = fc.first;
+      auto it = mark.find(key);
+      f.marker = it == mark.end() ? SIDE_NONE : it->second;
+      m.bfacets.push_back(f);
+    }
+}
+
 // ===========================================================================
 DofMap buildDofMap(Mesh const & m, RefFE const & fe)
 {
@@ -986,6 +1114,18 @@ void Builder::buildRhs(
     std::fill(Fe.begin(), Fe.end(), 0.);
     for (int q = 0; q < nq; ++q)
     {
+      if (rhsKind == RHS_PROJECTION)
+      {
+        // AssemblyS2SProjection::build (include/assembly_rhs.hpp:134-148):
+        // Fe += coef JxW_q phi_q (phi_q . f_local)
+        double fq = 0.;
+        for (int j = 0; j < n; ++j)
+          fq += cfe.phiRef[q * n + j] * qvalues[ids[j]];
+        double const w = params[0] * cfe.JxW[q];
+        for (int i = 0; i < n; ++i)
+          Fe[i] += (w * cfe.phiRef[q * n + i]) * fq;
+        continue;
+      }
       double const f = rhsKind == RHS_QVALUES ? qvalues[e * nq + q]
                                               : evalRhs(rhsKind, params, &cfe.qpt[3 * q]);
       for (int i = 0; i < n; ++i)

@@ -9,6 +9,8 @@
 // element definitions (vertex order, basis products) and the published Gauss rules.
 #pragma once
 
+#include <string>
+
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -140,6 +142,11 @@ void buildCubeTet5(Mesh & m, double const * origin, double const * length, u32 n
 void buildCubeTet6(Mesh & m, double const * origin, double const * length, u32 nx, u32 ny, u32 nz);
 // buildFacets + markFacetsCube, reference include/mesh.hpp:226-341, :443-481
 void buildBoundaryFacets(Mesh & m, double const * origin, double const * length);
+// readGMSH, reference include/mesh.hpp:785-1062 (format 2.2 ASCII): volume
+// elements of geometry `geo` (quadratic ones through their vertices), used
+// points renumbered in file order, boundary facets marked by the first tag of
+// the file's facet element on the same vertices (0 if none)
+void readGmsh(Mesh & m, std::string const & path, int geo);
 
 // ---------------------------------------------------------------------------
 struct DofMap
@@ -206,6 +213,7 @@ enum RhsKind : int
   RHS_SINSINSIN = 3,  // p0 * sin(p1 x) * sin(p2 y) * sin(p3 z)
   RHS_FEVAR1D = 4,    // sin(p1 x) - p0 * cos(p2 x)
   RHS_QVALUES = 5,    // values given per (element, quadrature point)
+  RHS_PROJECTION = 6, // AssemblyProjection(p0, f_h): nodal values f_h of the same FE space
 };
 
 double evalRhs(int kind, double const * p, double const * x);

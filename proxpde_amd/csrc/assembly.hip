@@ -704,6 +704,23 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// AssemblyProjection: wf = (coef JxW_q, f_h(x_q)) with f_h(x_q) = phi_q . f_local
+__global__ void __launch_bounds__(kBlock)
+    k_proj_qvalues(int64_t E, int n, int nq, double const * __restrict__ phi, uint32_t const * __restrict__ emap,
+                   double const * __restrict__ fh, double coef, double * __restrict__ wf)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= E * nq)
+    return;
+  int64_t const e = t / nq;
+  int const q = (int)(t - e * nq);
+  double f = 0.;
+  for (int j = 0; j < n; ++j)
+    f += phi[q * n + j] * fh[emap[e * n + j]];
+  wf[2 * t] = coef * wf[2 * t];
+  wf[2 * t + 1] = f;
+}
+
 // Fe_i = sum_q (JxW_q phi_i(q)) f_q, zeroed on constrained dofs
 __global__ void __launch_bounds__(kBlock)
     k_rhs_rows(int64_t E, int n, int nq, double const * __restrict__ wf, double const * __restrict__ phi,
@@ -1010,7 +1027,7 @@ void assembleRhs(pxg_ctx * c, int kind, double const * params, double const * qv
 {
   PXG_REQUIRE(c->fe.feType && c->nelem && c->ndofs && c->nnzS, PXG_E_STATE,
               "assemble_rhs needs set_fe, set_mesh, set_dofmap and build_pattern first");
-  PXG_REQUIRE(kind >= PXG_RHS_CONST && kind <= PXG_RHS_QVALUES, PXG_E_INVALID, "rhs kind");
+  PXG_REQUIRE(kind >= PXG_RHS_CONST && kind <= PXG_RHS_PROJECTION, PXG_E_INVALID, "rhs kind");
   hipStream_t s = c->stream;
   FeTables & T = c->fe;
   int64_t const E = c->nelem, m = E * T.nq;
@@ -1019,21 +1036,30 @@ void assembleRhs(pxg_ctx * c, int kind, double const * params, double const * qv
     for (int k = 0; k < 4; ++k)
       prm.p[k] = params[k];
   DevBuf<double> qv;
-  if (kind == PXG_RHS_QVALUES)
+  bool const proj = kind == PXG_RHS_PROJECTION;
+  if (kind == PXG_RHS_QVALUES || proj)
   {
     PXG_REQUIRE(qvaluesHost != nullptr, PXG_E_INVALID, "qvalues required");
-    qv.upload(qvaluesHost, m, s);
+    qv.upload(qvaluesHost, proj ? c->ndofs : m, s);
   }
+  // projection: JxW from the constant-zero form, then f_h at the points
+  int const qkind = proj ? PXG_RHS_CONST : kind;
+  RhsParams const qprm = proj ? RhsParams{{0., 0., 0., 0.}} : prm;
   c->qwork.resize(2 * m);
   dispatchGeo(T.geo,
               [&](auto dimC, auto npeC)
               {
                 constexpr int D = decltype(dimC)::value, NPE = decltype(npeC)::value;
                 k_qpoints<D, NPE><<<gridFor(m), kBlock, 0, s>>>(E, T.nq, T.geo, c->xyz.p, c->conn.p,
-                                                                 T.d_geoGrad.p, T.d_xi.p, T.d_w.p, kind, prm, qv.p,
-                                                                 nullptr, c->qwork.p);
+                                                                 T.d_geoGrad.p, T.d_xi.p, T.d_w.p, qkind, qprm,
+                                                                 qv.p, nullptr, c->qwork.p);
               });
   PXG_HIP(hipGetLastError());
+  if (proj)
+  {
+    k_proj_qvalues<<<gridFor(m), kBlock, 0, s>>>(E, T.n, T.nq, T.d_phi.p, c->elemMap.p, qv.p, prm.p[0], c->qwork.p);
+    PXG_HIP(hipGetLastError());
+  }
   c->feBuf.resize(E * T.n);
   k_rhs_rows<<<gridFor(E * T.n), kBlock, 0, s>>>(E, T.n, T.nq, c->qwork.p, T.d_phi.p, c->elemMap.p, c->bcFlag.p,
                                                  c->hasBC, c->feBuf.p);

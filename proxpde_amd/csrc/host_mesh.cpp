@@ -13,6 +13,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <string>
+#include <map>
+#include <fstream>
 #include <cstring>
 #include <new>
 #include <stdexcept>
@@ -38,6 +41,7 @@ struct pxh_mesh
 
 namespace
 {
+std::string g_hostError; // message of the last failed host call
 using u32 = uint32_t;
 using u64 = uint64_t;
 
@@ -145,23 +149,26 @@ void markAll(pxh_mesh & m)
     f.marker = markerOf(m, f.elem, f.side);
 }
 
-// boundary facets of a tet mesh: facets seen exactly once
-void tetBoundary(pxh_mesh & m)
+// boundary facets of an unstructured mesh: facets seen exactly once, in order
+// of first visit (element order, then the reference side order)
+void boundaryFacets(pxh_mesh & m)
 {
-  pxg::GeoInfo const & g = pxg::geoInfo(pxg::kTet);
+  pxg::GeoInfo const & g = pxg::geoInfo(m.geo);
   int64_t const ne = m.nelem();
-  KeyMap map((size_t)ne * 4);
+  int const nf = (int)g.facets.size();
+  KeyMap map((size_t)ne * nf);
   std::vector<std::pair<u32, int>> first; // (elem, side) of the first visit
   std::vector<int> count;
-  first.reserve(ne * 2 + 16);
+  first.reserve(ne * nf / 2 + 16);
   for (int64_t e = 0; e < ne; ++e)
-    for (int s = 0; s < 4; ++s)
+    for (int s = 0; s < nf; ++s)
     {
-      u32 v[3];
-      for (int p = 0; p < 3; ++p)
-        v[p] = m.conn[e * 4 + g.facets[s][p]];
+      u32 v[4];
+      int const np = (int)g.facets[s].size();
+      for (int p = 0; p < np; ++p)
+        v[p] = m.conn[e * m.npe + g.facets[s][p]];
       u64 k0, k1;
-      packSorted(v, 3, k0, k1);
+      packSorted(v, np, k0, k1);
       bool ins;
       u32 * slot = map.find_or_insert(k0, k1, ins);
       if (ins)
@@ -176,6 +183,204 @@ void tetBoundary(pxh_mesh & m)
   for (size_t k = 0; k < first.size(); ++k)
     if (count[k] == 1)
       m.bfacets.push_back({first[k].first, first[k].second, 0});
+}
+
+// ---- GMSH 2.2 ASCII reader (the reference's readGMSH, include/mesh.hpp:785-1062)
+// element type codes of the format: linear / quadratic volume types and the
+// number of nodes listed for each
+struct GmshType
+{
+  int lin, quad, quadNodes;
+};
+GmshType gmshTypeOf(int geo)
+{
+  switch (geo)
+  {
+  case pxg::kLine: return {1, 8, 3};
+  case pxg::kQuad: return {3, 10, 9};
+  case pxg::kHex: return {5, 12, 27};
+  case pxg::kTet: return {4, 11, 10};
+  default: return {-1, -1, 0};
+  }
+}
+// facet geometry of each volume geometry (line facets are points)
+GmshType gmshFacetTypeOf(int geo)
+{
+  switch (geo)
+  {
+  case pxg::kLine: return {15, 15, 1};
+  case pxg::kQuad: return {1, 8, 3};
+  case pxg::kHex: return {3, 10, 9};
+  case pxg::kTet: return {2, 9, 6};
+  default: return {-1, -1, 0};
+  }
+}
+// ridges (edges of 3D meshes, points of 2D meshes) are skipped
+GmshType gmshRidgeTypeOf(int geo)
+{
+  switch (geo)
+  {
+  case pxg::kQuad: return {15, 15, 1};
+  case pxg::kHex:
+  case pxg::kTet: return {1, 8, 3};
+  default: return {-1, -1, 0};
+  }
+}
+int nodesOfType(int type)
+{
+  switch (type)
+  {
+  case 15: return 1;
+  case 1: return 2;
+  case 2: return 3;
+  case 3: return 4;
+  case 4: return 4;
+  case 5: return 8;
+  case 8: return 3;
+  case 9: return 6;
+  case 10: return 9;
+  case 11: return 10;
+  case 12: return 27;
+  default: return -1;
+  }
+}
+
+void readGmsh(pxh_mesh & m, char const * path)
+{
+  std::ifstream in(path);
+  if (!in)
+    throw std::runtime_error(std::string("mesh file not found: ") + path);
+  std::string buf;
+  auto expect = [&](char const * tok)
+  {
+    in >> buf;
+    if (buf != tok)
+      throw std::runtime_error(std::string("gmsh: expected ") + tok + ", found " + buf);
+  };
+  expect("$MeshFormat");
+  std::string version;
+  int filetype = -1, datasize = -1;
+  in >> version >> filetype >> datasize;
+  if (version != "2.2" || filetype != 0 || datasize != 8)
+    throw std::runtime_error("gmsh: only the 2.2 ASCII format is read");
+  expect("$EndMeshFormat");
+  GmshType const vt = gmshTypeOf(m.geo), ft = gmshFacetTypeOf(m.geo), rt = gmshRidgeTypeOf(m.geo);
+  pxg::GeoInfo const & g = pxg::geoInfo(m.geo);
+  int const npe = g.npts, nfp = (int)g.facets[0].size();
+  std::vector<double> pts;      // file nodes
+  std::vector<int> used;        // elements touching each file node
+  std::vector<u32> vconn;       // volume connectivity, file node ids
+  std::vector<std::pair<std::vector<u32>, int>> ffacets; // file facets (sorted ids, marker)
+  while (in >> buf)
+  {
+    if (buf == "$Nodes")
+    {
+      int64_t nn = 0;
+      in >> nn;
+      pts.resize(3 * nn);
+      used.assign(nn, 0);
+      for (int64_t k = 0; k < nn; ++k)
+      {
+        int64_t id;
+        in >> id >> pts[3 * k] >> pts[3 * k + 1] >> pts[3 * k + 2];
+        if (id != k + 1)
+          throw std::runtime_error("gmsh: node ids must be consecutive from 1");
+      }
+      expect("$EndNodes");
+    }
+    else if (buf == "$Elements")
+    {
+      int64_t ne = 0;
+      in >> ne;
+      for (int64_t k = 0; k < ne; ++k)
+      {
+        int64_t id;
+        int type, ntags;
+        in >> id >> type >> ntags;
+        std::vector<int> tags(ntags);
+        for (int t = 0; t < ntags; ++t)
+          in >> tags[t];
+        int const nn = nodesOfType(type);
+        if (nn < 0)
+          throw std::runtime_error("gmsh: unknown element type " + std::to_string(type));
+        std::vector<u32> c(nn);
+        for (int t = 0; t < nn; ++t)
+        {
+          int64_t v;
+          in >> v;
+          c[t] = (u32)(v - 1);
+        }
+        if (type == vt.lin || type == vt.quad)
+        {
+          // quadratic elements: the vertices come first, the rest is skipped
+          for (int t = 0; t < npe; ++t)
+          {
+            vconn.push_back(c[t]);
+            used.at(c[t])++;
+          }
+        }
+        else if (type == ft.lin || type == ft.quad)
+        {
+          std::vector<u32> f(c.begin(), c.begin() + nfp);
+          std::sort(f.begin(), f.end());
+          ffacets.push_back({f, ntags > 0 ? tags[0] : 0});
+        }
+        else if (type == rt.lin || type == rt.quad)
+          continue;
+        else
+          throw std::runtime_error("gmsh: element type " + std::to_string(type) + " does not fit the mesh");
+      }
+      expect("$EndElements");
+    }
+    else if (!buf.empty() && buf[0] == '$')
+    {
+      // any other section ($PhysicalNames, ...) is skipped to its end tag
+      std::string const end = "$End" + buf.substr(1);
+      while (in >> buf && buf != end)
+      {
+      }
+    }
+    else
+      throw std::runtime_error("gmsh: unexpected token " + buf);
+  }
+  // points touched by volume elements, renumbered in file order
+  std::vector<int64_t> newId(used.size(), -1);
+  m.xyz.clear();
+  int64_t cnt = 0;
+  for (size_t k = 0; k < used.size(); ++k)
+    if (used[k] > 0)
+    {
+      newId[k] = cnt++;
+      m.xyz.insert(m.xyz.end(), {pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]});
+    }
+  m.npe = npe;
+  m.conn.resize(vconn.size());
+  for (size_t k = 0; k < vconn.size(); ++k)
+    m.conn[k] = (u32)newId[vconn[k]];
+  boundaryFacets(m);
+  // markers of the boundary facets listed in the file (matched by vertex set)
+  std::map<std::vector<u32>, int> fileMarker;
+  for (auto & f: ffacets)
+  {
+    std::vector<u32> key;
+    for (u32 v: f.first)
+    {
+      if (newId.at(v) < 0)
+        throw std::runtime_error("gmsh: facet on a node of no element");
+      key.push_back((u32)newId[v]);
+    }
+    std::sort(key.begin(), key.end());
+    fileMarker[key] = f.second;
+  }
+  for (auto & b: m.bfacets)
+  {
+    std::vector<u32> key;
+    for (int p: g.facets[b.side])
+      key.push_back(m.conn[(size_t)b.elem * m.npe + p]);
+    std::sort(key.begin(), key.end());
+    auto it = fileMarker.find(key);
+    b.marker = it == fileMarker.end() ? 0 : it->second;
+  }
 }
 
 } // namespace
@@ -293,7 +498,7 @@ int pxh_mesh_structured(pxh_mesh ** out, int kind, double const * origin, double
                 m->conn.insert(m->conn.end(), {c[r[0]], c[r[1]], c[r[2]], c[r[3]]});
           }
       if (!hex)
-        tetBoundary(*m);
+        boundaryFacets(*m);
       markAll(*m);
       break;
     }
@@ -311,6 +516,33 @@ int pxh_mesh_structured(pxh_mesh ** out, int kind, double const * origin, double
     return PXG_E_INVALID;
   }
 }
+
+int pxh_mesh_gmsh(pxh_mesh ** out, char const * path, int geo)
+{
+  if (!out || !path || geo < pxg::kLine || geo > pxg::kTet)
+    return PXG_E_INVALID;
+  *out = nullptr;
+  auto * m = new (std::nothrow) pxh_mesh;
+  if (!m)
+    return PXG_E_OOM;
+  try
+  {
+    m->geo = geo;
+    for (int k = 0; k < 3; ++k)
+      m->origin[k] = 0., m->length[k] = 1.;
+    readGmsh(*m, path);
+  }
+  catch (std::exception const & e)
+  {
+    g_hostError = e.what();
+    delete m;
+    return PXG_E_INVALID;
+  }
+  *out = m;
+  return PXG_OK;
+}
+
+const char * pxh_last_error(void) { return g_hostError.c_str(); }
 
 int pxh_mesh_destroy(pxh_mesh * m)
 {

@@ -19,7 +19,7 @@ FE_DIM = {1: 1, 2: 2, 3: 2, 4: 3, 5: 3, 6: 3, 7: 3}
 MESH = {"line": 1, "square": 2, "cube_hex": 3, "cube_tet5": 4, "cube_tet6": 5}
 SIDE = {"bottom": 1, "right": 2, "top": 3, "left": 4, "back": 5, "front": 6}
 TERM = {"stiffness": 1, "stiffness_fe": 2, "mass": 3, "mass_fe": 4}
-RHS = {"none": 0, "const": 1, "sinsin": 2, "sinsinsin": 3, "fevar1d": 4, "qvalues": 5}
+RHS = {"none": 0, "const": 1, "sinsin": 2, "sinsinsin": 3, "fevar1d": 4, "qvalues": 5, "projection": 6}
 FP64, FP32 = 0, 1
 
 _lib = None
@@ -56,6 +56,7 @@ def lib():
     pp = C.POINTER(C.c_void_p)
     sig = {
         "pxh_mesh_structured": [pp, i32, dp, dp, i32, i32, i32],
+        "pxh_mesh_gmsh": [pp, C.c_char_p, i32],
         "pxh_mesh_destroy": [vp],
         "pxh_mesh_info": [vp, i64p, i64p, C.POINTER(C.c_int), i64p],
         "pxh_mesh_get": [vp, dp, u32p],
@@ -106,6 +107,8 @@ def lib():
     L.pxg_last_error.argtypes = [vp]
     L.pxg_mg_last_error.restype = C.c_char_p
     L.pxg_mg_last_error.argtypes = [vp]
+    L.pxh_last_error.restype = C.c_char_p
+    L.pxh_last_error.argtypes = []
     _lib = L
     return L
 
@@ -125,23 +128,41 @@ def _check(rc, ctx=None):
 
 
 # ---------------------------------------------------------------------------
-class HostMesh:
-    """Structured mesh + dof numbering of the host layer (pxh_*)."""
+GEO = {"line": 1, "quad": 2, "hex": 3, "tet": 4}
 
-    def __init__(self, kind: str, n, origin=(0.0, 0.0, 0.0), length=(1.0, 1.0, 1.0)):
+
+class HostMesh:
+    """Mesh + dof numbering of the host layer (pxh_*): structured
+    (HostMesh(kind, n, ...)) or read from a GMSH 2.2 file (HostMesh.gmsh)."""
+
+    def __init__(self, kind: str, n, origin=(0.0, 0.0, 0.0), length=(1.0, 1.0, 1.0), _handle=None):
         L = lib()
         n = list(n) + [0] * (3 - len(n))
         self.kind = kind
         self.n = n
         self.origin = _f64(origin)
         self.length = _f64(length)
-        h = C.c_void_p()
-        _check(L.pxh_mesh_structured(C.byref(h), MESH[kind], _p(self.origin), _p(self.length), n[0], n[1], n[2]))
+        if _handle is None:
+            h = C.c_void_p()
+            _check(L.pxh_mesh_structured(C.byref(h), MESH[kind], _p(self.origin), _p(self.length), n[0], n[1],
+                                         n[2]))
+        else:
+            h = _handle
         self.h = h
         npts, nelem, nbf = C.c_int64(), C.c_int64(), C.c_int64()
         npe = C.c_int()
         L.pxh_mesh_info(h, C.byref(npts), C.byref(nelem), C.byref(npe), C.byref(nbf))
         self.npts, self.nelem, self.npe, self.nbfacets = npts.value, nelem.value, npe.value, nbf.value
+
+    @classmethod
+    def gmsh(cls, path: str, geo: str):
+        """readGMSH: volume elements of geometry `geo` ("line", "quad", "hex", "tet")."""
+        L = lib()
+        h = C.c_void_p()
+        rc = L.pxh_mesh_gmsh(C.byref(h), os.fsencode(path), GEO[geo])
+        if rc != 0:
+            raise PxgError(rc, L.pxh_last_error().decode())
+        return cls("gmsh:" + geo, (0, 0, 0), _handle=h)
 
     def __del__(self):
         try:

@@ -66,7 +66,9 @@ def grf(points, seed, std=1.0, ell=0.1, features=1024, device=None):
 class Problem:
     """Mesh, dof map and boundary data of one structured workload (host layer)."""
 
-    def __init__(self, fe: str, n, mesh: str | None = None, origin=(0.0, 0.0, 0.0), length=None):
+    def __init__(self, fe: str, n, mesh: str | None = None, origin=(0.0, 0.0, 0.0), length=None, gmsh=None):
+        """Structured mesh `mesh` with n elements per axis, or (gmsh = path) the
+        mesh of a GMSH 2.2 file with the geometry of `fe`."""
         self.fe = fe
         self.mesh_kind = mesh or MESH_OF_FE[fe]
         dim = N.FE_DIM[N.FE[fe]]
@@ -76,7 +78,13 @@ class Problem:
         self.origin = origin
         self.length = length
         self.n = list(n)
-        self.mesh = N.HostMesh(self.mesh_kind, n, origin, length)
+        if gmsh is not None:
+            geo = {"line_p1": "line", "quad_q1": "quad", "quad_q2": "quad", "hex_q1": "hex", "hex_q2": "hex",
+                   "tet_p1": "tet", "tet_p2": "tet"}[fe]
+            self.mesh_kind = "gmsh"
+            self.mesh = N.HostMesh.gmsh(gmsh, geo)
+        else:
+            self.mesh = N.HostMesh(self.mesh_kind, n, origin, length)
         self.xyz, self.conn = self.mesh.arrays()
         self.ndofs, self.emap = self.mesh.dofmap(fe)
 

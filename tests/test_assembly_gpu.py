@@ -327,3 +327,65 @@ def test_fp32_build_keeps_fp64_sparsity(gpu, fe, n, sides):
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     err = np.abs(out[1][2].astype(np.float64) - out[0][2]).max() / np.abs(out[0][2]).max()
     assert err < 1e-6
+
+
+def _gpu_projection_error(prob):
+    a = prob.assembler()
+    d = prob.bc_dofs(("left", "bottom"))
+    a.set_dirichlet(d, np.zeros(len(d)))
+    a.build_pattern()
+    a.assemble([("stiffness", 1.0)])
+    x = prob.dof_coords()
+    f = 2.5 * PI * PI * np.sin(0.5 * PI * x[:, 0]) * np.sin(1.5 * PI * x[:, 1])
+    a.assemble_rhs("projection", (1.0,), qvalues=f)
+    a.close()
+    rp, col, v, b = a.csr()
+    u = spl.spsolve(sp.csr_matrix((v, col, rp), shape=(prob.ndofs, prob.ndofs)).tocsc(), b)
+    return np.linalg.norm(u - np.sin(0.5 * PI * x[:, 0]) * np.sin(1.5 * PI * x[:, 1]))
+
+
+def test_known_answer_gmsh_tet_through_gpu(gpu):
+    """data/poisson3dtet.yaml:11 with the GPU-assembled system (GMSH mesh, P1
+    tets, projection right-hand side)."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "cube_uns.msh")
+    err = _gpu_projection_error(W.Problem("tet_p1", (0, 0, 0), gmsh=path))
+    assert abs(err - 0.88528925443) < 1e-10, err
+
+
+@pytest.mark.parametrize("n,expected", [(4, 0.08097818482375403), (8, 0.02265721054353444)])
+def test_known_answer_tet_p2_projection_through_gpu(gpu, n, expected):
+    """test/poisson3dtet_p2.cpp:124,134 with the GPU-assembled system (5-tet split)."""
+    err = _gpu_projection_error(W.Problem("tet_p2", (n, n, n), mesh="cube_tet5"))
+    assert abs(err - expected) < 1e-12, err
+
+
+def test_gmsh_assembly_matches_oracle(oracle, gpu):
+    """Unstructured P2 tets (GMSH mesh): pattern bit-exact, values and projection
+    right-hand side within 1e-12 of the oracle."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "cube_uns.msh")
+    prob = W.Problem("tet_p2", (0, 0, 0), gmsh=path)
+    o = oracle.Space("tet_p2", "gmsh", (0, 0, 0), gmsh=path)
+    x = prob.dof_coords()
+    kappa = np.exp(W.grf(x, seed=2, std=1.0, ell=0.3, features=64))
+    f = np.cos(x[:, 0]) + x[:, 1] * x[:, 2]
+    dofs = prob.bc_dofs(("left", "front"))
+    vals = _bc_values(x)[dofs]
+    flag = np.zeros(prob.ndofs, np.uint8)
+    flag[dofs] = 1
+    full = np.zeros(prob.ndofs)
+    full[dofs] = vals
+    terms = [("stiffness_fe", 1.0), ("mass", 1.0)]
+    rp_o, col_o, v_o, b_o = o.assemble(terms, kappa=kappa, bc=(flag, full), rhs="projection", rhs_params=(2.0,),
+                                       qvalues=f)
+    a = prob.assembler()
+    a.set_dirichlet(dofs, vals)
+    a.set_coefficient(kappa)
+    a.build_pattern()
+    a.assemble(terms)
+    a.assemble_rhs("projection", (2.0,), qvalues=f)
+    a.close()
+    rp, col, v, b = a.csr()
+    assert np.array_equal(rp, rp_o) and np.array_equal(col, col_o.astype(np.int32))
+    assert _rel_ok(v, v_o)[0] and _rel_ok(b, b_o)[0]

@@ -54,3 +54,36 @@ def test_large_q1_hex_numbering_matches_closed_form(native):
     perm = np.full(m.npts, -1)
     perm[conn.ravel()] = emap.ravel()
     assert (perm >= 0).all() and len(np.unique(perm)) == m.npts
+
+
+@pytest.mark.parametrize("fe", ["tet_p1", "tet_p2"])
+def test_gmsh_mesh_and_dofmap_bit_exact(oracle, native, fe):
+    """readGMSH on data/cube_uns.msh (fixture): points, elements, boundary
+    facets with the file's markers, dof numbering and Dirichlet sets."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "cube_uns.msh")
+    o = oracle.Space(fe, "gmsh", (0, 0, 0), gmsh=path)
+    m = native.HostMesh.gmsh(path, "tet")
+    xo, co = o.mesh()
+    xp, cp = m.arrays()
+    assert xp.shape == (237, 3) and cp.shape == (714, 4)
+    assert np.array_equal(xo.view(np.uint64), xp.view(np.uint64))
+    assert np.array_equal(co, cp)
+    nd, emap = m.dofmap(fe)
+    assert nd == o.ndofs
+    assert np.array_equal(emap, o.elem_map())
+    for sd in ("bottom", "right", "top", "left", "back", "front"):
+        assert np.array_equal(m.bc_dofs(fe, emap, sd), o.bc_dofs(sd)), sd
+    bo = {tuple(r) for r in o.bfacets()}
+    bp = {tuple(r) for r in m.bfacets()}
+    assert bo == bp and len(bp) == 408
+    assert sorted({r[2] for r in bp}) == [1, 2, 3, 4, 5, 6]
+
+
+def test_gmsh_errors(native, tmp_path):
+    bad = tmp_path / "bad.msh"
+    bad.write_text("$MeshFormat\n4.1 0 8\n$EndMeshFormat\n")
+    with pytest.raises(native.PxgError, match="2.2"):
+        native.HostMesh.gmsh(str(bad), "tet")
+    with pytest.raises(native.PxgError, match="not found"):
+        native.HostMesh.gmsh(str(tmp_path / "missing.msh"), "tet")

@@ -51,6 +51,39 @@ def test_known_answer_poisson(oracle, fe, mesh, n, length, qr, expected, ref):
     assert abs(err - expected) < 1e-12, (err, expected, ref)
 
 
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+
+
+def _poisson_projection(O, fe, mesh, n, gmsh=None):
+    """test/poisson3dtet*.cpp: stiffness(1), u = 0 on LEFT and BOTTOM, rhs =
+    AssemblyProjection(1, I_h f) with f = 2.5 pi^2 sin(pi x / 2) sin(3 pi y / 2)."""
+    s = O.Space(fe, mesh, n, gmsh=gmsh)
+    flag = np.zeros(s.ndofs, np.uint8)
+    for side in ("left", "bottom"):
+        flag[s.bc_dofs(side)] = 1
+    x = s.dof_coords()
+    f = 2.5 * PI * PI * np.sin(0.5 * PI * x[:, 0]) * np.sin(1.5 * PI * x[:, 1])
+    rp, col, v, b = s.assemble([("stiffness", 1.0)], bc=(flag, np.zeros(s.ndofs)), rhs="projection",
+                               rhs_params=(1.0,), qvalues=f)
+    u = _solve(rp, col, v, b)
+    return np.linalg.norm(u - np.sin(0.5 * PI * x[:, 0]) * np.sin(1.5 * PI * x[:, 1]))
+
+
+def test_known_answer_gmsh_tet(oracle):
+    """P1 tets read from data/cube_uns.msh (fixture tests/golden/cube_uns.msh),
+    expected error of data/poisson3dtet.yaml:11."""
+    import os
+    err = _poisson_projection(oracle, "tet_p1", "gmsh", (0, 0, 0), gmsh=os.path.join(GOLDEN, "cube_uns.msh"))
+    assert abs(err - 0.88528925443) < 1e-10, err
+
+
+@pytest.mark.parametrize("n,expected", [(4, 0.08097818482375403), (8, 0.02265721054353444)])
+def test_known_answer_tet_p2_projection(oracle, n, expected):
+    """P2 tets on the reference's 5-tet cube split (test/poisson3dtet_p2.cpp:124,134)."""
+    err = _poisson_projection(oracle, "tet_p2", "cube_tet5", (n, n, n))
+    assert abs(err - expected) < 1e-12, err
+
+
 def test_known_answer_fevar(oracle):
     """Variable-coefficient stiffness + mass, 1D (test/fevar.cpp:58-104)."""
     s = oracle.Space("line_p1", "line", (20,), length=(1, 0, 0))
