@@ -417,6 +417,14 @@ def main():
     spmv32_ms = time_spmv(x32, y32, N.FP32)
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     spmv32_gbs = spmv32_bytes / (spmv32_ms * 1e-3) / 1e9
+    # the block-packed copy (18-bit column offsets, what the multigrid levels use)
+    a.pack(True)
+    pk_ms = time_spmv(x, y, N.FP64)
+    pk32_ms = time_spmv(x32, y32, N.FP32)
+    pk_active = a.packed()
+    a.pack(False)
+    pk_bytes = nnz * 10.25 + 8 * (prob.ndofs + 1) + 16 * prob.ndofs
+    pk32_bytes = nnz * 6.25 + 8 * (prob.ndofs + 1) + 8 * prob.ndofs
     # the row-group kernel on the same matrix, for comparison
     os.environ["PXG_SPMV_KERNEL"] = "rowgroup"
     rg_ms = time_spmv(x, y, N.FP64)
@@ -528,7 +536,10 @@ def main():
         "host_setup_s": t_setup_host,
         "spmv": {"ms": spmv_ms, "GB/s": spmv_gbs, "frac_hbm": spmv_gbs / HBM_PEAK_GBS,
                  "fp32_ms": spmv32_ms, "fp32_GB/s": spmv32_gbs, "bytes": spmv_bytes,
-                 "rowgroup_kernel_ms": rg_ms, "rowgroup_kernel_fp32_ms": rg32_ms},
+                 "rowgroup_kernel_ms": rg_ms, "rowgroup_kernel_fp32_ms": rg32_ms,
+                 "packed": {"active": pk_active, "ms": pk_ms, "GB/s": pk_bytes / (pk_ms * 1e-3) / 1e9,
+                            "fp32_ms": pk32_ms, "fp32_GB/s": pk32_bytes / (pk32_ms * 1e-3) / 1e9,
+                            "bytes": pk_bytes, "fp32_bytes": pk32_bytes}},
         "device_copy": copy_cal,
         "reference_flops_per_elem": q2hex_flops_per_elem(),
         "reference_gflops_per_s": q2hex_flops_per_elem() * value / 1e9,

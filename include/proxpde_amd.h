@@ -166,6 +166,18 @@ int pxg_rhs_device(pxg_ctx * ctx, double ** b_dev);
  * No counterpart in the reference, whose matrices are plain values; it is
  * for callers that cache per-matrix state (autograd solve, multigrid).     */
 int pxg_matrix_stamp(pxg_ctx * ctx, uint64_t * stamp);
+/* enable 1: pxg_spmv / pxg_jacobi (and the multigrid on this level) read a
+ * block-packed copy of the closed matrix, made on first use after each
+ * close: one base column per block of <= 2048 entries, 18-bit column
+ * offsets, so an entry streams 6.25 (FP32) / 10.25 (FP64) bytes instead of
+ * 8 / 12; results are bitwise those of the plain CSR.  Skipped (plain CSR)
+ * when some block spans 2^18 columns or more.  enable 0: plain CSR;
+ * negative: unchanged.  *active (may be NULL): 1 if the packed copy of the
+ * current closed matrix exists (it is made by the first SpMV / Jacobi).
+ * No counterpart in the reference (Eigen's SpMV reads its CSR arrays); it
+ * is for callers that apply one matrix many times.  pxg_mg_create enables
+ * it on its levels.                                                        */
+int pxg_csr_pack(pxg_ctx * ctx, int enable, int * active);
 
 /* ---- matrix-free y = K(kappa) x and its gradients ------------------------
  * nb vectors interleaved as [n_dofs][nb]; use_bc applies the Dirichlet form
@@ -182,7 +194,9 @@ typedef struct pxg_mg pxg_mg;
  * element counts halve per level; dims: n_lev x 3 element counts.
  * Weighted Jacobi (omega, nu1 pre / nu2 post sweeps), R = P^T, coarsest
  * level solved directly in FP64 (dense inverse of its free-dof block, up to
- * 4096 free dofs; larger: Jacobi-CG to coarse_rtol).                      */
+ * 4096 free dofs; larger: Jacobi-CG to coarse_rtol).  Enables
+ * pxg_csr_pack on every level (pxg_csr_pack(level, 0) afterwards turns it
+ * off again).                                                              */
 int pxg_mg_create(pxg_mg ** mg, int n_lev, pxg_ctx ** levels, const int * dims, double omega, int nu1, int nu2,
                   double coarse_rtol);
 int pxg_mg_destroy(pxg_mg * mg);

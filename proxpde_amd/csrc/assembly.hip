@@ -1214,6 +1214,7 @@ void closeMatrix(pxg_ctx * c, double eps)
   c->val64Valid = !f32;
   c->closed = true;
   ++c->stamp;
+  ++c->patStamp; // rowptr / col rewritten
   c->layoutValid = eps == kDefaultPrune;
   c->layoutPrecision = c->valPrecision;
   if (c->layoutValid)

@@ -409,11 +409,7 @@ int pxg_spmv(pxg_ctx * c, int precision, const void * x, void * y)
                  {
                    PXG_REQUIRE(c->closed, PXG_E_STATE, "close before spmv");
                    PXG_REQUIRE(x && y && x != y, PXG_E_INVALID, "x and y must be distinct device buffers");
-                   if (precision == PXG_FP32)
-                     ensureVal32(c);
-                   else
-                     ensureVal64(c);
-                   ensureRowBlocks(c);
+                   ensureOperator(c, precision == PXG_FP32 ? PXG_FP32 : PXG_FP64);
                    CsrView const A = viewOf(c);
                    if (precision == PXG_FP32)
                      spmv<float>(A, (float const *)x, (float *)y, c->stream);
@@ -429,11 +425,7 @@ int pxg_jacobi(pxg_ctx * c, int precision, double omega, int sweeps, const void 
                  {
                    PXG_REQUIRE(c->closed, PXG_E_STATE, "close before jacobi");
                    PXG_REQUIRE(b && x && sweeps >= 0, PXG_E_INVALID, "jacobi arguments");
-                   if (precision == PXG_FP32)
-                     ensureVal32(c);
-                   else
-                     ensureVal64(c);
-                   ensureRowBlocks(c);
+                   ensureOperator(c, precision == PXG_FP32 ? PXG_FP32 : PXG_FP64);
                    CsrView const A = viewOf(c);
                    int64_t const n = c->ndofs;
                    if (precision == PXG_FP32)
@@ -497,6 +489,18 @@ int pxg_matrix_stamp(pxg_ctx * c, uint64_t * stamp)
                  {
                    PXG_REQUIRE(stamp, PXG_E_INVALID, "null stamp");
                    *stamp = c->stamp;
+                 });
+}
+
+int pxg_csr_pack(pxg_ctx * c, int enable, int * active)
+{
+  return guarded(c,
+                 [&]
+                 {
+                   if (enable >= 0)
+                     c->packWanted = enable != 0;
+                   if (active)
+                     *active = c->closed && c->packWanted && c->packOk && c->packStamp == c->patStamp ? 1 : 0;
                  });
 }
 

@@ -176,9 +176,21 @@ struct pxg_ctx
   pxg::DevBuf<float> val32;
   pxg::DevBuf<double> dinv;
   pxg::DevBuf<float> dinv32;
+  uint64_t patStamp = 0; // counts changes of the closed pattern (rowptr, col)
   pxg::DevBuf<int32_t> blockRow; // row blocks of the stream SpMV kernel
   int64_t nStreamBlocks = 0;
-  uint64_t blockStamp = 0; // matrix stamp the row blocks were built for (0: none)
+  uint64_t blockStamp = 0; // pattern stamp the row blocks were built for (0: none)
+  // block-packed copy of the closed matrix for the SpMV / Jacobi kernels
+  bool packWanted = false;
+  bool packOk = false;      // every row block spans fewer than 2^18 columns
+  uint64_t packStamp = 0;   // pattern stamp of the index part
+  uint64_t packValStamp = 0, packVal32Stamp = 0; // matrix stamps of the value parts
+  pxg::DevBuf<int32_t> packBase;
+  pxg::DevBuf<uint16_t> packLo;
+  pxg::DevBuf<uint16_t> packHi;
+  pxg::DevBuf<int> packFlag;
+  pxg::DevBuf<double> packVal;
+  pxg::DevBuf<float> packVal32;
   // work buffers
   pxg::DevBuf<double> geoBuf; // E*nq*8
   pxg::DevBuf<double> keBuf;  // E*n*n (fp64)

@@ -38,6 +38,14 @@ struct CsrView
   // at most kStreamSlots entries); null: the row-group kernel is used
   int32_t const * blockRow = nullptr;
   int64_t nblocks = 0;
+  // block-packed copy for the stream kernel (null: not built or not current):
+  // column of slot k of block q = pbase[q] + (plo[k] | hi2(k) << 16), values
+  // in kStreamSlots slots per block
+  int32_t const * pbase = nullptr;
+  uint16_t const * plo = nullptr;
+  uint16_t const * phi = nullptr;
+  double const * pval = nullptr;
+  float const * pval32 = nullptr;
 };
 
 // entries staged in LDS by one block of the stream kernel
@@ -46,8 +54,15 @@ constexpr int kStreamSlots = 2048;
 // when some row is too long for the stream kernel.  Synchronises the stream.
 void buildRowBlocks(int64_t nrows, int64_t nnz, int64_t const * rowptr, DevBuf<int32_t> & blockRow,
                     int64_t & nblocks, hipStream_t s);
-// row blocks of the closed matrix of a context, built on first use after a close
+// row blocks of the closed matrix of a context, built on first use after a
+// change of the closed pattern
 void ensureRowBlocks(pxg_ctx * c);
+// block-packed copy of the closed matrix in `precision` when the context asks
+// for one (pxg_csr_pack) and every block spans fewer than 2^18 columns
+void ensurePack(pxg_ctx * c, int precision);
+// values in `precision`, row blocks and (if asked for) the packed copy: what
+// the SpMV / Jacobi kernels read
+void ensureOperator(pxg_ctx * c, int precision);
 
 CsrView viewOf(pxg_ctx const * c);
 

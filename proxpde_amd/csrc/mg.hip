@@ -650,6 +650,9 @@ int pxg_mg_create(pxg_mg ** out, int nlev, pxg_ctx ** levels, const int * dims, 
     for (int l = 0; l < nlev; ++l)
     {
       PXG_REQUIRE(levels[l] && levels[l]->closed, PXG_E_STATE, "levels must be closed contexts");
+      // every level matrix is applied several times per cycle: the levels
+      // keep block-packed copies (pxg_csr_pack) for the smoother and residual
+      levels[l]->packWanted = true;
       m->ctx.push_back(levels[l]);
       m->dims.push_back({dims[3 * l], dims[3 * l + 1], dims[3 * l + 2]});
     }
@@ -747,20 +750,18 @@ int pxg_mg_solve(pxg_mg * m, int mode, const double * b, double * x, double rtol
     for (auto * c: m->ctx)
       PXG_REQUIRE(c->closed, PXG_E_STATE, "levels must be closed");
     factorCoarse(m);
-    for (auto * c: m->ctx)
-      ensureRowBlocks(c);
     double rr = 0.;
     int it = 0;
     if (mode == PXG_FP32)
     {
       for (auto * c: m->ctx)
-        ensureVal32(c);
+        ensureOperator(c, PXG_FP32);
       it = solveIR(m, b, x, rtol, maxit, rr);
     }
     else
     {
       for (auto * c: m->ctx)
-        ensureVal64(c);
+        ensureOperator(c, PXG_FP64);
       it = pcg<double>(m, b, x, rtol, maxit, rr, false);
     }
     PXG_HIP(hipStreamSynchronize(m->s));
@@ -794,8 +795,7 @@ int pxg_mg_vcycle(pxg_mg * m, const double * b, double * x)
     for (auto * c: m->ctx)
     {
       PXG_REQUIRE(c->closed, PXG_E_STATE, "levels must be closed");
-      ensureVal64(c);
-      ensureRowBlocks(c);
+      ensureOperator(c, PXG_FP64);
     }
     factorCoarse(m);
     vcycle<double>(m, 0, b, x);

@@ -9,6 +9,11 @@
 //    window of `cap` entries (at most kStreamSlots entries in all), streams
 //    their column/value arrays fully coalesced, multiplies by the gathered x
 //    into LDS, then reduces each row from LDS with a group of 8 lanes;
+//  * packed stream kernel: the same blocks and reduction over a block-packed
+//    copy of the matrix (one base column per block, 16 + 2 bits of column
+//    offset per entry, values in fixed slots of kStreamSlots per block), so
+//    an entry streams 6.25 (FP32) or 10.25 (FP64) bytes instead of 8 or 12;
+//    the products, and so the results, are bitwise those of the stream kernel;
 //  * row-group kernel (no row-block table, or rows too long): one group of
 //    LPR lanes per row, lanes stride the row, butterfly reduction.
 #include "kernels.hpp"
@@ -108,6 +113,43 @@ __device__ inline void csrOut(int64_t row, T s, T const * __restrict__ x, T cons
     y[row] += s;
 }
 
+// row sums of the products staged in LDS (entry k of the block in prod[k]):
+// G lanes per row (chosen from the mean row length); the lanes of one group
+// run the same rows, so the group stays converged
+template <typename T, int MODE, int G>
+__device__ inline void streamReduce(T const * prod, int64_t r0, int64_t r1, int64_t base,
+                                    int64_t const * __restrict__ rowptr, T const * __restrict__ x,
+                                    T const * __restrict__ b, T const * __restrict__ dinv, T omega,
+                                    T * __restrict__ y)
+{
+  int const t = threadIdx.x;
+  int const lane = t % G;
+  for (int64_t row = r0 + t / G; row < r1; row += kBlock / G)
+  {
+    int const k0 = (int)(rowptr[row] - base), k1 = (int)(rowptr[row + 1] - base);
+    // rows up to kRed * G entries in one unrolled pass of independent reads
+    constexpr int kRed = G >= 8 ? 16 : 8;
+    T s = T(0);
+#pragma unroll
+    for (int j = 0; j < kRed; ++j)
+    {
+      int const k = k0 + lane + j * G;
+      T const v = prod[min(k, kStreamSlots - 1)];
+      s += k < k1 ? v : T(0);
+    }
+    for (int k = k0 + lane + kRed * G; k < k1; k += G)
+      s += prod[k];
+    if constexpr (G > 1)
+    {
+#pragma unroll
+      for (int off = G / 2; off > 0; off >>= 1)
+        s += __shfl_xor(s, off, G);
+    }
+    if (lane == 0)
+      csrOut<T, MODE>(row, s, x, b, dinv, omega, y);
+  }
+}
+
 template <typename T, int MODE, int G>
 __global__ void __launch_bounds__(kBlock)
     k_csr_stream(int32_t const * __restrict__ blockRow, int64_t const * __restrict__ rowptr,
@@ -115,7 +157,6 @@ __global__ void __launch_bounds__(kBlock)
                  T const * __restrict__ b, T const * __restrict__ dinv, T omega, T * __restrict__ y)
 {
   constexpr int kSlots = kStreamSlots / kBlock; // entries per thread
-  // G lanes per row in the reduction (chosen from the mean row length)
   __shared__ T prod[kStreamSlots];
   int64_t const r0 = blockRow[blockIdx.x], r1 = blockRow[blockIdx.x + 1];
   if (r0 == r1)
@@ -144,32 +185,124 @@ __global__ void __launch_bounds__(kBlock)
       prod[t + m * kBlock] = base + t + m * kBlock < end ? vv[m] * xv[m] : T(0);
   }
   __syncthreads();
-  int const lane = t % G;
-  // the lanes of one group run the same rows, so the group stays converged
-  for (int64_t row = r0 + t / G; row < r1; row += kBlock / G)
+  streamReduce<T, MODE, G>(prod, r0, r1, base, rowptr, x, b, dinv, omega, y);
+}
+
+// packed stream kernel: thread t takes the slots t + m kBlock of its block
+// (m < 8) like the stream kernel, so value, offset and x loads of a wave are
+// consecutive; the 2 high offset bits of those eight slots share one 16-bit
+// word.  Slot indices are clamped into the block's entries.
+template <typename T, int MODE, int G>
+__global__ void __launch_bounds__(kBlock)
+    k_csr_packed(int32_t const * __restrict__ blockRow, int64_t const * __restrict__ rowptr,
+                 int32_t const * __restrict__ pbase, uint16_t const * __restrict__ plo,
+                 uint16_t const * __restrict__ phi, T const * __restrict__ pval, T const * __restrict__ x,
+                 T const * __restrict__ b, T const * __restrict__ dinv, T omega, T * __restrict__ y)
+{
+  constexpr int kSlots = kStreamSlots / kBlock;
+  static_assert(kSlots == 8, "eight 2-bit fields per 16-bit word");
+  __shared__ T prod[kStreamSlots];
+  int64_t const r0 = blockRow[blockIdx.x], r1 = blockRow[blockIdx.x + 1];
+  if (r0 == r1)
+    return;
+  int64_t const base = rowptr[r0];
+  int const len = (int)(rowptr[r1] - base);
+  int const t = threadIdx.x;
+  if (len > 0) // block-uniform
   {
-    int const k0 = (int)(rowptr[row] - base), k1 = (int)(rowptr[row + 1] - base);
-    // rows up to kRed * G entries in one unrolled pass of independent reads
-    constexpr int kRed = G >= 8 ? 16 : 8;
-    T s = T(0);
+    size_t const blk = (size_t)blockIdx.x * kStreamSlots;
+    uint32_t const hi = phi[(size_t)blockIdx.x * kBlock + t];
+    int32_t const cb = pbase[blockIdx.x];
+    T vv[kSlots];
+    int32_t cc[kSlots];
 #pragma unroll
-    for (int j = 0; j < kRed; ++j)
+    for (int m = 0; m < kSlots; ++m)
     {
-      int const k = k0 + lane + j * G;
-      T const v = prod[min(k, kStreamSlots - 1)];
-      s += k < k1 ? v : T(0);
+      int const k = min(t + m * kBlock, len - 1);
+      vv[m] = __builtin_nontemporal_load(pval + blk + k);
+      uint32_t const lo = __builtin_nontemporal_load(plo + blk + k);
+      // a slot past the entries reads the low bits of the last entry and its
+      // own high-bit field, which is zero: a column between cb and that
+      // entry's column, so in range; its product is masked below
+      cc[m] = cb + (int32_t)(lo | (((hi >> (2 * m)) & 3u) << 16));
     }
-    for (int k = k0 + lane + kRed * G; k < k1; k += G)
-      s += prod[k];
-    if constexpr (G > 1)
-    {
+    T xv[kSlots];
 #pragma unroll
-      for (int off = G / 2; off > 0; off >>= 1)
-        s += __shfl_xor(s, off, G);
-    }
-    if (lane == 0)
-      csrOut<T, MODE>(row, s, x, b, dinv, omega, y);
+    for (int m = 0; m < kSlots; ++m)
+      xv[m] = x[cc[m]];
+#pragma unroll
+    for (int m = 0; m < kSlots; ++m)
+      prod[t + m * kBlock] = t + m * kBlock < len ? vv[m] * xv[m] : T(0);
   }
+  __syncthreads();
+  streamReduce<T, MODE, G>(prod, r0, r1, base, rowptr, x, b, dinv, omega, y);
+}
+
+// index part of the block-packed copy: per block the smallest column, per
+// slot the low 16 bits of (col - base), and for thread t of the kernel the 2
+// high bits of its slots t + m kBlock in bits 2m of one 16-bit word;
+// bad = 1 when some block spans 2^18 columns or more
+__global__ void __launch_bounds__(kBlock)
+    k_pack_index(int32_t const * __restrict__ blockRow, int64_t const * __restrict__ rowptr,
+                 int32_t const * __restrict__ col, int32_t * __restrict__ pbase, uint16_t * __restrict__ plo,
+                 uint16_t * __restrict__ phi, int * __restrict__ bad)
+{
+  __shared__ int smin[kBlock], smax[kBlock];
+  int64_t const r0 = blockRow[blockIdx.x], r1 = blockRow[blockIdx.x + 1];
+  int64_t const base = rowptr[r0];
+  int const len = (int)(rowptr[r1] - base);
+  int const t = threadIdx.x;
+  int mn = 0x7fffffff, mx = -1;
+  for (int k = t; k < len; k += kBlock)
+  {
+    int const c = col[base + k];
+    mn = min(mn, c);
+    mx = max(mx, c);
+  }
+  smin[t] = mn;
+  smax[t] = mx;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1)
+  {
+    if (t < off)
+    {
+      smin[t] = min(smin[t], smin[t + off]);
+      smax[t] = max(smax[t], smax[t + off]);
+    }
+    __syncthreads();
+  }
+  int const cb = len > 0 ? smin[0] : 0;
+  if (t == 0)
+  {
+    pbase[blockIdx.x] = cb;
+    if (len > 0 && (int64_t)smax[0] - cb >= (1 << 18))
+      atomicOr(bad, 1);
+  }
+  uint32_t h = 0;
+  size_t const blk = (size_t)blockIdx.x * kStreamSlots;
+#pragma unroll
+  for (int m = 0; m < kStreamSlots / kBlock; ++m)
+  {
+    int const k = t + m * kBlock;
+    uint32_t const off = k < len ? (uint32_t)(col[base + k] - cb) : 0u;
+    plo[blk + k] = (uint16_t)(off & 0xffffu);
+    h |= ((off >> 16) & 3u) << (2 * m);
+  }
+  phi[(size_t)blockIdx.x * kBlock + t] = (uint16_t)h;
+}
+
+// value part of the block-packed copy (slots past a block's entries: 0)
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_pack_values(int32_t const * __restrict__ blockRow, int64_t const * __restrict__ rowptr,
+                  T const * __restrict__ val, T * __restrict__ pval)
+{
+  int64_t const r0 = blockRow[blockIdx.x], r1 = blockRow[blockIdx.x + 1];
+  int64_t const base = rowptr[r0];
+  int const len = (int)(rowptr[r1] - base);
+  size_t const slot = (size_t)blockIdx.x * kStreamSlots;
+  for (int k = threadIdx.x; k < kStreamSlots; k += kBlock)
+    pval[slot + k] = k < len ? val[base + k] : T(0);
 }
 
 __global__ void k_max_row_len(int64_t nrows, int64_t const * __restrict__ rowptr, int * __restrict__ out)
@@ -235,6 +368,15 @@ bool rowGroupForced()
   return e && e[0] == 'r';
 }
 
+template <typename T>
+T const * packValOf(CsrView const & A)
+{
+  if constexpr (sizeof(T) == 8)
+    return A.pval;
+  else
+    return A.pval32;
+}
+
 template <typename T, int MODE>
 void launchCsr(CsrView const & A, T const * val, T const * x, T const * b, T const * dinv, double omega, T * y,
                hipStream_t s)
@@ -243,6 +385,25 @@ void launchCsr(CsrView const & A, T const * val, T const * x, T const * b, T con
     return;
   T const om = (T)omega;
   double const mean = A.nnz > 0 ? (double)A.nnz / (double)A.nrows : 1.0;
+  T const * pval = packValOf<T>(A);
+  if (A.blockRow && A.nblocks > 0 && pval && A.plo && !rowGroupForced())
+  {
+    unsigned const g = (unsigned)A.nblocks;
+    if (mean > 48)
+      k_csr_packed<T, MODE, 8><<<g, kBlock, 0, s>>>(A.blockRow, A.rowptr, A.pbase, A.plo, A.phi, pval, x, b, dinv,
+                                                    om, y);
+    else if (mean > 20)
+      k_csr_packed<T, MODE, 4><<<g, kBlock, 0, s>>>(A.blockRow, A.rowptr, A.pbase, A.plo, A.phi, pval, x, b, dinv,
+                                                    om, y);
+    else if (mean > 8)
+      k_csr_packed<T, MODE, 2><<<g, kBlock, 0, s>>>(A.blockRow, A.rowptr, A.pbase, A.plo, A.phi, pval, x, b, dinv,
+                                                    om, y);
+    else
+      k_csr_packed<T, MODE, 1><<<g, kBlock, 0, s>>>(A.blockRow, A.rowptr, A.pbase, A.plo, A.phi, pval, x, b, dinv,
+                                                    om, y);
+    PXG_HIP(hipGetLastError());
+    return;
+  }
   if (A.blockRow && A.nblocks > 0 && !rowGroupForced())
   {
     unsigned const g = (unsigned)A.nblocks;
@@ -433,10 +594,67 @@ void buildRowBlocks(int64_t nrows, int64_t nnz, int64_t const * rowptr, DevBuf<i
 
 void ensureRowBlocks(pxg_ctx * c)
 {
-  if (!c->closed || c->blockStamp == c->stamp)
+  if (!c->closed || c->blockStamp == c->patStamp)
     return;
   buildRowBlocks(c->ndofs, c->nnz, c->rowptr.p, c->blockRow, c->nStreamBlocks, c->stream);
-  c->blockStamp = c->stamp;
+  c->blockStamp = c->patStamp;
+}
+
+void ensurePack(pxg_ctx * c, int precision)
+{
+  if (!c->packWanted || !c->closed)
+    return;
+  ensureRowBlocks(c);
+  if (c->nStreamBlocks <= 0)
+    return;
+  hipStream_t s = c->stream;
+  unsigned const g = (unsigned)c->nStreamBlocks;
+  size_t const slots = (size_t)c->nStreamBlocks * kStreamSlots;
+  if (c->packStamp != c->patStamp)
+  {
+    c->packBase.resize(c->nStreamBlocks);
+    c->packLo.resize(slots);
+    c->packHi.resize((size_t)c->nStreamBlocks * kBlock);
+    c->packFlag.resize(1);
+    PXG_HIP(hipMemsetAsync(c->packFlag.p, 0, sizeof(int), s));
+    k_pack_index<<<g, kBlock, 0, s>>>(c->blockRow.p, c->rowptr.p, c->col.p, c->packBase.p, c->packLo.p,
+                                      c->packHi.p, c->packFlag.p);
+    PXG_HIP(hipGetLastError());
+    int bad = 1;
+    PXG_HIP(hipMemcpyAsync(&bad, c->packFlag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    PXG_HIP(hipStreamSynchronize(s));
+    c->packOk = !bad;
+    c->packStamp = c->patStamp;
+    c->packValStamp = c->packVal32Stamp = 0;
+  }
+  if (!c->packOk)
+    return;
+  if (precision == PXG_FP32 && c->packVal32Stamp != c->stamp)
+  {
+    ensureVal32(c);
+    c->packVal32.resize(slots);
+    k_pack_values<float><<<g, kBlock, 0, s>>>(c->blockRow.p, c->rowptr.p, c->val32.p, c->packVal32.p);
+    PXG_HIP(hipGetLastError());
+    c->packVal32Stamp = c->stamp;
+  }
+  if (precision == PXG_FP64 && c->packValStamp != c->stamp)
+  {
+    ensureVal64(c);
+    c->packVal.resize(slots);
+    k_pack_values<double><<<g, kBlock, 0, s>>>(c->blockRow.p, c->rowptr.p, c->val.p, c->packVal.p);
+    PXG_HIP(hipGetLastError());
+    c->packValStamp = c->stamp;
+  }
+}
+
+void ensureOperator(pxg_ctx * c, int precision)
+{
+  if (precision == PXG_FP32)
+    ensureVal32(c);
+  else
+    ensureVal64(c);
+  ensureRowBlocks(c);
+  ensurePack(c, precision);
 }
 
 CsrView viewOf(pxg_ctx const * c)
@@ -451,9 +669,18 @@ CsrView viewOf(pxg_ctx const * c)
   v.val32 = c->val32.p;
   v.dinv = c->dinv.p;
   v.dinv32 = c->dinv32.p;
-  bool const blocks = c->nStreamBlocks > 0 && c->blockStamp == c->stamp;
+  bool const blocks = c->nStreamBlocks > 0 && c->blockStamp == c->patStamp;
   v.blockRow = blocks ? c->blockRow.p : nullptr;
   v.nblocks = blocks ? c->nStreamBlocks : 0;
+  bool const pack = blocks && c->packWanted && c->packOk && c->packStamp == c->patStamp;
+  if (pack)
+  {
+    v.pbase = c->packBase.p;
+    v.plo = c->packLo.p;
+    v.phi = c->packHi.p;
+    v.pval = c->packValStamp == c->stamp ? c->packVal.p : nullptr;
+    v.pval32 = c->packVal32Stamp == c->stamp ? c->packVal32.p : nullptr;
+  }
   return v;
 }
 

@@ -90,6 +90,7 @@ def lib():
         "pxg_csr_device": [vp, C.POINTER(i64p), C.POINTER(i32p), C.POINTER(dp), C.POINTER(fp), i64p],
         "pxg_rhs_device": [vp, C.POINTER(dp)],
         "pxg_matrix_stamp": [vp, C.POINTER(C.c_uint64)],
+        "pxg_csr_pack": [vp, i32, C.POINTER(C.c_int)],
         "pxg_mf_apply": [vp, vp, i32, vp, vp, i32],
         "pxg_mf_grad_kappa": [vp, i32, vp, vp, vp, i32],
         "pxg_mg_create": [pp, i32, pp, C.POINTER(C.c_int), C.c_double, i32, i32, C.c_double],
@@ -302,6 +303,16 @@ class Assembler:
         st = C.c_uint64()
         self._ck(lib().pxg_matrix_stamp(self.h, C.byref(st)))
         return st.value
+
+    def pack(self, enable=True):
+        """SpMV / Jacobi on a block-packed copy of the closed matrix (pxg_csr_pack)."""
+        self._ck(lib().pxg_csr_pack(self.h, 1 if enable else 0, None))
+
+    def packed(self):
+        """True if SpMV / Jacobi of the current closed matrix read the packed copy."""
+        act = C.c_int()
+        self._ck(lib().pxg_csr_pack(self.h, -1, C.byref(act)))
+        return bool(act.value)
 
     def csr(self, fp32=False):
         rowptr = np.empty(self.ndofs + 1, dtype=np.int64)

@@ -66,6 +66,45 @@ def test_spmv_and_jacobi(oracle, gpu):
     assert np.abs(x0.cpu().numpy() - x_ref).max() <= 1e-13 * np.abs(x_ref).max()
 
 
+@pytest.mark.parametrize("fe,n,packed", [("hex_q2", 6, True), ("hex_q1", 60, True), ("hex_q2", 32, False)])
+def test_packed_spmv_bitwise(gpu, fe, n, packed):
+    """SpMV and Jacobi on the block-packed copy equal the plain CSR kernels bit
+    for bit.  The dofs are renumbered at random, so row blocks span up to all
+    columns: Q1 60^3 (226,981 dofs) uses every high-bit pattern of the 18-bit
+    offsets; Q2 32^3 (274,625 dofs) has blocks spanning 2^18 columns or more,
+    where the pack is skipped and the plain kernel must run."""
+    import torch
+    prob = W.Problem(fe, (n, n, n))
+    perm = np.random.default_rng(1).permutation(prob.ndofs).astype(np.uint32)
+    a = prob.assembler()
+    a.set_dofmap(prob.ndofs, perm[prob.emap])
+    dofs = perm[prob.all_sides()]
+    a.set_dirichlet(dofs, np.zeros(len(dofs)))
+    a.set_coefficient(_kappa(prob.dof_coords())[np.argsort(perm)])
+    a.build_pattern()
+    a.assemble([("stiffness_fe", 1.0)])
+    a.close()
+    rng = np.random.default_rng(2)
+    x = torch.from_numpy(rng.uniform(-1, 1, prob.ndofs)).to(gpu)
+    bt = torch.from_numpy(rng.uniform(-1, 1, prob.ndofs)).to(gpu)
+    out = {}
+    for pack in (False, True):
+        a.pack(pack)
+        for prec, dt in ((0, torch.float64), (1, torch.float32)):
+            xv = x.to(dt)
+            y = torch.empty_like(xv)
+            a.spmv_dev(xv.data_ptr(), y.data_ptr(), precision=prec)
+            xj = torch.zeros_like(xv)
+            a.jacobi_dev(bt.to(dt).data_ptr(), xj.data_ptr(), 2.0 / 3.0, 3, precision=prec)
+            a.synchronize()
+            out[pack, prec] = (y.cpu().numpy(), xj.cpu().numpy())
+    for prec in (0, 1):
+        for k in range(2):
+            assert np.array_equal(out[False, prec][k], out[True, prec][k])
+    # the packed copy exists only where every block fits the 18-bit offsets
+    assert a.packed() == packed
+
+
 def _levels(oracle, fe, ns):
     gl, ol = [], []
     for n in ns:
