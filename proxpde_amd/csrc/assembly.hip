@@ -10,7 +10,7 @@
 //   elemMap  [E][n]      u32     kappa  [ndofs]    f64
 //   geoBuf   [E][nq][8]  f64     per quadrature point: s*(J^T J)^{-1} (6 sym.
 //                                entries), mass weight, JxW
-//   keBuf    [E][n][n]   T       element matrices after Dirichlet elimination
+//   keBuf    [E][n][ns]  T       element matrices after Dirichlet elimination (row stride ns)
 //   feBuf    [E][n]      f64     element right-hand sides / liftings
 //   contrib  [E*n*n]     u32     element-matrix entry ids sorted by (row, col, elem)
 //   segPtr   [nnzS+1]    u32     segment of each structural nonzero in contrib
@@ -258,8 +258,8 @@ __global__ void __launch_bounds__(kBlock)
       f = 0. - s;
     }
   }
-  // row i of the element matrix, row-major
-  T * out = ke + (e * N + i) * N;
+  // row i of the element matrix, row-major with row stride keRowStride(N)
+  T * out = ke + (e * N + i) * keRowStride(N);
 #pragma unroll
   for (int j = 0; j < N; ++j)
     out[j] = acc[j];
@@ -313,6 +313,18 @@ __global__ void k_pattern_fill(int64_t m, int64_t ndofs, int n, uint64_t const *
     if (t == 0 || keys[t - 1] / (uint64_t)ndofs != row)
       rowptrS[row] = id;
   }
+}
+
+// contribution ids e n^2 + i n + j -> positions e n ns + i ns + j in the
+// element-matrix buffers (row stride ns)
+__global__ void k_pad_ids(int64_t m, int n, int ns, uint32_t * __restrict__ ids)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m)
+    return;
+  uint32_t const id = ids[t];
+  uint32_t const r = id / (uint32_t)n; // e n + i
+  ids[t] = r * (uint32_t)ns + (id - r * (uint32_t)n);
 }
 
 __global__ void k_dof_keys(int64_t m, uint32_t const * __restrict__ emap, uint32_t * __restrict__ keys,
@@ -827,7 +839,9 @@ void buildPattern(pxg_ctx * c)
 {
   int const n = c->fe.n;
   int64_t const E = c->nelem, nn = (int64_t)n * n, m = E * nn;
-  PXG_REQUIRE(m < (int64_t)0xffffffffLL, PXG_E_INVALID, "too many element-matrix entries for 32-bit maps");
+  int const ns = keRowStride(n);
+  PXG_REQUIRE(E * n * ns < (int64_t)0xffffffffLL, PXG_E_INVALID,
+              "too many element-matrix entries for 32-bit maps");
   PXG_REQUIRE(c->ndofs > 0 && (double)c->ndofs * (double)c->ndofs < 1.8e19, PXG_E_INVALID, "dof count");
   hipStream_t s = c->stream;
   int endBit = 1;
@@ -868,6 +882,11 @@ void buildPattern(pxg_ctx * c)
     k_pattern_fill<<<gridFor(m), kBlock, 0, s>>>(m, c->ndofs, n, keysOut.p, idsOut.p, c->contrib.p, c->segPtr.p,
                                                   c->colS.p, c->rowptrS.p, c->emapNnz.p);
     PXG_HIP(hipGetLastError());
+    if (ns != n)
+    {
+      k_pad_ids<<<gridFor(m), kBlock, 0, s>>>(m, n, ns, c->contrib.p);
+      PXG_HIP(hipGetLastError());
+    }
     uint32_t const mm = (uint32_t)m;
     int64_t const nz = nnzS;
     PXG_HIP(hipMemcpyAsync(c->segPtr.p + nnzS, &mm, sizeof(uint32_t), hipMemcpyHostToDevice, s));
@@ -967,7 +986,7 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                {
                  if (!fused)
                  {
-                   c->keBuf.resize(E * N * N);
+                   c->keBuf.resize(E * N * keRowStride(N));
                    size_t const sh = (size_t)T.nq * N * (D + 1) * sizeof(double);
                    ProfScope ps(c, "element");
                    k_element_rows<double, N, D><<<gridFor(E * N), kBlock, sh, s>>>(
@@ -987,7 +1006,7 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                {
                  if (!fused)
                  {
-                   c->keBuf32.resize(E * N * N);
+                   c->keBuf32.resize(E * N * keRowStride(N));
                    size_t const sh = (size_t)T.nq * N * (D + 1) * sizeof(float);
                    ProfScope ps(c, "element32");
                    k_element_rows<float, N, D><<<gridFor(E * N), kBlock, sh, s>>>(

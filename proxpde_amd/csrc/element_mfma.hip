@@ -350,7 +350,8 @@ __global__ void __launch_bounds__(64 * kWaves)
     //         unit rows; the lifting f = -Ke bcv uses the unmodified Ke, its
     //         lower part through symmetry.
     bool const anyBC = __any(lane < N && W.flag[lane] != 0);
-    T * out = ke + e * (N * N);
+    constexpr int NS = keRowStride(N);
+    T * out = ke + e * (N * NS);
     if constexpr (sizeof(T) == 4)
     {
       // FP32: entries below 2^-19 sqrt(Ke_ii Ke_jj) are rounding noise of
@@ -419,9 +420,9 @@ __global__ void __launch_bounds__(64 * kWaves)
           }
           if (valid)
           {
-            out[i * N + j] = v;
+            out[i * NS + j] = v;
             if (j > i)
-              out[j * N + i] = v; // the lower triangle repeats the upper one exactly
+              out[j * NS + i] = v; // the lower triangle repeats the upper one exactly
           }
         }
     }
@@ -535,7 +536,7 @@ bool elementMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, doubl
   {
     using TT = decltype(tag);
     TT * ke = nullptr;
-    size_t const np = (size_t)T.n * T.n; // full row-major element matrix
+    size_t const np = (size_t)T.n * keRowStride(T.n); // full row-major element matrix, padded rows
     if constexpr (sizeof(TT) == 8)
     {
       c->keBuf.resize((size_t)c->nelem * np);

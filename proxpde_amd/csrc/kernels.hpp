@@ -24,6 +24,14 @@ void ensureVal64(pxg_ctx * c);
 // element_mfma.hip: fused geometry + MFMA element kernel; false if not applicable
 bool elementMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, double cMF, int precision);
 
+// Row stride of the element-matrix buffers (keBuf, keBuf32): rows of 16 or
+// more entries are padded to a multiple of 16 entries (128 B in FP64, 64 B in
+// FP32), so an element-matrix row fills whole cache lines and no line holds
+// parts of two rows.  Two rows of one element belong to two global rows,
+// which the scatter gathers at different times; a shared line would be
+// fetched twice.  Smaller element matrices are kept dense.
+constexpr int keRowStride(int n) { return n >= 16 ? (n + 15) / 16 * 16 : n; }
+
 // sparse.hip: CSR kernels on raw device arrays
 struct CsrView
 {
