@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <string>
+#include <vector>
 
 namespace pxg
 {
@@ -468,6 +469,221 @@ __global__ void k_diag_compact(int64_t nrows, uint32_t const * __restrict__ diag
 
 __global__ void k_set_int(int * p, int v) { *p = v; }
 
+// ---- rebuild into the pruned layout, staged by row blocks ------------------
+// A stage block is a run of rows whose element-matrix rows (the (element,
+// local row) pairs of their dofs, dofContrib) number at most kStagePairs.
+// The kernel copies those element-matrix rows to LDS with coalesced loads
+// and sums every structural entry of the block's rows from LDS in segment
+// order, so the sums are bitwise those of k_scatter.  Per structural entry
+// (info): its contribution offset in the block (bits 0-14) and the layout's
+// keep flag (bit 15); per contribution (sid): its LDS slot, pair * n + local
+// column.
+constexpr int kStagePairs = 128;
+static_assert(kStagePairs <= 256, "one pair id per thread of a stage block");
+
+struct StageBlock
+{
+  uint32_t s0, s1;   // structural entries
+  uint32_t p0, p1;   // (element, local row) pairs in dofContrib
+  uint32_t id0, id1; // contributions (contrib / sid order)
+  uint32_t k0;       // layout position of the first kept entry
+  uint32_t pad;
+};
+
+__global__ void k_stage_blocks(int64_t nb, int32_t const * __restrict__ r0s, int64_t const * __restrict__ rowptrS,
+                               uint32_t const * __restrict__ dofSeg, uint32_t const * __restrict__ segPtr,
+                               uint32_t const * __restrict__ pos, StageBlock * __restrict__ blocks)
+{
+  int64_t const b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb)
+    return;
+  int32_t const r0 = r0s[b], r1 = r0s[b + 1];
+  StageBlock B;
+  B.s0 = (uint32_t)rowptrS[r0];
+  B.s1 = (uint32_t)rowptrS[r1];
+  B.p0 = dofSeg[r0];
+  B.p1 = dofSeg[r1];
+  B.id0 = segPtr[B.s0];
+  B.id1 = segPtr[B.s1];
+  B.k0 = pos[B.s0];
+  B.pad = 0;
+  blocks[b] = B;
+}
+
+// largest number of (element, local row) pairs of a dof
+__global__ void k_max_pairs(int64_t nrows, uint32_t const * __restrict__ seg, int * __restrict__ out)
+{
+  __shared__ int sh[kBlock];
+  int m = 0;
+  for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * kBlock)
+    m = max(m, (int)(seg[r + 1] - seg[r]));
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1)
+  {
+    if ((int)threadIdx.x < off)
+      sh[threadIdx.x] = max(sh[threadIdx.x], sh[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    atomicMax(out, sh[0]);
+}
+
+// stage block q = the rows whose first pair falls in [q cap, (q+1) cap):
+// with cap = kStagePairs - (most pairs of a row) a block stages at most
+// kStagePairs pairs; every q is written exactly once
+__global__ void k_stage_rows(int64_t nrows, uint32_t const * __restrict__ seg, int64_t cap, int64_t nb,
+                             int32_t * __restrict__ r0s)
+{
+  int64_t const r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r > nrows)
+    return;
+  int64_t const lo = r == 0 ? 0 : (int64_t)seg[r - 1] / cap + 1;
+  int64_t const hi = r == nrows ? nb : min((int64_t)seg[r] / cap, nb);
+  for (int64_t q = lo; q <= hi; ++q)
+    r0s[q] = (int32_t)r;
+}
+
+// position of each (element, local row) pair in dofContrib
+__global__ void k_pair_pos(int64_t m, uint32_t const * __restrict__ dofContrib, uint32_t * __restrict__ pairPos)
+{
+  int64_t const p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p < m)
+    pairPos[dofContrib[p]] = (uint32_t)p;
+}
+
+// info of the structural entries and sid of the contributions of one stage
+// block; *bad is set if a contribution's pair lies outside the block's pairs
+// (a corrupt map)
+__global__ void k_stage_maps(int n, int ns, StageBlock const * __restrict__ blocks,
+                             uint32_t const * __restrict__ segPtr, uint32_t const * __restrict__ contrib,
+                             uint32_t const * __restrict__ keep, uint32_t const * __restrict__ pairPos,
+                             uint16_t * __restrict__ info, uint16_t * __restrict__ sid, int * __restrict__ bad)
+{
+  StageBlock const B = blocks[blockIdx.x];
+  for (uint32_t s = B.s0 + threadIdx.x; s < B.s1; s += kBlock)
+    info[s] = (uint16_t)((segPtr[s] - B.id0) | (keep[s] ? 0x8000u : 0u));
+  uint32_t const en = (uint32_t)n * (uint32_t)ns;
+  for (uint32_t g = B.id0 + threadIdx.x; g < B.id1; g += kBlock)
+  {
+    uint32_t const id = contrib[g];
+    uint32_t const e = id / en, rem = id - e * en, li = rem / (uint32_t)ns, lj = rem - li * (uint32_t)ns;
+    uint32_t const p = pairPos[e * (uint32_t)n + li];
+    if (p < B.p0 || p >= B.p1)
+      atomicOr(bad, 1);
+    sid[g] = (uint16_t)((p - B.p0) * (uint32_t)n + lj);
+  }
+}
+
+template <typename T, int N, int NS>
+__global__ void __launch_bounds__(kBlock)
+    k_scatter_staged(StageBlock const * __restrict__ blocks, uint32_t const * __restrict__ dofContrib,
+                     T const * __restrict__ ke, uint16_t const * __restrict__ info, uint16_t const * __restrict__ sid,
+                     T * __restrict__ valC, T * __restrict__ valS, int * __restrict__ changed)
+{
+  // a block has at most kStagePairs * N contributions, so at most that many
+  // structural entries and staged element-matrix values
+  constexpr int kMax = kStagePairs * N;
+  constexpr int kPer = (kMax + kBlock - 1) / kBlock;
+  __shared__ T lk[kMax];
+  __shared__ uint16_t linfo[kMax];
+  __shared__ uint16_t lsid[kMax];
+  __shared__ uint32_t lpc[kStagePairs];
+  __shared__ uint32_t wsum[kBlock / 64];
+  StageBlock const B = blocks[blockIdx.x];
+  int const t = threadIdx.x;
+  int const np = (int)(B.p1 - B.p0), ne = (int)(B.s1 - B.s0), nid = (int)(B.id1 - B.id0);
+  if (ne == 0)
+    return; // rows of dofs in no element (then np = nid = 0 too)
+  // 1. the block's pair ids, entry info and contribution slots -> LDS; all
+  //    loads are issued before the first LDS store (indices clamped)
+  {
+    uint32_t pc = 0;
+    if (t < kStagePairs)
+      pc = dofContrib[B.p0 + min(t, np - 1)];
+    uint32_t ri[kPer], rs[kPer];
+#pragma unroll
+    for (int m = 0; m < kPer; ++m)
+    {
+      int const i = t + m * kBlock;
+      ri[m] = info[B.s0 + min(i, ne - 1)];
+      rs[m] = sid[B.id0 + min(i, nid - 1)];
+    }
+    if (t < np)
+      lpc[t] = pc;
+#pragma unroll
+    for (int m = 0; m < kPer; ++m)
+    {
+      int const i = t + m * kBlock;
+      if (i < ne)
+        linfo[i] = (uint16_t)ri[m];
+      if (i < nid)
+        lsid[i] = (uint16_t)rs[m];
+    }
+  }
+  __syncthreads();
+  // 2. element-matrix rows of the pairs -> LDS
+  {
+    int const nst = np * N;
+    T rv[kPer];
+#pragma unroll
+    for (int m = 0; m < kPer; ++m)
+    {
+      int const i = min(t + m * kBlock, nst - 1);
+      int const p = i / N, lj = i - p * N;
+      rv[m] = ke[(size_t)lpc[p] * NS + lj];
+    }
+#pragma unroll
+    for (int m = 0; m < kPer; ++m)
+      if (t + m * kBlock < nst)
+        lk[t + m * kBlock] = rv[m];
+  }
+  __syncthreads();
+  // 3. structural entries in rounds of kBlock, from LDS; kept ones go to the
+  //    next layout positions in order, pruned ones to valS
+  uint32_t kept = B.k0;
+  int const lane = t & 63, w = t >> 6;
+  bool flip = false;
+  for (int r = 0; r < ne; r += kBlock)
+  {
+    int const i = r + t;
+    bool const v = i < ne;
+    uint32_t const inf = v ? linfo[i] : 0u;
+    uint32_t const nx = i + 1 < ne ? (uint32_t)(linfo[i + 1] & 0x7fffu) : (uint32_t)nid;
+    uint32_t const off = inf & 0x7fffu;
+    uint32_t const cnt = v ? nx - off : 0u;
+    bool const keep = v && (inf >> 15);
+    T acc = T(0);
+    for (uint32_t j = 0; j < cnt; ++j)
+      acc += lk[lsid[off + j]];
+    uint64_t const m = __ballot(keep);
+    uint32_t const below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0)
+      wsum[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q)
+    {
+      before += q < w ? wsum[q] : 0u;
+      total += wsum[q];
+    }
+    if (v)
+    {
+      bool const now = fabs((double)acc) > kDefaultPrune;
+      if (keep)
+        valC[kept + before + below] = acc;
+      else
+        valS[B.s0 + i] = acc;
+      flip |= now != keep;
+    }
+    kept += total;
+    __syncthreads(); // wsum is rewritten by the next round
+  }
+  if (flip)
+    *changed = 1;
+}
+
 // contributions loaded ahead per structural entry (PXG_SCATTER_UNROLL = 1, 2,
 // 4 or 8 overrides the default for launch tuning)
 int scatterUnroll()
@@ -503,6 +719,22 @@ void launchScatterDirect(int64_t nnzS, uint32_t const * segPtr, uint32_t const *
   case 8: k_scatter_direct<T, 8><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, pos, valC, valS, changed); break;
   default: k_scatter_direct<T, 4><<<g, kBlock, 0, s>>>(nnzS, segPtr, contrib, ke, pos, valC, valS, changed); break;
   }
+}
+
+// rebuild into the pruned layout: staged by row blocks when the stage maps
+// exist, else one thread per structural entry
+template <typename T, int N>
+void launchRebuild(pxg_ctx * c, T const * ke, T * valC, T * valS, hipStream_t s)
+{
+  if (c->stageValid && c->nStage > 0)
+  {
+    k_scatter_staged<T, N, keRowStride(N)><<<(unsigned)c->nStage, kBlock, 0, s>>>(
+        reinterpret_cast<StageBlock const *>(c->stageBlk.p), c->dofContrib.p, ke, c->stageInfo.p, c->stageSid.p,
+        valC, valS, c->changed.p);
+    PXG_HIP(hipGetLastError());
+    return;
+  }
+  launchScatterDirect<T>(c->nnzS, c->segPtr.p, c->contrib.p, ke, c->pos.p, valC, valS, c->changed.p, s);
 }
 
 template <typename T>
@@ -996,8 +1228,7 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                  PXG_HIP(hipGetLastError());
                  ProfScope ps2(c, "scatter");
                  if (direct)
-                   launchScatterDirect<double>(c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf.p, c->pos.p, c->val.p,
-                                               c->valS.p, c->changed.p, s);
+                   launchRebuild<double, N>(c, c->keBuf.p, c->val.p, c->valS.p, s);
                  else
                    launchScatter<double>(overwrite, c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf.p, c->valS.p,
                                          c->keep.p, s);
@@ -1016,8 +1247,7 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                  PXG_HIP(hipGetLastError());
                  ProfScope ps2(c, "scatter32");
                  if (direct)
-                   launchScatterDirect<float>(c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf32.p, c->pos.p,
-                                              c->val32.p, c->valS32.p, c->changed.p, s);
+                   launchRebuild<float, N>(c, c->keBuf32.p, c->val32.p, c->valS32.p, s);
                  else
                    launchScatter<float>(overwrite, c->nnzS, c->segPtr.p, c->contrib.p, c->keBuf32.p, c->valS32.p,
                                         c->keep.p, s);
@@ -1139,6 +1369,8 @@ void geometryOut(pxg_ctx * c, double * detJ, double * JxW)
   PXG_HIP(hipStreamSynchronize(s));
 }
 
+void buildStageMaps(pxg_ctx * c);
+
 void closeMatrix(pxg_ctx * c, double eps)
 {
   PXG_REQUIRE(c->nnzS > 0, PXG_E_STATE, "close needs build_pattern");
@@ -1241,7 +1473,54 @@ void closeMatrix(pxg_ctx * c, double eps)
     c->diagC.resize(nr);
     k_diag_compact<<<gridFor(nr), kBlock, 0, s>>>(nr, c->diagPos.p, c->keep.p, c->pos.p, c->diagC.p);
     PXG_HIP(hipGetLastError());
+    buildStageMaps(c);
   }
+  else
+    c->stageValid = false;
+}
+
+// stage blocks and maps of the pruned layout (k_scatter_staged); left
+// invalid (one thread per structural entry instead) when a dof has more
+// element-matrix rows than a block stages
+void buildStageMaps(pxg_ctx * c)
+{
+  hipStream_t s = c->stream;
+  int const n = c->fe.n;
+  int64_t const nr = c->ndofs, npairs = c->nelem * n;
+  c->stageValid = false;
+  if ((int64_t)kStagePairs * n > 0x7fff || nr >= ((int64_t)1 << 31) || npairs <= 0)
+    return; // contribution offsets of a block must fit 15 bits
+  c->changed.resize(1);
+  k_set_int<<<1, 1, 0, s>>>(c->changed.p, 0);
+  k_max_pairs<<<256, kBlock, 0, s>>>(nr, c->dofSeg.p, c->changed.p);
+  PXG_HIP(hipGetLastError());
+  int maxPairs = 0;
+  PXG_HIP(hipMemcpyAsync(&maxPairs, c->changed.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  PXG_HIP(hipStreamSynchronize(s));
+  int64_t const cap = kStagePairs - std::max(maxPairs, 1);
+  if (cap < 1)
+    return; // a dof with more element-matrix rows than a block stages
+  int64_t const nb = npairs / cap + 1;
+  c->stageR0.resize(nb + 1);
+  k_stage_rows<<<gridFor(nr + 1), kBlock, 0, s>>>(nr, c->dofSeg.p, cap, nb, c->stageR0.p);
+  c->stageBlk.resize((size_t)nb * (sizeof(StageBlock) / sizeof(uint32_t)));
+  auto * blocks = reinterpret_cast<StageBlock *>(c->stageBlk.p);
+  k_stage_blocks<<<gridFor(nb), kBlock, 0, s>>>(nb, c->stageR0.p, c->rowptrS.p, c->dofSeg.p, c->segPtr.p, c->pos.p,
+                                                blocks);
+  DevBuf<uint32_t> pairPos;
+  pairPos.resize(npairs);
+  k_pair_pos<<<gridFor(npairs), kBlock, 0, s>>>(npairs, c->dofContrib.p, pairPos.p);
+  c->stageInfo.resize(std::max<int64_t>(c->nnzS, 1));
+  c->stageSid.resize(std::max<size_t>(c->contrib.n, 1));
+  k_set_int<<<1, 1, 0, s>>>(c->changed.p, 0);
+  k_stage_maps<<<(unsigned)nb, kBlock, 0, s>>>(n, keRowStride(n), blocks, c->segPtr.p, c->contrib.p, c->keep.p,
+                                              pairPos.p, c->stageInfo.p, c->stageSid.p, c->changed.p);
+  PXG_HIP(hipGetLastError());
+  int bad = 1;
+  PXG_HIP(hipMemcpyAsync(&bad, c->changed.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  PXG_HIP(hipStreamSynchronize(s));
+  c->nStage = nb;
+  c->stageValid = bad == 0;
 }
 
 void unsplitValues(pxg_ctx * c)

@@ -163,6 +163,15 @@ struct pxg_ctx
   // pruned ones in valS / valS32
   bool valsSplit = false;
   pxg::DevBuf<int> changed; // direct scatter: some prune flag differs from the layout
+  // stage blocks of the rebuild into the pruned layout: runs of rows with at
+  // most a fixed number of element-matrix rows, staged in LDS; per structural
+  // entry its contribution offset in the block and keep flag, per
+  // contribution its LDS slot
+  bool stageValid = false;
+  int64_t nStage = 0;
+  pxg::DevBuf<int32_t> stageR0;   // nStage + 1 first rows
+  pxg::DevBuf<uint32_t> stageBlk; // 8 words per block
+  pxg::DevBuf<uint16_t> stageInfo, stageSid;
   uint64_t stamp = 0;      // counts closes: changes whenever the closed matrix may have changed
   pxg::DevBuf<double> valS;
   pxg::DevBuf<float> valS32;
