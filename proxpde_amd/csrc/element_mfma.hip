@@ -76,18 +76,19 @@ struct Shape
   static constexpr int STEPS = (K + 3) / 4;               // MFMA k steps of 4
 };
 
-// per-wave LDS
-template <typename T, int N, int NPE, int NQ>
+// per-wave LDS (kept small: two blocks of kWaves waves must fit one CU's LDS)
+template <typename T, int N, int NPE, int NQ, int DIM, bool MASS>
 struct WaveLds
 {
   double x[NPE * 3];
   double kap[N];
   double bcv[N];
   // rows of the per-point C operand factor: a < DIM the metric row
-  // s_q (J^T J)^{-1}[a][:], a = DIM the mass row (m_q, 0, 0)
-  double M[NQ][4][3];
+  // s_q (J^T J)^{-1}[a][:], a = DIM the mass row (m_q, 0, 0) when mass
+  // terms are built
+  double M[NQ][DIM + (MASS ? 1 : 0)][3];
   double f[N];  // lifting accumulator of the Dirichlet elimination
-  double dg[N]; // diagonal of Ke (FP32: the flush threshold of noise entries)
+  double dg[sizeof(T) == 4 ? N : 1]; // FP32: diagonal of Ke, the flush threshold of noise entries
   int flag[N];
 };
 
@@ -102,7 +103,7 @@ __global__ void __launch_bounds__(64 * kWaves)
                    T * __restrict__ ke, double * __restrict__ fe, int * __restrict__ status)
 {
   using S = Shape<N, DIM, NQ, MASS>;
-  using L = WaveLds<T, N, NPE, NQ>;
+  using L = WaveLds<T, N, NPE, NQ, DIM, MASS>;
   using Acc = typename Mfma<T>::Acc;
   constexpr int NT = S::NT, KS = S::KS, STEPS = S::STEPS;
   bool const needK = (flags & 10) != 0;
@@ -273,9 +274,12 @@ __global__ void __launch_bounds__(64 * kWaves)
         W.M[q][2][1] = s * I12;
         W.M[q][2][2] = s * I22;
       }
-      W.M[q][DIM][0] = m;
-      W.M[q][DIM][1] = 0.;
-      W.M[q][DIM][2] = 0.;
+      if constexpr (MASS)
+      {
+        W.M[q][DIM][0] = m;
+        W.M[q][DIM][1] = 0.;
+        W.M[q][DIM][2] = 0.;
+      }
     }
     waveSync();
     // next element's values: their latency hides behind the matrix products
