@@ -18,7 +18,7 @@
 // lifting Fe.
 //
 // One wavefront per element; kWaves wavefronts per block share the constant
-// tables in LDS; blocks stride over the elements.
+// tables in LDS; waves take elements from a shared counter.
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -57,6 +57,9 @@ struct Mfma<float>
 };
 
 constexpr int kWaves = 8;
+// element counters of the persistent kernel, one per 256-byte line
+constexpr int kCounters = 8;
+constexpr int kCounterStride = 64;
 
 // ordering point for the lanes of one wavefront: LDS accesses of a wave
 // execute in order, so a compiler fence plus a wave barrier suffices
@@ -93,14 +96,15 @@ struct WaveLds
 };
 
 template <typename T, int N, int DIM, int NPE, int NQ, bool MASS>
-__global__ void __launch_bounds__(64 * kWaves)
+__global__ void __launch_bounds__(64 * kWaves, 4)
     k_element_mfma(int64_t E, double const * __restrict__ xyz, uint32_t const * __restrict__ conn,
                    uint32_t const * __restrict__ emap, double const * __restrict__ kappa,
                    uint8_t const * __restrict__ bcFlag, double const * __restrict__ bcVal, int hasBC,
                    double const * __restrict__ geoGradG, double const * __restrict__ wG,
                    double const * __restrict__ phiG, double const * __restrict__ gradG,
                    T const * __restrict__ afrag, double cS, double cSF, double cM, double cMF, int flags,
-                   T * __restrict__ ke, double * __restrict__ fe, int * __restrict__ status)
+                   T * __restrict__ ke, double * __restrict__ fe, int * __restrict__ status,
+                   int * __restrict__ work)
 {
   using S = Shape<N, DIM, NQ, MASS>;
   using L = WaveLds<T, N, NPE, NQ, DIM, MASS>;
@@ -137,10 +141,35 @@ __global__ void __launch_bounds__(64 * kWaves)
   __syncthreads();
 
   int const kr = lane >> 4, jc = lane & 15;
-  // waves own disjoint element streams and LDS regions: after the tables are
-  // loaded they synchronise only with themselves
-  int64_t const stride = (int64_t)gridDim.x * kWaves;
-  int64_t e = (int64_t)blockIdx.x * kWaves + wave;
+  // waves own disjoint LDS regions and take elements from counters (waves of
+  // one SIMD are issued with different priorities, so a fixed share per wave
+  // would leave the slowest waves running alone at the end).  Blocks b with
+  // the same b % P (P = min(grid, kCounters)) share a counter over one
+  // contiguous range of elements (blocks are placed on the XCDs round-robin,
+  // so a range stays in one XCD's L2); every range has blocks, wherever they
+  // run.  After the
+  // tables are loaded waves synchronise only with themselves.  The element
+  // after next is requested one element ahead, so the atomic's latency hides
+  // behind an element's work.
+  int const parts = min((int)gridDim.x, kCounters);
+  int const part = (int)(blockIdx.x % parts);
+  int64_t const r0 = E * part / parts, r1 = E * (part + 1) / parts;
+  int * const ctr = work + part * kCounterStride;
+  auto grab = [&]() -> int
+  {
+    int v = 0;
+    if (lane == 0)
+      v = atomicAdd(ctr, 1);
+    return v;
+  };
+  // wave-uniform (scalar registers): lane 0's counter value
+  auto elem = [&](int v) -> int64_t
+  {
+    int64_t const k = r0 + __builtin_amdgcn_readfirstlane(v);
+    return k < r1 ? k : E;
+  };
+  int64_t e = elem(grab());
+  int64_t en = elem(grab());
   // gathered data of the element being processed, loaded one element ahead
   double rx = 0., rk = 0., rb = 0.;
   int rf = 0;
@@ -173,8 +202,9 @@ __global__ void __launch_bounds__(64 * kWaves)
     gatherIds(e, pid, id);
     gatherVals(e, pid, id);
   }
-  for (; e < E; e += stride)
+  while (e < E)
   {
+    int const ennReq = grab(); // element after next (lane 0)
     // ---- 1. element data -> LDS
     if (lane < NPE * 3)
       W.x[lane] = rx;
@@ -187,7 +217,7 @@ __global__ void __launch_bounds__(64 * kWaves)
     }
     waveSync();
     uint32_t npid = 0, nid = 0;
-    gatherIds(e + stride, npid, nid);
+    gatherIds(en, npid, nid);
     // ---- 2. geometry, one quadrature point per lane
     if (lane < NQ)
     {
@@ -283,7 +313,7 @@ __global__ void __launch_bounds__(64 * kWaves)
     }
     waveSync();
     // next element's values: their latency hides behind the matrix products
-    gatherVals(e + stride, npid, nid);
+    gatherVals(en, npid, nid);
     // ---- 3. Ke tiles on the matrix cores (upper tiles only); the C operand
     //         C[4s + kr][16 tj + jc] is formed in registers per step
     Acc acc[NT][NT];
@@ -447,7 +477,15 @@ __global__ void __launch_bounds__(64 * kWaves)
     if (lane < N)
       fe[e * N + lane] = anyBC ? (W.flag[lane] ? W.bcv[lane] : 0. - W.f[lane]) : 0.;
     waveSync(); // LDS of this element is reused by the next one
+    e = en;
+    en = elem(ennReq);
   }
+}
+
+__global__ void k_reset_counters(int * c)
+{
+  if (threadIdx.x < kCounters)
+    c[threadIdx.x * kCounterStride] = 0;
 }
 
 // host: MFMA fragments of the constant operand A (N x K)
@@ -513,10 +551,14 @@ void launchMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, double
   int64_t const blocksNeeded = (c->nelem + kWaves - 1) / kWaves;
   int const grid =
       (int)std::max<int64_t>(1, std::min<int64_t>(blocksNeeded, (int64_t)ncu * std::max(perCu, 1)));
+  PXG_REQUIRE(c->nelem < ((int64_t)1 << 31) - (int64_t)grid * kWaves * 2, PXG_E_INVALID,
+              "element count exceeds the 32-bit work counters");
+  c->workCounter.resize(kCounters * kCounterStride);
+  k_reset_counters<<<1, 64, 0, c->stream>>>(c->workCounter.p);
   k_element_mfma<T, N, DIM, NPE, NQ, MASS><<<grid, 64 * kWaves, 0, c->stream>>>(
       c->nelem, c->xyz.p, c->conn.p, c->elemMap.p, c->kappa.p, c->bcFlag.p, c->bcVal.p, c->hasBC ? 1 : 0,
       tab.d_geoGrad.p, tab.d_w.p, tab.d_phi.p, tab.d_grad.p, dfrag, cS, cSF, cM, cMF, flags, ke, c->feBuf.p,
-      c->status.p);
+      c->status.p, c->workCounter.p);
   PXG_HIP(hipGetLastError());
 }
 
