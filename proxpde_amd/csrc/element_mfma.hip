@@ -57,9 +57,8 @@ struct Mfma<float>
 };
 
 constexpr int kWaves = 8;
-// element counters of the persistent kernel, one per 256-byte line
-constexpr int kCounters = 8;
-constexpr int kCounterStride = 64;
+constexpr int kCounters = kWorkCounters;
+constexpr int kCounterStride = kWorkCounterStride;
 
 // ordering point for the lanes of one wavefront: LDS accesses of a wave
 // execute in order, so a compiler fence plus a wave barrier suffices

@@ -32,6 +32,12 @@ bool elementMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, doubl
 // fetched twice.  Smaller element matrices are kept dense.
 constexpr int keRowStride(int n) { return n >= 16 ? (n + 15) / 16 * 16 : n; }
 
+// Work counters of the persistent element kernels (element_mfma.hip,
+// matfree.hip): blocks b with the same b % P share one counter over a
+// contiguous range of the elements, one counter per 256-byte line.
+constexpr int kWorkCounters = 8;
+constexpr int kWorkCounterStride = 64;
+
 // sparse.hip: CSR kernels on raw device arrays
 struct CsrView
 {

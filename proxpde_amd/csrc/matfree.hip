@@ -64,7 +64,8 @@ __global__ void __launch_bounds__(64 * kMfWaves)
               double const * __restrict__ geoGradG, double const * __restrict__ wG, double const * __restrict__ phiG,
               double const * __restrict__ gradG, uint32_t const * __restrict__ emap,
               double const * __restrict__ kappa, uint8_t const * __restrict__ cons, double const * __restrict__ x,
-              double const * __restrict__ ybar, double * __restrict__ out, int * __restrict__ status)
+              double const * __restrict__ ybar, double * __restrict__ out, int * __restrict__ status,
+              int * __restrict__ work)
 {
   constexpr int EPW = 64 / SUB;
   using L = ElemLds<DIM, NPE, N, SUB, NBC, GRAD>;
@@ -88,7 +89,6 @@ __global__ void __launch_bounds__(64 * kMfWaves)
   int const wave = tid >> 6, lane = tid & 63;
   int const sub = lane / SUB, l = lane - sub * SUB;
   L & M = lds[wave * EPW + sub];
-  int64_t const stride = (int64_t)gridDim.x * kMfWaves * EPW;
   // gathered data of the next element, loaded one element ahead so the
   // global-memory latency hides behind the current element's work:
   // ids first (conn, emap), values (coordinates, kappa, flag, the first
@@ -132,12 +132,35 @@ __global__ void __launch_bounds__(64 * kMfWaves)
       }
     }
   };
-  int64_t const first = ((int64_t)blockIdx.x * kMfWaves + wave) * EPW;
-  gatherIds(first + sub);
-  gatherVals(first + sub);
-  // the wave loops uniformly; each group checks its own element
-  for (int64_t base = first; base < E; base += stride)
+  // waves take batches of EPW consecutive elements from counters: blocks b
+  // with the same b % P (P = min(grid, kWorkCounters)) share a counter over
+  // one contiguous range of batches, so waves of unequal issue priority end
+  // together; the batch after next is requested one batch ahead
+  int const parts = min((int)gridDim.x, kWorkCounters);
+  int const part = (int)(blockIdx.x % parts);
+  int64_t const nbat = (E + EPW - 1) / EPW;
+  int64_t const q0 = nbat * part / parts, q1 = nbat * (part + 1) / parts;
+  int * const ctr = work + part * kWorkCounterStride;
+  auto grab = [&]() -> int
   {
+    int v = 0;
+    if (lane == 0)
+      v = atomicAdd(ctr, 1);
+    return v;
+  };
+  auto batch = [&](int v) -> int64_t // first element of the batch (wave-uniform), E when done
+  {
+    int64_t const q = q0 + __builtin_amdgcn_readfirstlane(v);
+    return q < q1 ? q * EPW : E;
+  };
+  int64_t base = batch(grab());
+  int64_t nbase = batch(grab());
+  gatherIds(base + sub);
+  gatherVals(base + sub);
+  // the wave loops uniformly; each group checks its own element
+  while (base < E)
+  {
+    int const req = grab(); // batch after next (lane 0)
     int64_t const e = base + sub;
     bool const live = e < E;
     uint32_t id = 0;
@@ -162,7 +185,7 @@ __global__ void __launch_bounds__(64 * kMfWaves)
       }
     }
     waveSync();
-    gatherIds(e + stride);
+    gatherIds(nbase + sub);
     // geometry of point q = l, kept in registers across the batch
     double I00 = 0., I01 = 0., I02 = 0., I11 = 0., I12 = 0., I22 = 0., sq = 0.;
     if (live && l < nq)
@@ -229,7 +252,7 @@ __global__ void __launch_bounds__(64 * kMfWaves)
         sq = kq * jxw;
       }
     }
-    gatherVals(e + stride);
+    gatherVals(nbase + sub);
     double cq = 0.; // gradient: sum over the batch at point q
     for (int b0 = 0; b0 < nb; b0 += NBC)
     {
@@ -332,7 +355,15 @@ __global__ void __launch_bounds__(64 * kMfWaves)
       }
       waveSync();
     }
+    base = nbase;
+    nbase = batch(req);
   }
+}
+
+__global__ void k_reset_work(int * c)
+{
+  if (threadIdx.x < kWorkCounters)
+    c[threadIdx.x * kWorkCounterStride] = 0;
 }
 
 // y[d][b] = sum over the (element, local) entries of dof d, in element order;
@@ -396,7 +427,8 @@ void prepare(pxg_ctx * c, int nb)
 }
 
 // one launch of the element kernel: lanes per element = the smallest power of
-// two (at least 8) holding n and nq; one resident wave of blocks striding over E
+// two (at least 8) holding n and nq; one resident wave of blocks taking batches
+// of elements from the work counters
 template <bool GRAD>
 void launchElem(pxg_ctx * c, double const * kappa, uint8_t const * cons, int nb, double const * x,
                 double const * ybar, double * outBuf)
@@ -421,9 +453,13 @@ void launchElem(pxg_ctx * c, double const * kappa, uint8_t const * cons, int nb,
                    int64_t const per = (int64_t)kMfWaves * (64 / SUB);
                    int64_t const need = (c->nelem + per - 1) / per;
                    int const grid = (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)ncu * std::max(perCu, 1)));
+                   PXG_REQUIRE(need < ((int64_t)1 << 31) - 2 * per * grid, PXG_E_INVALID,
+                               "element count exceeds the 32-bit work counters");
+                   c->workCounter.resize(kWorkCounters * kWorkCounterStride);
+                   k_reset_work<<<1, 64, 0, c->stream>>>(c->workCounter.p);
                    kern<<<grid, 64 * kMfWaves, 0, c->stream>>>(c->nelem, T.nq, nb, c->xyz.p, c->conn.p, T.d_geoGrad.p,
                                                                T.d_w.p, T.d_phi.p, T.d_grad.p, c->elemMap.p, kappa,
-                                                               cons, x, ybar, outBuf, c->status.p);
+                                                               cons, x, ybar, outBuf, c->status.p, c->workCounter.p);
                    }
                  };
                  auto pick = [&](auto subC)
