@@ -445,7 +445,17 @@ def main():
         ev1.record(stream)
         torch.cuda.synchronize()
         cms = ev0.elapsed_time(ev1) / 3
-        copy_cal = {"bytes_each_way": nel * 8, "ms": cms, "GB/s_read_plus_write": 2 * nel * 8 / (cms * 1e-3) / 1e9}
+        # the same copy as a compute kernel (an elementwise product by one),
+        # which the FETCH_SIZE / WRITE_SIZE counter runs always see
+        torch.mul(src, 1.0, out=dst)
+        ev0.record(stream)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        kms = ev0.elapsed_time(ev1) / 3
+        copy_cal = {"bytes_each_way": nel * 8, "ms": cms, "GB/s_read_plus_write": 2 * nel * 8 / (cms * 1e-3) / 1e9,
+                    "kernel_ms": kms, "kernel_GB/s_read_plus_write": 2 * nel * 8 / (kms * 1e-3) / 1e9}
         del src, dst
 
     # --- rooflines: the dominant kernel of the build step (the JSON "roofline"),

@@ -205,8 +205,10 @@ int pxg_mg_destroy(pxg_mg * mg);
  * coarsest matrix closed again since the last factorisation)               */
 int pxg_mg_update(pxg_mg * mg);
 /* solve A x = b (device pointers, FP64); mode PXG_FP64: MG-PCG in FP64;
- * PXG_FP32: FP32 MG-PCG inner solves inside FP64 iterative refinement
- * (FP64 residual b - A x, FP64 CG inner products, FP64 coarsest solve).   */
+ * PXG_FP32: MG-PCG on FP32 matrices and vectors with an FP64 iterate,
+ * refined by replacing the CG residual with the FP64 residual b - A x each
+ * time it has fallen a hundredfold (FP64 inner products, FP64 coarsest
+ * solve; convergence judged on the FP64 residual).                        */
 int pxg_mg_solve(pxg_mg * mg, int mode, const double * b_dev, double * x_dev, double rtol, int maxit,
                  int * iters, double * relres);
 /* one V-cycle x = M^{-1} b (FP64), for testing                            */

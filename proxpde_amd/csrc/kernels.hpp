@@ -107,6 +107,9 @@ void jacobiDiag(int64_t n, T const * dinv, T const * b, T * x, double omega, hip
 // x += alpha p ; r -= alpha q with alpha = sc[ia] / sc[ib]  (scalars on device)
 template <typename T>
 void cgUpdate(int64_t n, T * x, T const * p, T * r, T const * q, double const * sc, int ia, int ib, hipStream_t s);
+// x += alpha p (FP64 iterate) ; r -= alpha q (FP32 residual), alpha = sc[ia] / sc[ib]
+void cgUpdateMixed(int64_t n, double * x, float const * p, float * r, float const * q, double const * sc, int ia,
+                   int ib, hipStream_t s);
 // p = z + beta p with beta = sc[ia] / sc[ib]
 template <typename T>
 void cgDirection(int64_t n, T * p, T const * z, double const * sc, int ia, int ib, hipStream_t s);

@@ -20,8 +20,9 @@
 // with rows of constrained fine dofs and columns of constrained coarse dofs
 // removed so Dirichlet rows never receive a coarse correction.
 // FP64 mode runs everything in double.  FP32 mode runs the preconditioned CG
-// and the cycle in float (FP64 dot products, FP64 coarsest solve) inside an
-// FP64 iterative refinement whose residual b - A x is formed in FP64.
+// and the cycle in float (FP64 dot products, FP64 coarsest solve) on an FP64
+// iterate, refined by replacing the CG residual with the FP64 residual
+// b - A x (see solveIR).
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -45,7 +46,6 @@ struct pxg_mg
   std::vector<std::array<int, 3>> dims;
   double omega = 2. / 3., coarseRtol = 1e-10;
   int nu1 = 2, nu2 = 2;
-  double innerRtol = 1e-4;
   hipStream_t s = nullptr;
   struct Level
   {
@@ -65,8 +65,8 @@ struct pxg_mg
   };
   std::vector<Level> lv;
   DevBuf<double> sc, partial;
-  DevBuf<double> r, z, p, q, d;
-  DevBuf<float> r32, z32, p32, q32, d32;
+  DevBuf<double> r, z, p, q;
+  DevBuf<float> r32, z32, p32, q32;
   // coarsest-level CG vectors
   DevBuf<double> cr, cz, cp, cq;
   // coarsest-level direct solve: inverse of the free-dof block
@@ -591,11 +591,21 @@ void residual64(pxg_mg * m, double const * b, double const * x, double * r)
   }
 }
 
+// FP32 + FP64 refinement without restarts: preconditioned CG on FP32
+// vectors and matrices (FP64 inner products) whose iterate is kept in FP64,
+// with the recursive FP32 residual replaced by the FP64 residual b - A x
+// whenever it has fallen by kReplace since the last replacement or reaches
+// the tolerance (reliable updates, van der Vorst and Ye, SISC 22 (2000)).
+// Every refinement is thereby one more CG step instead of a restarted inner
+// solve; convergence is decided on the FP64 residual only.
 int solveIR(pxg_mg * m, double const * b, double * x, double rtol, int maxit, double & relres)
 {
+  constexpr double kReplace = 1e-2;
   hipStream_t s = m->s;
   int64_t const n = m->lv[0].n;
-  double * r = m->r.p;
+  CsrView const A = viewOf(m->ctx[0]);
+  double * r64 = m->r.p;
+  float * r = m->r32.p, * z = m->z32.p, * p = m->p32.p, * q = m->q32.p;
   double * sc = m->sc.p;
   dot<double>(n, b, b, sc + 5, m->partial.p, s);
   double const bn = std::sqrt(readScalar(m, 5));
@@ -605,27 +615,47 @@ int solveIR(pxg_mg * m, double const * b, double * x, double rtol, int maxit, do
     relres = 0.;
     return 0;
   }
-  int total = 0;
-  for (int outer = 0; outer < 50; ++outer)
+  // true residual of x: its norm (the convergence measure) and its FP32
+  // rounding (the CG residual from here on)
+  auto trueResidual = [&]() -> double
   {
-    residual64(m, b, x, r);
-    dot<double>(n, r, r, sc + 6, m->partial.p, s);
+    residual64(m, b, x, r64);
+    dot<double>(n, r64, r64, sc + 6, m->partial.p, s);
     double const rn = std::sqrt(readScalar(m, 6));
     PXG_REQUIRE(std::isfinite(rn), PXG_E_NUMERIC, "non-finite residual in refinement");
-    relres = rn / bn;
-    if (relres <= rtol || total >= maxit)
-      break;
-    convert<float, double>(n, r, m->r32.p, s);
-    // inner FP32 solve of A d = r to a relative tolerance that need not go
-    // below what the outer target still requires
-    double const inner = std::max(m->innerRtol, 0.5 * rtol / relres);
-    double innerRel = 0.;
-    // inner CG uses its own r32/z32/p32/q32; the right-hand side is copied to b32
-    PXG_HIP(hipMemcpyAsync(m->lv[0].b32.p, m->r32.p, n * sizeof(float), hipMemcpyDeviceToDevice, s));
-    total += pcg<float>(m, m->lv[0].b32.p, m->d32.p, inner, maxit - total, innerRel, true);
-    addTo<double, float>(n, x, m->d32.p, s);
+    convert<float, double>(n, r64, r, s);
+    return rn;
+  };
+  double last = trueResidual();
+  relres = last / bn;
+  if (relres <= rtol)
+    return 0;
+  vcycle<float>(m, 0, r, z);
+  PXG_HIP(hipMemcpyAsync(p, z, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+  dot<float>(n, r, z, sc + 2, m->partial.p, s);
+  int it = 0;
+  while (it < maxit)
+  {
+    spmv<float>(A, p, q, s);
+    dot<float>(n, p, q, sc + 3, m->partial.p, s);
+    cgUpdateMixed(n, x, p, r, q, sc, 2, 3, s);
+    ++it;
+    dot<float>(n, r, r, sc + 1, m->partial.p, s);
+    double const rr = std::sqrt(readScalar(m, 1));
+    PXG_REQUIRE(std::isfinite(rr), PXG_E_NUMERIC, "CG breakdown (non-finite residual)");
+    if (rr <= rtol * bn || rr <= kReplace * last)
+    {
+      last = trueResidual();
+      relres = last / bn;
+      if (relres <= rtol)
+        break;
+    }
+    vcycle<float>(m, 0, r, z);
+    dot<float>(n, r, z, sc + 4, m->partial.p, s);
+    cgDirection<float>(n, p, z, sc, 4, 2, s);
+    copyScalar(m, 2, 4);
   }
-  return total;
+  return it;
 }
 } // namespace
 
@@ -682,12 +712,10 @@ int pxg_mg_create(pxg_mg ** out, int nlev, pxg_ctx ** levels, const int * dims, 
     m->z.resize(n0);
     m->p.resize(n0);
     m->q.resize(n0);
-    m->d.resize(n0);
     m->r32.resize(n0);
     m->z32.resize(n0);
     m->p32.resize(n0);
     m->q32.resize(n0);
-    m->d32.resize(n0);
     m->cr.resize(nc);
     m->cz.resize(nc);
     m->cp.resize(nc);

@@ -535,6 +535,19 @@ __global__ void k_cg_update(int64_t n, T * __restrict__ x, T const * __restrict_
   r[i] -= alpha * q[i];
 }
 
+// FP64 iterate, FP32 residual: x += alpha p (in FP64) ; r -= alpha q
+__global__ void k_cg_update_mixed(int64_t n, double * __restrict__ x, float const * __restrict__ p,
+                                  float * __restrict__ r, float const * __restrict__ q, double const * __restrict__ sc,
+                                  int ia, int ib)
+{
+  int64_t const i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n)
+    return;
+  double const alpha = sc[ib] != 0. ? sc[ia] / sc[ib] : 0.;
+  x[i] += alpha * (double)p[i];
+  r[i] -= (float)alpha * q[i];
+}
+
 template <typename T>
 __global__ void k_cg_direction(int64_t n, T * __restrict__ p, T const * __restrict__ z, double const * __restrict__ sc,
                                int ia, int ib)
@@ -738,6 +751,13 @@ template <typename T>
 void cgUpdate(int64_t n, T * x, T const * p, T * r, T const * q, double const * sc, int ia, int ib, hipStream_t s)
 {
   k_cg_update<T><<<blocksFor(n), kBlock, 0, s>>>(n, x, p, r, q, sc, ia, ib);
+  PXG_HIP(hipGetLastError());
+}
+
+void cgUpdateMixed(int64_t n, double * x, float const * p, float * r, float const * q, double const * sc, int ia,
+                   int ib, hipStream_t s)
+{
+  k_cg_update_mixed<<<blocksFor(n), kBlock, 0, s>>>(n, x, p, r, q, sc, ia, ib);
   PXG_HIP(hipGetLastError());
 }
 

@@ -9,7 +9,7 @@
 //   proxpde::kappa_solve(kappa, b, ctxs, inject, mg, stream, rtol, mode, maxit, fp32_levels, fine_fp64)
 //       forward : re-assembles K(kappa) on every multigrid level (kappa injected to
 //                 the coarse dofs through `inject`), closes, and solves K u = b with
-//                 MG-PCG (mode 0: FP64; mode 1: FP32 inner solves in FP64 refinement).
+//                 MG-PCG (mode 0: FP64; mode 1: FP32 CG refined by FP64 residuals).
 //                 fp32_levels: element matrices of all but the coarsest level in FP32;
 //                 the refinement residual then uses the FP64 matrix-free operator.
 //                 fine_fp64: the finest level is still assembled in FP64 (its FP32
