@@ -113,6 +113,19 @@ __device__ inline void csrOut(int64_t row, T s, T const * __restrict__ x, T cons
     y[row] += s;
 }
 
+// Row block of a workgroup.  Workgroups are dispatched to the 8 XCDs in
+// turn (workgroup b to XCD b % 8); each XCD has its own L2.  Mapping the
+// workgroups of XCD x to the x-th contiguous run of row blocks keeps the x
+// entries an XCD gathers within a narrow column band, so they stay in its
+// L2 instead of every XCD streaming all of x.  A bijection for any count.
+__device__ inline unsigned xcdRowBlock()
+{
+  constexpr unsigned kXcd = 8;
+  unsigned const nb = gridDim.x, b = blockIdx.x;
+  unsigned const x = b % kXcd, k = b / kXcd, base = nb / kXcd, extra = nb % kXcd;
+  return x * base + min(x, extra) + k;
+}
+
 // row sums of the products staged in LDS (entry k of the block in prod[k]):
 // G lanes per row (chosen from the mean row length); the lanes of one group
 // run the same rows, so the group stays converged
@@ -158,7 +171,8 @@ __global__ void __launch_bounds__(kBlock)
 {
   constexpr int kSlots = kStreamSlots / kBlock; // entries per thread
   __shared__ T prod[kStreamSlots];
-  int64_t const r0 = blockRow[blockIdx.x], r1 = blockRow[blockIdx.x + 1];
+  unsigned const qb = xcdRowBlock();
+  int64_t const r0 = blockRow[qb], r1 = blockRow[qb + 1];
   if (r0 == r1)
     return; // a window covered by a row that started earlier
   int64_t const base = rowptr[r0], end = rowptr[r1];
@@ -202,7 +216,8 @@ __global__ void __launch_bounds__(kBlock)
   constexpr int kSlots = kStreamSlots / kBlock;
   static_assert(kSlots == 8, "eight 2-bit fields per 16-bit word");
   __shared__ T prod[kStreamSlots];
-  int64_t const r0 = blockRow[blockIdx.x], r1 = blockRow[blockIdx.x + 1];
+  unsigned const qb = xcdRowBlock();
+  int64_t const r0 = blockRow[qb], r1 = blockRow[qb + 1];
   if (r0 == r1)
     return;
   int64_t const base = rowptr[r0];
@@ -210,9 +225,9 @@ __global__ void __launch_bounds__(kBlock)
   int const t = threadIdx.x;
   if (len > 0) // block-uniform
   {
-    size_t const blk = (size_t)blockIdx.x * kStreamSlots;
-    uint32_t const hi = phi[(size_t)blockIdx.x * kBlock + t];
-    int32_t const cb = pbase[blockIdx.x];
+    size_t const blk = (size_t)qb * kStreamSlots;
+    uint32_t const hi = phi[(size_t)qb * kBlock + t];
+    int32_t const cb = pbase[qb];
     T vv[kSlots];
     int32_t cc[kSlots];
 #pragma unroll

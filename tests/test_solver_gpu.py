@@ -12,9 +12,18 @@ from proxpde_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
+def _torch_stream(a):
+    """Run the context on torch's current stream: the device-pointer entry
+    points read torch tensors made on that stream (the context's own stream
+    is non-blocking and would not wait for them)."""
+    import torch
+    a.set_stream(torch.cuda.current_stream().cuda_stream)
+    return a
+
+
 def _hex_level(fe, n, kappa_fn, sides=True):
     prob = W.Problem(fe, (n, n, n))
-    a = prob.assembler()
+    a = _torch_stream(prob.assembler())
     dofs = prob.all_sides() if sides else np.zeros(0, np.uint32)
     a.set_dirichlet(dofs, np.zeros(len(dofs)))
     kap = kappa_fn(prob.dof_coords())
@@ -76,7 +85,7 @@ def test_packed_spmv_bitwise(gpu, fe, n, packed):
     import torch
     prob = W.Problem(fe, (n, n, n))
     perm = np.random.default_rng(1).permutation(prob.ndofs).astype(np.uint32)
-    a = prob.assembler()
+    a = _torch_stream(prob.assembler())
     a.set_dofmap(prob.ndofs, perm[prob.emap])
     dofs = perm[prob.all_sides()]
     a.set_dirichlet(dofs, np.zeros(len(dofs)))
@@ -167,7 +176,7 @@ def test_prolongation_known_answer(native, gpu):
     ctxs = []
     for n in ((6, 4, 2), (3, 2, 1)):
         prob = W.Problem("hex_q1", n)
-        a = prob.assembler()
+        a = _torch_stream(prob.assembler())
         a.set_dirichlet(np.zeros(0, np.uint32), np.zeros(0))
         a.set_coefficient(np.ones(prob.ndofs))
         a.build_pattern()
@@ -219,7 +228,7 @@ def test_matrix_free_apply_and_gradient(oracle, gpu, fe, mesh, n, pert, use_bc):
         xyz = W.perturb_interior(prob.xyz, 1.0 / max(n), seed=1, amount=pert)
         prob.set_coords(xyz)
         o.set_coords(xyz)
-    a = prob.assembler()
+    a = _torch_stream(prob.assembler())
     dofs = prob.all_sides()
     a.set_dirichlet(dofs, np.zeros(len(dofs)))
     cons = np.zeros(prob.ndofs, np.uint8)
@@ -255,7 +264,7 @@ def test_gradient_central_differences(gpu):
     """d/dkappa (ybar . K(kappa) x) against central differences, step 1e-6."""
     import torch
     prob = W.Problem("hex_q2", (2, 2, 2))
-    a = prob.assembler()
+    a = _torch_stream(prob.assembler())
     rng = np.random.default_rng(4)
     kap = np.exp(0.3 * rng.normal(size=prob.ndofs))
     x = rng.normal(size=(prob.ndofs, 2))
