@@ -76,10 +76,14 @@ struct Shape
   static constexpr int KS = NQ * DIM;                     // stiffness rows of C
   static constexpr int K = KS + (MASS ? NQ : 0);          // all rows of C
   static constexpr int STEPS = (K + 3) / 4;               // MFMA k steps of 4
+  // single-tile elements (n <= 16): q-major rows, one quadrature point per
+  // step with rows a = lane/16 (see k_element_mfma)
+  static constexpr bool QM = NT == 1;
+  static constexpr int MROWS = QM ? 4 : DIM + (MASS ? 1 : 0); // rows of the C factor per point
 };
 
 // per-wave LDS (kept small: two blocks of kWaves waves must fit one CU's LDS)
-template <typename T, int N, int NPE, int NQ, int DIM, bool MASS>
+template <typename T, int N, int NPE, int NQ, int MROWS>
 struct WaveLds
 {
   double x[NPE * 3];
@@ -87,8 +91,8 @@ struct WaveLds
   double bcv[N];
   // rows of the per-point C operand factor: a < DIM the metric row
   // s_q (J^T J)^{-1}[a][:], a = DIM the mass row (m_q, 0, 0) when mass
-  // terms are built
-  double M[NQ][DIM + (MASS ? 1 : 0)][3];
+  // terms are built (always, and zero rows after it, for q-major elements)
+  double M[NQ][MROWS][3];
   double f[N];  // lifting accumulator of the Dirichlet elimination
   double dg[sizeof(T) == 4 ? N : 1]; // FP32: diagonal of Ke, the flush threshold of noise entries
   int flag[N];
@@ -106,17 +110,20 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
                    int * __restrict__ work)
 {
   using S = Shape<N, DIM, NQ, MASS>;
-  using L = WaveLds<T, N, NPE, NQ, DIM, MASS>;
+  using L = WaveLds<T, N, NPE, NQ, Shape<N, DIM, NQ, MASS>::MROWS>;
   using Acc = typename Mfma<T>::Acc;
   constexpr int NT = S::NT, KS = S::KS, STEPS = S::STEPS;
+  constexpr bool QM = S::QM;
   bool const needK = (flags & 10) != 0;
 
   __shared__ double sGeo[NQ * NPE * DIM];
   __shared__ double sPhi[NQ * N];
+  // reference gradients: g_j(q, b) at (q N + j) DIM + b, or at (q DIM + b) N + j
+  // for q-major elements
   __shared__ double sGrad[NQ * N * DIM];
   __shared__ double sW[NQ];
-  __shared__ T sA[NT * STEPS * 64]; // constant operand fragments
-  __shared__ int sKA[STEPS * 4];      // row k = 4 s + kr of C: (q << 2 | a), -1 for padding
+  __shared__ T sA[QM ? 1 : NT * STEPS * 64]; // constant operand fragments
+  __shared__ int sKA[QM ? 1 : STEPS * 4];     // row k = 4 s + kr of C: (q << 2 | a), -1 for padding
   __shared__ L lds[kWaves];
 
   int const tid = threadIdx.x;
@@ -125,7 +132,15 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
   for (int k = tid; k < NQ * N; k += blockDim.x)
     sPhi[k] = phiG[k];
   for (int k = tid; k < NQ * N * DIM; k += blockDim.x)
-    sGrad[k] = gradG[k];
+  {
+    if constexpr (QM)
+    {
+      int const q = k / (N * DIM), j = (k / DIM) % N, b = k % DIM;
+      sGrad[(q * DIM + b) * N + j] = gradG[k];
+    }
+    else
+      sGrad[k] = gradG[k];
+  }
   for (int k = tid; k < NQ; k += blockDim.x)
     sW[k] = wG[k];
 
@@ -133,10 +148,13 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
   L & W = lds[wave];
 
   // constant operand fragments: sA[(ti STEPS + s) 64 + lane] = A[16 ti + lane%16][4 s + lane/16]
-  for (int k = tid; k < NT * STEPS * 64; k += blockDim.x)
-    sA[k] = afrag[k];
-  for (int k = tid; k < STEPS * 4; k += blockDim.x)
-    sKA[k] = k < KS ? (((k / DIM) << 2) | (k % DIM)) : ((MASS && k < KS + NQ) ? (((k - KS) << 2) | DIM) : -1);
+  if constexpr (!QM)
+  {
+    for (int k = tid; k < NT * STEPS * 64; k += blockDim.x)
+      sA[k] = afrag[k];
+    for (int k = tid; k < STEPS * 4; k += blockDim.x)
+      sKA[k] = k < KS ? (((k / DIM) << 2) | (k % DIM)) : ((MASS && k < KS + NQ) ? (((k - KS) << 2) | DIM) : -1);
+  }
   __syncthreads();
 
   int const kr = lane >> 4, jc = lane & 15;
@@ -303,11 +321,21 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
         W.M[q][2][1] = s * I12;
         W.M[q][2][2] = s * I22;
       }
-      if constexpr (MASS)
+      if constexpr (MASS || QM)
       {
         W.M[q][DIM][0] = m;
         W.M[q][DIM][1] = 0.;
         W.M[q][DIM][2] = 0.;
+      }
+      if constexpr (QM)
+      {
+#pragma unroll
+        for (int a = DIM + 1; a < 4; ++a)
+        {
+          W.M[q][a][0] = 0.;
+          W.M[q][a][1] = 0.;
+          W.M[q][a][2] = 0.;
+        }
       }
     }
     waveSync();
@@ -321,58 +349,92 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
 #pragma unroll
       for (int tj = 0; tj < NT; ++tj)
         acc[ti][tj] = Acc{0, 0, 0, 0};
-    // C[4s + kr][16 tj + jc] for row code (q, a) of step s: the metric row
-    // (or the mass row) times the reference gradient (or phi) of dof j at q;
-    // indices are clamped and the value masked, so there are no branches
-    auto cval = [&](int s, int tj) -> T
+    if constexpr (QM)
     {
-      int const code = sKA[4 * s + kr];
-      int const j = 16 * tj + jc;
-      bool const valid = code >= 0 && j < N;
-      int const c = code < 0 ? 0 : code;
-      int const q = c >> 2, a = c & 3;
-      int const jj = j < N ? j : N - 1;
-      double const * m = W.M[q][a];
-      double const * g = sGrad + (q * N + jj) * DIM;
-      double g0 = g[0];
-      if constexpr (MASS)
-        g0 = a == DIM ? sPhi[q * N + jj] : g0;
-      double v = m[0] * g0 + m[1] * g[1];
-      if constexpr (DIM == 3)
-        v += m[2] * g[2];
-      return valid ? (T)v : T(0);
-    };
-    // software-pipelined: the operands of step s+1 are read from LDS while the
-    // matrix instructions of step s run.  Not unrolled: it keeps the kernel
-    // at or below 128 VGPRs, i.e. two blocks (16 waves) per CU
-    T aC[NT], bC[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-    {
-      aC[t] = sA[(t * STEPS) * 64 + lane];
-      bC[t] = cval(0, t);
-    }
+      // q-major (one tile): step q, lane row a = kr.  The lane's dof j = jc
+      // gives A[j][4q + a] = g_j(q, a) (phi_j(q) on the mass row) and
+      // C[4q + a][j] = row a of the C factor . g_j(q); rows past the mass row
+      // are zero in the factor, their A value is any finite table value
+      int const jj = jc < N ? jc : N - 1;
+      bool const massRow = MASS && kr == DIM;
+      auto operands = [&](int q, T & aV, T & bV)
+      {
+        double const m0 = W.M[q][kr][0], m1 = W.M[q][kr][1], m2 = W.M[q][kr][2];
+        double g0 = sGrad[(q * DIM + 0) * N + jj];
+        double const g1 = sGrad[(q * DIM + 1) * N + jj];
+        double const g2 = DIM == 3 ? sGrad[(q * DIM + (DIM - 1)) * N + jj] : 0.;
+        if constexpr (MASS)
+          g0 = massRow ? sPhi[q * N + jj] : g0;
+        bV = (T)(m0 * g0 + m1 * g1 + m2 * g2);
+        aV = (T)(kr == 1 ? g1 : (kr == 2 && DIM == 3) ? g2 : g0);
+      };
+      T aC, bC;
+      operands(0, aC, bC);
 #pragma unroll 1
-    for (int s = 0; s < STEPS; ++s)
-    {
-      int const sn = s + 1 < STEPS ? s + 1 : s;
-      T aN[NT], bN[NT];
-#pragma unroll
-      for (int t = 0; t < NT; ++t)
+      for (int q = 0; q < NQ; ++q)
       {
-        aN[t] = sA[(t * STEPS + sn) * 64 + lane];
-        bN[t] = cval(sn, t);
+        T aN, bN;
+        operands(q + 1 < NQ ? q + 1 : q, aN, bN);
+        acc[0][0] = Mfma<T>::mma(aC, bC, acc[0][0]);
+        aC = aN;
+        bC = bN;
       }
-#pragma unroll
-      for (int tj = 0; tj < NT; ++tj)
-#pragma unroll
-        for (int ti = 0; ti <= tj; ++ti)
-          acc[ti][tj] = Mfma<T>::mma(aC[ti], bC[tj], acc[ti][tj]);
-#pragma unroll
+    }
+    else
+    {
+      // C[4s + kr][16 tj + jc] for row code (q, a) of step s: the metric row
+      // (or the mass row) times the reference gradient (or phi) of dof j at q;
+      // indices are clamped and the value masked, so there are no branches
+      auto cval = [&](int s, int tj) -> T
+      {
+        int const code = sKA[4 * s + kr];
+        int const j = 16 * tj + jc;
+        bool const valid = code >= 0 && j < N;
+        int const c = code < 0 ? 0 : code;
+        int const q = c >> 2, a = c & 3;
+        int const jj = j < N ? j : N - 1;
+        double const * m = W.M[q][a];
+        double const * g = sGrad + (q * N + jj) * DIM;
+        double g0 = g[0];
+        if constexpr (MASS)
+          g0 = a == DIM ? sPhi[q * N + jj] : g0;
+        double v = m[0] * g0 + m[1] * g[1];
+        if constexpr (DIM == 3)
+          v += m[2] * g[2];
+        return valid ? (T)v : T(0);
+      };
+      // software-pipelined: the operands of step s+1 are read from LDS while the
+      // matrix instructions of step s run.  Not unrolled: it keeps the kernel
+      // at or below 128 VGPRs, i.e. two blocks (16 waves) per CU
+      T aC[NT], bC[NT];
+  #pragma unroll
       for (int t = 0; t < NT; ++t)
       {
-        aC[t] = aN[t];
-        bC[t] = bN[t];
+        aC[t] = sA[(t * STEPS) * 64 + lane];
+        bC[t] = cval(0, t);
+      }
+  #pragma unroll 1
+      for (int s = 0; s < STEPS; ++s)
+      {
+        int const sn = s + 1 < STEPS ? s + 1 : s;
+        T aN[NT], bN[NT];
+  #pragma unroll
+        for (int t = 0; t < NT; ++t)
+        {
+          aN[t] = sA[(t * STEPS + sn) * 64 + lane];
+          bN[t] = cval(sn, t);
+        }
+  #pragma unroll
+        for (int tj = 0; tj < NT; ++tj)
+  #pragma unroll
+          for (int ti = 0; ti <= tj; ++ti)
+            acc[ti][tj] = Mfma<T>::mma(aC[ti], bC[tj], acc[ti][tj]);
+  #pragma unroll
+        for (int t = 0; t < NT; ++t)
+        {
+          aC[t] = aN[t];
+          bC[t] = bN[t];
+        }
       }
     }
     // ---- 4. the upper triangle goes from the accumulators straight to the
@@ -518,7 +580,9 @@ void launchMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, double
   // cache key: precision and mass rows
   int const key = (sizeof(T) == 8 ? 1 : 2) | (MASS ? 4 : 0);
   T const * dfrag = nullptr;
-  if constexpr (sizeof(T) == 8)
+  if constexpr (Shape<N, DIM, NQ, MASS>::QM)
+    ; // q-major elements read the gradient table directly
+  else if constexpr (sizeof(T) == 8)
   {
     if (tab.afragKey64 != key)
     {
