@@ -471,15 +471,27 @@ __global__ void k_set_int(int * p, int v) { *p = v; }
 
 // ---- rebuild into the pruned layout, staged by row blocks ------------------
 // A stage block is a run of rows whose element-matrix rows (the (element,
-// local row) pairs of their dofs, dofContrib) number at most kStagePairs.
+// local row) pairs of their dofs, dofContrib) number at most P (stagePairs()).
 // The kernel copies those element-matrix rows to LDS with coalesced loads
 // and sums every structural entry of the block's rows from LDS in segment
 // order, so the sums are bitwise those of k_scatter.  Per structural entry
 // (info): its contribution offset in the block (bits 0-14) and the layout's
 // keep flag (bit 15); per contribution (sid): its LDS slot, pair * n + local
 // column.
-constexpr int kStagePairs = 128;
-static_assert(kStagePairs <= 256, "one pair id per thread of a stage block");
+constexpr int kStagePairsMax = 128;
+static_assert(kStagePairsMax <= 256, "one pair id per thread of a stage block");
+// pairs per stage block (PXG_STAGE_PAIRS = 32, 64 or 128 overrides the
+// default for launch tuning; fewer pairs, less LDS, more blocks per CU)
+int stagePairs()
+{
+  static int const v = []
+  {
+    char const * e = std::getenv("PXG_STAGE_PAIRS");
+    int x = e ? std::atoi(e) : 64;
+    return (x == 32 || x == 128) ? x : 64;
+  }();
+  return v;
+}
 
 struct StageBlock
 {
@@ -530,8 +542,8 @@ __global__ void k_max_pairs(int64_t nrows, uint32_t const * __restrict__ seg, in
 }
 
 // stage block q = the rows whose first pair falls in [q cap, (q+1) cap):
-// with cap = kStagePairs - (most pairs of a row) a block stages at most
-// kStagePairs pairs; every q is written exactly once
+// with cap = P - (most pairs of a row) a block stages at most P pairs; every q
+// is written exactly once
 __global__ void k_stage_rows(int64_t nrows, uint32_t const * __restrict__ seg, int64_t cap, int64_t nb,
                              int32_t * __restrict__ r0s)
 {
@@ -575,20 +587,20 @@ __global__ void k_stage_maps(int n, int ns, StageBlock const * __restrict__ bloc
   }
 }
 
-template <typename T, int N, int NS>
+template <typename T, int N, int NS, int P>
 __global__ void __launch_bounds__(kBlock)
     k_scatter_staged(StageBlock const * __restrict__ blocks, uint32_t const * __restrict__ dofContrib,
                      T const * __restrict__ ke, uint16_t const * __restrict__ info, uint16_t const * __restrict__ sid,
                      T * __restrict__ valC, T * __restrict__ valS, int * __restrict__ changed)
 {
-  // a block has at most kStagePairs * N contributions, so at most that many
+  // a block has at most P * N contributions, so at most that many
   // structural entries and staged element-matrix values
-  constexpr int kMax = kStagePairs * N;
+  constexpr int kMax = P * N;
   constexpr int kPer = (kMax + kBlock - 1) / kBlock;
   __shared__ T lk[kMax];
   __shared__ uint16_t linfo[kMax];
   __shared__ uint16_t lsid[kMax];
-  __shared__ uint32_t lpc[kStagePairs];
+  __shared__ uint32_t lpc[P];
   __shared__ uint32_t wsum[kBlock / 64];
   StageBlock const B = blocks[blockIdx.x];
   int const t = threadIdx.x;
@@ -599,7 +611,7 @@ __global__ void __launch_bounds__(kBlock)
   //    loads are issued before the first LDS store (indices clamped)
   {
     uint32_t pc = 0;
-    if (t < kStagePairs)
+    if (t < P)
       pc = dofContrib[B.p0 + min(t, np - 1)];
     uint32_t ri[kPer], rs[kPer];
 #pragma unroll
@@ -728,9 +740,19 @@ void launchRebuild(pxg_ctx * c, T const * ke, T * valC, T * valS, hipStream_t s)
 {
   if (c->stageValid && c->nStage > 0)
   {
-    k_scatter_staged<T, N, keRowStride(N)><<<(unsigned)c->nStage, kBlock, 0, s>>>(
-        reinterpret_cast<StageBlock const *>(c->stageBlk.p), c->dofContrib.p, ke, c->stageInfo.p, c->stageSid.p,
-        valC, valS, c->changed.p);
+    auto go = [&](auto pc)
+    {
+      constexpr int P = decltype(pc)::value;
+      k_scatter_staged<T, N, keRowStride(N), P><<<(unsigned)c->nStage, kBlock, 0, s>>>(
+          reinterpret_cast<StageBlock const *>(c->stageBlk.p), c->dofContrib.p, ke, c->stageInfo.p, c->stageSid.p,
+          valC, valS, c->changed.p);
+    };
+    if (c->stagePairs == 32)
+      go(std::integral_constant<int, 32>{});
+    else if (c->stagePairs == 128)
+      go(std::integral_constant<int, 128>{});
+    else
+      go(std::integral_constant<int, 64>{});
     PXG_HIP(hipGetLastError());
     return;
   }
@@ -1488,7 +1510,8 @@ void buildStageMaps(pxg_ctx * c)
   int const n = c->fe.n;
   int64_t const nr = c->ndofs, npairs = c->nelem * n;
   c->stageValid = false;
-  if ((int64_t)kStagePairs * n > 0x7fff || nr >= ((int64_t)1 << 31) || npairs <= 0)
+  int const P = stagePairs();
+  if ((int64_t)P * n > 0x7fff || nr >= ((int64_t)1 << 31) || npairs <= 0)
     return; // contribution offsets of a block must fit 15 bits
   c->changed.resize(1);
   k_set_int<<<1, 1, 0, s>>>(c->changed.p, 0);
@@ -1497,7 +1520,7 @@ void buildStageMaps(pxg_ctx * c)
   int maxPairs = 0;
   PXG_HIP(hipMemcpyAsync(&maxPairs, c->changed.p, sizeof(int), hipMemcpyDeviceToHost, s));
   PXG_HIP(hipStreamSynchronize(s));
-  int64_t const cap = kStagePairs - std::max(maxPairs, 1);
+  int64_t const cap = P - std::max(maxPairs, 1);
   if (cap < 1)
     return; // a dof with more element-matrix rows than a block stages
   int64_t const nb = npairs / cap + 1;
@@ -1520,6 +1543,7 @@ void buildStageMaps(pxg_ctx * c)
   PXG_HIP(hipMemcpyAsync(&bad, c->changed.p, sizeof(int), hipMemcpyDeviceToHost, s));
   PXG_HIP(hipStreamSynchronize(s));
   c->nStage = nb;
+  c->stagePairs = P;
   c->stageValid = bad == 0;
 }
 
