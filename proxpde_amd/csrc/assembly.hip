@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace pxg
@@ -475,9 +476,9 @@ __global__ void k_set_int(int * p, int v) { *p = v; }
 // The kernel copies those element-matrix rows to LDS with coalesced loads
 // and sums every structural entry of the block's rows from LDS in segment
 // order, so the sums are bitwise those of k_scatter.  Per structural entry
-// (info): its contribution offset in the block (bits 0-14) and the layout's
-// keep flag (bit 15); per contribution (sid): its LDS slot, pair * n + local
-// column.
+// (info): its contribution offset in the block (bits 0-13), the diagonal flag
+// (bit 14) and the layout's keep flag (bit 15); per contribution (sid): its
+// LDS slot, pair * n + local column.
 constexpr int kStagePairsMax = 128;
 static_assert(kStagePairsMax <= 256, "one pair id per thread of a stage block");
 // pairs per stage block (PXG_STAGE_PAIRS = 32, 64 or 128 overrides the
@@ -493,12 +494,16 @@ int stagePairs()
   return v;
 }
 
+template <typename T>
+using OtherT = std::conditional_t<sizeof(T) == 8, float, double>;
+
 struct StageBlock
 {
   uint32_t s0, s1;   // structural entries
   uint32_t p0, p1;   // (element, local row) pairs in dofContrib
   uint32_t id0, id1; // contributions (contrib / sid order)
   uint32_t k0;       // layout position of the first kept entry
+  uint32_t r0, r1;   // rows
   uint32_t pad;
 };
 
@@ -518,17 +523,34 @@ __global__ void k_stage_blocks(int64_t nb, int32_t const * __restrict__ r0s, int
   B.id0 = segPtr[B.s0];
   B.id1 = segPtr[B.s1];
   B.k0 = pos[B.s0];
+  B.r0 = (uint32_t)r0;
+  B.r1 = (uint32_t)r1;
   B.pad = 0;
   blocks[b] = B;
 }
 
-// largest number of (element, local row) pairs of a dof
+// bit 14 of the info word marks the diagonal entry of each row
+__global__ void k_stage_diag(int64_t nrows, uint32_t const * __restrict__ diagPos, uint16_t * __restrict__ info)
+{
+  int64_t const r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nrows)
+    info[diagPos[r]] |= (uint16_t)0x4000u;
+}
+
+// largest number of (element, local row) pairs of a dof in out[0]; out[1]
+// set if some dof has none
 __global__ void k_max_pairs(int64_t nrows, uint32_t const * __restrict__ seg, int * __restrict__ out)
 {
   __shared__ int sh[kBlock];
-  int m = 0;
+  int m = 0, empty = 0;
   for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * kBlock)
-    m = max(m, (int)(seg[r + 1] - seg[r]));
+  {
+    int const np = (int)(seg[r + 1] - seg[r]);
+    m = max(m, np);
+    empty |= np == 0;
+  }
+  if (empty)
+    atomicOr(out + 1, 1);
   sh[threadIdx.x] = m;
   __syncthreads();
   for (int off = kBlock / 2; off > 0; off >>= 1)
@@ -587,11 +609,17 @@ __global__ void k_stage_maps(int n, int ns, StageBlock const * __restrict__ bloc
   }
 }
 
+// Besides the matrix values the block writes, for its rows, the inverse
+// diagonal of the closed matrix (dinv in T, dinvO in the other precision, as
+// k_diag_inverse_split and the cast of the close would) and adds the Dirichlet
+// lifting of the elements to b (as k_rhs_scatter would, same order).
 template <typename T, int N, int NS, int P>
 __global__ void __launch_bounds__(kBlock)
     k_scatter_staged(StageBlock const * __restrict__ blocks, uint32_t const * __restrict__ dofContrib,
                      T const * __restrict__ ke, uint16_t const * __restrict__ info, uint16_t const * __restrict__ sid,
-                     T * __restrict__ valC, T * __restrict__ valS, int * __restrict__ changed)
+                     T * __restrict__ valC, T * __restrict__ valS, int * __restrict__ changed,
+                     uint32_t const * __restrict__ dofSeg, double const * __restrict__ fe, double * __restrict__ b,
+                     T * __restrict__ dinv, OtherT<T> * __restrict__ dinvO)
 {
   // a block has at most P * N contributions, so at most that many
   // structural entries and staged element-matrix values
@@ -601,7 +629,9 @@ __global__ void __launch_bounds__(kBlock)
   __shared__ uint16_t linfo[kMax];
   __shared__ uint16_t lsid[kMax];
   __shared__ uint32_t lpc[P];
-  __shared__ uint32_t wsum[kBlock / 64];
+  __shared__ double lfe[P];       // element lifting value of each pair
+  __shared__ uint32_t lseg[P + 1]; // first pair of each row of the block
+  __shared__ uint32_t wsum[2 * (kBlock / 64)];
   StageBlock const B = blocks[blockIdx.x];
   int const t = threadIdx.x;
   int const np = (int)(B.p1 - B.p0), ne = (int)(B.s1 - B.s0), nid = (int)(B.id1 - B.id0);
@@ -609,10 +639,16 @@ __global__ void __launch_bounds__(kBlock)
     return; // rows of dofs in no element (then np = nid = 0 too)
   // 1. the block's pair ids, entry info and contribution slots -> LDS; all
   //    loads are issued before the first LDS store (indices clamped)
+  int const nr = (int)(B.r1 - B.r0); // at most P: every row has a pair
+  double bRow = 0.;
   {
-    uint32_t pc = 0;
+    uint32_t pc = 0, sg = 0;
     if (t < P)
       pc = dofContrib[B.p0 + min(t, np - 1)];
+    if (t <= P)
+      sg = dofSeg[B.r0 + min(t, nr)];
+    if (t < nr)
+      bRow = b[B.r0 + t];
     uint32_t ri[kPer], rs[kPer];
 #pragma unroll
     for (int m = 0; m < kPer; ++m)
@@ -623,6 +659,8 @@ __global__ void __launch_bounds__(kBlock)
     }
     if (t < np)
       lpc[t] = pc;
+    if (t <= nr)
+      lseg[t] = sg - B.p0;
 #pragma unroll
     for (int m = 0; m < kPer; ++m)
     {
@@ -634,7 +672,7 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
   __syncthreads();
-  // 2. element-matrix rows of the pairs -> LDS
+  // 2. element-matrix rows and lifting values of the pairs -> LDS
   {
     int const nst = np * N;
     T rv[kPer];
@@ -645,55 +683,82 @@ __global__ void __launch_bounds__(kBlock)
       int const p = i / N, lj = i - p * N;
       rv[m] = ke[(size_t)lpc[p] * NS + lj];
     }
+    double const fv = t < np ? fe[lpc[t]] : 0.;
 #pragma unroll
     for (int m = 0; m < kPer; ++m)
       if (t + m * kBlock < nst)
         lk[t + m * kBlock] = rv[m];
+    if (t < np)
+      lfe[t] = fv;
   }
   __syncthreads();
   // 3. structural entries in rounds of kBlock, from LDS; kept ones go to the
-  //    next layout positions in order, pruned ones to valS
-  uint32_t kept = B.k0;
+  //    next layout positions in order, pruned ones to valS; the k-th
+  //    diagonal entry of the block is that of row r0 + k
+  uint32_t kept = B.k0, diag = B.r0;
   int const lane = t & 63, w = t >> 6;
+  uint64_t const below = (1ull << lane) - 1ull;
   bool flip = false;
   for (int r = 0; r < ne; r += kBlock)
   {
     int const i = r + t;
     bool const v = i < ne;
     uint32_t const inf = v ? linfo[i] : 0u;
-    uint32_t const nx = i + 1 < ne ? (uint32_t)(linfo[i + 1] & 0x7fffu) : (uint32_t)nid;
-    uint32_t const off = inf & 0x7fffu;
+    uint32_t const nx = i + 1 < ne ? (uint32_t)(linfo[i + 1] & 0x3fffu) : (uint32_t)nid;
+    uint32_t const off = inf & 0x3fffu;
     uint32_t const cnt = v ? nx - off : 0u;
-    bool const keep = v && (inf >> 15);
+    bool const keep = v && (inf & 0x8000u);
+    bool const isDiag = v && (inf & 0x4000u);
     T acc = T(0);
     for (uint32_t j = 0; j < cnt; ++j)
       acc += lk[lsid[off + j]];
-    uint64_t const m = __ballot(keep);
-    uint32_t const below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    uint64_t const mk = __ballot(keep), md = __ballot(isDiag);
     if (lane == 0)
-      wsum[w] = (uint32_t)__popcll(m);
+    {
+      wsum[w] = (uint32_t)__popcll(mk);
+      wsum[kBlock / 64 + w] = (uint32_t)__popcll(md);
+    }
     __syncthreads();
-    uint32_t before = 0, total = 0;
+    uint32_t beforeK = 0, totalK = 0, beforeD = 0, totalD = 0;
 #pragma unroll
     for (int q = 0; q < kBlock / 64; ++q)
     {
-      before += q < w ? wsum[q] : 0u;
-      total += wsum[q];
+      beforeK += q < w ? wsum[q] : 0u;
+      totalK += wsum[q];
+      beforeD += q < w ? wsum[kBlock / 64 + q] : 0u;
+      totalD += wsum[kBlock / 64 + q];
     }
     if (v)
     {
       bool const now = fabs((double)acc) > kDefaultPrune;
       if (keep)
-        valC[kept + before + below] = acc;
+        valC[kept + beforeK + (uint32_t)__popcll(mk & below)] = acc;
       else
         valS[B.s0 + i] = acc;
       flip |= now != keep;
+      if (isDiag)
+      {
+        uint32_t const row = diag + beforeD + (uint32_t)__popcll(md & below);
+        T const di = (fabs((double)acc) <= kDefaultPrune || acc == T(0)) ? T(0) : T(1) / acc;
+        dinv[row] = di;
+        dinvO[row] = (OtherT<T>)di;
+      }
     }
-    kept += total;
+    kept += totalK;
+    diag += totalD;
     __syncthreads(); // wsum is rewritten by the next round
   }
   if (flip)
     *changed = 1;
+  // 4. Dirichlet lifting of the block's rows: b[r] += the element values of
+  //    r's pairs, in element order (staged in lfe)
+  if (t < nr)
+  {
+    double acc = bRow;
+    for (uint32_t q = lseg[t]; q < lseg[t + 1]; ++q)
+      acc += lfe[q];
+    b[B.r0 + t] = acc;
+  }
 }
 
 // contributions loaded ahead per structural entry (PXG_SCATTER_UNROLL = 1, 2,
@@ -743,9 +808,21 @@ void launchRebuild(pxg_ctx * c, T const * ke, T * valC, T * valS, hipStream_t s)
     auto go = [&](auto pc)
     {
       constexpr int P = decltype(pc)::value;
+      T * dinv;
+      OtherT<T> * dinvO;
+      if constexpr (sizeof(T) == 8)
+      {
+        dinv = c->dinv.p;
+        dinvO = c->dinv32.p;
+      }
+      else
+      {
+        dinv = c->dinv32.p;
+        dinvO = c->dinv.p;
+      }
       k_scatter_staged<T, N, keRowStride(N), P><<<(unsigned)c->nStage, kBlock, 0, s>>>(
           reinterpret_cast<StageBlock const *>(c->stageBlk.p), c->dofContrib.p, ke, c->stageInfo.p, c->stageSid.p,
-          valC, valS, c->changed.p);
+          valC, valS, c->changed.p, c->dofSeg.p, c->feBuf.p, c->b.p, dinv, dinvO);
     };
     if (c->stagePairs == 32)
       go(std::integral_constant<int, 32>{});
@@ -754,6 +831,7 @@ void launchRebuild(pxg_ctx * c, T const * ke, T * valC, T * valS, hipStream_t s)
     else
       go(std::integral_constant<int, 64>{});
     PXG_HIP(hipGetLastError());
+    c->stagedFused = true; // dinv and the lifting are done
     return;
   }
   launchScatterDirect<T>(c->nnzS, c->segPtr.p, c->contrib.p, ke, c->pos.p, valC, valS, c->changed.p, s);
@@ -1221,6 +1299,7 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
   bool const overwrite = c->valsZero;
   // a rebuild after clear writes straight into the pruned layout of the last close
   bool const direct = overwrite && c->layoutValid && c->layoutPrecision == precision;
+  c->stagedFused = false;
   if (!direct && !overwrite)
     unsplitValues(c); // accumulating: structural values first
   c->valsSplit = false;
@@ -1276,6 +1355,7 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                }
              });
   PXG_HIP(hipGetLastError());
+  if (!c->stagedFused) // (the staged rebuild adds the lifting itself)
   {
     ProfScope ps(c, "rhs_scatter");
     k_rhs_scatter<<<gridFor(c->ndofs), kBlock, 0, s>>>(c->ndofs, c->dofSeg.p, c->dofContrib.p, c->feBuf.p, c->b.p);
@@ -1405,7 +1485,9 @@ void closeMatrix(pxg_ctx * c, double eps)
     // if every prune flag matches the layout, rowptr and col stand and the
     // kept values are in place: only the inverse diagonal is new.  It is
     // computed before the flag is read back, so one synchronisation suffices.
-    if (f32)
+    if (c->stagedFused)
+      ; // the staged rebuild wrote both inverse diagonals
+    else if (f32)
     {
       k_diag_inverse_split<float><<<gridFor(nr), kBlock, 0, s>>>(nr, c->diagC.p, c->diagPos.p, c->val32.p,
                                                                   c->valS32.p, eps, c->dinv32.p);
@@ -1511,18 +1593,21 @@ void buildStageMaps(pxg_ctx * c)
   int64_t const nr = c->ndofs, npairs = c->nelem * n;
   c->stageValid = false;
   int const P = stagePairs();
-  if ((int64_t)P * n > 0x7fff || nr >= ((int64_t)1 << 31) || npairs <= 0)
-    return; // contribution offsets of a block must fit 15 bits
-  c->changed.resize(1);
-  k_set_int<<<1, 1, 0, s>>>(c->changed.p, 0);
-  k_max_pairs<<<256, kBlock, 0, s>>>(nr, c->dofSeg.p, c->changed.p);
+  if ((int64_t)P * n > 0x3fff || nr >= ((int64_t)1 << 31) || npairs <= 0)
+    return; // contribution offsets of a block must fit 14 bits
+  DevBuf<int> mp;
+  mp.resize(2);
+  k_set_int<<<1, 1, 0, s>>>(mp.p, 0);
+  k_set_int<<<1, 1, 0, s>>>(mp.p + 1, 0);
+  k_max_pairs<<<256, kBlock, 0, s>>>(nr, c->dofSeg.p, mp.p);
   PXG_HIP(hipGetLastError());
-  int maxPairs = 0;
-  PXG_HIP(hipMemcpyAsync(&maxPairs, c->changed.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  int hp[2] = {0, 0};
+  PXG_HIP(hipMemcpyAsync(hp, mp.p, sizeof(hp), hipMemcpyDeviceToHost, s));
   PXG_HIP(hipStreamSynchronize(s));
-  int64_t const cap = P - std::max(maxPairs, 1);
-  if (cap < 1)
-    return; // a dof with more element-matrix rows than a block stages
+  int64_t const cap = P - std::max(hp[0], 1);
+  if (cap < 1 || hp[1])
+    return; // a dof with more element-matrix rows than a block stages, or with none
+  c->changed.resize(1);
   int64_t const nb = npairs / cap + 1;
   c->stageR0.resize(nb + 1);
   k_stage_rows<<<gridFor(nr + 1), kBlock, 0, s>>>(nr, c->dofSeg.p, cap, nb, c->stageR0.p);
@@ -1538,6 +1623,7 @@ void buildStageMaps(pxg_ctx * c)
   k_set_int<<<1, 1, 0, s>>>(c->changed.p, 0);
   k_stage_maps<<<(unsigned)nb, kBlock, 0, s>>>(n, keRowStride(n), blocks, c->segPtr.p, c->contrib.p, c->keep.p,
                                               pairPos.p, c->stageInfo.p, c->stageSid.p, c->changed.p);
+  k_stage_diag<<<gridFor(nr), kBlock, 0, s>>>(nr, c->diagPos.p, c->stageInfo.p);
   PXG_HIP(hipGetLastError());
   int bad = 1;
   PXG_HIP(hipMemcpyAsync(&bad, c->changed.p, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -170,6 +170,7 @@ struct pxg_ctx
   bool stageValid = false;
   int64_t nStage = 0;
   int stagePairs = 0; // element-matrix rows staged per block
+  bool stagedFused = false; // the last build's staged rebuild wrote dinv and the lifting
   pxg::DevBuf<int32_t> stageR0;   // nStage + 1 first rows
   pxg::DevBuf<uint32_t> stageBlk; // 8 words per block
   pxg::DevBuf<uint16_t> stageInfo, stageSid;
