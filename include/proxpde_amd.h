@@ -123,7 +123,10 @@ int pxg_build_pattern(pxg_ctx * ctx, int64_t * nnz_structural);
  * given terms, per-element Dirichlet elimination, deterministic scatter into
  * the structural CSR values and the lifting into b.  Accumulates, as
  * successive buildLhs calls accumulate triplets.  precision: PXG_FP64 or
- * PXG_FP32 (element kernel and values in float).                          */
+ * PXG_FP32 (element kernel and values in float).  Returns without waiting
+ * for the GPU: an element with a non-positive Jacobian determinant is
+ * reported (PXG_E_NUMERIC) by the next pxg_close, or by the next
+ * pxg_assemble if that comes first.                                       */
 int pxg_assemble(pxg_ctx * ctx, int n_terms, const pxg_term * terms, int precision);
 
 /* Builder::buildRhs with AssemblyRhsAnalytic or AssemblyProjection

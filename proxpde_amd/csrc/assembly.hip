@@ -1265,6 +1265,11 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
   hipStream_t s = c->stream;
   FeTables & T = c->fe;
   int64_t const E = c->nelem;
+  if (c->statusPending) // a previous build not closed yet: its status first
+  {
+    c->statusPending = false;
+    checkStatus(c);
+  }
   PXG_HIP(hipMemsetAsync(c->status.p, 0, sizeof(int), s));
   bool const hasMass = (flags & (kHasMass | kHasMassFE)) != 0;
   c->feBuf.resize(E * T.n);
@@ -1371,7 +1376,9 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
   c->valsSplit = direct;
   c->closed = false;
   ++c->stamp;
-  checkStatus(c);
+  // the element kernels' status (a non-positive Jacobian) is read by the
+  // next close, with the close's own flag, instead of waiting for it here
+  c->statusPending = true;
 }
 
 void assembleRhs(pxg_ctx * c, int kind, double const * params, double const * qvaluesHost)
@@ -1500,9 +1507,13 @@ void closeMatrix(pxg_ctx * c, double eps)
       k_cast<float, double><<<gridFor(nr), kBlock, 0, s>>>(nr, c->dinv.p, c->dinv32.p);
     }
     PXG_HIP(hipGetLastError());
-    int ch = 1;
+    int ch = 1, st = 0;
     PXG_HIP(hipMemcpyAsync(&ch, c->changed.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (c->statusPending)
+      PXG_HIP(hipMemcpyAsync(&st, c->status.p, sizeof(int), hipMemcpyDeviceToHost, s));
     PXG_HIP(hipStreamSynchronize(s));
+    c->statusPending = false;
+    PXG_REQUIRE(st == 0, PXG_E_NUMERIC, "non-positive Jacobian determinant in an element");
     if (!ch)
     {
       c->val32Valid = f32;
@@ -1511,6 +1522,11 @@ void closeMatrix(pxg_ctx * c, double eps)
       ++c->stamp;
       return;
     }
+  }
+  if (c->statusPending)
+  {
+    c->statusPending = false;
+    checkStatus(c);
   }
   unsplitValues(c); // the prune set changed (or another threshold): structural values back
   c->keep.resize(ns + 1);
@@ -1674,7 +1690,6 @@ void clearValues(pxg_ctx * c)
   c->valsSplit = false;
   if (c->b.p)
     PXG_HIP(hipMemsetAsync(c->b.p, 0, c->ndofs * sizeof(double), s));
-  PXG_HIP(hipStreamSynchronize(s));
   c->valsDirty = false;
   c->closed = false;
   ++c->stamp;

@@ -210,6 +210,7 @@ struct pxg_ctx
   pxg::DevBuf<double> qwork;  // E*nq scratch
   pxg::DevBuf<double> mfBuf;  // matrix-free element results [E][n][nb]
   pxg::DevBuf<int> status;    // device error flag
+  bool statusPending = false; // status of the last build not read yet (read by close)
   pxg::DevBuf<int> workCounter; // element counter of the persistent element kernels
   pxg::DevBuf<unsigned char> sortTmp;
   // kernel timing with HIP events on the context stream (off by default)

@@ -389,3 +389,30 @@ def test_gmsh_assembly_matches_oracle(oracle, gpu):
     rp, col, v, b = a.csr()
     assert np.array_equal(rp, rp_o) and np.array_equal(col, col_o.astype(np.int32))
     assert _rel_ok(v, v_o)[0] and _rel_ok(b, b_o)[0]
+
+
+@pytest.mark.parametrize("fe", ["hex_q2", "hex_q1"])
+def test_inverted_element_reported_at_close(gpu, fe):
+    """A mesh collapsed onto x = 0 has singular Jacobians (det J^T J = 0): the
+    element kernels flag it (no fault, the kernels finish normally), assemble
+    returns at once and the next close reports PXG_E_NUMERIC (-4); a later
+    clean build on the same context succeeds.  (Mirrored elements are not an
+    error: detJ = sqrt(det J^T J) > 0 as in CurFE::reinit,
+    include/curfe.hpp:73-77.)"""
+    from proxpde_amd import native as N
+    prob = W.Problem(fe, (2, 2, 2))
+    a = prob.assembler()
+    a.set_coefficient(np.ones(prob.ndofs))
+    a.build_pattern()
+    xyz = prob.xyz.copy()
+    xyz[:, 0] = 0.0
+    a.set_mesh(xyz, prob.conn)
+    a.assemble([("stiffness_fe", 1.0)])
+    with pytest.raises(N.PxgError) as err:
+        a.close()
+    assert err.value.code == -4
+    a.set_mesh(prob.xyz, prob.conn)
+    a.clear()
+    a.assemble([("stiffness_fe", 1.0)])
+    a.close()
+    assert a.nnz > 0
