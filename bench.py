@@ -482,10 +482,12 @@ def main():
     # rebuild into the pruned layout (staged by row blocks): per structural
     # entry its 16-bit info word and the value written (8 B, kept or pruned);
     # per element-matrix entry its 16-bit LDS slot and its value (8 B); per
-    # element-matrix row its pair id (4 B)
-    sc_by = nnz_s * (2 + 8) + E * 27 * 27 * (2 + 8) + E * 27 * 4
+    # element-matrix row its pair id and lifting value (4 + 8 B); per row its
+    # pair offset, b read and written, both inverse diagonals (4 + 16 + 12 B)
+    sc_by = nnz_s * (2 + 8) + E * 27 * 27 * (2 + 8) + E * 27 * (4 + 8) + prob.ndofs * (4 + 16 + 12)
     sc_ach = sc_by / (per["scatter"] * 1e-3) / 1e9 if per["scatter"] else None
-    rf_scatter = {"kernel": "k_scatter_staged<double,27,32>", "bound": "hbm", "achieved": sc_ach, "peak": HBM_PEAK_GBS,
+    rf_scatter = {"kernel": "k_scatter_staged<double,27,32,64>", "bound": "hbm", "achieved": sc_ach,
+                  "peak": HBM_PEAK_GBS,
                   "unit": "GB/s", "frac": sc_ach / HBM_PEAK_GBS if sc_ach else None, "traffic": tr("scatter")}
     rf_spmv = {"kernel": "k_csr_stream<double,0>", "bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS,
                "unit": "GB/s", "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": tr("spmv_fp64")}
