@@ -79,6 +79,7 @@ def lib():
         L.orc_transfer.argtypes = [vp, vp, i32, i64p, u32p, dp]
         L.orc_mf_apply.argtypes = [vp, dp, u8p, i32, dp, dp]
         L.orc_mf_grad_kappa.argtypes = [vp, u8p, i32, dp, dp, dp]
+        L.orc_stokes.argtypes = [vp, i32, C.c_double, C.c_double, C.c_double, u8p, dp, i32, C.c_double, i64p, i64p]
         _lib = L
     return _lib
 
@@ -207,6 +208,27 @@ class Space:
         b = np.empty(self.ndofs)
         self.L.orc_get_csr(self.h, _p(rowptr, C.c_int64), _p(col, C.c_uint32), _p(valv), _p(b))
         return rowptr, col, valv, b
+
+    def stokes(self, interleaved, coefs, vel_flag, vel_val, mode=0, prune=1e-16):
+        """Stokes block system of the reference's test/assemblyvector.cpp on this
+        QUAD_Q2 space (velocity: 2 components, BLOCK or INTERLEAVED dofs; pressure
+        Q1 after them).  coefs = (stiffness, grad, div); vel_flag / vel_val: the
+        velocity Dirichlet data (2 * ndofs entries).  mode 0: vector spaces,
+        mode 1: the scalar component spaces.  Returns (rowptr, col, val, b)."""
+        flag = np.ascontiguousarray(vel_flag, dtype=np.uint8)
+        val = _f64(vel_val)
+        nd = C.c_int64()
+        nnz = C.c_int64()
+        rc = self.L.orc_stokes(self.h, int(interleaved), float(coefs[0]), float(coefs[1]), float(coefs[2]),
+                               _p(flag, C.c_uint8), _p(val), int(mode), prune, C.byref(nd), C.byref(nnz))
+        if rc != 0:
+            raise RuntimeError(self.L.orc_error(self.h).decode())
+        rowptr = np.empty(nd.value + 1, dtype=np.int64)
+        col = np.empty(nnz.value, dtype=np.uint32)
+        v = np.empty(nnz.value)
+        b = np.empty(nd.value)
+        self.L.orc_get_csr(self.h, _p(rowptr, C.c_int64), _p(col, C.c_uint32), _p(v), _p(b))
+        return rowptr, col, v, b
 
     def pattern(self):
         nnz = self.L.orc_pattern(self.h, None, None, None)

@@ -1100,6 +1100,171 @@ void Builder::buildLhs(
   }
 }
 
+DofMap vectorDofMap(DofMap const & s, int dim, bool interleaved, u32 offset)
+{
+  DofMap v;
+  v.n = s.n * dim;
+  v.ndofs = s.ndofs * dim;
+  i64 const ne = s.n ? (i64)s.elemMap.size() / s.n : 0;
+  v.elemMap.resize((size_t)ne * v.n);
+  for (i64 e = 0; e < ne; ++e)
+    for (int d = 0; d < dim; ++d)
+      for (int i = 0; i < s.n; ++i)
+      {
+        u32 const id = s.elemMap[e * s.n + i];
+        v.elemMap[e * v.n + i + d * s.n] = offset + (interleaved ? id * dim + d : id + d * s.ndofs);
+      }
+  return v;
+}
+
+void Builder::buildLhsVectorStiffness(
+    Mesh const & m, CurFE & cfe, DofMap const & d, int dim, double coef, std::vector<uint8_t> const & bcFlag,
+    std::vector<double> const & bcVal)
+{
+  int const n = cfe.n, nq = cfe.nq, nn = n * dim;
+  std::vector<double> Ke(nn * nn), Fe(nn), h(nn), vtx;
+  std::vector<int> cons(nn);
+  bool const hasBC = !bcFlag.empty();
+  for (i64 e = 0; e < m.nelem(); ++e)
+  {
+    gatherVtx(m, e, vtx);
+    cfe.reinit(vtx.data());
+    u32 const * ids = &d.elemMap[e * nn];
+    std::fill(Ke.begin(), Ke.end(), 0.);
+    std::fill(Fe.begin(), Fe.end(), 0.);
+    // AssemblyStiffness::build: for q, for d: block(d, d) += coef JxW dphi dphi^T
+    for (int q = 0; q < nq; ++q)
+    {
+      double const s = coef * cfe.JxW[q];
+      for (int c = 0; c < dim; ++c)
+        for (int i = 0; i < n; ++i)
+          for (int j = 0; j < n; ++j)
+          {
+            double const * gi = &cfe.dphi[(q * n + i) * 3];
+            double const * gj = &cfe.dphi[(q * n + j) * 3];
+            Ke[(i + c * n) * nn + j + c * n] += s * (gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2]);
+          }
+    }
+    if (hasBC)
+    {
+      bool any = false;
+      for (int i = 0; i < nn; ++i)
+      {
+        cons[i] = bcFlag[ids[i]];
+        h[i] = cons[i] ? bcVal[ids[i]] : 0.;
+        any = any || cons[i];
+      }
+      if (any)
+      {
+        for (int i = 0; i < nn; ++i)
+        {
+          double s = 0.;
+          for (int j = 0; j < nn; ++j)
+            s += Ke[i * nn + j] * h[j];
+          Fe[i] = cons[i] ? 0. : (Fe[i] - s);
+        }
+        for (int i = 0; i < nn; ++i)
+          for (int j = 0; j < nn; ++j)
+            if (cons[i] || cons[j])
+              Ke[i * nn + j] = 0.;
+        for (int i = 0; i < nn; ++i)
+          if (cons[i])
+          {
+            Ke[i * nn + i] = 1.0; // BCEss::diag
+            Fe[i] = h[i];
+          }
+      }
+    }
+    // storage order of builder.hpp:128-152: i, d1, j, d2
+    for (int i = 0; i < n; ++i)
+      for (int d1 = 0; d1 < dim; ++d1)
+      {
+        int const r = i + d1 * n;
+        b[ids[r]] += Fe[r];
+        for (int j = 0; j < n; ++j)
+          for (int d2 = 0; d2 < dim; ++d2)
+          {
+            int const c = j + d2 * n;
+            trow.push_back(ids[r]);
+            tcol.push_back(ids[c]);
+            tval.push_back(Ke[r * nn + c]);
+          }
+      }
+  }
+}
+
+void Builder::buildCoupling(
+    Mesh const & m, CurFE & row, CurFE & clm, DofMap const & dr, int dimR, DofMap const & dc, int dimC, int kind,
+    double coef, std::vector<int> const & comps, std::vector<uint8_t> const & bcRowFlag,
+    std::vector<uint8_t> const & bcClmFlag, std::vector<double> const & bcClmVal)
+{
+  int const nR = row.n, nC = clm.n, nq = row.nq;
+  int const NR = nR * dimR, NC = nC * dimC;
+  std::vector<double> Ke((size_t)NR * NC), Fe(NR), h(NC), vtx;
+  std::vector<int> crow(NR), cclm(NC);
+  for (i64 e = 0; e < m.nelem(); ++e)
+  {
+    gatherVtx(m, e, vtx);
+    row.reinit(vtx.data());
+    clm.reinit(vtx.data());
+    u32 const * ir = &dr.elemMap[e * NR];
+    u32 const * ic = &dc.elemMap[e * NC];
+    std::fill(Ke.begin(), Ke.end(), 0.);
+    // AssemblyGrad: block (d nR, 0) += coef JxW dphiR.col(c) phiC^T
+    // AssemblyDiv:  block (0, d nC) += coef JxW phiR dphiC.col(c)^T
+    int blk = 0;
+    for (int c: comps)
+    {
+      for (int q = 0; q < nq; ++q)
+      {
+        double const s = coef * row.JxW[q];
+        if (kind == COUPLING_GRAD)
+          for (int i = 0; i < nR; ++i)
+            for (int j = 0; j < nC; ++j)
+              Ke[(size_t)(i + blk * nR) * NC + j] += (s * row.dphi[(q * nR + i) * 3 + c]) * clm.phiRef[q * nC + j];
+        else
+          for (int i = 0; i < nR; ++i)
+            for (int j = 0; j < nC; ++j)
+              Ke[(size_t)i * NC + j + blk * nC] += (s * row.phiRef[q * nR + i]) * clm.dphi[(q * nC + j) * 3 + c];
+      }
+      ++blk;
+    }
+    // Crow clears constrained rows, Cclm constrained columns; Fe = -Crow Ke h
+    for (int i = 0; i < NR; ++i)
+      crow[i] = bcRowFlag.empty() ? 0 : bcRowFlag[ir[i]];
+    for (int j = 0; j < NC; ++j)
+    {
+      cclm[j] = bcClmFlag.empty() ? 0 : bcClmFlag[ic[j]];
+      h[j] = cclm[j] ? bcClmVal[ic[j]] : 0.;
+    }
+    for (int i = 0; i < NR; ++i)
+    {
+      double s = 0.;
+      for (int j = 0; j < NC; ++j)
+        s += Ke[(size_t)i * NC + j] * h[j];
+      Fe[i] = crow[i] ? 0. : -s;
+    }
+    for (int i = 0; i < NR; ++i)
+      for (int j = 0; j < NC; ++j)
+        if (crow[i] || cclm[j])
+          Ke[(size_t)i * NC + j] = 0.;
+    for (int i = 0; i < nR; ++i)
+      for (int d1 = 0; d1 < dimR; ++d1)
+      {
+        int const r = i + d1 * nR;
+        b[ir[r]] += Fe[r];
+        for (int j = 0; j < nC; ++j)
+          for (int d2 = 0; d2 < dimC; ++d2)
+          {
+            int const c = j + d2 * nC;
+            trow.push_back(ir[r]);
+            tcol.push_back(ic[c]);
+            tval.push_back(Ke[(size_t)r * NC + c]);
+          }
+      }
+  }
+}
+
 void Builder::buildRhs(
     Mesh const & m, CurFE & cfe, DofMap const & d, int rhsKind, double const * params,
     double const * qvalues, std::vector<uint8_t> const & bcFlag)

@@ -250,7 +250,31 @@ struct Builder
       double const * qvalues, std::vector<uint8_t> const & bcFlag);
   // setFromTriplets (sum duplicates in insertion order, sorted columns) + prune
   Csr close(double pruneEps) const;
+  // buildLhs(AssemblyStiffness{coef, vector space}) on a space of `dim`
+  // components (reference builder.hpp:31-154, assembly_lhs.hpp:36-75): the
+  // map has n*dim columns, local dof i of component d at i + d*n
+  void buildLhsVectorStiffness(
+      Mesh const & m, CurFE & cfe, DofMap const & d, int dim, double coef, std::vector<uint8_t> const & bcFlag,
+      std::vector<double> const & bcVal);
+  // buildCoupling(AssemblyGrad / AssemblyDiv, bcsRow, bcsClm), reference
+  // builder.hpp:156-277, assembly_coupling.hpp:42-132: rows of the row space
+  // (dimR components, map nR*dimR wide), columns of the column space; comps:
+  // the physical derivative of each component block (the CompList)
+  void buildCoupling(
+      Mesh const & m, CurFE & row, CurFE & clm, DofMap const & dr, int dimR, DofMap const & dc, int dimC, int kind,
+      double coef, std::vector<int> const & comps, std::vector<uint8_t> const & bcRowFlag,
+      std::vector<uint8_t> const & bcClmFlag, std::vector<double> const & bcClmVal);
 };
+
+enum CouplingKind : int
+{
+  COUPLING_GRAD = 1, // coef * (phiP_j, d phiV_i / dx_c): rows velocity, columns pressure
+  COUPLING_DIV = 2,  // coef * (d phiV_j / dx_c, phiP_i): rows pressure, columns velocity
+};
+
+// vector dof map from a scalar one, reference include/dof.hpp:112-329:
+// BLOCK id + d*size, INTERLEAVED id*dim + d, plus the space offset
+DofMap vectorDofMap(DofMap const & s, int dim, bool interleaved, u32 offset);
 
 // structural pattern and element->nnz map (position of each (e,i,j) in the
 // unpruned pattern); the reference never materialises this map, it is the
