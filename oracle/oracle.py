@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import shutil
 import subprocess
 
 import numpy as np
@@ -29,8 +30,10 @@ _lib = None
 
 
 def build(force: bool = False) -> str:
-    """Compile the oracle with its Makefile (plain g++)."""
-    if force or not os.path.exists(_LIB):
+    """Compile the oracle with its Makefile (plain g++); make rebuilds the
+    library when a source is newer.  Where the sources cannot be compiled (no
+    g++) an existing library is used as it is."""
+    if force or not os.path.exists(_LIB) or shutil.which(os.environ.get("CXX", "g++")):
         subprocess.check_call(["make", "-s", "-B" if force else "-s", "-C", _HERE, "liboracle.so"])
     return _LIB
 
@@ -79,7 +82,7 @@ def lib():
         L.orc_transfer.argtypes = [vp, vp, i32, i64p, u32p, dp]
         L.orc_mf_apply.argtypes = [vp, dp, u8p, i32, dp, dp]
         L.orc_mf_grad_kappa.argtypes = [vp, u8p, i32, dp, dp, dp]
-        L.orc_stokes.argtypes = [vp, i32, C.c_double, C.c_double, C.c_double, u8p, dp, i32, C.c_double, i64p, i64p]
+        L.orc_stokes.argtypes = [vp, i32] + [C.c_double] * 5 + [u8p, dp, i32, C.c_double, i64p, i64p]
         _lib = L
     return _lib
 
@@ -209,26 +212,35 @@ class Space:
         self.L.orc_get_csr(self.h, _p(rowptr, C.c_int64), _p(col, C.c_uint32), _p(valv), _p(b))
         return rowptr, col, valv, b
 
-    def stokes(self, interleaved, coefs, vel_flag, vel_val, mode=0, prune=1e-16):
-        """Stokes block system of the reference's test/assemblyvector.cpp on this
-        QUAD_Q2 space (velocity: 2 components, BLOCK or INTERLEAVED dofs; pressure
-        Q1 after them).  coefs = (stiffness, grad, div); vel_flag / vel_val: the
-        velocity Dirichlet data (2 * ndofs entries).  mode 0: vector spaces,
-        mode 1: the scalar component spaces.  Returns (rowptr, col, val, b)."""
-        flag = np.ascontiguousarray(vel_flag, dtype=np.uint8)
-        val = _f64(vel_val)
+    def stokes(self, interleaved, coefs, flag, val, mode=0, prune=1e-16):
+        """Stokes block system of the reference's test/assemblyvector.cpp and
+        test/stokes2dquad.cpp on this QUAD_Q2 space (velocity: 2 components,
+        BLOCK or INTERLEAVED dofs; pressure Q1 after them).  coefs: dict with
+        keys stiffness, tensor, mass (velocity block), grad, div (missing: 0).
+        flag / val: Dirichlet data per dof of the velocity (2 * ndofs entries)
+        or of the whole system; a constrained pressure dof acts as
+        buildLhs(AssemblyDummy{p}, bcsP) plus the couplings' bcsP.  mode 0:
+        vector spaces, mode 1: the scalar component spaces.  Returns
+        (rowptr, col, val, b)."""
         nd = C.c_int64()
         nnz = C.c_int64()
-        rc = self.L.orc_stokes(self.h, int(interleaved), float(coefs[0]), float(coefs[1]), float(coefs[2]),
-                               _p(flag, C.c_uint8), _p(val), int(mode), prune, C.byref(nd), C.byref(nnz))
+        k = {"stiffness": 0.0, "tensor": 0.0, "mass": 0.0, "grad": 0.0, "div": 0.0}
+        k.update(coefs)
+        f = np.zeros(max(len(flag), 2 * self.ndofs + self.nelem * 4), dtype=np.uint8)
+        v = np.zeros(len(f))
+        f[: len(flag)] = np.asarray(flag, dtype=np.uint8)
+        v[: len(val)] = np.asarray(val, dtype=np.float64)
+        rc = self.L.orc_stokes(self.h, int(interleaved), float(k["stiffness"]), float(k["tensor"]),
+                               float(k["mass"]), float(k["grad"]), float(k["div"]), _p(f, C.c_uint8), _p(v),
+                               int(mode), prune, C.byref(nd), C.byref(nnz))
         if rc != 0:
             raise RuntimeError(self.L.orc_error(self.h).decode())
         rowptr = np.empty(nd.value + 1, dtype=np.int64)
         col = np.empty(nnz.value, dtype=np.uint32)
-        v = np.empty(nnz.value)
+        vv = np.empty(nnz.value)
         b = np.empty(nd.value)
-        self.L.orc_get_csr(self.h, _p(rowptr, C.c_int64), _p(col, C.c_uint32), _p(v), _p(b))
-        return rowptr, col, v, b
+        self.L.orc_get_csr(self.h, _p(rowptr, C.c_int64), _p(col, C.c_uint32), _p(vv), _p(b))
+        return rowptr, col, vv, b
 
     def pattern(self):
         nnz = self.L.orc_pattern(self.h, None, None, None)

@@ -281,17 +281,23 @@ int orc_get_csr(void * p, int64_t * rowptr, uint32_t * col, double * val, double
   return 0;
 }
 
-// Stokes block system of the reference's test/assemblyvector.cpp on the quad
-// mesh of a QUAD_Q2 context: velocity Q2 with 2 components (BLOCK or
-// INTERLEAVED dofs) at [0, 2 nU), pressure Q1 at [2 nU, 2 nU + nP), both with
-// the context's rule.  velFlag / velVal: Dirichlet data per velocity dof
-// (global numbering); the pressure has none.  mode 0: vector spaces,
-//   buildLhs(AssemblyStiffness{cS, vel}) + buildCoupling(AssemblyGrad{cG, vel, p})
-//   + buildCoupling(AssemblyDiv{cD, p, vel});
-// mode 1: the scalar component spaces U, V of the same test, comp lists {0}, {1}.
-// The closed matrix and b replace the context's (orc_get_csr).
-int orc_stokes(void * p, int interleaved, double cS, double cG, double cD, uint8_t const * velFlag,
-               double const * velVal, int mode, double pruneEps, int64_t * ndofsOut, int64_t * nnzOut)
+// Stokes block system of the reference's test/assemblyvector.cpp and
+// test/stokes2dquad.cpp on the quad mesh of a QUAD_Q2 context: velocity Q2
+// with 2 components (BLOCK or INTERLEAVED dofs) at [0, 2 nU), pressure Q1 at
+// [2 nU, 2 nU + nP), both with the context's rule.  flag / val: Dirichlet
+// data per dof of the whole system (global numbering).  mode 0: vector spaces,
+//   buildLhs({AssemblyStiffness{cS, vel}, AssemblyTensorStiffness{cT, vel},
+//             AssemblyScalarMass{cM, vel}}, bcsVel)
+//   [+ buildLhs(AssemblyDummy{p}, bcsP) when some pressure dof is constrained]
+//   + buildCoupling(AssemblyGrad{cG, vel, p}, bcsVel, bcsP)
+//   + buildCoupling(AssemblyDiv{cD, p, vel}, bcsP, bcsVel);
+// mode 1: the scalar component spaces U, V of assemblyvector.cpp, comp lists
+// {0}, {1} (no tensor term: it couples the components).  Terms with a zero
+// coefficient are left out.  The closed matrix and b replace the context's
+// (orc_get_csr).
+int orc_stokes(void * p, int interleaved, double cS, double cT, double cM, double cG, double cD,
+               uint8_t const * flagIn, double const * valIn, int mode, double pruneEps, int64_t * ndofsOut,
+               int64_t * nnzOut)
 {
   auto * c = static_cast<Ctx *>(p);
   return guard(
@@ -300,6 +306,8 @@ int orc_stokes(void * p, int interleaved, double cS, double cG, double cD, uint8
       {
         if (c->feType != FE_QUAD_Q2)
           throw std::runtime_error("orc_stokes needs a QUAD_Q2 context");
+        if (mode != 0 && cT != 0.)
+          throw std::runtime_error("the tensor stiffness has no component-space form");
         RefFE const & fv = refFE(FE_QUAD_Q2);
         RefFE const & fp = refFE(FE_QUAD_Q1);
         CurFE cv(fv, c->qr), cp(fp, c->qr);
@@ -310,22 +318,21 @@ int orc_stokes(void * p, int interleaved, double cS, double cG, double cD, uint8
           id += off;
         dp.ndofs = nP;
         i64 const nd = (i64)off + nP;
-        std::vector<uint8_t> flag(nd, 0);
-        std::vector<double> val(nd, 0.);
-        for (u32 k = 0; k < off; ++k)
-        {
-          flag[k] = velFlag[k];
-          val[k] = velVal[k];
-        }
-        std::vector<uint8_t> none;
-        std::vector<double> noneV;
+        std::vector<uint8_t> flag(flagIn, flagIn + nd);
+        std::vector<double> val(valIn, valIn + nd);
+        bool pinned = false;
+        for (i64 k = off; k < nd; ++k)
+          pinned = pinned || flag[k];
+        std::vector<Term> dummy; // AssemblyDummy: no term, Ke = 0
         c->builder.init(nd);
         if (mode == 0)
         {
           DofMap vel = vectorDofMap(sv, 2, interleaved != 0, 0);
-          c->builder.buildLhsVectorStiffness(c->mesh, cv, vel, 2, cS, flag, val);
-          c->builder.buildCoupling(c->mesh, cv, cp, vel, 2, dp, 1, COUPLING_GRAD, cG, {0, 1}, flag, none, noneV);
-          c->builder.buildCoupling(c->mesh, cp, cv, dp, 1, vel, 2, COUPLING_DIV, cD, {0, 1}, none, flag, val);
+          c->builder.buildLhsVector(c->mesh, cv, vel, 2, cS, cT, cM, flag, val);
+          if (pinned)
+            c->builder.buildLhs(c->mesh, cp, dp, dummy, flag, val);
+          c->builder.buildCoupling(c->mesh, cv, cp, vel, 2, dp, 1, COUPLING_GRAD, cG, {0, 1}, flag, flag, val);
+          c->builder.buildCoupling(c->mesh, cp, cv, dp, 1, vel, 2, COUPLING_DIV, cD, {0, 1}, flag, flag, val);
         }
         else
         {
@@ -336,13 +343,19 @@ int orc_stokes(void * p, int interleaved, double cS, double cG, double cD, uint8
             u.elemMap[k] = interleaved ? 2 * id : id;
             v.elemMap[k] = interleaved ? 2 * id + 1 : id + nU;
           }
-          std::vector<Term> stiff{Term{TERM_STIFFNESS, cS, nullptr}};
-          c->builder.buildLhs(c->mesh, cv, u, stiff, flag, val);
-          c->builder.buildCoupling(c->mesh, cv, cp, u, 1, dp, 1, COUPLING_GRAD, cG, {0}, flag, none, noneV);
-          c->builder.buildCoupling(c->mesh, cp, cv, dp, 1, u, 1, COUPLING_DIV, cD, {0}, none, flag, val);
-          c->builder.buildLhs(c->mesh, cv, v, stiff, flag, val);
-          c->builder.buildCoupling(c->mesh, cv, cp, v, 1, dp, 1, COUPLING_GRAD, cG, {1}, flag, none, noneV);
-          c->builder.buildCoupling(c->mesh, cp, cv, dp, 1, v, 1, COUPLING_DIV, cD, {1}, none, flag, val);
+          std::vector<Term> lhs;
+          if (cS != 0.)
+            lhs.push_back(Term{TERM_STIFFNESS, cS, nullptr});
+          if (cM != 0.)
+            lhs.push_back(Term{TERM_MASS, cM, nullptr});
+          c->builder.buildLhs(c->mesh, cv, u, lhs, flag, val);
+          if (pinned)
+            c->builder.buildLhs(c->mesh, cp, dp, dummy, flag, val);
+          c->builder.buildCoupling(c->mesh, cv, cp, u, 1, dp, 1, COUPLING_GRAD, cG, {0}, flag, flag, val);
+          c->builder.buildCoupling(c->mesh, cp, cv, dp, 1, u, 1, COUPLING_DIV, cD, {0}, flag, flag, val);
+          c->builder.buildLhs(c->mesh, cv, v, lhs, flag, val);
+          c->builder.buildCoupling(c->mesh, cv, cp, v, 1, dp, 1, COUPLING_GRAD, cG, {1}, flag, flag, val);
+          c->builder.buildCoupling(c->mesh, cp, cv, dp, 1, v, 1, COUPLING_DIV, cD, {1}, flag, flag, val);
         }
         c->A = c->builder.close(pruneEps);
         c->builder.trow.clear();

@@ -1117,9 +1117,9 @@ DofMap vectorDofMap(DofMap const & s, int dim, bool interleaved, u32 offset)
   return v;
 }
 
-void Builder::buildLhsVectorStiffness(
-    Mesh const & m, CurFE & cfe, DofMap const & d, int dim, double coef, std::vector<uint8_t> const & bcFlag,
-    std::vector<double> const & bcVal)
+void Builder::buildLhsVector(
+    Mesh const & m, CurFE & cfe, DofMap const & d, int dim, double cS, double cT, double cM,
+    std::vector<uint8_t> const & bcFlag, std::vector<double> const & bcVal)
 {
   int const n = cfe.n, nq = cfe.nq, nn = n * dim;
   std::vector<double> Ke(nn * nn), Fe(nn), h(nn), vtx;
@@ -1132,18 +1132,61 @@ void Builder::buildLhsVectorStiffness(
     u32 const * ids = &d.elemMap[e * nn];
     std::fill(Ke.begin(), Ke.end(), 0.);
     std::fill(Fe.begin(), Fe.end(), 0.);
-    // AssemblyStiffness::build: for q, for d: block(d, d) += coef JxW dphi dphi^T
-    for (int q = 0; q < nq; ++q)
+    // the terms in the order of the tuple {stiffness, tensor stiffness, mass}
+    auto blockStiff = [&](double coef)
     {
-      double const s = coef * cfe.JxW[q];
-      for (int c = 0; c < dim; ++c)
-        for (int i = 0; i < n; ++i)
-          for (int j = 0; j < n; ++j)
-          {
-            double const * gi = &cfe.dphi[(q * n + i) * 3];
-            double const * gj = &cfe.dphi[(q * n + j) * 3];
-            Ke[(i + c * n) * nn + j + c * n] += s * (gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2]);
-          }
+      // AssemblyStiffness::build: for q, for d: block(d, d) += coef JxW dphi dphi^T
+      for (int q = 0; q < nq; ++q)
+      {
+        double const s = coef * cfe.JxW[q];
+        for (int c = 0; c < dim; ++c)
+          for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j)
+            {
+              double const * gi = &cfe.dphi[(q * n + i) * 3];
+              double const * gj = &cfe.dphi[(q * n + j) * 3];
+              Ke[(i + c * n) * nn + j + c * n] += s * (gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2]);
+            }
+      }
+    };
+    if (cS != 0.)
+      blockStiff(cS);
+    if (cT != 0.)
+    {
+      // AssemblyTensorStiffness::build (assembly_lhs.hpp:131-163): per q and
+      // component di the block-diagonal term, then for every dj
+      // block(di, dj) += coef JxW dphi.col(di) dphi.col(dj)^T
+      for (int q = 0; q < nq; ++q)
+      {
+        double const s = cT * cfe.JxW[q];
+        for (int di = 0; di < dim; ++di)
+        {
+          for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j)
+            {
+              double const * gi = &cfe.dphi[(q * n + i) * 3];
+              double const * gj = &cfe.dphi[(q * n + j) * 3];
+              Ke[(i + di * n) * nn + j + di * n] += s * (gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2]);
+            }
+          for (int dj = 0; dj < dim; ++dj)
+            for (int i = 0; i < n; ++i)
+              for (int j = 0; j < n; ++j)
+                Ke[(i + di * n) * nn + j + dj * n] +=
+                    s * (cfe.dphi[(q * n + i) * 3 + di] * cfe.dphi[(q * n + j) * 3 + dj]);
+        }
+      }
+    }
+    if (cM != 0.)
+    {
+      // AssemblyScalarMass::build (assembly_lhs.hpp:207-221): block(d, d) += coef JxW phi phi^T
+      for (int q = 0; q < nq; ++q)
+      {
+        double const s = cM * cfe.JxW[q];
+        for (int c = 0; c < dim; ++c)
+          for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j)
+              Ke[(i + c * n) * nn + j + c * n] += s * (cfe.phiRef[q * n + i] * cfe.phiRef[q * n + j]);
+      }
     }
     if (hasBC)
     {

@@ -250,12 +250,14 @@ struct Builder
       double const * qvalues, std::vector<uint8_t> const & bcFlag);
   // setFromTriplets (sum duplicates in insertion order, sorted columns) + prune
   Csr close(double pruneEps) const;
-  // buildLhs(AssemblyStiffness{coef, vector space}) on a space of `dim`
-  // components (reference builder.hpp:31-154, assembly_lhs.hpp:36-75): the
-  // map has n*dim columns, local dof i of component d at i + d*n
-  void buildLhsVectorStiffness(
-      Mesh const & m, CurFE & cfe, DofMap const & d, int dim, double coef, std::vector<uint8_t> const & bcFlag,
-      std::vector<double> const & bcVal);
+  // buildLhs({AssemblyStiffness(cS), AssemblyTensorStiffness(cT),
+  // AssemblyScalarMass(cM)}) on a space of `dim` components (reference
+  // builder.hpp:31-154, assembly_lhs.hpp:36-75, 115-173, 190-229; a zero
+  // coefficient leaves its term out): the map has n*dim columns, local dof i
+  // of component d at i + d*n
+  void buildLhsVector(
+      Mesh const & m, CurFE & cfe, DofMap const & d, int dim, double cS, double cT, double cM,
+      std::vector<uint8_t> const & bcFlag, std::vector<double> const & bcVal);
   // buildCoupling(AssemblyGrad / AssemblyDiv, bcsRow, bcsClm), reference
   // builder.hpp:156-277, assembly_coupling.hpp:42-132: rows of the row space
   // (dimR components, map nR*dimR wide), columns of the column space; comps:

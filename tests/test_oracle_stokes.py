@@ -8,7 +8,9 @@ component by component on scalar spaces whose dof maps are shifted by hand, for
 the BLOCK and the INTERLEAVED orderings.  The others pin the coupling integrals
 against closed forms: the grad rows of interior velocity functions sum to zero
 over the pressure basis, div applied to the interpolant of u = (x, 0) gives the
-integrals of the pressure basis (sum 1 on the unit square), and grad = div^T."""
+integrals of the pressure basis (sum 1 on the unit square), and grad = div^T;
+the cross term of the tensor stiffness is the gradient of the divergence, and
+a pinned pressure dof acts as AssemblyDummy plus the couplings' bcsP."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -47,13 +49,14 @@ def _csr(rp, col, val):
 
 
 @pytest.mark.parametrize("interleaved", [False, True])
-@pytest.mark.parametrize("n", [(3, 3), (4, 2)])
-def test_vector_spaces_match_component_spaces(interleaved, n):
+@pytest.mark.parametrize("n,mass", [((3, 3), 0.0), ((4, 2), 0.0), ((3, 2), 2.5)])
+def test_vector_spaces_match_component_spaces(interleaved, n, mass):
     O.build()
     o = O.Space("quad_q2", "square", n)
     flag, val = _vel_bc(o, interleaved)
-    rv, cv, vv, bv = o.stokes(interleaved, (1.0, -1.0, -1.0), flag, val, mode=0)
-    rs, cs, vs, bs = o.stokes(interleaved, (1.0, -1.0, -1.0), flag, val, mode=1)
+    k = {"stiffness": 1.0, "mass": mass, "grad": -1.0, "div": -1.0}
+    rv, cv, vv, bv = o.stokes(interleaved, k, flag, val, mode=0)
+    rs, cs, vs, bs = o.stokes(interleaved, k, flag, val, mode=1)
     assert np.array_equal(rv, rs) and np.array_equal(cv, cs)
     assert np.array_equal(vv, vs)
     assert np.abs(bv - bs).max() <= 1e-15 * max(1.0, np.abs(bv).max())
@@ -67,7 +70,7 @@ def test_coupling_closed_forms():
     no = np.zeros(2 * nU, np.uint8)
     zero = np.zeros(2 * nU)
     cg, cd = -1.5, 0.75
-    rp, col, v, b = o.stokes(False, (0.0, cg, cd), no, zero, mode=0, prune=-1.0)
+    rp, col, v, b = o.stokes(False, {"grad": cg, "div": cd}, no, zero, mode=0, prune=-1.0)
     A = _csr(rp, col, v).toarray()
     nd = A.shape[0]
     nP = nd - 2 * nU
@@ -88,3 +91,70 @@ def test_coupling_closed_forms():
     h = (1.0 / n[0]) * (1.0 / n[1])
     assert abs(w.max() - cd * h) <= 1e-14
     assert nP == (n[0] + 1) * (n[1] + 1)
+
+
+def test_tensor_stiffness_is_stiffness_plus_grad_div():
+    """AssemblyTensorStiffness = AssemblyStiffness + X with X_(a,di),(j,dj) =
+    int d phi_a/dx_di d phi_j/dx_dj: X u = int grad phi_a div u, so X applied
+    to the interpolant of a divergence-free field vanishes and for u = (x, 0)
+    it equals D^T 1 / cd (div coupling, sum over the pressure basis)."""
+    O.build()
+    n = (3, 2)
+    o = O.Space("quad_q2", "square", n)
+    nU = o.ndofs
+    no = np.zeros(2 * nU, np.uint8)
+    zero = np.zeros(2 * nU)
+    ct, cd = 0.7, -1.25
+    At = _csr(*o.stokes(False, {"tensor": ct}, no, zero, prune=-1.0)[:3]).toarray()[:2 * nU, :2 * nU]
+    As = _csr(*o.stokes(False, {"stiffness": ct}, no, zero, prune=-1.0)[:3]).toarray()[:2 * nU, :2 * nU]
+    Ad = _csr(*o.stokes(False, {"div": cd}, no, zero, prune=-1.0)[:3]).toarray()
+    D = Ad[2 * nU:, :2 * nU]
+    X = At - As
+    scale = np.abs(At).max()
+    assert np.abs(At - At.T).max() <= 1e-14 * scale
+    x = o.dof_coords()
+    for u in (np.concatenate([x[:, 1], np.zeros(nU)]), np.concatenate([x[:, 0], -x[:, 1]]),
+              np.concatenate([np.zeros(nU), x[:, 0]])):
+        assert np.abs(X @ u).max() <= 1e-13 * scale
+    u = np.concatenate([x[:, 0], np.zeros(nU)])
+    ref = (ct / cd) * (D.T @ np.ones(D.shape[0]))
+    assert np.abs(ref).max() > 0.1 * ct
+    assert np.abs(X @ u - ref).max() <= 1e-13 * scale
+
+
+def test_pinned_pressure_acts_as_dummy_with_bcs():
+    """A constrained pressure dof: buildLhs(AssemblyDummy{p}, bcsP) puts 1 on
+    its diagonal per element and h in b per element; the div rows of that dof
+    are cleared, its grad column is lifted into the velocity rhs."""
+    O.build()
+    n = (3, 3)
+    o = O.Space("quad_q2", "square", n)
+    nU = o.ndofs
+    vflag, vval = _vel_bc(o, False)
+    k = {"stiffness": 1.0, "grad": -1.0, "div": -1.0}
+    r0, c0, v0, b0 = o.stokes(False, k, vflag, vval)
+    A0 = _csr(r0, c0, v0).toarray()
+    nd = A0.shape[0]
+    flag = np.zeros(nd, np.uint8)
+    val = np.zeros(nd)
+    flag[:2 * nU] = vflag
+    val[:2 * nU] = vval
+    pin = 2 * nU + 5  # an interior-edge pressure dof of the 4 x 4 vertex grid (2 elements)
+    flag[pin] = 1
+    val[pin] = 2.5
+    r1, c1, v1, b1 = o.stokes(False, k, flag, val)
+    A1 = _csr(r1, c1, v1).toarray()
+    count = A1[pin, pin]
+    assert count in (1.0, 2.0, 4.0)
+    row = A1[pin].copy()
+    row[pin] = 0.0
+    assert np.abs(row).max() == 0.0 and np.abs(A1[:, pin]).sum() == count
+    assert b1[pin] == 2.5 * count
+    free = np.where(flag[:2 * nU] == 0)[0]
+    assert np.abs((b1 - b0)[free] + 2.5 * A0[free, pin]).max() <= 1e-14
+    others = [r for r in range(2 * nU, nd) if r != pin]
+    assert np.array_equal(b1[others], b0[others])
+    A0[:, pin] = 0.0
+    A0[pin, :] = 0.0
+    A0[pin, pin] = count
+    assert np.abs(A1 - A0).max() == 0.0
