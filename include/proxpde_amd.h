@@ -45,12 +45,30 @@ extern "C" {
 #define PXG_FE_HEX_Q2 5  /* RefHexahedronQ2,  GaussQR<Hexahedron,27>   */
 #define PXG_FE_TET_P1 6  /* RefTetrahedronP1, GaussQR<Tetrahedron,4>   */
 #define PXG_FE_TET_P2 7  /* RefTetrahedronP2, GaussQR<Tetrahedron,11>  */
+/* Taylor-Hood pair on quads as one element of 22 local dofs: the velocity
+ * FESpace<Mesh<Quad>, RefQuadQ2, GaussQR<Quad,9>, 2> (local i + 9 d for
+ * component d, include/fespace.hpp) followed by the pressure
+ * FESpace<Mesh<Quad>, RefQuadQ1, GaussQR<Quad,9>> (local 18 + k), as used
+ * together by test/assemblyvector.cpp and test/stokes2dquad.cpp.  The dof
+ * map row of an element holds the 18 velocity ids (BLOCK: id + d nU,
+ * INTERLEAVED: 2 id + d, include/dof.hpp:139-166) and the 4 pressure ids
+ * (after the velocity, the FESpace offset).  Terms: STIFFNESS,
+ * TENSOR_STIFFNESS, MASS on the velocity block, GRAD and DIV couplings.
+ * A constrained pressure dof acts as buildLhs(AssemblyDummy{p}, bcsP) plus
+ * bcsP in both couplings.                                                  */
+#define PXG_FE_QUAD_Q2Q1 8
 
 /* ---- assembly terms (reference include/assembly_lhs.hpp) --------------- */
 #define PXG_TERM_STIFFNESS 1    /* AssemblyStiffness(coef)          :36-75   */
 #define PXG_TERM_STIFFNESS_FE 2 /* AssemblyStiffnessFE(kappa)       :77-113  */
 #define PXG_TERM_MASS 3         /* AssemblyMass / ScalarMass(coef)  :191-231 */
 #define PXG_TERM_MASS_FE 4      /* AssemblyMassFE(coef, kappa)      :233-283 */
+/* couplings of PXG_FE_QUAD_Q2Q1 (reference include/assembly_coupling.hpp and
+ * Builder::buildCoupling, include/builder.hpp:156-277), comp list {0, 1}:   */
+#define PXG_TERM_GRAD 5 /* AssemblyGrad(coef, vel, p)  :41-86, rows velocity  */
+#define PXG_TERM_DIV 6  /* AssemblyDiv(coef, p, vel)   :88-133, rows pressure */
+/* AssemblyTensorStiffness(coef, vel), assembly_lhs.hpp:115-173 (velocity)   */
+#define PXG_TERM_TENSOR_STIFFNESS 7
 
 /* ---- analytic right-hand sides (AssemblyRhsAnalytic, assembly_rhs.hpp:34-91)
  * The reference takes a host std::function; the closed forms used by its

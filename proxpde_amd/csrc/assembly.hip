@@ -269,6 +269,183 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // --------------------------------------------------------------------------
+// Taylor-Hood Q2-Q1 element: one thread per (element, local row), local order
+// [velocity x (9) | velocity y (9) | pressure (4)], tables nq x 22 (values)
+// and nq x 22 x 2 (reference gradients).  Per quadrature point the thread
+// rebuilds the vertex map J (3 x 2), JxW = sqrt(det J^T J) w_q and the
+// physical gradients dphi = gref (J^T J)^-1 J^T (include/curfe.hpp), then adds
+//   velocity row (a, d), s = coef JxW:
+//     AssemblyStiffness        (d, d):  s dphi_a . dphi_j
+//     AssemblyTensorStiffness  (d, d):  s dphi_a . dphi_j and for every
+//                              (d, dj): s dphi_a[d] dphi_j[dj]
+//                              (include/assembly_lhs.hpp:131-163)
+//     AssemblyScalarMass       (d, d):  s phi_a phi_j
+//     AssemblyGrad             pressure column k: s dphi_a[d] psi_k
+//                              (include/assembly_coupling.hpp:66-83)
+//   pressure row k:
+//     AssemblyDiv              (d) column j: s psi_k dphi_j[d]
+//                              (include/assembly_coupling.hpp:113-130)
+// and eliminates the constrained dofs of the element like k_element_rows:
+// a constrained row becomes the unit row with Fe = h (Builder::buildLhs,
+// include/builder.hpp:83-124; for a pressure dof: AssemblyDummy with bcsP),
+// constrained columns are cleared and lifted, Fe_i = -sum_j Ke_ij h_j, which
+// is buildCoupling's Fe = -Crow Ke h (include/builder.hpp:193-248).
+struct ThCoef
+{
+  double stiff, tensor, mass, grad, div;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_element_th(int64_t E, int nq, double const * __restrict__ xyz, uint32_t const * __restrict__ conn,
+                 double const * __restrict__ geoGrad, double const * __restrict__ w, T const * __restrict__ gradTab,
+                 T const * __restrict__ phiTab, ThCoef cf, uint32_t const * __restrict__ emap,
+                 uint8_t const * __restrict__ bcFlag, double const * __restrict__ bcVal, int hasBC,
+                 T * __restrict__ ke, double * __restrict__ fe, int * __restrict__ status)
+{
+  constexpr int N = 22, NV = 9, NP = 4;
+  extern __shared__ unsigned char smem_raw[];
+  double * sGeo = reinterpret_cast<double *>(smem_raw); // nq x 4 x 2
+  T * sGrad = reinterpret_cast<T *>(sGeo + nq * 8);     // nq x 22 x 2
+  T * sPhi = sGrad + nq * N * 2;                        // nq x 22
+  for (int k = threadIdx.x; k < nq * 8; k += blockDim.x)
+    sGeo[k] = geoGrad[k];
+  for (int k = threadIdx.x; k < nq * N * 2; k += blockDim.x)
+    sGrad[k] = gradTab[k];
+  for (int k = threadIdx.x; k < nq * N; k += blockDim.x)
+    sPhi[k] = phiTab[k];
+  __syncthreads();
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= E * N)
+    return;
+  int64_t const e = t / N;
+  int const i = (int)(t - e * N);
+  bool const vel = i < 2 * NV;
+  int const d = vel ? i / NV : 0;     // velocity component of the row
+  int const a = vel ? i - d * NV : 0; // its Q2 basis
+  int const k = vel ? 0 : i - 2 * NV; // pressure basis of a pressure row
+  double x[4][3];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+  {
+    uint32_t const pid = conn[e * 4 + v];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      x[v][r] = xyz[3ull * pid + r];
+  }
+  T acc[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+    acc[j] = T(0);
+  for (int q = 0; q < nq; ++q)
+  {
+    double J[3][2] = {{0., 0.}, {0., 0.}, {0., 0.}};
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+    {
+      double const g0 = sGeo[(q * 4 + v) * 2], g1 = sGeo[(q * 4 + v) * 2 + 1];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+      {
+        J[r][0] += x[v][r] * g0;
+        J[r][1] += x[v][r] * g1;
+      }
+    }
+    double const A00 = J[0][0] * J[0][0] + J[1][0] * J[1][0] + J[2][0] * J[2][0];
+    double const A01 = J[0][0] * J[0][1] + J[1][0] * J[1][1] + J[2][0] * J[2][1];
+    double const A11 = J[0][1] * J[0][1] + J[1][1] * J[1][1] + J[2][1] * J[2][1];
+    double const det = A00 * A11 - A01 * A01;
+    if (!(det > 0.) && i == 0)
+      atomicOr(status, 1);
+    double const I00 = A11 / det, I01 = -A01 / det, I11 = A00 / det;
+    // pseudo-inverse rows P[a][c] = sum_b I[a][b] J[c][b]
+    T P[2][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+    {
+      P[0][c] = (T)(I00 * J[c][0] + I01 * J[c][1]);
+      P[1][c] = (T)(I01 * J[c][0] + I11 * J[c][1]);
+    }
+    double const JxW = sqrt(det) * w[q];
+    T const * gq = sGrad + q * N * 2;
+    T const * pq = sPhi + q * N;
+    auto dphi = [&](int b, int c) { return gq[b * 2] * P[0][c] + gq[b * 2 + 1] * P[1][c]; };
+    if (vel)
+    {
+      T gi[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        gi[c] = dphi(i, c);
+      T const s = (T)(cf.stiff * JxW), ts = (T)(cf.tensor * JxW), m = (T)(cf.mass * JxW);
+      T const gs = (T)(cf.grad * JxW) * gi[d];
+      T const pa = pq[i];
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+      {
+        T gj[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          gj[c] = dphi(j, c); // (the x and y copies of basis j share one gradient)
+        T const dd = gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2];
+        T diag = s * dd + ts * dd + m * (pa * pq[j]);
+        if (d == 0)
+        {
+          acc[j] += diag + ts * (gi[0] * gj[0]);
+          acc[NV + j] += ts * (gi[0] * gj[1]);
+        }
+        else
+        {
+          acc[j] += ts * (gi[1] * gj[0]);
+          acc[NV + j] += diag + ts * (gi[1] * gj[1]);
+        }
+      }
+#pragma unroll
+      for (int kk = 0; kk < NP; ++kk)
+        acc[2 * NV + kk] += gs * pq[2 * NV + kk];
+    }
+    else
+    {
+      T const sp = (T)(cf.div * JxW) * pq[2 * NV + k];
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+      {
+        acc[j] += sp * dphi(j, 0);
+        acc[NV + j] += sp * dphi(j, 1);
+      }
+    }
+  }
+  double f = 0.;
+  if (hasBC)
+  {
+    uint32_t const * ids = emap + e * N;
+    if (bcFlag[ids[i]])
+    {
+      f = bcVal[ids[i]];
+#pragma unroll
+      for (int j = 0; j < N; ++j)
+        acc[j] = (j == i) ? T(1) : T(0);
+    }
+    else
+    {
+      double sum = 0.;
+#pragma unroll
+      for (int j = 0; j < N; ++j)
+        if (bcFlag[ids[j]])
+        {
+          sum += (double)acc[j] * bcVal[ids[j]];
+          acc[j] = T(0);
+        }
+      f = 0. - sum;
+    }
+  }
+  T * out = ke + (e * N + i) * keRowStride(N);
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+    out[j] = acc[j];
+  fe[t] = f;
+}
+
+// --------------------------------------------------------------------------
 // pattern construction: keys row*ndofs+col of every element-matrix entry
 __global__ void k_entry_keys(int64_t E, int n, int64_t ndofs, uint32_t const * __restrict__ emap,
                              uint64_t * __restrict__ keys, uint32_t * __restrict__ ids)
@@ -1120,6 +1297,7 @@ void dispatchFe(int feType, F && f)
   case PXG_FE_HEX_Q2: f(std::integral_constant<int, 27>{}, std::integral_constant<int, 3>{}); break;
   case PXG_FE_TET_P1: f(std::integral_constant<int, 4>{}, std::integral_constant<int, 3>{}); break;
   case PXG_FE_TET_P2: f(std::integral_constant<int, 10>{}, std::integral_constant<int, 3>{}); break;
+  case PXG_FE_QUAD_Q2Q1: f(std::integral_constant<int, 22>{}, std::integral_constant<int, 2>{}); break;
   default: throw Error(PXG_E_INVALID, "unknown FE type");
   }
 }
@@ -1247,17 +1425,29 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
   PXG_REQUIRE(c->fe.feType && c->nelem && c->ndofs && c->nnzS, PXG_E_STATE,
               "assemble needs set_fe, set_mesh, set_dofmap and build_pattern first");
   PXG_REQUIRE(precision == PXG_FP64 || precision == PXG_FP32, PXG_E_INVALID, "precision");
+  bool const mixed = c->fe.feType == PXG_FE_QUAD_Q2Q1;
   int flags = 0;
   double cS = 0., cSF = 0., cM = 0., cMF = 0.;
+  ThCoef th{0., 0., 0., 0., 0.};
   for (int k = 0; k < nterms; ++k)
-    switch (terms[k].kind)
+  {
+    int const kind = terms[k].kind;
+    bool const coupling = kind == PXG_TERM_GRAD || kind == PXG_TERM_DIV || kind == PXG_TERM_TENSOR_STIFFNESS;
+    PXG_REQUIRE(!coupling || mixed, PXG_E_INVALID, "grad, div and tensor stiffness need the Taylor-Hood element");
+    PXG_REQUIRE(!mixed || !(kind == PXG_TERM_STIFFNESS_FE || kind == PXG_TERM_MASS_FE), PXG_E_INVALID,
+                "FE coefficient terms are not available on the Taylor-Hood element");
+    switch (kind)
     {
-    case PXG_TERM_STIFFNESS: flags |= kHasStiff, cS += terms[k].coef; break;
+    case PXG_TERM_STIFFNESS: flags |= kHasStiff, cS += terms[k].coef, th.stiff += terms[k].coef; break;
     case PXG_TERM_STIFFNESS_FE: flags |= kHasStiffFE, cSF += terms[k].coef; break;
-    case PXG_TERM_MASS: flags |= kHasMass, cM += terms[k].coef; break;
+    case PXG_TERM_MASS: flags |= kHasMass, cM += terms[k].coef, th.mass += terms[k].coef; break;
     case PXG_TERM_MASS_FE: flags |= kHasMassFE, cMF += terms[k].coef; break;
+    case PXG_TERM_GRAD: th.grad += terms[k].coef; break;
+    case PXG_TERM_DIV: th.div += terms[k].coef; break;
+    case PXG_TERM_TENSOR_STIFFNESS: th.tensor += terms[k].coef; break;
     default: throw Error(PXG_E_INVALID, "unknown term kind");
     }
+  }
   PXG_REQUIRE(!(flags & (kHasStiffFE | kHasMassFE)) || c->hasKappa, PXG_E_STATE,
               "FE coefficient terms need pxg_set_coefficient");
   if (c->valsDirty && precision != c->valPrecision)
@@ -1281,12 +1471,12 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
     return v && std::string(v) == "rows";
   }();
   bool fused = false;
-  if (!rowsOnly)
+  if (!rowsOnly && !mixed)
   {
     ProfScope ps(c, precision == PXG_FP64 ? "element" : "element32");
     fused = elementMfma(c, flags, cS, cSF, cM, cMF, precision);
   }
-  if (!fused)
+  if (!fused && !mixed)
   {
     c->geoBuf.resize(E * T.nq * 8);
     dispatchGeo(T.geo,
@@ -1320,16 +1510,31 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
              [&](auto nC, auto dimC)
              {
                constexpr int N = decltype(nC)::value, D = decltype(dimC)::value;
+               // Taylor-Hood pair: geometry and element matrix in one kernel
+               auto elementTh = [&](auto tag, auto * ke, auto const * gradTab, auto const * phiTab)
+               {
+                 using TT = decltype(tag);
+                 size_t const sh = (size_t)T.nq * 8 * sizeof(double) + (size_t)T.nq * N * 3 * sizeof(TT);
+                 ProfScope ps(c, sizeof(TT) == 8 ? "element" : "element32");
+                 k_element_th<TT><<<gridFor(E * N), kBlock, sh, s>>>(
+                     E, T.nq, c->xyz.p, c->conn.p, T.d_geoGrad.p, T.d_w.p, gradTab, phiTab, th, c->elemMap.p,
+                     c->bcFlag.p, c->bcVal.p, c->hasBC, ke, c->feBuf.p, c->status.p);
+               };
                if (precision == PXG_FP64)
                {
                  if (!fused)
                  {
                    c->keBuf.resize(E * N * keRowStride(N));
-                   size_t const sh = (size_t)T.nq * N * (D + 1) * sizeof(double);
-                   ProfScope ps(c, "element");
-                   k_element_rows<double, N, D><<<gridFor(E * N), kBlock, sh, s>>>(
-                       E, T.nq, c->geoBuf.p, T.d_grad.p, T.d_phi.p, hasMass, c->elemMap.p, c->bcFlag.p,
-                       c->bcVal.p, c->hasBC, c->keBuf.p, c->feBuf.p);
+                   if constexpr (N == 22)
+                     elementTh(double{}, c->keBuf.p, T.d_grad.p, T.d_phi.p);
+                   else
+                   {
+                     size_t const sh = (size_t)T.nq * N * (D + 1) * sizeof(double);
+                     ProfScope ps(c, "element");
+                     k_element_rows<double, N, D><<<gridFor(E * N), kBlock, sh, s>>>(
+                         E, T.nq, c->geoBuf.p, T.d_grad.p, T.d_phi.p, hasMass, c->elemMap.p, c->bcFlag.p,
+                         c->bcVal.p, c->hasBC, c->keBuf.p, c->feBuf.p);
+                   }
                  }
                  PXG_HIP(hipGetLastError());
                  ProfScope ps2(c, "scatter");
@@ -1344,11 +1549,16 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                  if (!fused)
                  {
                    c->keBuf32.resize(E * N * keRowStride(N));
-                   size_t const sh = (size_t)T.nq * N * (D + 1) * sizeof(float);
-                   ProfScope ps(c, "element32");
-                   k_element_rows<float, N, D><<<gridFor(E * N), kBlock, sh, s>>>(
-                       E, T.nq, c->geoBuf.p, T.d_grad32.p, T.d_phi32.p, hasMass, c->elemMap.p, c->bcFlag.p,
-                       c->bcVal.p, c->hasBC, c->keBuf32.p, c->feBuf.p);
+                   if constexpr (N == 22)
+                     elementTh(float{}, c->keBuf32.p, T.d_grad32.p, T.d_phi32.p);
+                   else
+                   {
+                     size_t const sh = (size_t)T.nq * N * (D + 1) * sizeof(float);
+                     ProfScope ps(c, "element32");
+                     k_element_rows<float, N, D><<<gridFor(E * N), kBlock, sh, s>>>(
+                         E, T.nq, c->geoBuf.p, T.d_grad32.p, T.d_phi32.p, hasMass, c->elemMap.p, c->bcFlag.p,
+                         c->bcVal.p, c->hasBC, c->keBuf32.p, c->feBuf.p);
+                   }
                  }
                  PXG_HIP(hipGetLastError());
                  ProfScope ps2(c, "scatter32");
@@ -1386,6 +1596,8 @@ void assembleRhs(pxg_ctx * c, int kind, double const * params, double const * qv
   PXG_REQUIRE(c->fe.feType && c->nelem && c->ndofs && c->nnzS, PXG_E_STATE,
               "assemble_rhs needs set_fe, set_mesh, set_dofmap and build_pattern first");
   PXG_REQUIRE(kind >= PXG_RHS_CONST && kind <= PXG_RHS_PROJECTION, PXG_E_INVALID, "rhs kind");
+  PXG_REQUIRE(c->fe.feType != PXG_FE_QUAD_Q2Q1, PXG_E_INVALID,
+              "right-hand sides of the Taylor-Hood pair are assembled per component space");
   hipStream_t s = c->stream;
   FeTables & T = c->fe;
   int64_t const E = c->nelem, m = E * T.nq;

@@ -42,13 +42,35 @@ int guarded(pxg_ctx * c, F && f)
   }
 }
 
+// local basis b of the FE of a context: the FE itself, or for the Taylor-Hood
+// pair (22 local dofs) Q2 for both velocity components and Q1 for the pressure
+struct LocalBasis
+{
+  FeInfo const * f;
+  int i;
+};
+LocalBasis localBasis(int feType, int b)
+{
+  if (feType != PXG_FE_QUAD_Q2Q1)
+    return {&feInfo(feType), b};
+  if (b < 18)
+    return {&feInfo(PXG_FE_QUAD_Q2), b % 9};
+  return {&feInfo(PXG_FE_QUAD_Q1), b - 18};
+}
+
 void buildTables(pxg_ctx * c, int feType, int nq, double const * nodes, double const * weights)
 {
-  FeInfo const & fe = feInfo(feType);
+  bool const mixed = feType == PXG_FE_QUAD_Q2Q1;
+  FeInfo fe = feInfo(mixed ? PXG_FE_QUAD_Q2 : feType);
+  if (mixed)
+  {
+    fe.type = feType;
+    fe.n = 22;
+  }
   FeTables & T = c->fe;
   Quadrature q;
-  if (nq <= 0)
-    q = recommendedQuadrature(feType);
+  if (nq <= 0) // the pair shares the velocity's rule (both spaces use GaussQR<Quad,9>)
+    q = recommendedQuadrature(mixed ? PXG_FE_QUAD_Q2 : feType);
   else
   {
     PXG_REQUIRE(nodes && weights, PXG_E_INVALID, "quadrature nodes and weights required");
@@ -74,8 +96,9 @@ void buildTables(pxg_ctx * c, int feType, int nq, double const * nodes, double c
     double const * xi = &q.xi[(size_t)iq * fe.dim];
     for (int i = 0; i < fe.n; ++i)
     {
-      T.phi[iq * fe.n + i] = fe.phi(i, xi);
-      fe.grad(i, xi, &T.grad[((size_t)iq * fe.n + i) * fe.dim]);
+      LocalBasis const lb = localBasis(feType, i);
+      T.phi[iq * fe.n + i] = lb.f->phi(lb.i, xi);
+      lb.f->grad(lb.i, xi, &T.grad[((size_t)iq * fe.n + i) * fe.dim]);
     }
     for (int v = 0; v < npe; ++v)
       geoGrad(fe.geo, v, xi, &T.geoGrad[((size_t)iq * npe + v) * fe.dim]);
@@ -196,7 +219,7 @@ int pxg_set_fe(pxg_ctx * c, int feType, int nq, const double * nodes, const doub
   return guarded(c,
                  [&]
                  {
-                   PXG_REQUIRE(feType >= PXG_FE_LINE_P1 && feType <= PXG_FE_TET_P2, PXG_E_INVALID,
+                   PXG_REQUIRE(feType >= PXG_FE_LINE_P1 && feType <= PXG_FE_QUAD_Q2Q1, PXG_E_INVALID,
                                "unknown FE type");
                    PXG_REQUIRE(nq >= 0 && nq <= 64, PXG_E_INVALID, "quadrature size must be 0..64");
                    buildTables(c, feType, nq, nodes, weights);

@@ -413,7 +413,7 @@ void dispatchFeMf(int feType, F && f)
   case PXG_FE_HEX_Q2: f(integral_constant<int, 3>{}, integral_constant<int, 8>{}, integral_constant<int, 27>{}); break;
   case PXG_FE_TET_P1: f(integral_constant<int, 3>{}, integral_constant<int, 4>{}, integral_constant<int, 4>{}); break;
   case PXG_FE_TET_P2: f(integral_constant<int, 3>{}, integral_constant<int, 4>{}, integral_constant<int, 10>{}); break;
-  default: throw Error(PXG_E_INVALID, "unknown FE type");
+  default: throw Error(PXG_E_INVALID, "no matrix-free operator for this FE type");
   }
 }
 

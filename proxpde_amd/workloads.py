@@ -108,3 +108,76 @@ class Problem:
         a.set_mesh(self.xyz, self.conn)
         a.set_dofmap(self.ndofs, self.emap)
         return a
+
+
+class StokesProblem:
+    """Taylor-Hood Q2-Q1 system on a structured quad mesh (host layer).
+
+    Velocity: the Q2 space with 2 components, BLOCK (id + d nU) or INTERLEAVED
+    (2 id + d) dof ordering (reference include/dof.hpp:139-166); pressure: the
+    Q1 space after the velocity (offset 2 nU), as in the reference's
+    test/assemblyvector.cpp and test/stokes2dquad.cpp.  `emap` is the dof map
+    of the 22-dof element PXG_FE_QUAD_Q2Q1: [u (9) | v (9) | p (4)]."""
+
+    def __init__(self, n, interleaved=False, length=(1.0, 1.0, 0.0)):
+        self.interleaved = bool(interleaved)
+        self.vel = Problem("quad_q2", n, length=length)
+        self.mesh = self.vel.mesh
+        self.xyz, self.conn = self.vel.xyz, self.vel.conn
+        self.nU = self.vel.ndofs
+        self.nP, emapP = self.mesh.dofmap("quad_q1")
+        self.emapP = emapP
+        ev = self.vel.emap.astype(np.int64)
+        E = ev.shape[0]
+        emap = np.empty((E, 22), dtype=np.uint32)
+        for d in range(2):
+            emap[:, 9 * d:9 * d + 9] = self.vel_id(ev, d)
+        emap[:, 18:] = emapP.astype(np.int64) + 2 * self.nU
+        self.emap = emap
+        self.ndofs = 2 * self.nU + self.nP
+
+    def vel_id(self, i, d):
+        """Global id of component d of velocity dof i."""
+        i = np.asarray(i, dtype=np.int64)
+        return 2 * i + d if self.interleaved else i + d * self.nU
+
+    def set_coords(self, xyz):
+        self.vel.set_coords(xyz)
+        self.xyz = self.vel.xyz
+
+    def dof_coords(self):
+        """Coordinates of the Q2 (velocity) dofs."""
+        return self.vel.dof_coords()
+
+    def dirichlet(self, bcs, pressure=()):
+        """Dirichlet data of the whole system from BCEss-like entries
+        (side, value, comps): value is a pair (u, v) or a function of the dof
+        coordinates returning an (n, 2) array, comps the constrained
+        components (default both).  Later entries win where sides share dofs,
+        as the reference's BCEss list does.  pressure: (pressure dof, value)
+        pairs.  Returns (dofs, values) for set_dirichlet and the full flag /
+        value arrays."""
+        flag = np.zeros(self.ndofs, np.uint8)
+        val = np.zeros(self.ndofs)
+        x = self.dof_coords()
+        for entry in bcs:
+            side, value = entry[0], entry[1]
+            comps = entry[2] if len(entry) > 2 else (0, 1)
+            ids = self.vel.bc_dofs((side,)).astype(np.int64)
+            vv = value(x[ids]) if callable(value) else np.tile(np.asarray(value, np.float64), (len(ids), 1))
+            for d in comps:
+                g = self.vel_id(ids, d)
+                flag[g] = 1
+                val[g] = vv[:, d]
+        for k, v in pressure:
+            flag[2 * self.nU + k] = 1
+            val[2 * self.nU + k] = v
+        dofs = np.nonzero(flag)[0].astype(np.uint32)
+        return dofs, val[dofs], flag, val
+
+    def assembler(self, qr=None, device=0):
+        a = N.Assembler(device)
+        a.set_fe("quad_q2q1", qr)
+        a.set_mesh(self.xyz, self.conn)
+        a.set_dofmap(self.ndofs, self.emap)
+        return a

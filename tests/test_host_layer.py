@@ -87,3 +87,30 @@ def test_gmsh_errors(native, tmp_path):
         native.HostMesh.gmsh(str(bad), "tet")
     with pytest.raises(native.PxgError, match="not found"):
         native.HostMesh.gmsh(str(tmp_path / "missing.msh"), "tet")
+
+
+@pytest.mark.parametrize("interleaved", [False, True])
+def test_stokes_dofmap_matches_oracle(oracle, native, interleaved):
+    """The 22-dof Taylor-Hood map: velocity components in BLOCK or INTERLEAVED
+    ordering (reference include/dof.hpp:139-166), then the Q1 pressure after
+    2 nU; Dirichlet data with later entries winning on shared dofs."""
+    from proxpde_amd import workloads as W
+    n = (4, 3)
+    prob = W.StokesProblem(n, interleaved)
+    ov = oracle.Space("quad_q2", "square", n)
+    op = oracle.Space("quad_q1", "square", n)
+    ev = ov.elem_map().astype(np.int64)
+    nU = ov.ndofs
+    ref = np.concatenate([2 * ev if interleaved else ev, 2 * ev + 1 if interleaved else ev + nU,
+                          op.elem_map().astype(np.int64) + 2 * nU], axis=1)
+    assert np.array_equal(prob.emap, ref.astype(np.uint32))
+    assert prob.ndofs == 2 * nU + op.ndofs
+    dofs, vals, flag, full = prob.dirichlet([("bottom", (0.0, 2.0)), ("top", (1.0, 0.0), (0,)),
+                                             ("left", (3.0, 4.0))], pressure=((2, 0.5),))
+    left = np.asarray(ov.bc_dofs("left"), np.int64)
+    top = np.asarray(ov.bc_dofs("top"), np.int64)
+    assert np.all(full[prob.vel_id(left, 0)] == 3.0) and np.all(full[prob.vel_id(left, 1)] == 4.0)
+    top_only = np.setdiff1d(top, left)
+    assert np.all(full[prob.vel_id(top_only, 0)] == 1.0) and not flag[prob.vel_id(top_only, 1)].any()
+    assert flag[2 * nU + 2] == 1 and full[2 * nU + 2] == 0.5
+    assert np.array_equal(dofs, np.nonzero(flag)[0]) and np.array_equal(vals, full[dofs])

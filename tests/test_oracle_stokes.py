@@ -158,3 +158,60 @@ def test_pinned_pressure_acts_as_dummy_with_bcs():
     A0[pin, :] = 0.0
     A0[pin, pin] = count
     assert np.abs(A1 - A0).max() == 0.0
+
+
+def lu_solve(A, b):
+    """Direct solve standing in for the reference's LUSolver: SuperLU and one
+    step of iterative refinement (without it the pressure error of the
+    stokes2dquad case is 7e-14, above the reference's 3e-14, from the
+    factorisation's rounding alone)."""
+    import scipy.sparse.linalg as spl
+    A = A.tocsc()
+    lu = spl.splu(A)
+    x = lu.solve(b)
+    return x + lu.solve(b - A @ x)
+
+
+def stokes2dquad_known_answer(space, pspace, solve_system, n=(10, 10)):
+    """The reference's stokes2dquad known answer: the Q2-Q1 solution equals the
+    interpolant of u = (0, (1 - x^2) / 2), p = nu (1 - y) (errors below 1e-14,
+    2e-14, 3e-14 in the 2-norm of the nodal vectors, stokes2dquad.cpp:142-152).
+    solve_system(flag, val) returns the solution vector."""
+    nu = 0.1
+    x = space.dof_coords()
+    nU = len(x)
+    xp = pspace.dof_coords()
+    nd = 2 * nU + len(xp)
+    flag = np.zeros(nd, np.uint8)
+    val = np.zeros(nd)
+    for side, f, comps in (("bottom", lambda p: np.stack([0 * p[:, 0], 0.5 * (1 - p[:, 0] ** 2)], 1), (0, 1)),
+                           ("right", lambda p: np.zeros((len(p), 2)), (0, 1)),
+                           ("top", lambda p: np.zeros((len(p), 2)), (0,)),
+                           ("left", lambda p: np.zeros((len(p), 2)), (0,))):
+        ids = np.asarray(space.bc_dofs(side), np.int64)
+        v = f(x[ids])
+        for d in comps:
+            flag[ids + d * nU] = 1
+            val[ids + d * nU] = v[:, d]
+    sol = solve_system(flag, val, nu)
+    u_ex = np.zeros(nU)
+    v_ex = 0.5 * (1 - x[:, 0] ** 2)
+    p_ex = nu * (1 - xp[:, 1])
+    eu = np.linalg.norm(sol[:nU] - u_ex)
+    ev = np.linalg.norm(sol[nU:2 * nU] - v_ex)
+    ep = np.linalg.norm(sol[2 * nU:] - p_ex)
+    return eu, ev, ep
+
+
+def test_stokes2dquad_known_answer():
+    O.build()
+    n = (10, 10)
+    o = O.Space("quad_q2", "square", n)
+    op = O.Space("quad_q1", "square", n)
+
+    def solve(flag, val, nu):
+        rp, col, v, b = o.stokes(False, {"tensor": nu, "grad": -1.0, "div": -1.0}, flag, val)
+        return lu_solve(_csr(rp, col, v), b)
+    eu, ev, ep = stokes2dquad_known_answer(o, op, solve, n)
+    print(f"errors u {eu:.2e} v {ev:.2e} p {ep:.2e}")
+    assert eu <= 1e-14 and ev <= 2e-14 and ep <= 3e-14

@@ -1,0 +1,182 @@
+"""Parity of the Taylor-Hood (Q2-Q1) Stokes assembly on the GPU with the oracle.
+
+The HIP path assembles the velocity block (AssemblyStiffness,
+AssemblyTensorStiffness, AssemblyScalarMass), the AssemblyGrad / AssemblyDiv
+couplings and the Dirichlet elimination of Builder::buildLhs / buildCoupling
+(reference include/builder.hpp:31-277) as one 22-dof element through the C ABI.
+Bar: FP64 values of A and b within 1e-12 relative to the oracle (the bar of
+the scalar elements) and the closed pattern equal except at rounding level:
+many entries of the coupling blocks are zero in exact arithmetic and carry
+residues of 1e-19 to 1e-15, and whether one survives prune(1e-16) depends on
+the order of evaluation (the reference's own pattern depends on Eigen's), so an
+entry present on one side only must be below 1e-14 of the matrix scale on
+both.  Further: the reference's stokes2dquad
+known answer with the GPU-assembled system; a rebuild after clear (the staged
+path into the closed layout) bitwise equal to the first build; FP32 within
+2e-6 of the FP64 oracle in max norm.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from proxpde_amd import native as N
+from proxpde_amd import workloads as W
+
+from test_oracle_stokes import lu_solve, stokes2dquad_known_answer
+
+pytestmark = pytest.mark.gpu
+
+AV_BC = [("bottom", (0.0, 0.0)), ("right", (0.0, 0.0)), ("top", (1.0, 0.0)), ("left", (0.0, 0.0))]
+
+
+def _inlet(p):
+    return np.stack([0.0 * p[:, 0], 0.5 * (1.0 - p[:, 0] ** 2)], 1)
+
+
+S2D_BC = [("bottom", _inlet), ("right", (0.0, 0.0)), ("top", (0.0, 0.0), (0,)), ("left", (0.0, 0.0), (0,))]
+TERM = {"stiffness": "stiffness", "tensor": "tensor_stiffness", "mass": "mass", "grad": "grad", "div": "div"}
+
+CASES = [
+    # n, interleaved, perturb, coefficients, velocity BCs, pressure pins
+    ((3, 3), False, 0.0, {"stiffness": 1.0, "grad": -1.0, "div": -1.0}, AV_BC, ()),
+    ((3, 3), True, 0.0, {"stiffness": 1.0, "grad": -1.0, "div": -1.0}, AV_BC, ()),
+    ((7, 5), True, 0.2, {"stiffness": 1.0, "grad": -1.0, "div": -1.0}, AV_BC, ()),
+    ((10, 10), False, 0.2, {"tensor": 0.1, "mass": 3.0, "grad": -1.0, "div": -0.5}, S2D_BC, ((5, 0.7),)),
+    ((12, 9), True, 0.15, {"stiffness": 0.5, "tensor": 0.25, "grad": 1.5, "div": -1.0}, S2D_BC, ()),
+]
+
+
+def _pair(oracle, n, interleaved, perturb):
+    prob = W.StokesProblem(n, interleaved)
+    o = oracle.Space("quad_q2", "square", n)
+    if perturb:
+        xyz = W.perturb_interior(prob.xyz, 1.0 / max(n), seed=3, amount=perturb, length=(1.0, 1.0, 0.0))
+        prob.set_coords(xyz)
+        o.set_coords(xyz)
+    assert np.array_equal(o.elem_map().astype(np.uint32), prob.vel.emap)
+    return prob, o
+
+
+def _rel_ok(a, b, tol=1e-12):
+    scale = np.maximum(np.abs(b), 1e-4 * np.abs(b).max())
+    return np.abs(a - b).max() <= tol * scale.max(), np.max(np.abs(a - b) / scale)
+
+
+def _assemble(prob, dofs, vals, coefs, precision=N.FP64):
+    a = prob.assembler()
+    a.set_dirichlet(dofs, vals)
+    a.build_pattern()
+    a.assemble([(TERM[k], c) for k, c in coefs.items()], precision)
+    a.close(1e-16)
+    return a
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0][0]}x{c[0][1]}-{'il' if c[1] else 'blk'}" for c in CASES])
+def test_stokes_matches_oracle(oracle, gpu, case):
+    n, interleaved, pert, coefs, bcs, pins = case
+    print(f"stokes case n={n} interleaved={interleaved} coefs={coefs}", flush=True)
+    prob, o = _pair(oracle, n, interleaved, pert)
+    dofs, vals, flag, full = prob.dirichlet(bcs, pins)
+    rp_o, col_o, v_o, b_o = o.stokes(interleaved, coefs, flag, full)
+    a = _assemble(prob, dofs, vals, coefs)
+    rp, col, v, b = a.csr()
+    nd = prob.ndofs
+    G = sp.csr_matrix((v, col, rp), shape=(nd, nd))
+    R = sp.csr_matrix((v_o, col_o.astype(np.int64), rp_o), shape=(nd, nd))
+    scale = np.abs(v_o).max()
+    pg, pr = G.copy(), R.copy()
+    pg.data[:] = 1.0
+    pr.data[:] = 1.0
+    only = abs(pg - pr).tocoo()
+    only_r, only_c = only.row[only.data != 0], only.col[only.data != 0]
+    if len(only_r):
+        tiny = max(np.abs(G[only_r, only_c]).max(), np.abs(R[only_r, only_c]).max())
+        assert tiny <= 1e-14 * scale, f"pattern differs at a non-rounding entry ({tiny:.3e})"
+    err = abs(G - R).max() / scale
+    assert err <= 1e-12, f"values rel err {err:.3e}"
+    okb, errb = _rel_ok(b, b_o)
+    assert okb, f"rhs rel err {errb:.3e}"
+    print(f"  nnz={len(v)} (oracle {len(v_o)}, {len(only_r)} rounding-level entries on one side only) "
+          f"max err A={err:.2e} b={errb:.2e}")
+
+
+def test_stokes_rebuild_after_clear_is_bitwise(oracle, gpu):
+    n, interleaved, pert, coefs, bcs, pins = CASES[3]
+    prob, o = _pair(oracle, n, interleaved, pert)
+    dofs, vals, flag, full = prob.dirichlet(bcs, pins)
+    a = _assemble(prob, dofs, vals, coefs)
+    first = a.csr()
+    terms = [(TERM[k], c) for k, c in coefs.items()]
+    for _ in range(2):
+        a.clear()
+        a.assemble(terms)
+        a.close(1e-16)
+        again = a.csr()
+        for x, y in zip(first, again):
+            assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
+
+
+def test_stokes_fp32_against_fp64_oracle(oracle, gpu):
+    n, interleaved, pert, coefs, bcs, pins = CASES[2]
+    prob, o = _pair(oracle, n, interleaved, pert)
+    dofs, vals, flag, full = prob.dirichlet(bcs, pins)
+    rp_o, col_o, v_o, b_o = o.stokes(interleaved, coefs, flag, full)
+    a = _assemble(prob, dofs, vals, coefs, precision=N.FP32)
+    rp, col, v, b = a.csr(fp32=True)
+    nd = prob.ndofs
+    A = sp.csr_matrix((v.astype(np.float64), col, rp), shape=(nd, nd)).toarray()
+    Ao = sp.csr_matrix((v_o, col_o.astype(np.int64), rp_o), shape=(nd, nd)).toarray()
+    err = np.abs(A - Ao).max() / np.abs(Ao).max()
+    errb = np.abs(b - b_o).max() / np.abs(b_o).max()
+    print(f"  fp32 max err A={err:.2e} b={errb:.2e}")
+    assert err <= 2e-6 and errb <= 2e-6
+
+
+class _VelSpace:
+    def __init__(self, prob):
+        self.prob = prob
+
+    def dof_coords(self):
+        return self.prob.dof_coords()
+
+    def bc_dofs(self, side):
+        return self.prob.vel.bc_dofs((side,))
+
+
+class _PressureSpace:
+    def __init__(self, prob):
+        self.prob = prob
+
+    def dof_coords(self):
+        return self.prob.mesh.dof_coords("quad_q1", self.prob.emapP, self.prob.nP)
+
+
+def test_stokes2dquad_known_answer_gpu(gpu):
+    """test/stokes2dquad.cpp on the GPU-assembled system: the discrete
+    solution is the interpolant of the exact one to the reference's bounds."""
+    n = (10, 10)
+    prob = W.StokesProblem(n, interleaved=False)
+
+    def solve(flag, val, nu):
+        dofs = np.nonzero(flag)[0].astype(np.uint32)
+        a = _assemble(prob, dofs, val[dofs], {"tensor": nu, "grad": -1.0, "div": -1.0})
+        rp, col, v, b = a.csr()
+        return lu_solve(sp.csr_matrix((v, col, rp), shape=(prob.ndofs, prob.ndofs)), b)
+    eu, ev, ep = stokes2dquad_known_answer(_VelSpace(prob), _PressureSpace(prob), solve, n)
+    print(f"  errors u {eu:.2e} v {ev:.2e} p {ep:.2e}")
+    assert eu <= 1e-14 and ev <= 2e-14 and ep <= 3e-14
+
+
+def test_stokes_rejects_unsupported(gpu):
+    prob = W.StokesProblem((3, 2))
+    a = prob.assembler()
+    a.build_pattern()
+    with pytest.raises(N.PxgError):
+        a.assemble([("mass_fe", 1.0)])
+    with pytest.raises(N.PxgError):
+        a.assemble_rhs("const", (1.0,))
+    scalar = W.Problem("quad_q2", (3, 2)).assembler()
+    scalar.build_pattern()
+    for kind in ("grad", "div", "tensor_stiffness"):
+        with pytest.raises(N.PxgError):
+            scalar.assemble([(kind, 1.0)])
