@@ -301,6 +301,80 @@ def config_c5(args, dev, N, W):
             "dofs": solver.ndofs}, "results": res}
 
 
+def config_stokes(args, dev, N, W):
+    """Taylor-Hood Q2-Q1 Stokes system on 1024^2 quads (BLOCK velocity dofs),
+    interior vertices perturbed U(-0.2h, 0.2h) seed 3: tensor stiffness 0.1 +
+    grad -1 + div -1 (test/stokes2dquad.cpp's operator), Dirichlet velocity on
+    all four sides; full builds (clear + assemble + close) in FP64 and FP32, the
+    22-dof element kernel and the staged scatter timed with HIP events."""
+    n = 1024
+    prob = W.StokesProblem((n, n))
+    prob.set_coords(W.perturb_interior(prob.xyz, 1.0 / n, seed=3, amount=0.2, length=(1.0, 1.0, 0.0)))
+    bcs = [("bottom", (0.0, 1.0)), ("right", (0.0, 0.0)), ("top", (0.0, 0.0)), ("left", (0.0, 0.0))]
+    dofs, vals, _, _ = prob.dirichlet(bcs)
+    a = prob.assembler()
+    a.set_dirichlet(dofs, vals)
+    t0 = time.time()
+    nnz_s = a.build_pattern()
+    t_pattern = time.time() - t0
+    E = prob.mesh.nelem
+    terms = [("tensor_stiffness", 0.1), ("grad", -1.0), ("div", -1.0)]
+    res = {"pattern_s": t_pattern, "nnz_structural": int(nnz_s)}
+    for name, prec in (("fp64", N.FP64), ("fp32", N.FP32)):
+        def build():
+            a.clear()
+            a.assemble(terms, precision=prec)
+            a.close(1e-16)
+        build()
+        build()
+        a.profile(True)
+        t, _ = _sync_time(build, 5)
+        el = "element" if prec == N.FP64 else "element32"
+        sc = "scatter" if prec == N.FP64 else "scatter32"
+        pe, ps = a.profile_get(el), a.profile_get(sc)
+        a.profile(False)
+        res[f"build_{name}_ms"] = 1e3 * t
+        res[f"elements_per_s_{name}"] = E / t
+        s = 8 if prec == N.FP64 else 4
+        ke_ms = pe[0] / max(pe[1], 1)
+        sc_ms = ps[0] / max(ps[1], 1)
+        # element kernel, algorithmic bytes per element: the 22 x 22 matrix and
+        # 22 lifting values written, 4 vertex ids + 4 x 3 coordinates, 22 dof ids,
+        # flags and values read
+        ke_bytes = E * (22 * 22 * s + 22 * 8 + 4 * 4 + 4 * 24 + 22 * (4 + 1 + 8))
+        # staged scatter: per structural entry its info word and value written,
+        # per contribution its slot and element-matrix value, per element row
+        # its pair id and lifting value, per row the segment offset, b read and
+        # written and both inverse diagonals
+        sc_bytes = nnz_s * (2 + s) + E * 484 * (2 + s) + E * 22 * (4 + 8) + prob.ndofs * (4 + 16 + 12)
+        res[f"element_{name}_ms"] = ke_ms
+        res[f"element_{name}_GB/s"] = ke_bytes / (ke_ms * 1e-3) / 1e9 if ke_ms else None
+        res[f"scatter_{name}_ms"] = sc_ms
+        res[f"scatter_{name}_GB/s"] = sc_bytes / (sc_ms * 1e-3) / 1e9 if sc_ms else None
+    res["nnz"] = int(a.nnz)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.build()
+        m = 128
+        o = O.Space("quad_q2", "square", (m, m))
+        sp_ = W.StokesProblem((m, m))
+        _, _, flag, full = sp_.dirichlet(bcs)
+        coefs = {"tensor": 0.1, "grad": -1.0, "div": -1.0}
+        o.stokes(False, coefs, flag, full)
+        tc = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            o.stokes(False, coefs, flag, full)
+        tcpu = (time.perf_counter() - tc) / reps
+        cpu = {"value": m * m / tcpu, "unit": "elements/s", "cores": 1, "kind": "port",
+               "sample": f"oracle buildLhs + 2 x buildCoupling + close, {m}^2 quads, {reps} reps"}
+    return {"metric": "Stokes Q2-Q1 1024^2 build (tensor stiffness + grad + div), elements/s",
+            "value": res["elements_per_s_fp64"], "unit": "elements/s",
+            "config": {"workload": "Taylor-Hood Q2-Q1 on 1024^2 perturbed quads, BLOCK dofs, Dirichlet velocity",
+                       "elements": E, "dofs": prob.ndofs}, "results": res, "cpu_baseline": cpu}
+
+
 def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -317,7 +391,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--solve-levels", type=int, default=5)
     ap.add_argument("--solve-reps", type=int, default=2)
-    ap.add_argument("--config", choices=["c2", "c3", "c5"], default=None,
+    ap.add_argument("--config", choices=["c2", "c3", "c5", "stokes"], default=None,
                     help="run one of the other workloads instead of the headline one (one JSON line)")
     ap.add_argument("--copy-gib", type=float, default=1.0,
                     help="size of a plain device copy timed for reference (0: none)")
@@ -335,7 +409,7 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.init()
     if args.config:
-        fn = {"c2": config_c2, "c3": config_c3, "c5": config_c5}[args.config]
+        fn = {"c2": config_c2, "c3": config_c3, "c5": config_c5, "stokes": config_stokes}[args.config]
         progress(f"config {args.config}")
         out = fn(args, dev, N, W)
         out.update({"n_gpus": 1, "data": "synthetic", "dtype": "f64 / f32"})
