@@ -295,108 +295,130 @@ struct ThCoef
   double stiff, tensor, mass, grad, div;
 };
 
+// Work split: a block takes G elements (G <= kBlock / 22, from the LDS
+// budget).  Phase 1, one thread per (element, q): J, JxW and the
+// pseudo-inverse rows P = (J^T J)^-1 J^T into LDS; phase 2, one thread per
+// (element, q, Q2 basis): the physical gradient into LDS; phase 3, one thread
+// per (element, local row): the row of the element matrix from LDS (every
+// thread of an element reads the same gradients, LDS broadcasts them), the
+// elimination and the store.
+constexpr int kThN = 22, kThNV = 9, kThNP = 4;
+
+// LDS bytes per element of k_element_th: P (6) and JxW per point in FP64,
+// three gradient components per (point, Q2 basis) in T
+template <typename T>
+constexpr size_t thLdsPerElem(int nq)
+{
+  return (size_t)nq * 7 * sizeof(double) + (size_t)nq * kThNV * 3 * sizeof(T);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-    k_element_th(int64_t E, int nq, double const * __restrict__ xyz, uint32_t const * __restrict__ conn,
+    k_element_th(int64_t E, int G, int nq, double const * __restrict__ xyz, uint32_t const * __restrict__ conn,
                  double const * __restrict__ geoGrad, double const * __restrict__ w, T const * __restrict__ gradTab,
                  T const * __restrict__ phiTab, ThCoef cf, uint32_t const * __restrict__ emap,
                  uint8_t const * __restrict__ bcFlag, double const * __restrict__ bcVal, int hasBC,
                  T * __restrict__ ke, double * __restrict__ fe, int * __restrict__ status)
 {
-  constexpr int N = 22, NV = 9, NP = 4;
+  constexpr int N = kThN, NV = kThNV, NP = kThNP;
   extern __shared__ unsigned char smem_raw[];
-  double * sGeo = reinterpret_cast<double *>(smem_raw); // nq x 4 x 2
-  T * sGrad = reinterpret_cast<T *>(sGeo + nq * 8);     // nq x 22 x 2
-  T * sPhi = sGrad + nq * N * 2;                        // nq x 22
-  for (int k = threadIdx.x; k < nq * 8; k += blockDim.x)
-    sGeo[k] = geoGrad[k];
-  for (int k = threadIdx.x; k < nq * N * 2; k += blockDim.x)
-    sGrad[k] = gradTab[k];
-  for (int k = threadIdx.x; k < nq * N; k += blockDim.x)
+  double * sGeo = reinterpret_cast<double *>(smem_raw);  // [G][nq][7]: P row-major, JxW
+  T * sG = reinterpret_cast<T *>(sGeo + (size_t)G * nq * 7); // [G][nq][9][3]
+  T * sPhi = sG + (size_t)G * nq * NV * 3;                 // [nq][22]
+  int const t = threadIdx.x;
+  int64_t const e0 = (int64_t)blockIdx.x * G;
+  int const ne = (int)min<int64_t>(G, E - e0);
+  for (int k = t; k < nq * N; k += blockDim.x)
     sPhi[k] = phiTab[k];
-  __syncthreads();
-  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= E * N)
-    return;
-  int64_t const e = t / N;
-  int const i = (int)(t - e * N);
-  bool const vel = i < 2 * NV;
-  int const d = vel ? i / NV : 0;     // velocity component of the row
-  int const a = vel ? i - d * NV : 0; // its Q2 basis
-  int const k = vel ? 0 : i - 2 * NV; // pressure basis of a pressure row
-  double x[4][3];
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
+  // 1. geometry per (element, q)
+  for (int it = t; it < ne * nq; it += blockDim.x)
   {
-    uint32_t const pid = conn[e * 4 + v];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-      x[v][r] = xyz[3ull * pid + r];
-  }
-  T acc[N];
-#pragma unroll
-  for (int j = 0; j < N; ++j)
-    acc[j] = T(0);
-  for (int q = 0; q < nq; ++q)
-  {
+    int const el = it / nq, q = it - el * nq;
+    int64_t const e = e0 + el;
     double J[3][2] = {{0., 0.}, {0., 0.}, {0., 0.}};
 #pragma unroll
     for (int v = 0; v < 4; ++v)
     {
-      double const g0 = sGeo[(q * 4 + v) * 2], g1 = sGeo[(q * 4 + v) * 2 + 1];
+      uint32_t const pid = conn[e * 4 + v];
+      double const g0 = geoGrad[(q * 4 + v) * 2], g1 = geoGrad[(q * 4 + v) * 2 + 1];
 #pragma unroll
       for (int r = 0; r < 3; ++r)
       {
-        J[r][0] += x[v][r] * g0;
-        J[r][1] += x[v][r] * g1;
+        double const xr = xyz[3ull * pid + r];
+        J[r][0] += xr * g0;
+        J[r][1] += xr * g1;
       }
     }
     double const A00 = J[0][0] * J[0][0] + J[1][0] * J[1][0] + J[2][0] * J[2][0];
     double const A01 = J[0][0] * J[0][1] + J[1][0] * J[1][1] + J[2][0] * J[2][1];
     double const A11 = J[0][1] * J[0][1] + J[1][1] * J[1][1] + J[2][1] * J[2][1];
     double const det = A00 * A11 - A01 * A01;
-    if (!(det > 0.) && i == 0)
+    if (!(det > 0.))
       atomicOr(status, 1);
     double const I00 = A11 / det, I01 = -A01 / det, I11 = A00 / det;
-    // pseudo-inverse rows P[a][c] = sum_b I[a][b] J[c][b]
-    T P[2][3];
+    double * o = sGeo + (size_t)it * 7;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
     {
-      P[0][c] = (T)(I00 * J[c][0] + I01 * J[c][1]);
-      P[1][c] = (T)(I01 * J[c][0] + I11 * J[c][1]);
+      o[c] = I00 * J[c][0] + I01 * J[c][1];
+      o[3 + c] = I01 * J[c][0] + I11 * J[c][1];
     }
-    double const JxW = sqrt(det) * w[q];
-    T const * gq = sGrad + q * N * 2;
+    o[6] = sqrt(det) * w[q];
+  }
+  __syncthreads();
+  // 2. physical gradients dphi_b = gref_b P per (element, q, Q2 basis)
+  for (int it = t; it < ne * nq * NV; it += blockDim.x)
+  {
+    int const eq = it / NV, b = it - eq * NV, q = eq % nq;
+    double const * o = sGeo + (size_t)eq * 7;
+    T const g0 = gradTab[(q * N + b) * 2], g1 = gradTab[(q * N + b) * 2 + 1];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      sG[(size_t)it * 3 + c] = g0 * (T)o[c] + g1 * (T)o[3 + c];
+  }
+  __syncthreads();
+  // 3. one row per thread
+  if (t >= ne * N)
+    return;
+  int const el = t / N, i = t - el * N;
+  int64_t const e = e0 + el;
+  bool const vel = i < 2 * NV;
+  int const d = vel ? i / NV : 0;     // velocity component of the row
+  int const a = vel ? i - d * NV : 0; // its Q2 basis
+  int const k = vel ? 0 : i - 2 * NV; // pressure basis of a pressure row
+  T acc[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+    acc[j] = T(0);
+  for (int q = 0; q < nq; ++q)
+  {
+    double const JxW = sGeo[((size_t)el * nq + q) * 7 + 6];
+    T const * gq = sG + ((size_t)el * nq + q) * NV * 3;
     T const * pq = sPhi + q * N;
-    auto dphi = [&](int b, int c) { return gq[b * 2] * P[0][c] + gq[b * 2 + 1] * P[1][c]; };
     if (vel)
     {
-      T gi[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        gi[c] = dphi(i, c);
+      T const gi0 = gq[a * 3], gi1 = gq[a * 3 + 1], gi2 = gq[a * 3 + 2];
+      T const gid = d == 0 ? gi0 : gi1;
       T const s = (T)(cf.stiff * JxW), ts = (T)(cf.tensor * JxW), m = (T)(cf.mass * JxW);
-      T const gs = (T)(cf.grad * JxW) * gi[d];
+      T const gs = (T)(cf.grad * JxW) * gid;
       T const pa = pq[i];
 #pragma unroll
       for (int j = 0; j < NV; ++j)
       {
-        T gj[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          gj[c] = dphi(j, c); // (the x and y copies of basis j share one gradient)
-        T const dd = gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2];
-        T diag = s * dd + ts * dd + m * (pa * pq[j]);
+        T const gj0 = gq[j * 3], gj1 = gq[j * 3 + 1], gj2 = gq[j * 3 + 2];
+        T const dd = gi0 * gj0 + gi1 * gj1 + gi2 * gj2;
+        T const diag = s * dd + ts * dd + m * (pa * pq[j]);
+        // tensor cross term: (d, dj) block gets ts dphi_a[d] dphi_j[dj]
+        T const c0 = ts * (gid * gj0), c1 = ts * (gid * gj1);
         if (d == 0)
         {
-          acc[j] += diag + ts * (gi[0] * gj[0]);
-          acc[NV + j] += ts * (gi[0] * gj[1]);
+          acc[j] += diag + c0;
+          acc[NV + j] += c1;
         }
         else
         {
-          acc[j] += ts * (gi[1] * gj[0]);
-          acc[NV + j] += diag + ts * (gi[1] * gj[1]);
+          acc[j] += c0;
+          acc[NV + j] += diag + c1;
         }
       }
 #pragma unroll
@@ -409,8 +431,8 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
       for (int j = 0; j < NV; ++j)
       {
-        acc[j] += sp * dphi(j, 0);
-        acc[NV + j] += sp * dphi(j, 1);
+        acc[j] += sp * gq[j * 3];
+        acc[NV + j] += sp * gq[j * 3 + 1];
       }
     }
   }
@@ -442,7 +464,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int j = 0; j < N; ++j)
     out[j] = acc[j];
-  fe[t] = f;
+  fe[e * N + i] = f;
 }
 
 // --------------------------------------------------------------------------
@@ -1514,11 +1536,16 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                auto elementTh = [&](auto tag, auto * ke, auto const * gradTab, auto const * phiTab)
                {
                  using TT = decltype(tag);
-                 size_t const sh = (size_t)T.nq * 8 * sizeof(double) + (size_t)T.nq * N * 3 * sizeof(TT);
+                 // elements per block: all rows in one block, LDS within 64 KB
+                 size_t const per = thLdsPerElem<TT>(T.nq), tab = (size_t)T.nq * kThN * sizeof(TT);
+                 int const G = (int)std::max<size_t>(
+                     1, std::min<size_t>(kBlock / kThN, (65536 - tab) / per));
+                 size_t const sh = (size_t)G * per + tab;
                  ProfScope ps(c, sizeof(TT) == 8 ? "element" : "element32");
-                 k_element_th<TT><<<gridFor(E * N), kBlock, sh, s>>>(
-                     E, T.nq, c->xyz.p, c->conn.p, T.d_geoGrad.p, T.d_w.p, gradTab, phiTab, th, c->elemMap.p,
-                     c->bcFlag.p, c->bcVal.p, c->hasBC, ke, c->feBuf.p, c->status.p);
+                 unsigned const grid = (unsigned)((E + G - 1) / G);
+                 k_element_th<TT><<<grid, kBlock, sh, s>>>(E, G, T.nq, c->xyz.p, c->conn.p, T.d_geoGrad.p, T.d_w.p,
+                                                          gradTab, phiTab, th, c->elemMap.p, c->bcFlag.p,
+                                                          c->bcVal.p, c->hasBC, ke, c->feBuf.p, c->status.p);
                };
                if (precision == PXG_FP64)
                {
