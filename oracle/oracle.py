@@ -213,9 +213,10 @@ class Space:
         return rowptr, col, valv, b
 
     def stokes(self, interleaved, coefs, flag, val, mode=0, prune=1e-16):
-        """Stokes block system of the reference's test/assemblyvector.cpp and
-        test/stokes2dquad.cpp on this QUAD_Q2 space (velocity: 2 components,
-        BLOCK or INTERLEAVED dofs; pressure Q1 after them).  coefs: dict with
+        """Stokes block system of the reference's test/assemblyvector.cpp,
+        test/stokes2dquad.cpp, stokes3dhex.cpp and stokes3dtet.cpp on this
+        QUAD_Q2 / HEX_Q2 / TET_P2 space (velocity: dim components, BLOCK or
+        INTERLEAVED dofs; pressure Q1 / P1 after them).  coefs: dict with
         keys stiffness, tensor, mass (velocity block), grad, div (missing: 0).
         flag / val: Dirichlet data per dof of the velocity (2 * ndofs entries)
         or of the whole system; a constrained pressure dof acts as
@@ -226,7 +227,8 @@ class Space:
         nnz = C.c_int64()
         k = {"stiffness": 0.0, "tensor": 0.0, "mass": 0.0, "grad": 0.0, "div": 0.0}
         k.update(coefs)
-        f = np.zeros(max(len(flag), 2 * self.ndofs + self.nelem * 4), dtype=np.uint8)
+        dim = {"quad_q2": 2, "hex_q2": 3, "tet_p2": 3}[self.fe]
+        f = np.zeros(max(len(flag), dim * self.ndofs + self.nelem * 8), dtype=np.uint8)
         v = np.zeros(len(f))
         f[: len(flag)] = np.asarray(flag, dtype=np.uint8)
         v[: len(val)] = np.asarray(val, dtype=np.float64)

@@ -281,18 +281,20 @@ int orc_get_csr(void * p, int64_t * rowptr, uint32_t * col, double * val, double
   return 0;
 }
 
-// Stokes block system of the reference's test/assemblyvector.cpp and
-// test/stokes2dquad.cpp on the quad mesh of a QUAD_Q2 context: velocity Q2
-// with 2 components (BLOCK or INTERLEAVED dofs) at [0, 2 nU), pressure Q1 at
-// [2 nU, 2 nU + nP), both with the context's rule.  flag / val: Dirichlet
-// data per dof of the whole system (global numbering).  mode 0: vector spaces,
+// Stokes block system of the reference's test/assemblyvector.cpp,
+// test/stokes2dquad.cpp, test/stokes3dhex.cpp and test/stokes3dtet.cpp on the
+// mesh of a QUAD_Q2, HEX_Q2 or TET_P2 context: velocity with dim components
+// (BLOCK or INTERLEAVED dofs) at [0, dim nU), pressure in the linear space of
+// the same geometry (Q1 / P1) at [dim nU, dim nU + nP), both with the
+// context's rule.  flag / val: Dirichlet data per dof of the whole system
+// (global numbering).  mode 0: vector spaces,
 //   buildLhs({AssemblyStiffness{cS, vel}, AssemblyTensorStiffness{cT, vel},
 //             AssemblyScalarMass{cM, vel}}, bcsVel)
 //   [+ buildLhs(AssemblyDummy{p}, bcsP) when some pressure dof is constrained]
 //   + buildCoupling(AssemblyGrad{cG, vel, p}, bcsVel, bcsP)
 //   + buildCoupling(AssemblyDiv{cD, p, vel}, bcsP, bcsVel);
-// mode 1: the scalar component spaces U, V of assemblyvector.cpp, comp lists
-// {0}, {1} (no tensor term: it couples the components).  Terms with a zero
+// mode 1: the scalar component spaces of assemblyvector.cpp, comp lists {d}
+// (no tensor term: it couples the components).  Terms with a zero
 // coefficient are left out.  The closed matrix and b replace the context's
 // (orc_get_csr).
 int orc_stokes(void * p, int interleaved, double cS, double cT, double cM, double cG, double cD,
@@ -304,16 +306,23 @@ int orc_stokes(void * p, int interleaved, double cS, double cT, double cM, doubl
       c,
       [&]
       {
-        if (c->feType != FE_QUAD_Q2)
-          throw std::runtime_error("orc_stokes needs a QUAD_Q2 context");
+        int pType = 0;
+        switch (c->feType)
+        {
+        case FE_QUAD_Q2: pType = FE_QUAD_Q1; break;
+        case FE_HEX_Q2: pType = FE_HEX_Q1; break;
+        case FE_TET_P2: pType = FE_TET_P1; break;
+        default: throw std::runtime_error("orc_stokes needs a QUAD_Q2, HEX_Q2 or TET_P2 context");
+        }
         if (mode != 0 && cT != 0.)
           throw std::runtime_error("the tensor stiffness has no component-space form");
-        RefFE const & fv = refFE(FE_QUAD_Q2);
-        RefFE const & fp = refFE(FE_QUAD_Q1);
+        RefFE const & fv = refFE(c->feType);
+        RefFE const & fp = refFE(pType);
+        int const dim = fv.dim;
         CurFE cv(fv, c->qr), cp(fp, c->qr);
         DofMap const & sv = c->dofs;
         DofMap dp = buildDofMap(c->mesh, fp);
-        u32 const nU = sv.ndofs, nP = dp.ndofs, off = 2 * nU;
+        u32 const nU = sv.ndofs, nP = dp.ndofs, off = dim * nU;
         for (auto & id: dp.elemMap)
           id += off;
         dp.ndofs = nP;
@@ -324,38 +333,40 @@ int orc_stokes(void * p, int interleaved, double cS, double cT, double cM, doubl
         for (i64 k = off; k < nd; ++k)
           pinned = pinned || flag[k];
         std::vector<Term> dummy; // AssemblyDummy: no term, Ke = 0
+        std::vector<int> all;
+        for (int d = 0; d < dim; ++d)
+          all.push_back(d);
         c->builder.init(nd);
         if (mode == 0)
         {
-          DofMap vel = vectorDofMap(sv, 2, interleaved != 0, 0);
-          c->builder.buildLhsVector(c->mesh, cv, vel, 2, cS, cT, cM, flag, val);
+          DofMap vel = vectorDofMap(sv, dim, interleaved != 0, 0);
+          c->builder.buildLhsVector(c->mesh, cv, vel, dim, cS, cT, cM, flag, val);
           if (pinned)
             c->builder.buildLhs(c->mesh, cp, dp, dummy, flag, val);
-          c->builder.buildCoupling(c->mesh, cv, cp, vel, 2, dp, 1, COUPLING_GRAD, cG, {0, 1}, flag, flag, val);
-          c->builder.buildCoupling(c->mesh, cp, cv, dp, 1, vel, 2, COUPLING_DIV, cD, {0, 1}, flag, flag, val);
+          c->builder.buildCoupling(c->mesh, cv, cp, vel, dim, dp, 1, COUPLING_GRAD, cG, all, flag, flag, val);
+          c->builder.buildCoupling(c->mesh, cp, cv, dp, 1, vel, dim, COUPLING_DIV, cD, all, flag, flag, val);
         }
         else
         {
-          DofMap u = sv, v = sv;
-          for (size_t k = 0; k < u.elemMap.size(); ++k)
-          {
-            u32 const id = sv.elemMap[k];
-            u.elemMap[k] = interleaved ? 2 * id : id;
-            v.elemMap[k] = interleaved ? 2 * id + 1 : id + nU;
-          }
           std::vector<Term> lhs;
           if (cS != 0.)
             lhs.push_back(Term{TERM_STIFFNESS, cS, nullptr});
           if (cM != 0.)
             lhs.push_back(Term{TERM_MASS, cM, nullptr});
-          c->builder.buildLhs(c->mesh, cv, u, lhs, flag, val);
-          if (pinned)
-            c->builder.buildLhs(c->mesh, cp, dp, dummy, flag, val);
-          c->builder.buildCoupling(c->mesh, cv, cp, u, 1, dp, 1, COUPLING_GRAD, cG, {0}, flag, flag, val);
-          c->builder.buildCoupling(c->mesh, cp, cv, dp, 1, u, 1, COUPLING_DIV, cD, {0}, flag, flag, val);
-          c->builder.buildLhs(c->mesh, cv, v, lhs, flag, val);
-          c->builder.buildCoupling(c->mesh, cv, cp, v, 1, dp, 1, COUPLING_GRAD, cG, {1}, flag, flag, val);
-          c->builder.buildCoupling(c->mesh, cp, cv, dp, 1, v, 1, COUPLING_DIV, cD, {1}, flag, flag, val);
+          for (int d = 0; d < dim; ++d)
+          {
+            DofMap u = sv;
+            for (size_t k = 0; k < u.elemMap.size(); ++k)
+            {
+              u32 const id = sv.elemMap[k];
+              u.elemMap[k] = interleaved ? dim * id + d : id + d * nU;
+            }
+            c->builder.buildLhs(c->mesh, cv, u, lhs, flag, val);
+            if (pinned && d == 0)
+              c->builder.buildLhs(c->mesh, cp, dp, dummy, flag, val);
+            c->builder.buildCoupling(c->mesh, cv, cp, u, 1, dp, 1, COUPLING_GRAD, cG, {d}, flag, flag, val);
+            c->builder.buildCoupling(c->mesh, cp, cv, dp, 1, u, 1, COUPLING_DIV, cD, {d}, flag, flag, val);
+          }
         }
         c->A = c->builder.close(pruneEps);
         c->builder.trow.clear();

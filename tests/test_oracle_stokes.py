@@ -172,35 +172,41 @@ def lu_solve(A, b):
     return x + lu.solve(b - A @ x)
 
 
-def stokes2dquad_known_answer(space, pspace, solve_system, n=(10, 10)):
-    """The reference's stokes2dquad known answer: the Q2-Q1 solution equals the
-    interpolant of u = (0, (1 - x^2) / 2), p = nu (1 - y) (errors below 1e-14,
-    2e-14, 3e-14 in the 2-norm of the nodal vectors, stokes2dquad.cpp:142-152).
-    solve_system(flag, val) returns the solution vector."""
-    nu = 0.1
+def stokes_known_answer(space, pspace, solve_system, dim, bcs, nu, ly=1.0):
+    """Known answer of the reference's Stokes tests: with the inlet
+    u = (0, (1 - x^2) / 2 [, 0]) and the velocity conditions `bcs` (side,
+    constrained components; the inlet on bottom, zero elsewhere), the Q2-Q1 /
+    P2-P1 solution is the interpolant of that u and of p = nu (ly - y).
+    solve_system(flag, val, nu) returns the solution (BLOCK ordering).
+    Returns the 2-norms of the nodal errors of every component and of p."""
     x = space.dof_coords()
     nU = len(x)
     xp = pspace.dof_coords()
-    nd = 2 * nU + len(xp)
+    nd = dim * nU + len(xp)
     flag = np.zeros(nd, np.uint8)
     val = np.zeros(nd)
-    for side, f, comps in (("bottom", lambda p: np.stack([0 * p[:, 0], 0.5 * (1 - p[:, 0] ** 2)], 1), (0, 1)),
-                           ("right", lambda p: np.zeros((len(p), 2)), (0, 1)),
-                           ("top", lambda p: np.zeros((len(p), 2)), (0,)),
-                           ("left", lambda p: np.zeros((len(p), 2)), (0,))):
+    exact = np.zeros((nU, dim))
+    exact[:, 1] = 0.5 * (1 - x[:, 0] ** 2)
+    for side, comps in bcs:
         ids = np.asarray(space.bc_dofs(side), np.int64)
-        v = f(x[ids])
         for d in comps:
             flag[ids + d * nU] = 1
-            val[ids + d * nU] = v[:, d]
+            val[ids + d * nU] = exact[ids, d] if side == "bottom" else 0.0
     sol = solve_system(flag, val, nu)
-    u_ex = np.zeros(nU)
-    v_ex = 0.5 * (1 - x[:, 0] ** 2)
-    p_ex = nu * (1 - xp[:, 1])
-    eu = np.linalg.norm(sol[:nU] - u_ex)
-    ev = np.linalg.norm(sol[nU:2 * nU] - v_ex)
-    ep = np.linalg.norm(sol[2 * nU:] - p_ex)
-    return eu, ev, ep
+    errs = [np.linalg.norm(sol[d * nU:(d + 1) * nU] - exact[:, d]) for d in range(dim)]
+    errs.append(np.linalg.norm(sol[dim * nU:] - nu * (ly - xp[:, 1])))
+    return errs
+
+
+S2D_SIDES = (("bottom", (0, 1)), ("right", (0, 1)), ("top", (0,)), ("left", (0,)))
+S3D_SIDES = (("bottom", (0, 1, 2)), ("right", (0, 1, 2)), ("top", (0, 2)), ("left", (0, 2)),
+             ("back", (2,)), ("front", (2,)))
+
+
+def stokes2dquad_known_answer(space, pspace, solve_system, n=(10, 10)):
+    """test/stokes2dquad.cpp: errors below 1e-14, 2e-14, 3e-14 for u, v, p
+    (stokes2dquad.cpp:142-152) on 10^2 quads, nu = 0.1."""
+    return tuple(stokes_known_answer(space, pspace, solve_system, 2, S2D_SIDES, 0.1))
 
 
 def test_stokes2dquad_known_answer():
@@ -215,3 +221,55 @@ def test_stokes2dquad_known_answer():
     eu, ev, ep = stokes2dquad_known_answer(o, op, solve, n)
     print(f"errors u {eu:.2e} v {ev:.2e} p {ep:.2e}")
     assert eu <= 1e-14 and ev <= 2e-14 and ep <= 3e-14
+
+
+@pytest.mark.parametrize("fe,mesh", [("tet_p2", "cube_tet5"), ("hex_q2", "cube_hex")])
+@pytest.mark.parametrize("interleaved", [False, True])
+def test_vector_spaces_match_component_spaces_3d(fe, mesh, interleaved):
+    """assemblyvector.cpp's equivalence on the 3D pairs (P2-P1 tets, Q2-Q1 hexes)."""
+    O.build()
+    n = (2, 2, 2)
+    o = O.Space(fe, mesh, n)
+    nU = o.ndofs
+    flag = np.zeros(3 * nU, np.uint8)
+    val = np.zeros(3 * nU)
+    x = o.dof_coords()
+    for side in ("bottom", "left", "front"):
+        ids = np.asarray(o.bc_dofs(side), np.int64)
+        for d in range(3):
+            g = 3 * ids + d if interleaved else ids + d * nU
+            flag[g] = 1
+            val[g] = x[ids, d] + 0.5 * d
+    k = {"stiffness": 0.7, "mass": 1.5, "grad": -1.0, "div": -2.0}
+    rv, cv, vv, bv = o.stokes(interleaved, k, flag, val, mode=0)
+    rs, cs, vs, bs = o.stokes(interleaved, k, flag, val, mode=1)
+    assert np.array_equal(rv, rs) and np.array_equal(cv, cs)
+    assert np.array_equal(vv, vs)
+    assert np.abs(bv - bs).max() <= 1e-15 * max(1.0, np.abs(bv).max())
+
+
+# test/stokes3dtet.cpp:122-127 and test/stokes3dhex.cpp:126-131: checkError
+# (include/def.hpp:357-367) asks |error - ref| < 1e-12 for every norm
+STOKES3D_REF = {
+    "tet_p2": (1.191610761242353e-15, 4.347071660478249e-15, 1.168951507916659e-15, 5.36156027085171e-14),
+    "hex_q2": (1.309442419567852e-15, 1.021847252919293e-14, 1.800953966048908e-15, 1.097009563518918e-14),
+}
+STOKES3D_MESH = {"tet_p2": ("cube_tet5", "tet_p1"), "hex_q2": ("cube_hex", "hex_q1")}
+
+
+@pytest.mark.parametrize("fe", ["tet_p2", "hex_q2"])
+def test_stokes3d_known_answer(fe):
+    O.build()
+    n = (4, 4, 4)
+    mesh, pfe = STOKES3D_MESH[fe]
+    ln = (1.0, 2.0, 1.0)
+    o = O.Space(fe, mesh, n, length=ln)
+    op = O.Space(pfe, mesh, n, length=ln)
+
+    def solve(flag, val, nu):
+        rp, col, v, b = o.stokes(False, {"tensor": nu, "grad": -1.0, "div": -1.0}, flag, val)
+        return lu_solve(_csr(rp, col, v), b)
+    errs = stokes_known_answer(o, op, solve, 3, S3D_SIDES, 0.1, ly=2.0)
+    print(fe, ["%.3e" % e for e in errs])
+    for e, r in zip(errs, STOKES3D_REF[fe]):
+        assert abs(e - r) < 1e-12
