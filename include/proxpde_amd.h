@@ -45,18 +45,20 @@ extern "C" {
 #define PXG_FE_HEX_Q2 5  /* RefHexahedronQ2,  GaussQR<Hexahedron,27>   */
 #define PXG_FE_TET_P1 6  /* RefTetrahedronP1, GaussQR<Tetrahedron,4>   */
 #define PXG_FE_TET_P2 7  /* RefTetrahedronP2, GaussQR<Tetrahedron,11>  */
-/* Taylor-Hood pair on quads as one element of 22 local dofs: the velocity
- * FESpace<Mesh<Quad>, RefQuadQ2, GaussQR<Quad,9>, 2> (local i + 9 d for
- * component d, include/fespace.hpp) followed by the pressure
- * FESpace<Mesh<Quad>, RefQuadQ1, GaussQR<Quad,9>> (local 18 + k), as used
- * together by test/assemblyvector.cpp and test/stokes2dquad.cpp.  The dof
- * map row of an element holds the 18 velocity ids (BLOCK: id + d nU,
- * INTERLEAVED: 2 id + d, include/dof.hpp:139-166) and the 4 pressure ids
- * (after the velocity, the FESpace offset).  Terms: STIFFNESS,
+/* Taylor-Hood pairs as one element: the velocity FESpace<Mesh, Q2 / P2,
+ * RecommendedQR, dim> (local i + NV d for component d, include/fespace.hpp)
+ * followed by the pressure FESpace<Mesh, Q1 / P1, same rule> (local
+ * dim NV + k), as used together by test/assemblyvector.cpp,
+ * test/stokes2dquad.cpp, test/stokes3dhex.cpp and test/stokes3dtet.cpp.
+ * The dof map row of an element holds the dim NV velocity ids (BLOCK:
+ * id + d nU, INTERLEAVED: dim id + d, include/dof.hpp:139-166) and the
+ * pressure ids (after the velocity, the FESpace offset).  Terms: STIFFNESS,
  * TENSOR_STIFFNESS, MASS on the velocity block, GRAD and DIV couplings.
  * A constrained pressure dof acts as buildLhs(AssemblyDummy{p}, bcsP) plus
  * bcsP in both couplings.                                                  */
-#define PXG_FE_QUAD_Q2Q1 8
+#define PXG_FE_QUAD_Q2Q1 8  /* 2 x 9 + 4 = 22 local dofs, GaussQR<Quad,9>         */
+#define PXG_FE_HEX_Q2Q1 9   /* 3 x 27 + 8 = 89 local dofs, GaussQR<Hexahedron,27> */
+#define PXG_FE_TET_P2P1 10  /* 3 x 10 + 4 = 34 local dofs, GaussQR<Tetrahedron,11> */
 
 /* ---- assembly terms (reference include/assembly_lhs.hpp) --------------- */
 #define PXG_TERM_STIFFNESS 1    /* AssemblyStiffness(coef)          :36-75   */

@@ -295,24 +295,43 @@ struct ThCoef
   double stiff, tensor, mass, grad, div;
 };
 
-// Work split: a block takes G elements (G <= kBlock / 22, from the LDS
-// budget).  Phase 1, one thread per (element, q): J, JxW and the
-// pseudo-inverse rows P = (J^T J)^-1 J^T into LDS; phase 2, one thread per
-// (element, q, Q2 basis): the physical gradient into LDS; phase 3, one thread
-// per (element, local row): the row of the element matrix from LDS (every
-// thread of an element reads the same gradients, LDS broadcasts them), the
-// elimination and the store.
-constexpr int kThN = 22, kThNV = 9, kThNP = 4;
-
-// LDS bytes per element of k_element_th: P (6) and JxW per point in FP64,
-// three gradient components per (point, Q2 basis) in T
-template <typename T>
-constexpr size_t thLdsPerElem(int nq)
+// Shape of a Taylor-Hood pair: DIM velocity components of NV basis
+// functions, NP pressure basis functions, NPE mapping vertices.
+template <int DIM_, int NV_, int NP_, int NPE_>
+struct ThShape
 {
-  return (size_t)nq * 7 * sizeof(double) + (size_t)nq * kThNV * 3 * sizeof(T);
+  static constexpr int DIM = DIM_, NV = NV_, NP = NP_, NPE = NPE_;
+  static constexpr int N = DIM * NV + NP;
+  static constexpr int GEO = 3 * DIM + 1; // P (DIM x 3) and JxW per point
+};
+using ThQuad = ThShape<2, 9, 4, 4>;  // Q2-Q1 quads, 22 local dofs
+using ThTet = ThShape<3, 10, 4, 4>;  // P2-P1 tets, 34
+using ThHex = ThShape<3, 27, 8, 8>;  // Q2-Q1 hexes, 89
+// the pair whose element has N local dofs (void: N is a scalar element)
+template <int N>
+using ThOf = std::conditional_t<N == ThQuad::N, ThQuad,
+                                std::conditional_t<N == ThTet::N, ThTet, std::conditional_t<N == ThHex::N, ThHex, void>>>;
+bool isThPair(int feType)
+{
+  return feType == PXG_FE_QUAD_Q2Q1 || feType == PXG_FE_HEX_Q2Q1 || feType == PXG_FE_TET_P2P1;
 }
 
-template <typename T>
+// LDS bytes per element of k_element_th: P and JxW per point in FP64, three
+// gradient components per (point, velocity basis) in T
+template <typename T, typename S>
+constexpr size_t thLdsPerElem(int nq)
+{
+  return (size_t)nq * S::GEO * sizeof(double) + (size_t)nq * S::NV * 3 * sizeof(T);
+}
+
+// Work split: a block takes G elements (G <= kBlock / N, from the LDS
+// budget).  Phase 1, one thread per (element, q): J, JxW and the
+// pseudo-inverse rows P = (J^T J)^-1 J^T into LDS; phase 2, one thread per
+// (element, q, velocity basis): the physical gradient into LDS; phase 3, one
+// thread per (element, local row): the row of the element matrix from LDS
+// (every thread of an element reads the same gradients, LDS broadcasts them),
+// the elimination and the store.
+template <typename T, typename S>
 __global__ void __launch_bounds__(kBlock)
     k_element_th(int64_t E, int G, int nq, double const * __restrict__ xyz, uint32_t const * __restrict__ conn,
                  double const * __restrict__ geoGrad, double const * __restrict__ w, T const * __restrict__ gradTab,
@@ -320,11 +339,11 @@ __global__ void __launch_bounds__(kBlock)
                  uint8_t const * __restrict__ bcFlag, double const * __restrict__ bcVal, int hasBC,
                  T * __restrict__ ke, double * __restrict__ fe, int * __restrict__ status)
 {
-  constexpr int N = kThN, NV = kThNV, NP = kThNP;
+  constexpr int DIM = S::DIM, NV = S::NV, NP = S::NP, NPE = S::NPE, N = S::N, GEO = S::GEO;
   extern __shared__ unsigned char smem_raw[];
-  double * sGeo = reinterpret_cast<double *>(smem_raw);  // [G][nq][7]: P row-major, JxW
-  T * sG = reinterpret_cast<T *>(sGeo + (size_t)G * nq * 7); // [G][nq][9][3]
-  T * sPhi = sG + (size_t)G * nq * NV * 3;                 // [nq][22]
+  double * sGeo = reinterpret_cast<double *>(smem_raw);   // [G][nq][GEO]: P row-major, JxW
+  T * sG = reinterpret_cast<T *>(sGeo + (size_t)G * nq * GEO); // [G][nq][NV][3]
+  T * sPhi = sG + (size_t)G * nq * NV * 3;                   // [nq][N]
   int const t = threadIdx.x;
   int64_t const e0 = (int64_t)blockIdx.x * G;
   int const ne = (int)min<int64_t>(G, E - e0);
@@ -335,46 +354,91 @@ __global__ void __launch_bounds__(kBlock)
   {
     int const el = it / nq, q = it - el * nq;
     int64_t const e = e0 + el;
-    double J[3][2] = {{0., 0.}, {0., 0.}, {0., 0.}};
+    double J[3][DIM];
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < DIM; ++c)
+        J[r][c] = 0.;
+#pragma unroll
+    for (int v = 0; v < NPE; ++v)
     {
-      uint32_t const pid = conn[e * 4 + v];
-      double const g0 = geoGrad[(q * 4 + v) * 2], g1 = geoGrad[(q * 4 + v) * 2 + 1];
+      uint32_t const pid = conn[e * NPE + v];
+      double gv[DIM];
+#pragma unroll
+      for (int c = 0; c < DIM; ++c)
+        gv[c] = geoGrad[(q * NPE + v) * DIM + c];
 #pragma unroll
       for (int r = 0; r < 3; ++r)
       {
         double const xr = xyz[3ull * pid + r];
-        J[r][0] += xr * g0;
-        J[r][1] += xr * g1;
+#pragma unroll
+        for (int c = 0; c < DIM; ++c)
+          J[r][c] += xr * gv[c];
       }
     }
-    double const A00 = J[0][0] * J[0][0] + J[1][0] * J[1][0] + J[2][0] * J[2][0];
-    double const A01 = J[0][0] * J[0][1] + J[1][0] * J[1][1] + J[2][0] * J[2][1];
-    double const A11 = J[0][1] * J[0][1] + J[1][1] * J[1][1] + J[2][1] * J[2][1];
-    double const det = A00 * A11 - A01 * A01;
+    double A[DIM][DIM];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a)
+#pragma unroll
+      for (int b = 0; b < DIM; ++b)
+        A[a][b] = J[0][a] * J[0][b] + J[1][a] * J[1][b] + J[2][a] * J[2][b];
+    double I[DIM][DIM], det;
+    if constexpr (DIM == 2)
+    {
+      det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+      I[0][0] = A[1][1] / det;
+      I[0][1] = I[1][0] = -A[0][1] / det;
+      I[1][1] = A[0][0] / det;
+    }
+    else
+    {
+      double const c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+      double const c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+      double const c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+      det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+      I[0][0] = c00 / det;
+      I[0][1] = I[1][0] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+      I[0][2] = I[2][0] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+      I[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+      I[1][2] = I[2][1] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;
+      I[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
+    }
     if (!(det > 0.))
       atomicOr(status, 1);
-    double const I00 = A11 / det, I01 = -A01 / det, I11 = A00 / det;
-    double * o = sGeo + (size_t)it * 7;
+    double * o = sGeo + (size_t)it * GEO;
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-    {
-      o[c] = I00 * J[c][0] + I01 * J[c][1];
-      o[3 + c] = I01 * J[c][0] + I11 * J[c][1];
-    }
-    o[6] = sqrt(det) * w[q];
+    for (int a = 0; a < DIM; ++a)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+      {
+        double v = 0.;
+#pragma unroll
+        for (int b = 0; b < DIM; ++b)
+          v += I[a][b] * J[c][b];
+        o[a * 3 + c] = v;
+      }
+    o[3 * DIM] = sqrt(det) * w[q];
   }
   __syncthreads();
-  // 2. physical gradients dphi_b = gref_b P per (element, q, Q2 basis)
+  // 2. physical gradients dphi_b = gref_b P per (element, q, velocity basis)
   for (int it = t; it < ne * nq * NV; it += blockDim.x)
   {
     int const eq = it / NV, b = it - eq * NV, q = eq % nq;
-    double const * o = sGeo + (size_t)eq * 7;
-    T const g0 = gradTab[(q * N + b) * 2], g1 = gradTab[(q * N + b) * 2 + 1];
+    double const * o = sGeo + (size_t)eq * GEO;
+    T g[DIM];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a)
+      g[a] = gradTab[(q * N + b) * DIM + a];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-      sG[(size_t)it * 3 + c] = g0 * (T)o[c] + g1 * (T)o[3 + c];
+    {
+      T v = g[0] * (T)o[c];
+#pragma unroll
+      for (int a = 1; a < DIM; ++a)
+        v += g[a] * (T)o[a * 3 + c];
+      sG[(size_t)it * 3 + c] = v;
+    }
   }
   __syncthreads();
   // 3. one row per thread
@@ -382,58 +446,52 @@ __global__ void __launch_bounds__(kBlock)
     return;
   int const el = t / N, i = t - el * N;
   int64_t const e = e0 + el;
-  bool const vel = i < 2 * NV;
+  bool const vel = i < DIM * NV;
   int const d = vel ? i / NV : 0;     // velocity component of the row
-  int const a = vel ? i - d * NV : 0; // its Q2 basis
-  int const k = vel ? 0 : i - 2 * NV; // pressure basis of a pressure row
+  int const a = vel ? i - d * NV : 0; // its velocity basis
+  int const k = vel ? 0 : i - DIM * NV; // pressure basis of a pressure row
   T acc[N];
 #pragma unroll
   for (int j = 0; j < N; ++j)
     acc[j] = T(0);
   for (int q = 0; q < nq; ++q)
   {
-    double const JxW = sGeo[((size_t)el * nq + q) * 7 + 6];
+    double const JxW = sGeo[((size_t)el * nq + q) * GEO + 3 * DIM];
     T const * gq = sG + ((size_t)el * nq + q) * NV * 3;
     T const * pq = sPhi + q * N;
     if (vel)
     {
       T const gi0 = gq[a * 3], gi1 = gq[a * 3 + 1], gi2 = gq[a * 3 + 2];
-      T const gid = d == 0 ? gi0 : gi1;
+      T const gid = d == 0 ? gi0 : (d == 1 ? gi1 : gi2);
       T const s = (T)(cf.stiff * JxW), ts = (T)(cf.tensor * JxW), m = (T)(cf.mass * JxW);
       T const gs = (T)(cf.grad * JxW) * gid;
       T const pa = pq[i];
 #pragma unroll
       for (int j = 0; j < NV; ++j)
       {
-        T const gj0 = gq[j * 3], gj1 = gq[j * 3 + 1], gj2 = gq[j * 3 + 2];
-        T const dd = gi0 * gj0 + gi1 * gj1 + gi2 * gj2;
+        T const gj[3] = {gq[j * 3], gq[j * 3 + 1], gq[j * 3 + 2]};
+        T const dd = gi0 * gj[0] + gi1 * gj[1] + gi2 * gj[2];
         T const diag = s * dd + ts * dd + m * (pa * pq[j]);
-        // tensor cross term: (d, dj) block gets ts dphi_a[d] dphi_j[dj]
-        T const c0 = ts * (gid * gj0), c1 = ts * (gid * gj1);
-        if (d == 0)
+        // tensor cross term: block (d, dj) gets ts dphi_a[d] dphi_j[dj]
+#pragma unroll
+        for (int dj = 0; dj < DIM; ++dj)
         {
-          acc[j] += diag + c0;
-          acc[NV + j] += c1;
-        }
-        else
-        {
-          acc[j] += c0;
-          acc[NV + j] += diag + c1;
+          T const cr = ts * (gid * gj[dj]);
+          acc[dj * NV + j] += dj == d ? diag + cr : cr;
         }
       }
 #pragma unroll
       for (int kk = 0; kk < NP; ++kk)
-        acc[2 * NV + kk] += gs * pq[2 * NV + kk];
+        acc[DIM * NV + kk] += gs * pq[DIM * NV + kk];
     }
     else
     {
-      T const sp = (T)(cf.div * JxW) * pq[2 * NV + k];
+      T const sp = (T)(cf.div * JxW) * pq[DIM * NV + k];
 #pragma unroll
       for (int j = 0; j < NV; ++j)
-      {
-        acc[j] += sp * gq[j * 3];
-        acc[NV + j] += sp * gq[j * 3 + 1];
-      }
+#pragma unroll
+        for (int dj = 0; dj < DIM; ++dj)
+          acc[dj * NV + j] += sp * gq[j * 3 + dj];
     }
   }
   double f = 0.;
@@ -691,6 +749,17 @@ int stagePairs()
     return (x == 32 || x == 128) ? x : 64;
   }();
   return v;
+}
+
+// pairs per stage block for element rows of n entries: the default (or
+// PXG_STAGE_PAIRS) halved while a block's LDS (n values, an info word and a
+// slot per staged entry, FP64) would pass 48 KB
+int stagePairsFor(int n)
+{
+  int P = stagePairs();
+  while (P > 32 && (size_t)P * n * (sizeof(double) + 4) > 49152)
+    P /= 2;
+  return P;
 }
 
 template <typename T>
@@ -1007,21 +1076,27 @@ void launchRebuild(pxg_ctx * c, T const * ke, T * valC, T * valS, hipStream_t s)
     auto go = [&](auto pc)
     {
       constexpr int P = decltype(pc)::value;
-      T * dinv;
-      OtherT<T> * dinvO;
-      if constexpr (sizeof(T) == 8)
-      {
-        dinv = c->dinv.p;
-        dinvO = c->dinv32.p;
-      }
+      // (stage maps are built with stagePairsFor(N), whose blocks fit 48 KB)
+      if constexpr ((size_t)P * N * (sizeof(T) + 4) > 65536)
+        throw Error(PXG_E_INVALID, "stage block too large for the element size");
       else
       {
-        dinv = c->dinv32.p;
-        dinvO = c->dinv.p;
+        T * dinv;
+        OtherT<T> * dinvO;
+        if constexpr (sizeof(T) == 8)
+        {
+          dinv = c->dinv.p;
+          dinvO = c->dinv32.p;
+        }
+        else
+        {
+          dinv = c->dinv32.p;
+          dinvO = c->dinv.p;
+        }
+        k_scatter_staged<T, N, keRowStride(N), P><<<(unsigned)c->nStage, kBlock, 0, s>>>(
+            reinterpret_cast<StageBlock const *>(c->stageBlk.p), c->dofContrib.p, ke, c->stageInfo.p,
+            c->stageSid.p, valC, valS, c->changed.p, c->dofSeg.p, c->feBuf.p, c->b.p, dinv, dinvO);
       }
-      k_scatter_staged<T, N, keRowStride(N), P><<<(unsigned)c->nStage, kBlock, 0, s>>>(
-          reinterpret_cast<StageBlock const *>(c->stageBlk.p), c->dofContrib.p, ke, c->stageInfo.p, c->stageSid.p,
-          valC, valS, c->changed.p, c->dofSeg.p, c->feBuf.p, c->b.p, dinv, dinvO);
     };
     if (c->stagePairs == 32)
       go(std::integral_constant<int, 32>{});
@@ -1319,7 +1394,9 @@ void dispatchFe(int feType, F && f)
   case PXG_FE_HEX_Q2: f(std::integral_constant<int, 27>{}, std::integral_constant<int, 3>{}); break;
   case PXG_FE_TET_P1: f(std::integral_constant<int, 4>{}, std::integral_constant<int, 3>{}); break;
   case PXG_FE_TET_P2: f(std::integral_constant<int, 10>{}, std::integral_constant<int, 3>{}); break;
-  case PXG_FE_QUAD_Q2Q1: f(std::integral_constant<int, 22>{}, std::integral_constant<int, 2>{}); break;
+  case PXG_FE_QUAD_Q2Q1: f(std::integral_constant<int, ThQuad::N>{}, std::integral_constant<int, 2>{}); break;
+  case PXG_FE_HEX_Q2Q1: f(std::integral_constant<int, ThHex::N>{}, std::integral_constant<int, 3>{}); break;
+  case PXG_FE_TET_P2P1: f(std::integral_constant<int, ThTet::N>{}, std::integral_constant<int, 3>{}); break;
   default: throw Error(PXG_E_INVALID, "unknown FE type");
   }
 }
@@ -1447,7 +1524,7 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
   PXG_REQUIRE(c->fe.feType && c->nelem && c->ndofs && c->nnzS, PXG_E_STATE,
               "assemble needs set_fe, set_mesh, set_dofmap and build_pattern first");
   PXG_REQUIRE(precision == PXG_FP64 || precision == PXG_FP32, PXG_E_INVALID, "precision");
-  bool const mixed = c->fe.feType == PXG_FE_QUAD_Q2Q1;
+  bool const mixed = isThPair(c->fe.feType);
   int flags = 0;
   double cS = 0., cSF = 0., cM = 0., cMF = 0.;
   ThCoef th{0., 0., 0., 0., 0.};
@@ -1536,23 +1613,27 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                auto elementTh = [&](auto tag, auto * ke, auto const * gradTab, auto const * phiTab)
                {
                  using TT = decltype(tag);
-                 // elements per block: all rows in one block, LDS within 64 KB
-                 size_t const per = thLdsPerElem<TT>(T.nq), tab = (size_t)T.nq * kThN * sizeof(TT);
-                 int const G = (int)std::max<size_t>(
-                     1, std::min<size_t>(kBlock / kThN, (65536 - tab) / per));
+                 using Sh = ThOf<N>;
+                 // elements per block: all rows of an element in one block,
+                 // LDS within 64 KB
+                 size_t const per = thLdsPerElem<TT, Sh>(T.nq), tab = (size_t)T.nq * N * sizeof(TT);
+                 PXG_REQUIRE(tab + per <= 65536, PXG_E_INVALID,
+                             "quadrature too large for the Taylor-Hood element kernel");
+                 int const G = (int)std::min<size_t>(kBlock / N, (65536 - tab) / per);
                  size_t const sh = (size_t)G * per + tab;
                  ProfScope ps(c, sizeof(TT) == 8 ? "element" : "element32");
                  unsigned const grid = (unsigned)((E + G - 1) / G);
-                 k_element_th<TT><<<grid, kBlock, sh, s>>>(E, G, T.nq, c->xyz.p, c->conn.p, T.d_geoGrad.p, T.d_w.p,
-                                                          gradTab, phiTab, th, c->elemMap.p, c->bcFlag.p,
-                                                          c->bcVal.p, c->hasBC, ke, c->feBuf.p, c->status.p);
+                 k_element_th<TT, Sh><<<grid, kBlock, sh, s>>>(E, G, T.nq, c->xyz.p, c->conn.p, T.d_geoGrad.p,
+                                                              T.d_w.p, gradTab, phiTab, th, c->elemMap.p,
+                                                              c->bcFlag.p, c->bcVal.p, c->hasBC, ke, c->feBuf.p,
+                                                              c->status.p);
                };
                if (precision == PXG_FP64)
                {
                  if (!fused)
                  {
                    c->keBuf.resize(E * N * keRowStride(N));
-                   if constexpr (N == 22)
+                   if constexpr (!std::is_void_v<ThOf<N>>)
                      elementTh(double{}, c->keBuf.p, T.d_grad.p, T.d_phi.p);
                    else
                    {
@@ -1576,7 +1657,7 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                  if (!fused)
                  {
                    c->keBuf32.resize(E * N * keRowStride(N));
-                   if constexpr (N == 22)
+                   if constexpr (!std::is_void_v<ThOf<N>>)
                      elementTh(float{}, c->keBuf32.p, T.d_grad32.p, T.d_phi32.p);
                    else
                    {
@@ -1623,7 +1704,7 @@ void assembleRhs(pxg_ctx * c, int kind, double const * params, double const * qv
   PXG_REQUIRE(c->fe.feType && c->nelem && c->ndofs && c->nnzS, PXG_E_STATE,
               "assemble_rhs needs set_fe, set_mesh, set_dofmap and build_pattern first");
   PXG_REQUIRE(kind >= PXG_RHS_CONST && kind <= PXG_RHS_PROJECTION, PXG_E_INVALID, "rhs kind");
-  PXG_REQUIRE(c->fe.feType != PXG_FE_QUAD_Q2Q1, PXG_E_INVALID,
+  PXG_REQUIRE(!isThPair(c->fe.feType), PXG_E_INVALID,
               "right-hand sides of the Taylor-Hood pair are assembled per component space");
   hipStream_t s = c->stream;
   FeTables & T = c->fe;
@@ -1847,7 +1928,7 @@ void buildStageMaps(pxg_ctx * c)
   int const n = c->fe.n;
   int64_t const nr = c->ndofs, npairs = c->nelem * n;
   c->stageValid = false;
-  int const P = stagePairs();
+  int const P = stagePairsFor(n);
   if ((int64_t)P * n > 0x3fff || nr >= ((int64_t)1 << 31) || npairs <= 0)
     return; // contribution offsets of a block must fit 14 bits
   DevBuf<int> mp;

@@ -42,8 +42,24 @@ int guarded(pxg_ctx * c, F && f)
   }
 }
 
-// local basis b of the FE of a context: the FE itself, or for the Taylor-Hood
-// pair (22 local dofs) Q2 for both velocity components and Q1 for the pressure
+// Taylor-Hood pairs: velocity FE, pressure FE, velocity components
+struct ThPair
+{
+  int vel, pre, dim;
+};
+bool thPair(int feType, ThPair & p)
+{
+  switch (feType)
+  {
+  case PXG_FE_QUAD_Q2Q1: p = {PXG_FE_QUAD_Q2, PXG_FE_QUAD_Q1, 2}; return true;
+  case PXG_FE_HEX_Q2Q1: p = {PXG_FE_HEX_Q2, PXG_FE_HEX_Q1, 3}; return true;
+  case PXG_FE_TET_P2P1: p = {PXG_FE_TET_P2, PXG_FE_TET_P1, 3}; return true;
+  default: return false;
+  }
+}
+
+// local basis b of the FE of a context: the FE itself, or for a Taylor-Hood
+// pair the velocity basis b % NV of every component, then the pressure basis
 struct LocalBasis
 {
   FeInfo const * f;
@@ -51,26 +67,29 @@ struct LocalBasis
 };
 LocalBasis localBasis(int feType, int b)
 {
-  if (feType != PXG_FE_QUAD_Q2Q1)
+  ThPair p;
+  if (!thPair(feType, p))
     return {&feInfo(feType), b};
-  if (b < 18)
-    return {&feInfo(PXG_FE_QUAD_Q2), b % 9};
-  return {&feInfo(PXG_FE_QUAD_Q1), b - 18};
+  int const nv = feInfo(p.vel).n;
+  if (b < p.dim * nv)
+    return {&feInfo(p.vel), b % nv};
+  return {&feInfo(p.pre), b - p.dim * nv};
 }
 
 void buildTables(pxg_ctx * c, int feType, int nq, double const * nodes, double const * weights)
 {
-  bool const mixed = feType == PXG_FE_QUAD_Q2Q1;
-  FeInfo fe = feInfo(mixed ? PXG_FE_QUAD_Q2 : feType);
+  ThPair pair{0, 0, 0};
+  bool const mixed = thPair(feType, pair);
+  FeInfo fe = feInfo(mixed ? pair.vel : feType);
   if (mixed)
   {
     fe.type = feType;
-    fe.n = 22;
+    fe.n = pair.dim * fe.n + feInfo(pair.pre).n;
   }
   FeTables & T = c->fe;
   Quadrature q;
-  if (nq <= 0) // the pair shares the velocity's rule (both spaces use GaussQR<Quad,9>)
-    q = recommendedQuadrature(mixed ? PXG_FE_QUAD_Q2 : feType);
+  if (nq <= 0) // a pair uses the velocity's rule for both spaces, as the reference tests do
+    q = recommendedQuadrature(mixed ? pair.vel : feType);
   else
   {
     PXG_REQUIRE(nodes && weights, PXG_E_INVALID, "quadrature nodes and weights required");
@@ -219,7 +238,7 @@ int pxg_set_fe(pxg_ctx * c, int feType, int nq, const double * nodes, const doub
   return guarded(c,
                  [&]
                  {
-                   PXG_REQUIRE(feType >= PXG_FE_LINE_P1 && feType <= PXG_FE_QUAD_Q2Q1, PXG_E_INVALID,
+                   PXG_REQUIRE(feType >= PXG_FE_LINE_P1 && feType <= PXG_FE_TET_P2P1, PXG_E_INVALID,
                                "unknown FE type");
                    PXG_REQUIRE(nq >= 0 && nq <= 64, PXG_E_INVALID, "quadrature size must be 0..64");
                    buildTables(c, feType, nq, nodes, weights);

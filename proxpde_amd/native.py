@@ -15,8 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libproxpde_amd.so")
 
 FE = {"line_p1": 1, "quad_q1": 2, "quad_q2": 3, "hex_q1": 4, "hex_q2": 5, "tet_p1": 6, "tet_p2": 7,
-      "quad_q2q1": 8}
-FE_DIM = {1: 1, 2: 2, 3: 2, 4: 3, 5: 3, 6: 3, 7: 3, 8: 2}
+      "quad_q2q1": 8, "hex_q2q1": 9, "tet_p2p1": 10}
+FE_DIM = {1: 1, 2: 2, 3: 2, 4: 3, 5: 3, 6: 3, 7: 3, 8: 2, 9: 3, 10: 3}
 MESH = {"line": 1, "square": 2, "cube_hex": 3, "cube_tet5": 4, "cube_tet6": 5}
 SIDE = {"bottom": 1, "right": 2, "top": 3, "left": 4, "back": 5, "front": 6}
 TERM = {"stiffness": 1, "stiffness_fe": 2, "mass": 3, "mass_fe": 4, "grad": 5, "div": 6, "tensor_stiffness": 7}

@@ -110,50 +110,64 @@ class Problem:
         return a
 
 
+PAIRS = {"quad_q2q1": ("quad_q2", "quad_q1", 2, "square"), "hex_q2q1": ("hex_q2", "hex_q1", 3, "cube_hex"),
+         "tet_p2p1": ("tet_p2", "tet_p1", 3, "cube_tet5")}
+
+
 class StokesProblem:
-    """Taylor-Hood Q2-Q1 system on a structured quad mesh (host layer).
+    """Taylor-Hood system on a structured mesh (host layer).
 
-    Velocity: the Q2 space with 2 components, BLOCK (id + d nU) or INTERLEAVED
-    (2 id + d) dof ordering (reference include/dof.hpp:139-166); pressure: the
-    Q1 space after the velocity (offset 2 nU), as in the reference's
-    test/assemblyvector.cpp and test/stokes2dquad.cpp.  `emap` is the dof map
-    of the 22-dof element PXG_FE_QUAD_Q2Q1: [u (9) | v (9) | p (4)]."""
+    Velocity: the Q2 / P2 space with dim components, BLOCK (id + d nU) or
+    INTERLEAVED (dim id + d) dof ordering (reference include/dof.hpp:139-166);
+    pressure: the Q1 / P1 space after the velocity (offset dim nU), as in the
+    reference's test/assemblyvector.cpp, stokes2dquad.cpp, stokes3dhex.cpp and
+    stokes3dtet.cpp.  `emap` is the dof map of the pair's element
+    (PXG_FE_QUAD_Q2Q1 / HEX_Q2Q1 / TET_P2P1): [u_0 (NV) | ... | u_dim-1 | p]."""
 
-    def __init__(self, n, interleaved=False, length=(1.0, 1.0, 0.0)):
+    def __init__(self, n, interleaved=False, pair="quad_q2q1", mesh=None, length=None):
+        vfe, pfe, dim, kind = PAIRS[pair]
+        self.pair = pair
+        self.dim = dim
         self.interleaved = bool(interleaved)
-        self.vel = Problem("quad_q2", n, length=length)
+        if length is None:
+            length = (1.0, 1.0, 0.0) if dim == 2 else (1.0, 1.0, 1.0)
+        self.vel = Problem(vfe, n, mesh=mesh or kind, length=length)
         self.mesh = self.vel.mesh
         self.xyz, self.conn = self.vel.xyz, self.vel.conn
         self.nU = self.vel.ndofs
-        self.nP, emapP = self.mesh.dofmap("quad_q1")
+        self.pfe = pfe
+        self.nP, emapP = self.mesh.dofmap(pfe)
         self.emapP = emapP
         ev = self.vel.emap.astype(np.int64)
-        E = ev.shape[0]
-        emap = np.empty((E, 22), dtype=np.uint32)
-        for d in range(2):
-            emap[:, 9 * d:9 * d + 9] = self.vel_id(ev, d)
-        emap[:, 18:] = emapP.astype(np.int64) + 2 * self.nU
+        E, nv = ev.shape
+        emap = np.empty((E, dim * nv + emapP.shape[1]), dtype=np.uint32)
+        for d in range(dim):
+            emap[:, nv * d:nv * (d + 1)] = self.vel_id(ev, d)
+        emap[:, dim * nv:] = emapP.astype(np.int64) + dim * self.nU
         self.emap = emap
-        self.ndofs = 2 * self.nU + self.nP
+        self.ndofs = dim * self.nU + self.nP
 
     def vel_id(self, i, d):
         """Global id of component d of velocity dof i."""
         i = np.asarray(i, dtype=np.int64)
-        return 2 * i + d if self.interleaved else i + d * self.nU
+        return self.dim * i + d if self.interleaved else i + d * self.nU
 
     def set_coords(self, xyz):
         self.vel.set_coords(xyz)
         self.xyz = self.vel.xyz
 
     def dof_coords(self):
-        """Coordinates of the Q2 (velocity) dofs."""
+        """Coordinates of the velocity dofs."""
         return self.vel.dof_coords()
+
+    def pressure_coords(self):
+        return self.mesh.dof_coords(self.pfe, self.emapP, self.nP)
 
     def dirichlet(self, bcs, pressure=()):
         """Dirichlet data of the whole system from BCEss-like entries
-        (side, value, comps): value is a pair (u, v) or a function of the dof
-        coordinates returning an (n, 2) array, comps the constrained
-        components (default both).  Later entries win where sides share dofs,
+        (side, value, comps): value is a dim-tuple or a function of the dof
+        coordinates returning an (n, dim) array, comps the constrained
+        components (default all).  Later entries win where sides share dofs,
         as the reference's BCEss list does.  pressure: (pressure dof, value)
         pairs.  Returns (dofs, values) for set_dirichlet and the full flag /
         value arrays."""
@@ -162,7 +176,7 @@ class StokesProblem:
         x = self.dof_coords()
         for entry in bcs:
             side, value = entry[0], entry[1]
-            comps = entry[2] if len(entry) > 2 else (0, 1)
+            comps = entry[2] if len(entry) > 2 else tuple(range(self.dim))
             ids = self.vel.bc_dofs((side,)).astype(np.int64)
             vv = value(x[ids]) if callable(value) else np.tile(np.asarray(value, np.float64), (len(ids), 1))
             for d in comps:
@@ -170,14 +184,14 @@ class StokesProblem:
                 flag[g] = 1
                 val[g] = vv[:, d]
         for k, v in pressure:
-            flag[2 * self.nU + k] = 1
-            val[2 * self.nU + k] = v
+            flag[self.dim * self.nU + k] = 1
+            val[self.dim * self.nU + k] = v
         dofs = np.nonzero(flag)[0].astype(np.uint32)
         return dofs, val[dofs], flag, val
 
     def assembler(self, qr=None, device=0):
         a = N.Assembler(device)
-        a.set_fe("quad_q2q1", qr)
+        a.set_fe(self.pair, qr)
         a.set_mesh(self.xyz, self.conn)
         a.set_dofmap(self.ndofs, self.emap)
         return a

@@ -3,15 +3,16 @@
 The HIP path assembles the velocity block (AssemblyStiffness,
 AssemblyTensorStiffness, AssemblyScalarMass), the AssemblyGrad / AssemblyDiv
 couplings and the Dirichlet elimination of Builder::buildLhs / buildCoupling
-(reference include/builder.hpp:31-277) as one 22-dof element through the C ABI.
+(reference include/builder.hpp:31-277) as one element of the pair (22 dofs for
+Q2-Q1 quads, 34 for P2-P1 tets, 89 for Q2-Q1 hexes) through the C ABI.
 Bar: FP64 values of A and b within 1e-12 relative to the oracle (the bar of
 the scalar elements) and the closed pattern equal except at rounding level:
 many entries of the coupling blocks are zero in exact arithmetic and carry
 residues of 1e-19 to 1e-15, and whether one survives prune(1e-16) depends on
 the order of evaluation (the reference's own pattern depends on Eigen's), so an
 entry present on one side only must be below 1e-14 of the matrix scale on
-both.  Further: the reference's stokes2dquad
-known answer with the GPU-assembled system; a rebuild after clear (the staged
+both.  Further: the reference's stokes2dquad,
+stokes3dtet and stokes3dhex known answers with the GPU-assembled systems; a rebuild after clear (the staged
 path into the closed layout) bitwise equal to the first build; FP32 within
 2e-6 of the FP64 oracle in max norm.
 """
@@ -36,21 +37,42 @@ def _inlet(p):
 S2D_BC = [("bottom", _inlet), ("right", (0.0, 0.0)), ("top", (0.0, 0.0), (0,)), ("left", (0.0, 0.0), (0,))]
 TERM = {"stiffness": "stiffness", "tensor": "tensor_stiffness", "mass": "mass", "grad": "grad", "div": "div"}
 
+def _inlet3(p):
+    return np.stack([0.0 * p[:, 0], 0.5 * (1.0 - p[:, 0] ** 2), 0.0 * p[:, 0]], 1)
+
+
+Z3 = (0.0, 0.0, 0.0)
+# test/stokes3dhex.cpp, stokes3dtet.cpp
+S3D_BC = [("bottom", _inlet3), ("right", Z3), ("top", Z3, (0, 2)), ("left", Z3, (0, 2)), ("back", Z3, (2,)),
+          ("front", Z3, (2,))]
+# a mixed set: values that differ per side and component
+MIX3_BC = [("bottom", (0.5, 1.0, -0.25)), ("left", (0.0, 2.0, 0.0), (1,)), ("front", _inlet3, (0, 1))]
+
 CASES = [
-    # n, interleaved, perturb, coefficients, velocity BCs, pressure pins
-    ((3, 3), False, 0.0, {"stiffness": 1.0, "grad": -1.0, "div": -1.0}, AV_BC, ()),
-    ((3, 3), True, 0.0, {"stiffness": 1.0, "grad": -1.0, "div": -1.0}, AV_BC, ()),
-    ((7, 5), True, 0.2, {"stiffness": 1.0, "grad": -1.0, "div": -1.0}, AV_BC, ()),
-    ((10, 10), False, 0.2, {"tensor": 0.1, "mass": 3.0, "grad": -1.0, "div": -0.5}, S2D_BC, ((5, 0.7),)),
-    ((12, 9), True, 0.15, {"stiffness": 0.5, "tensor": 0.25, "grad": 1.5, "div": -1.0}, S2D_BC, ()),
+    # pair, n, interleaved, perturb, coefficients, velocity BCs, pressure pins
+    ("quad_q2q1", (3, 3), False, 0.0, {"stiffness": 1.0, "grad": -1.0, "div": -1.0}, AV_BC, ()),
+    ("quad_q2q1", (3, 3), True, 0.0, {"stiffness": 1.0, "grad": -1.0, "div": -1.0}, AV_BC, ()),
+    ("quad_q2q1", (7, 5), True, 0.2, {"stiffness": 1.0, "grad": -1.0, "div": -1.0}, AV_BC, ()),
+    ("quad_q2q1", (10, 10), False, 0.2, {"tensor": 0.1, "mass": 3.0, "grad": -1.0, "div": -0.5}, S2D_BC, ((5, 0.7),)),
+    ("quad_q2q1", (12, 9), True, 0.15, {"stiffness": 0.5, "tensor": 0.25, "grad": 1.5, "div": -1.0}, S2D_BC, ()),
+    ("tet_p2p1", (3, 2, 2), False, 0.15, {"tensor": 0.1, "grad": -1.0, "div": -1.0}, S3D_BC, ()),
+    ("tet_p2p1", (2, 3, 2), True, 0.1, {"stiffness": 0.5, "mass": 2.0, "grad": 1.0, "div": -0.5}, MIX3_BC,
+     ((3, 0.25),)),
+    ("hex_q2q1", (3, 2, 2), False, 0.15, {"tensor": 0.1, "grad": -1.0, "div": -1.0}, S3D_BC, ()),
+    ("hex_q2q1", (2, 2, 3), True, 0.1, {"stiffness": 0.5, "tensor": 0.2, "mass": 1.0, "grad": 1.0, "div": -0.5},
+     MIX3_BC, ((7, -0.5),)),
 ]
+VFE = {"quad_q2q1": ("quad_q2", "square"), "tet_p2p1": ("tet_p2", "cube_tet5"), "hex_q2q1": ("hex_q2", "cube_hex")}
 
 
-def _pair(oracle, n, interleaved, perturb):
-    prob = W.StokesProblem(n, interleaved)
-    o = oracle.Space("quad_q2", "square", n)
+def _pair(oracle, n, interleaved, perturb, pair="quad_q2q1", length=None):
+    prob = W.StokesProblem(n, interleaved, pair=pair, length=length)
+    vfe, mesh = VFE[pair]
+    ln = prob.vel.length
+    o = oracle.Space(vfe, mesh, n, length=ln)
     if perturb:
-        xyz = W.perturb_interior(prob.xyz, 1.0 / max(n), seed=3, amount=perturb, length=(1.0, 1.0, 0.0))
+        xyz = W.perturb_interior(prob.xyz, min(ln[k] / n[k] for k in range(len(n))), seed=3, amount=perturb,
+                                 length=ln)
         prob.set_coords(xyz)
         o.set_coords(xyz)
     assert np.array_equal(o.elem_map().astype(np.uint32), prob.vel.emap)
@@ -71,11 +93,12 @@ def _assemble(prob, dofs, vals, coefs, precision=N.FP64):
     return a
 
 
-@pytest.mark.parametrize("case", CASES, ids=[f"{c[0][0]}x{c[0][1]}-{'il' if c[1] else 'blk'}" for c in CASES])
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{'x'.join(map(str, c[1]))}-{'il' if c[2] else 'blk'}"
+                                              for c in CASES])
 def test_stokes_matches_oracle(oracle, gpu, case):
-    n, interleaved, pert, coefs, bcs, pins = case
-    print(f"stokes case n={n} interleaved={interleaved} coefs={coefs}", flush=True)
-    prob, o = _pair(oracle, n, interleaved, pert)
+    pair, n, interleaved, pert, coefs, bcs, pins = case
+    print(f"stokes case {pair} n={n} interleaved={interleaved} coefs={coefs}", flush=True)
+    prob, o = _pair(oracle, n, interleaved, pert, pair)
     dofs, vals, flag, full = prob.dirichlet(bcs, pins)
     rp_o, col_o, v_o, b_o = o.stokes(interleaved, coefs, flag, full)
     a = _assemble(prob, dofs, vals, coefs)
@@ -100,9 +123,13 @@ def test_stokes_matches_oracle(oracle, gpu, case):
           f"max err A={err:.2e} b={errb:.2e}")
 
 
-def test_stokes_rebuild_after_clear_is_bitwise(oracle, gpu):
-    n, interleaved, pert, coefs, bcs, pins = CASES[3]
-    prob, o = _pair(oracle, n, interleaved, pert)
+@pytest.mark.parametrize("ci", [3, 8])
+def test_stokes_rebuild_after_clear_is_bitwise(oracle, gpu, ci):
+    """The second and third builds go through the staged rebuild (for the
+    89-dof hex pair with 32 element rows per stage block)."""
+    pair, n, interleaved, pert, coefs, bcs, pins = CASES[ci]
+    print(f"rebuild case {pair} n={n}", flush=True)
+    prob, o = _pair(oracle, n, interleaved, pert, pair)
     dofs, vals, flag, full = prob.dirichlet(bcs, pins)
     a = _assemble(prob, dofs, vals, coefs)
     first = a.csr()
@@ -116,9 +143,11 @@ def test_stokes_rebuild_after_clear_is_bitwise(oracle, gpu):
             assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
 
 
-def test_stokes_fp32_against_fp64_oracle(oracle, gpu):
-    n, interleaved, pert, coefs, bcs, pins = CASES[2]
-    prob, o = _pair(oracle, n, interleaved, pert)
+@pytest.mark.parametrize("ci", [2, 6, 8])
+def test_stokes_fp32_against_fp64_oracle(oracle, gpu, ci):
+    pair, n, interleaved, pert, coefs, bcs, pins = CASES[ci]
+    print(f"fp32 case {pair} n={n}", flush=True)
+    prob, o = _pair(oracle, n, interleaved, pert, pair)
     dofs, vals, flag, full = prob.dirichlet(bcs, pins)
     rp_o, col_o, v_o, b_o = o.stokes(interleaved, coefs, flag, full)
     a = _assemble(prob, dofs, vals, coefs, precision=N.FP32)
@@ -148,7 +177,7 @@ class _PressureSpace:
         self.prob = prob
 
     def dof_coords(self):
-        return self.prob.mesh.dof_coords("quad_q1", self.prob.emapP, self.prob.nP)
+        return self.prob.pressure_coords()
 
 
 def test_stokes2dquad_known_answer_gpu(gpu):
@@ -165,6 +194,25 @@ def test_stokes2dquad_known_answer_gpu(gpu):
     eu, ev, ep = stokes2dquad_known_answer(_VelSpace(prob), _PressureSpace(prob), solve, n)
     print(f"  errors u {eu:.2e} v {ev:.2e} p {ep:.2e}")
     assert eu <= 1e-14 and ev <= 2e-14 and ep <= 3e-14
+
+
+@pytest.mark.parametrize("pair", ["tet_p2p1", "hex_q2q1"])
+def test_stokes3d_known_answer_gpu(gpu, pair):
+    """test/stokes3dtet.cpp / stokes3dhex.cpp on the GPU-assembled system:
+    every error norm within 1e-12 of the reference's (checkError)."""
+    from test_oracle_stokes import S3D_SIDES, STOKES3D_REF, stokes_known_answer
+    n = (4, 4, 4)
+    prob = W.StokesProblem(n, pair=pair, length=(1.0, 2.0, 1.0))
+
+    def solve(flag, val, nu):
+        dofs = np.nonzero(flag)[0].astype(np.uint32)
+        a = _assemble(prob, dofs, val[dofs], {"tensor": nu, "grad": -1.0, "div": -1.0})
+        rp, col, v, b = a.csr()
+        return lu_solve(sp.csr_matrix((v, col, rp), shape=(prob.ndofs, prob.ndofs)), b)
+    errs = stokes_known_answer(_VelSpace(prob), _PressureSpace(prob), solve, 3, S3D_SIDES, 0.1, ly=2.0)
+    print(f"  {pair} errors " + " ".join(f"{e:.2e}" for e in errs))
+    for e, r in zip(errs, STOKES3D_REF[VFE[pair][0]]):
+        assert abs(e - r) < 1e-12
 
 
 def test_stokes_rejects_unsupported(gpu):
