@@ -301,16 +301,26 @@ def config_c5(args, dev, N, W):
             "dofs": solver.ndofs}, "results": res}
 
 
+STOKES_SIZES = {"quad_q2q1": (1024, 1024), "tet_p2p1": (40, 40, 40), "hex_q2q1": (48, 48, 48)}
+
+
 def config_stokes(args, dev, N, W):
-    """Taylor-Hood Q2-Q1 Stokes system on 1024^2 quads (BLOCK velocity dofs),
-    interior vertices perturbed U(-0.2h, 0.2h) seed 3: tensor stiffness 0.1 +
-    grad -1 + div -1 (test/stokes2dquad.cpp's operator), Dirichlet velocity on
-    all four sides; full builds (clear + assemble + close) in FP64 and FP32, the
-    22-dof element kernel and the staged scatter timed with HIP events."""
-    n = 1024
-    prob = W.StokesProblem((n, n))
-    prob.set_coords(W.perturb_interior(prob.xyz, 1.0 / n, seed=3, amount=0.2, length=(1.0, 1.0, 0.0)))
-    bcs = [("bottom", (0.0, 1.0)), ("right", (0.0, 0.0)), ("top", (0.0, 0.0)), ("left", (0.0, 0.0))]
+    """Taylor-Hood Stokes system (--pair quad_q2q1: Q2-Q1 on 1024^2 quads;
+    tet_p2p1: P2-P1 on 40^3 cubes x 5 tets; hex_q2q1: Q2-Q1 on 48^3 hexes),
+    BLOCK velocity dofs, interior vertices perturbed U(-0.2h, 0.2h) seed 3:
+    tensor stiffness 0.1 + grad -1 + div -1 (the operator of the reference's
+    stokes2dquad / stokes3dhex / stokes3dtet tests), Dirichlet velocity on
+    every side; full builds (clear + assemble + close) in FP64 and FP32, the
+    element kernel and the staged scatter timed with HIP events."""
+    pair = args.pair
+    n = STOKES_SIZES[pair]
+    prob = W.StokesProblem(n, pair=pair)
+    ln = prob.vel.length
+    prob.set_coords(W.perturb_interior(prob.xyz, 1.0 / max(n), seed=3, amount=0.2, length=ln))
+    dim = prob.dim
+    sides = ("bottom", "right", "top", "left") + (("back", "front") if dim == 3 else ())
+    inflow = tuple(1.0 if d == 1 else 0.0 for d in range(dim))
+    bcs = [("bottom", inflow)] + [(sd, (0.0,) * dim) for sd in sides[1:]]
     dofs, vals, _, _ = prob.dirichlet(bcs)
     a = prob.assembler()
     a.set_dirichlet(dofs, vals)
@@ -318,8 +328,9 @@ def config_stokes(args, dev, N, W):
     nnz_s = a.build_pattern()
     t_pattern = time.time() - t0
     E = prob.mesh.nelem
+    nl = prob.emap.shape[1]
     terms = [("tensor_stiffness", 0.1), ("grad", -1.0), ("div", -1.0)]
-    res = {"pattern_s": t_pattern, "nnz_structural": int(nnz_s)}
+    res = {"pattern_s": t_pattern, "nnz_structural": int(nnz_s), "local_dofs": int(nl)}
     for name, prec in (("fp64", N.FP64), ("fp32", N.FP32)):
         def build():
             a.clear()
@@ -338,15 +349,16 @@ def config_stokes(args, dev, N, W):
         s = 8 if prec == N.FP64 else 4
         ke_ms = pe[0] / max(pe[1], 1)
         sc_ms = ps[0] / max(ps[1], 1)
-        # element kernel, algorithmic bytes per element: the 22 x 22 matrix and
-        # 22 lifting values written, 4 vertex ids + 4 x 3 coordinates, 22 dof ids,
-        # flags and values read
-        ke_bytes = E * (22 * 22 * s + 22 * 8 + 4 * 4 + 4 * 24 + 22 * (4 + 1 + 8))
+        npe = prob.conn.shape[1]
+        # element kernel, algorithmic bytes per element: the element matrix and
+        # its lifting values written, the vertex ids and coordinates, the dof
+        # ids, flags and values read
+        ke_bytes = E * (nl * nl * s + nl * 8 + npe * 4 + npe * 24 + nl * (4 + 1 + 8))
         # staged scatter: per structural entry its info word and value written,
         # per contribution its slot and element-matrix value, per element row
         # its pair id and lifting value, per row the segment offset, b read and
         # written and both inverse diagonals
-        sc_bytes = nnz_s * (2 + s) + E * 484 * (2 + s) + E * 22 * (4 + 8) + prob.ndofs * (4 + 16 + 12)
+        sc_bytes = nnz_s * (2 + s) + E * nl * nl * (2 + s) + E * nl * (4 + 8) + prob.ndofs * (4 + 16 + 12)
         res[f"element_{name}_ms"] = ke_ms
         res[f"element_{name}_GB/s"] = ke_bytes / (ke_ms * 1e-3) / 1e9 if ke_ms else None
         res[f"scatter_{name}_ms"] = sc_ms
@@ -356,23 +368,26 @@ def config_stokes(args, dev, N, W):
     if not args.no_cpu:
         from oracle import oracle as O
         O.build()
-        m = 128
-        o = O.Space("quad_q2", "square", (m, m))
-        sp_ = W.StokesProblem((m, m))
+        m = {"quad_q2q1": (128, 128), "tet_p2p1": (8, 8, 8), "hex_q2q1": (8, 8, 8)}[pair]
+        vfe, mesh = {"quad_q2q1": ("quad_q2", "square"), "tet_p2p1": ("tet_p2", "cube_tet5"),
+                     "hex_q2q1": ("hex_q2", "cube_hex")}[pair]
+        o = O.Space(vfe, mesh, m, length=ln)
+        sp_ = W.StokesProblem(m, pair=pair)
         _, _, flag, full = sp_.dirichlet(bcs)
         coefs = {"tensor": 0.1, "grad": -1.0, "div": -1.0}
         o.stokes(False, coefs, flag, full)
         tc = time.perf_counter()
-        reps = 5
+        reps = 3
         for _ in range(reps):
             o.stokes(False, coefs, flag, full)
         tcpu = (time.perf_counter() - tc) / reps
-        cpu = {"value": m * m / tcpu, "unit": "elements/s", "cores": 1, "kind": "port",
-               "sample": f"oracle buildLhs + 2 x buildCoupling + close, {m}^2 quads, {reps} reps"}
-    return {"metric": "Stokes Q2-Q1 1024^2 build (tensor stiffness + grad + div), elements/s",
+        cpu = {"value": o.nelem / tcpu, "unit": "elements/s", "cores": 1, "kind": "port",
+               "sample": f"oracle buildLhs + 2 x buildCoupling + close, {'x'.join(map(str, m))} {mesh}, {reps} reps"}
+    return {"metric": f"Stokes {pair} {'x'.join(map(str, n))} build (tensor stiffness + grad + div), elements/s",
             "value": res["elements_per_s_fp64"], "unit": "elements/s",
-            "config": {"workload": "Taylor-Hood Q2-Q1 on 1024^2 perturbed quads, BLOCK dofs, Dirichlet velocity",
-                       "elements": E, "dofs": prob.ndofs}, "results": res, "cpu_baseline": cpu}
+            "config": {"workload": f"Taylor-Hood {pair} on {'x'.join(map(str, n))} perturbed {prob.vel.mesh_kind}, "
+                                   "BLOCK dofs, Dirichlet velocity", "elements": E, "dofs": prob.ndofs},
+            "results": res, "cpu_baseline": cpu}
 
 
 def progress(msg):
@@ -393,6 +408,8 @@ def main():
     ap.add_argument("--solve-reps", type=int, default=2)
     ap.add_argument("--config", choices=["c2", "c3", "c5", "stokes"], default=None,
                     help="run one of the other workloads instead of the headline one (one JSON line)")
+    ap.add_argument("--pair", choices=["quad_q2q1", "tet_p2p1", "hex_q2q1"], default="quad_q2q1",
+                    help="Taylor-Hood pair of --config stokes")
     ap.add_argument("--copy-gib", type=float, default=1.0,
                     help="size of a plain device copy timed for reference (0: none)")
     args = ap.parse_args()

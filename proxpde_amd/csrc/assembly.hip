@@ -303,6 +303,8 @@ struct ThShape
   static constexpr int DIM = DIM_, NV = NV_, NP = NP_, NPE = NPE_;
   static constexpr int N = DIM * NV + NP;
   static constexpr int GEO = 3 * DIM + 1; // P (DIM x 3) and JxW per point
+  // rows accumulated one column block at a time (register budget)
+  static constexpr bool BLOCKED = N > 48;
 };
 using ThQuad = ThShape<2, 9, 4, 4>;  // Q2-Q1 quads, 22 local dofs
 using ThTet = ThShape<3, 10, 4, 4>;  // P2-P1 tets, 34
@@ -317,11 +319,17 @@ bool isThPair(int feType)
 }
 
 // LDS bytes per element of k_element_th: P and JxW per point in FP64, three
-// gradient components per (point, velocity basis) in T
+// gradient components per (point, velocity basis) in T (plus, per block, the
+// velocity and pressure basis values per point)
 template <typename T, typename S>
 constexpr size_t thLdsPerElem(int nq)
 {
   return (size_t)nq * S::GEO * sizeof(double) + (size_t)nq * S::NV * 3 * sizeof(T);
+}
+template <typename T, typename S>
+constexpr size_t thLdsTable(int nq)
+{
+  return (size_t)nq * (S::NV + S::NP) * sizeof(T);
 }
 
 // Work split: a block takes G elements (G <= kBlock / N, from the LDS
@@ -343,12 +351,16 @@ __global__ void __launch_bounds__(kBlock)
   extern __shared__ unsigned char smem_raw[];
   double * sGeo = reinterpret_cast<double *>(smem_raw);   // [G][nq][GEO]: P row-major, JxW
   T * sG = reinterpret_cast<T *>(sGeo + (size_t)G * nq * GEO); // [G][nq][NV][3]
-  T * sPhi = sG + (size_t)G * nq * NV * 3;                   // [nq][N]
+  T * sPhi = sG + (size_t)G * nq * NV * 3;                   // [nq][NV + NP]
+  constexpr int NB = NV + NP; // distinct basis values per point: velocity, pressure
   int const t = threadIdx.x;
   int64_t const e0 = (int64_t)blockIdx.x * G;
   int const ne = (int)min<int64_t>(G, E - e0);
-  for (int k = t; k < nq * N; k += blockDim.x)
-    sPhi[k] = phiTab[k];
+  for (int k = t; k < nq * NB; k += blockDim.x)
+  {
+    int const q = k / NB, r = k - q * NB;
+    sPhi[k] = phiTab[q * N + (r < NV ? r : DIM * NV + (r - NV))];
+  }
   // 1. geometry per (element, q)
   for (int it = t; it < ne * nq; it += blockDim.x)
   {
@@ -450,6 +462,88 @@ __global__ void __launch_bounds__(kBlock)
   int const d = vel ? i / NV : 0;     // velocity component of the row
   int const a = vel ? i - d * NV : 0; // its velocity basis
   int const k = vel ? 0 : i - DIM * NV; // pressure basis of a pressure row
+  if constexpr (S::BLOCKED)
+  {
+    // wide elements: one column block (a velocity component, then the
+    // pressure) at a time, so NV accumulators are live instead of N; the
+    // entries and the lifting sum (column order) are those of the one-pass
+    // form below
+    uint32_t const * ids = emap + e * N;
+    bool const ci = hasBC && bcFlag[ids[i]];
+    double sum = 0.;
+    T * out = ke + (e * N + i) * keRowStride(N);
+    for (int cb = 0; cb <= DIM; ++cb)
+    {
+      T acc[NV];
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+        acc[j] = T(0);
+      for (int q = 0; q < nq; ++q)
+      {
+        double const JxW = sGeo[((size_t)el * nq + q) * GEO + 3 * DIM];
+        T const * gq = sG + ((size_t)el * nq + q) * NV * 3;
+        T const * pq = sPhi + q * NB;
+        if (vel)
+        {
+          T const gi0 = gq[a * 3], gi1 = gq[a * 3 + 1], gi2 = gq[a * 3 + 2];
+          T const gid = d == 0 ? gi0 : (d == 1 ? gi1 : gi2);
+          T const ts = (T)(cf.tensor * JxW);
+          if (cb == d)
+          {
+            T const s = (T)(cf.stiff * JxW), m = (T)(cf.mass * JxW);
+            T const pa = pq[a];
+#pragma unroll
+            for (int j = 0; j < NV; ++j)
+            {
+              T const gj0 = gq[j * 3], gj1 = gq[j * 3 + 1], gj2 = gq[j * 3 + 2];
+              T const dd = gi0 * gj0 + gi1 * gj1 + gi2 * gj2;
+              T const diag = s * dd + ts * dd + m * (pa * pq[j]);
+              T const gjc = cb == 0 ? gj0 : (cb == 1 ? gj1 : gj2);
+              acc[j] += diag + ts * (gid * gjc);
+            }
+          }
+          else if (cb < DIM)
+          {
+#pragma unroll
+            for (int j = 0; j < NV; ++j)
+              acc[j] += ts * (gid * gq[j * 3 + cb]);
+          }
+          else
+          {
+            T const gs = (T)(cf.grad * JxW) * gid;
+#pragma unroll
+            for (int kk = 0; kk < NP; ++kk)
+              acc[kk] += gs * pq[NV + kk];
+          }
+        }
+        else if (cb < DIM)
+        {
+          T const sp = (T)(cf.div * JxW) * pq[NV + k];
+#pragma unroll
+          for (int j = 0; j < NV; ++j)
+            acc[j] += sp * gq[j * 3 + cb];
+        }
+      }
+      int const c0 = cb * NV, width = cb < DIM ? NV : NP;
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+        if (j < width)
+        {
+          int const col = c0 + j;
+          T v = acc[j];
+          if (ci)
+            v = col == i ? T(1) : T(0);
+          else if (hasBC && bcFlag[ids[col]])
+          {
+            sum += (double)v * bcVal[ids[col]];
+            v = T(0);
+          }
+          out[col] = v;
+        }
+    }
+    fe[e * N + i] = ci ? bcVal[ids[i]] : (hasBC ? 0. - sum : 0.);
+    return;
+  }
   T acc[N];
 #pragma unroll
   for (int j = 0; j < N; ++j)
@@ -458,14 +552,14 @@ __global__ void __launch_bounds__(kBlock)
   {
     double const JxW = sGeo[((size_t)el * nq + q) * GEO + 3 * DIM];
     T const * gq = sG + ((size_t)el * nq + q) * NV * 3;
-    T const * pq = sPhi + q * N;
+    T const * pq = sPhi + q * NB;
     if (vel)
     {
       T const gi0 = gq[a * 3], gi1 = gq[a * 3 + 1], gi2 = gq[a * 3 + 2];
       T const gid = d == 0 ? gi0 : (d == 1 ? gi1 : gi2);
       T const s = (T)(cf.stiff * JxW), ts = (T)(cf.tensor * JxW), m = (T)(cf.mass * JxW);
       T const gs = (T)(cf.grad * JxW) * gid;
-      T const pa = pq[i];
+      T const pa = pq[a];
 #pragma unroll
       for (int j = 0; j < NV; ++j)
       {
@@ -482,11 +576,11 @@ __global__ void __launch_bounds__(kBlock)
       }
 #pragma unroll
       for (int kk = 0; kk < NP; ++kk)
-        acc[DIM * NV + kk] += gs * pq[DIM * NV + kk];
+        acc[DIM * NV + kk] += gs * pq[NV + kk];
     }
     else
     {
-      T const sp = (T)(cf.div * JxW) * pq[DIM * NV + k];
+      T const sp = (T)(cf.div * JxW) * pq[NV + k];
 #pragma unroll
       for (int j = 0; j < NV; ++j)
 #pragma unroll
@@ -1616,7 +1710,7 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                  using Sh = ThOf<N>;
                  // elements per block: all rows of an element in one block,
                  // LDS within 64 KB
-                 size_t const per = thLdsPerElem<TT, Sh>(T.nq), tab = (size_t)T.nq * N * sizeof(TT);
+                 size_t const per = thLdsPerElem<TT, Sh>(T.nq), tab = thLdsTable<TT, Sh>(T.nq);
                  PXG_REQUIRE(tab + per <= 65536, PXG_E_INVALID,
                              "quadrature too large for the Taylor-Hood element kernel");
                  int const G = (int)std::min<size_t>(kBlock / N, (65536 - tab) / per);
