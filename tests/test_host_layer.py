@@ -89,28 +89,37 @@ def test_gmsh_errors(native, tmp_path):
         native.HostMesh.gmsh(str(tmp_path / "missing.msh"), "tet")
 
 
+STOKES_PAIRS = [("quad_q2q1", "quad_q2", "quad_q1", "square", (4, 3)),
+                ("tet_p2p1", "tet_p2", "tet_p1", "cube_tet5", (2, 3, 2)),
+                ("hex_q2q1", "hex_q2", "hex_q1", "cube_hex", (3, 2, 2))]
+
+
+@pytest.mark.parametrize("pair,vfe,pfe,mesh,n", STOKES_PAIRS, ids=[c[0] for c in STOKES_PAIRS])
 @pytest.mark.parametrize("interleaved", [False, True])
-def test_stokes_dofmap_matches_oracle(oracle, native, interleaved):
-    """The 22-dof Taylor-Hood map: velocity components in BLOCK or INTERLEAVED
-    ordering (reference include/dof.hpp:139-166), then the Q1 pressure after
-    2 nU; Dirichlet data with later entries winning on shared dofs."""
+def test_stokes_dofmap_matches_oracle(oracle, native, interleaved, pair, vfe, pfe, mesh, n):
+    """Taylor-Hood maps: velocity components in BLOCK or INTERLEAVED ordering
+    (reference include/dof.hpp:139-166), then the linear pressure after
+    dim nU; Dirichlet data with later entries winning on shared dofs."""
     from proxpde_amd import workloads as W
-    n = (4, 3)
-    prob = W.StokesProblem(n, interleaved)
-    ov = oracle.Space("quad_q2", "square", n)
-    op = oracle.Space("quad_q1", "square", n)
+    prob = W.StokesProblem(n, interleaved, pair=pair)
+    dim = prob.dim
+    ln = prob.vel.length
+    ov = oracle.Space(vfe, mesh, n, length=ln)
+    op = oracle.Space(pfe, mesh, n, length=ln)
     ev = ov.elem_map().astype(np.int64)
     nU = ov.ndofs
-    ref = np.concatenate([2 * ev if interleaved else ev, 2 * ev + 1 if interleaved else ev + nU,
-                          op.elem_map().astype(np.int64) + 2 * nU], axis=1)
+    ref = np.concatenate([dim * ev + d if interleaved else ev + d * nU for d in range(dim)]
+                         + [op.elem_map().astype(np.int64) + dim * nU], axis=1)
     assert np.array_equal(prob.emap, ref.astype(np.uint32))
-    assert prob.ndofs == 2 * nU + op.ndofs
-    dofs, vals, flag, full = prob.dirichlet([("bottom", (0.0, 2.0)), ("top", (1.0, 0.0), (0,)),
-                                             ("left", (3.0, 4.0))], pressure=((2, 0.5),))
+    assert prob.ndofs == dim * nU + op.ndofs
+    assert np.array_equal(prob.pressure_coords().view(np.uint64), op.dof_coords().view(np.uint64))
+    v = lambda *c: tuple(c[:dim])  # noqa: E731
+    dofs, vals, flag, full = prob.dirichlet([("bottom", v(0.0, 2.0, 0.0)), ("top", v(1.0, 0.0, 0.0), (0,)),
+                                             ("left", v(3.0, 4.0, 5.0))], pressure=((2, 0.5),))
     left = np.asarray(ov.bc_dofs("left"), np.int64)
     top = np.asarray(ov.bc_dofs("top"), np.int64)
     assert np.all(full[prob.vel_id(left, 0)] == 3.0) and np.all(full[prob.vel_id(left, 1)] == 4.0)
     top_only = np.setdiff1d(top, left)
     assert np.all(full[prob.vel_id(top_only, 0)] == 1.0) and not flag[prob.vel_id(top_only, 1)].any()
-    assert flag[2 * nU + 2] == 1 and full[2 * nU + 2] == 0.5
+    assert flag[dim * nU + 2] == 1 and full[dim * nU + 2] == 0.5
     assert np.array_equal(dofs, np.nonzero(flag)[0]) and np.array_equal(vals, full[dofs])
