@@ -116,8 +116,12 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
   constexpr bool QM = S::QM;
   bool const needK = (flags & 10) != 0;
 
+  // mapping gradients as [vertex][component][q] and basis values as [node][q]
+  // (sPhiT, for the coefficient at the points): the geometry phase has lane q
+  // read them, consecutive words across lanes and so no bank conflicts
   __shared__ double sGeo[NQ * NPE * DIM];
   __shared__ double sPhi[NQ * N];
+  __shared__ double sPhiT[N * NQ];
   // reference gradients: g_j(q, b) at (q N + j) DIM + b, or at (q DIM + b) N + j
   // for q-major elements
   __shared__ double sGrad[NQ * N * DIM];
@@ -128,9 +132,16 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
 
   int const tid = threadIdx.x;
   for (int k = tid; k < NQ * NPE * DIM; k += blockDim.x)
-    sGeo[k] = geoGradG[k];
+  {
+    int const q = k / (NPE * DIM), vc = k - q * (NPE * DIM); // source: [q][v][c]
+    sGeo[vc * NQ + q] = geoGradG[k];
+  }
   for (int k = tid; k < NQ * N; k += blockDim.x)
+  {
     sPhi[k] = phiG[k];
+    int const q = k / N, i = k - q * N;
+    sPhiT[i * NQ + q] = phiG[k];
+  }
   for (int k = tid; k < NQ * N * DIM; k += blockDim.x)
   {
     if constexpr (QM)
@@ -253,7 +264,7 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
         double gv[DIM];
 #pragma unroll
         for (int c = 0; c < DIM; ++c)
-          gv[c] = sGeo[(q * NPE + v) * DIM + c];
+          gv[c] = sGeo[(v * DIM + c) * NQ + q];
 #pragma unroll
         for (int r = 0; r < 3; ++r)
         {
@@ -299,7 +310,7 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
       if (needK)
 #pragma unroll 3
         for (int i = 0; i < N; ++i)
-          kq += sPhi[q * N + i] * W.kap[i];
+          kq += sPhiT[i * NQ + q] * W.kap[i];
       double s = 0., m = 0.;
       if (flags & 1)
         s += cS * JxW;
