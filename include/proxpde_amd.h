@@ -114,7 +114,8 @@ int pxg_profile(pxg_ctx * ctx, int enable);
 int pxg_profile_get(pxg_ctx * ctx, const char * name, double * total_ms, int64_t * count);
 
 /* FESpace / CurFE: FE type and quadrature rule.  nq = 0 selects the
- * reference's RecommendedQR; otherwise nodes (nq x dim) and weights (nq).
+ * reference's RecommendedQR (of the velocity FE for a Taylor-Hood pair);
+ * otherwise nodes (nq x dim) and weights (nq).
  * Replaces CurFE<RefFE,QR>::CurFE (include/curfe.hpp:20-45).             */
 int pxg_set_fe(pxg_ctx * ctx, int fe_type, int nq, const double * qr_nodes, const double * qr_weights);
 
@@ -146,11 +147,17 @@ int pxg_build_pattern(pxg_ctx * ctx, int64_t * nnz_structural);
  * PXG_FP32 (element kernel and values in float).  Returns without waiting
  * for the GPU: an element with a non-positive Jacobian determinant is
  * reported (PXG_E_NUMERIC) by the next pxg_close, or by the next
- * pxg_assemble if that comes first.                                       */
+ * pxg_assemble if that comes first.  Scalar FE types take STIFFNESS,
+ * STIFFNESS_FE, MASS, MASS_FE; the Taylor-Hood pairs take STIFFNESS,
+ * TENSOR_STIFFNESS, MASS (velocity block), GRAD and DIV, which together are
+ * buildLhs on the velocity plus both buildCoupling calls
+ * (include/builder.hpp:156-277); any other combination is PXG_E_INVALID. */
 int pxg_assemble(pxg_ctx * ctx, int n_terms, const pxg_term * terms, int precision);
 
 /* Builder::buildRhs with AssemblyRhsAnalytic or AssemblyProjection
- * (include/builder.hpp:279-364)                                           */
+ * (include/builder.hpp:279-364); scalar FE types only (a Taylor-Hood
+ * context returns PXG_E_INVALID: its right-hand sides are assembled on the
+ * component spaces)                                                       */
 int pxg_assemble_rhs(pxg_ctx * ctx, int rhs_kind, const double * params, const double * qvalues);
 
 /* quadrature points x_q of every element (n_elem x nq x 3), for host
@@ -204,7 +211,8 @@ int pxg_csr_pack(pxg_ctx * ctx, int enable, int * active);
 
 /* ---- matrix-free y = K(kappa) x and its gradients ------------------------
  * nb vectors interleaved as [n_dofs][nb]; use_bc applies the Dirichlet form
- * C K C + D_c of the assembled matrix.  Geometry is recomputed per call.     */
+ * C K C + D_c of the assembled matrix.  Geometry is recomputed per call.
+ * Scalar FE types only (PXG_E_INVALID for a Taylor-Hood context).          */
 int pxg_mf_apply(pxg_ctx * ctx, const double * kappa_dev, int nb, const double * x_dev, double * y_dev,
                  int use_bc);
 /* gk[n] = sum_b d/dkappa_n ( ybar_b . K(kappa) x_b )                         */
