@@ -14,7 +14,10 @@ correct assembly (reference include/builder.hpp:31-154, 366-371):
   u = x + 2y - z gives 6;
 * the FP32 build applied to a vector agrees with the FP64 build to FP32 accuracy;
 * MG-PCG (levels 64/32/16/8/4) in FP64 and in the FP32-refined form reach the
-  same solution (1e-8 relative residual each).
+  same solution (1e-8 relative residual each);
+* the Taylor-Hood system at the Stokes bench size (1024^2 quads): symmetric for
+  equal grad / div coefficients, exact velocity energy of a linear field,
+  constant pressure in the kernel of the grad block on interior rows.
 Bars are written in each test.
 """
 import numpy as np
@@ -136,3 +139,59 @@ def test_mg_pcg_fp64_and_refined_agree(full, gpu):
     print(f"  relres fp64 {r64:.2e}, fp32-refined {r32:.2e}, |u32 - u64| / |u64| = {diff:.2e}")
     assert r64 <= 1.05e-8 and r32 <= 1.05e-8
     assert diff <= 1e-7
+
+
+def test_stokes_full_size_properties(gpu):
+    """Taylor-Hood Q2-Q1 on 1024^2 quads (the Stokes bench size, 9.4 M dofs):
+    with equal grad and div coefficients the eliminated saddle-point system is
+    symmetric; the velocity block (stiffness 1) gives a linear field u = (x, 2y)
+    its energy 1 + 4 = 5; without Dirichlet data the grad block maps the constant
+    pressure to zero on interior velocity rows (the pressure basis sums to one)."""
+    import torch
+    n = (1024, 1024)
+    sp_ = W.StokesProblem(n)
+    print(f"full size Stokes {n}: {sp_.ndofs} dofs", flush=True)
+    bcs = [("bottom", (0.0, 1.0)), ("right", (0.0, 0.0)), ("top", (0.0, 0.0), (0,)), ("left", (0.0, 0.0), (0,))]
+    dofs, vals, _, _ = sp_.dirichlet(bcs)
+
+    def ctx(with_bc, terms):
+        a = sp_.assembler()
+        a.set_stream(torch.cuda.current_stream().cuda_stream)
+        d = dofs if with_bc else np.zeros(0, np.uint32)
+        a.set_dirichlet(d, vals if with_bc else np.zeros(0))
+        a.build_pattern()
+        a.assemble(terms)
+        a.close(1e-16)
+        return a
+
+    a = ctx(True, [("tensor_stiffness", 0.1), ("grad", -1.0), ("div", -1.0)])
+    g = torch.Generator(device=gpu)
+    g.manual_seed(21)
+    x = torch.rand(sp_.ndofs, dtype=torch.float64, device=gpu, generator=g) * 2 - 1
+    y = torch.rand(sp_.ndofs, dtype=torch.float64, device=gpu, generator=g) * 2 - 1
+    ax, ay = _apply(a, x), _apply(a, y)
+    lhs, rhs = float(torch.dot(y, ax)), float(torch.dot(x, ay))
+    print(f"  y.Ax = {lhs:.15e}, x.Ay = {rhs:.15e}")
+    assert abs(lhs - rhs) <= 1e-12 * float(y.norm() * ax.norm())
+    del a
+    b = ctx(False, [("stiffness", 1.0), ("grad", 1.0)])
+    xv = sp_.dof_coords()
+    u = np.zeros(sp_.ndofs)
+    u[sp_.vel_id(np.arange(sp_.nU), 0)] = xv[:, 0]
+    u[sp_.vel_id(np.arange(sp_.nU), 1)] = 2.0 * xv[:, 1]
+    ut = torch.from_numpy(u).to(gpu)
+    au = _apply(b, ut)
+    nv = 2 * sp_.nU
+    e = float(torch.dot(ut[:nv], au[:nv]))
+    print(f"  velocity energy of (x, 2y): {e:.15f} (exact 5)")
+    # u . A u sums 9.4 M rows whose O(1) terms cancel (interior rows of A u
+    # vanish to rounding): a rounding bar of 1e-10 relative at this size
+    assert abs(e - 5.0) <= 1e-10 * 5.0
+    p1 = torch.zeros(sp_.ndofs, dtype=torch.float64, device=gpu)
+    p1[nv:] = 1.0
+    gp = _apply(b, p1).cpu().numpy()
+    interior = (xv[:, 0] > 1e-12) & (xv[:, 0] < 1 - 1e-12) & (xv[:, 1] > 1e-12) & (xv[:, 1] < 1 - 1e-12)
+    rows = np.concatenate([sp_.vel_id(np.nonzero(interior)[0], d) for d in range(2)])
+    worst = np.abs(gp[rows]).max()
+    print(f"  |G 1| on interior velocity rows: {worst:.2e}")
+    assert worst <= 1e-14
