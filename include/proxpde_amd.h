@@ -184,7 +184,7 @@ int pxg_get_geometry(pxg_ctx * ctx, double * detJ, double * JxW);
 /* ---- operator apply (device pointers, the context's stream) ------------- */
 /* y = A x with the closed matrix (SparseMatrix<RowMajor> * Vec)            */
 int pxg_spmv(pxg_ctx * ctx, int precision, const void * x_dev, void * y_dev);
-/* sweeps of x <- x + omega D^-1 (b - A x)                                  */
+/* sweeps of x <- x + omega D^-1 (b - A x); returns when they are complete */
 int pxg_jacobi(pxg_ctx * ctx, int precision, double omega, int sweeps, const void * b_dev, void * x_dev);
 /* device pointers to the closed CSR arrays (valid until the next close)    */
 int pxg_csr_device(pxg_ctx * ctx, const int64_t ** rowptr, const int32_t ** col, const double ** val,

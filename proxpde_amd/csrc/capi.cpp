@@ -4,6 +4,8 @@
 #include "kernels.hpp"
 
 #include <cstring>
+#include <type_traits>
+#include <utility>
 #include <new>
 
 using namespace pxg;
@@ -470,31 +472,27 @@ int pxg_jacobi(pxg_ctx * c, int precision, double omega, int sweeps, const void 
                    ensureOperator(c, precision == PXG_FP32 ? PXG_FP32 : PXG_FP64);
                    CsrView const A = viewOf(c);
                    int64_t const n = c->ndofs;
+                   // sweeps alternate between x and a buffer of the context; an
+                   // odd count ends with one copy back into x
+                   auto run = [&](auto * xs, auto const * bs, auto & tmp)
+                   {
+                     using T = std::remove_pointer_t<decltype(xs)>;
+                     tmp.resize(n);
+                     T * cur = xs;
+                     T * nxt = tmp.p;
+                     for (int k = 0; k < sweeps; ++k)
+                     {
+                       jacobiStep<T>(A, bs, cur, nxt, omega, c->stream);
+                       std::swap(cur, nxt);
+                     }
+                     if (cur != xs)
+                       PXG_HIP(hipMemcpyAsync(xs, cur, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+                   };
                    if (precision == PXG_FP32)
-                   {
-                     DevBuf<float> tmp;
-                     tmp.resize(n);
-                     float * xs = (float *)x;
-                     for (int k = 0; k < sweeps; ++k)
-                     {
-                       jacobiStep<float>(A, (float const *)b, xs, tmp.p, omega, c->stream);
-                       PXG_HIP(hipMemcpyAsync(xs, tmp.p, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-                     }
-                     PXG_HIP(hipStreamSynchronize(c->stream));
-                   }
+                     run((float *)x, (float const *)b, c->jacTmp32);
                    else
-                   {
-                     DevBuf<double> tmp;
-                     tmp.resize(n);
-                     double * xs = (double *)x;
-                     for (int k = 0; k < sweeps; ++k)
-                     {
-                       jacobiStep<double>(A, (double const *)b, xs, tmp.p, omega, c->stream);
-                       PXG_HIP(
-                           hipMemcpyAsync(xs, tmp.p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-                     }
-                     PXG_HIP(hipStreamSynchronize(c->stream));
-                   }
+                     run((double *)x, (double const *)b, c->jacTmp);
+                   PXG_HIP(hipStreamSynchronize(c->stream));
                  });
 }
 

@@ -212,6 +212,8 @@ struct pxg_ctx
   pxg::DevBuf<int> status;    // device error flag
   bool statusPending = false; // status of the last build not read yet (read by close)
   pxg::DevBuf<int> workCounter; // element counter of the persistent element kernels
+  pxg::DevBuf<double> jacTmp;   // pxg_jacobi: the other vector of the sweeps
+  pxg::DevBuf<float> jacTmp32;
   pxg::DevBuf<unsigned char> sortTmp;
   // kernel timing with HIP events on the context stream (off by default)
   bool prof = false;
