@@ -246,6 +246,10 @@ int pxg_mg_solve(pxg_mg * mg, int mode, const double * b_dev, double * x_dev, do
 int pxg_mg_vcycle(pxg_mg * mg, const double * b_dev, double * x_dev);
 /* prolongation P_l (level l+1 -> l) as host CSR; returns nnz via *nnz     */
 int pxg_mg_prolongation(pxg_mg * mg, int level, int64_t * rowptr, int32_t * col, double * val, int64_t * nnz);
+/* Restrictor of the reference (include/multigrid.hpp:108-175, level l -> l+1):
+ * P_l^T with every row scaled to sum one, as host CSR (the cycle applies
+ * P^T itself, the Galerkin choice for CG)                                  */
+int pxg_mg_restriction(pxg_mg * mg, int level, int64_t * rowptr, int32_t * col, double * val, int64_t * nnz);
 const char * pxg_mg_last_error(const pxg_mg * mg);
 
 #ifdef __cplusplus

@@ -859,4 +859,51 @@ int pxg_mg_prolongation(pxg_mg * m, int l, int64_t * rowptr, int32_t * col, doub
   return PXG_OK;
 }
 
+// Restrictor of the reference (include/multigrid.hpp:108-175): the entries of
+// P^T, every row scaled to sum one (sum in increasing fine index, then each
+// value divided by it).  The cycle itself uses P^T; this is the reference's
+// operator as a host CSR, like pxg_mg_prolongation (rows and columns of
+// constrained dofs are the ones P leaves out).
+int pxg_mg_restriction(pxg_mg * m, int l, int64_t * rowptr, int32_t * col, double * val, int64_t * nnz)
+{
+  if (!m || l < 0 || l + 1 >= m->nlev)
+    return PXG_E_INVALID;
+  auto const & L = m->lv[l];
+  int64_t const nf = (int64_t)L.hPrp.size() - 1, np = (int64_t)L.hPcol.size(), nc = m->lv[l + 1].n;
+  if (nnz)
+    *nnz = np;
+  if (!rowptr && !col && !val)
+    return PXG_OK;
+  std::vector<int64_t> rp(nc + 1, 0);
+  for (int64_t k = 0; k < np; ++k)
+    rp[L.hPcol[k] + 1]++;
+  for (int64_t r = 0; r < nc; ++r)
+    rp[r + 1] += rp[r];
+  std::vector<int32_t> rc(np);
+  std::vector<double> rv(np);
+  std::vector<int64_t> pos(rp.begin(), rp.end() - 1);
+  for (int64_t r = 0; r < nf; ++r)
+    for (int64_t k = L.hPrp[r]; k < L.hPrp[r + 1]; ++k)
+    {
+      int64_t const o = pos[L.hPcol[k]]++;
+      rc[o] = (int32_t)r;
+      rv[o] = L.hPval[k];
+    }
+  for (int64_t r = 0; r < nc; ++r)
+  {
+    double sum = 0.;
+    for (int64_t k = rp[r]; k < rp[r + 1]; ++k)
+      sum += rv[k];
+    for (int64_t k = rp[r]; k < rp[r + 1]; ++k)
+      rv[k] /= sum;
+  }
+  if (rowptr)
+    std::memcpy(rowptr, rp.data(), rp.size() * sizeof(int64_t));
+  if (col)
+    std::memcpy(col, rc.data(), rc.size() * sizeof(int32_t));
+  if (val)
+    std::memcpy(val, rv.data(), rv.size() * sizeof(double));
+  return PXG_OK;
+}
+
 } // extern "C"

@@ -100,6 +100,7 @@ def lib():
         "pxg_mg_solve": [vp, i32, vp, vp, C.c_double, i32, C.POINTER(C.c_int), dp],
         "pxg_mg_vcycle": [vp, vp, vp],
         "pxg_mg_prolongation": [vp, i32, i64p, i32p, dp, i64p],
+        "pxg_mg_restriction": [vp, i32, i64p, i32p, dp, i64p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -419,4 +420,17 @@ class Multigrid:
         val = np.empty(nnz.value)
         self._ck(L.pxg_mg_prolongation(self.h, level, _p(rowptr, C.c_int64), _p(col, C.c_int32), _p(val),
                                        C.byref(nnz)))
+        return rowptr, col, val
+
+    def restriction(self, level):
+        """The reference's Restrictor (rows of P^T scaled to sum one), host CSR."""
+        L = lib()
+        nnz = C.c_int64()
+        self._ck(L.pxg_mg_restriction(self.h, level, None, None, None, C.byref(nnz)))
+        nr = self.levels[level + 1].ndofs
+        rowptr = np.empty(nr + 1, dtype=np.int64)
+        col = np.empty(nnz.value, dtype=np.int32)
+        val = np.empty(nnz.value)
+        self._ck(L.pxg_mg_restriction(self.h, level, _p(rowptr, C.c_int64), _p(col, C.c_int32), _p(val),
+                                      C.byref(nnz)))
         return rowptr, col, val

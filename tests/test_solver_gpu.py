@@ -190,6 +190,36 @@ def test_prolongation_known_answer(native, gpu):
     assert abs(np.linalg.norm(P @ uc) - np.sqrt(3.0) * 3.009757793463) <= 1e-11
 
 
+def test_restriction_known_answer_and_oracle(oracle, native, gpu):
+    """Reference test/restriction.cpp:102 (Q1, 6x4 -> 3x2, u = (1-x)^2,
+    |R u_f| = 1.759929209669, R = rows of P^T scaled to sum one), extruded to
+    hex 6x4x2 -> 3x2x1 where each of the 2 coarse z-layers repeats the 2D
+    values: |R u_f| = sqrt(2) * 1.759929209669.  The exported restriction is
+    also the oracle's (pattern bit-exact, values within 1e-15)."""
+    ctxs = []
+    for n in ((6, 4, 2), (3, 2, 1)):
+        prob = W.Problem("hex_q1", n)
+        a = _torch_stream(prob.assembler())
+        a.set_dirichlet(np.zeros(0, np.uint32), np.zeros(0))
+        a.set_coefficient(np.ones(prob.ndofs))
+        a.build_pattern()
+        a.assemble([("stiffness_fe", 1.0), ("mass", 1.0)])
+        a.close()
+        ctxs.append((prob, a))
+    mg = native.Multigrid([a for _, a in ctxs], [(6, 4, 2), (3, 2, 1)])
+    rp, col, val = mg.restriction(0)
+    nf, nc = ctxs[0][0].ndofs, ctxs[1][0].ndofs
+    R = sp.csr_matrix((val, col, rp), shape=(nc, nf))
+    uf = (1.0 - ctxs[0][0].dof_coords()[:, 0]) ** 2
+    assert abs(np.linalg.norm(R @ uf) - np.sqrt(2.0) * 1.759929209669) <= 1e-11
+    assert np.abs(R @ np.ones(nf) - 1.0).max() <= 1e-14
+    fine = oracle.Space("hex_q1", "cube_hex", (6, 4, 2))
+    coarse = oracle.Space("hex_q1", "cube_hex", (3, 2, 1))
+    orp, ocol, oval = oracle.transfer(fine, coarse, 1)
+    assert np.array_equal(rp, orp) and np.array_equal(col, ocol.astype(np.int32))
+    assert np.abs(val - oval).max() <= 1e-15
+
+
 @pytest.mark.parametrize("fe,ns", [("hex_q1", (16, 8, 4, 2)), ("hex_q2", (8, 4, 2))])
 def test_mg_pcg_fp64_and_fp32_refined(oracle, native, gpu, fe, ns):
     import torch
