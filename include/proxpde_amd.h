@@ -155,9 +155,11 @@ int pxg_build_pattern(pxg_ctx * ctx, int64_t * nnz_structural);
 int pxg_assemble(pxg_ctx * ctx, int n_terms, const pxg_term * terms, int precision);
 
 /* Builder::buildRhs with AssemblyRhsAnalytic or AssemblyProjection
- * (include/builder.hpp:279-364); scalar FE types only (a Taylor-Hood
- * context returns PXG_E_INVALID: its right-hand sides are assembled on the
- * component spaces)                                                       */
+ * (include/builder.hpp:279-364).  A Taylor-Hood context takes the velocity
+ * forcing of AssemblyAnalyticRhs on its vector space: PXG_RHS_CONST with
+ * params[d] the constant of component d, or PXG_RHS_QVALUES with qvalues
+ * n_elem x nq x dim (component fastest) at pxg_get_qpoints() points; the
+ * pressure rows get nothing (other kinds: PXG_E_INVALID)                  */
 int pxg_assemble_rhs(pxg_ctx * ctx, int rhs_kind, const double * params, const double * qvalues);
 
 /* quadrature points x_q of every element (n_elem x nq x 3), for host

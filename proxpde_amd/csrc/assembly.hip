@@ -1452,6 +1452,36 @@ __global__ void __launch_bounds__(kBlock)
   fe[t] = s;
 }
 
+// Taylor-Hood pairs: Fe of velocity row (d, a) = sum_q (JxW_q phi_a(q)) f_d
+// (AssemblyAnalyticRhs on the vector space, the expression of k_rhs_rows per
+// component); pressure rows 0; f_d constant (perPoint 0: fq[d]) or per point
+// (fq[(e nq + q) dim + d]); constrained rows 0
+__global__ void __launch_bounds__(kBlock)
+    k_rhs_rows_pair(int64_t E, int n, int nq, int nv, int dim, double const * __restrict__ wf,
+                    double const * __restrict__ phi, double const * __restrict__ fq, int perPoint,
+                    uint32_t const * __restrict__ emap, uint8_t const * __restrict__ bcFlag, int hasBC,
+                    double * __restrict__ fe)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= E * n)
+    return;
+  int64_t const e = t / n;
+  int const i = (int)(t - e * n);
+  double s = 0.;
+  if (i < dim * nv)
+  {
+    int const d = i / nv;
+    for (int q = 0; q < nq; ++q)
+    {
+      double const f = perPoint ? fq[(e * nq + q) * dim + d] : fq[d];
+      s += (wf[2 * (e * nq + q)] * phi[q * n + i]) * f;
+    }
+  }
+  if (hasBC && bcFlag[emap[t]])
+    s = 0.;
+  fe[t] = s;
+}
+
 __global__ void k_jxw(int64_t m, double const * __restrict__ geo, double * __restrict__ detJ,
                       double * __restrict__ JxW, double const * __restrict__ w, int nq)
 {
@@ -1798,11 +1828,41 @@ void assembleRhs(pxg_ctx * c, int kind, double const * params, double const * qv
   PXG_REQUIRE(c->fe.feType && c->nelem && c->ndofs && c->nnzS, PXG_E_STATE,
               "assemble_rhs needs set_fe, set_mesh, set_dofmap and build_pattern first");
   PXG_REQUIRE(kind >= PXG_RHS_CONST && kind <= PXG_RHS_PROJECTION, PXG_E_INVALID, "rhs kind");
-  PXG_REQUIRE(!isThPair(c->fe.feType), PXG_E_INVALID,
-              "right-hand sides of the Taylor-Hood pair are assembled per component space");
   hipStream_t s = c->stream;
   FeTables & T = c->fe;
   int64_t const E = c->nelem, m = E * T.nq;
+  if (isThPair(T.feType))
+  {
+    // velocity forcing of a pair: constant per component (params[0..dim-1]) or
+    // values per (element, point, component)
+    PXG_REQUIRE(kind == PXG_RHS_CONST || kind == PXG_RHS_QVALUES, PXG_E_INVALID,
+                "a Taylor-Hood context takes PXG_RHS_CONST or PXG_RHS_QVALUES");
+    int const dim = T.dim, nv = T.feType == PXG_FE_QUAD_Q2Q1 ? 9 : (T.feType == PXG_FE_TET_P2P1 ? 10 : 27);
+    bool const perPoint = kind == PXG_RHS_QVALUES;
+    PXG_REQUIRE(perPoint ? qvaluesHost != nullptr : params != nullptr, PXG_E_INVALID, "forcing values required");
+    DevBuf<double> fq;
+    fq.upload(perPoint ? qvaluesHost : params, perPoint ? (size_t)m * dim : (size_t)dim, s);
+    RhsParams const one{{1., 0., 0., 0.}};
+    c->qwork.resize(2 * m);
+    dispatchGeo(T.geo,
+                [&](auto dimC, auto npeC)
+                {
+                  constexpr int D = decltype(dimC)::value, NPE = decltype(npeC)::value;
+                  k_qpoints<D, NPE><<<gridFor(m), kBlock, 0, s>>>(E, T.nq, T.geo, c->xyz.p, c->conn.p,
+                                                                   T.d_geoGrad.p, T.d_xi.p, T.d_w.p, PXG_RHS_CONST,
+                                                                   one, nullptr, nullptr, c->qwork.p);
+                });
+    PXG_HIP(hipGetLastError());
+    c->feBuf.resize(E * T.n);
+    k_rhs_rows_pair<<<gridFor(E * T.n), kBlock, 0, s>>>(E, T.n, T.nq, nv, dim, c->qwork.p, T.d_phi.p, fq.p,
+                                                        perPoint ? 1 : 0, c->elemMap.p, c->bcFlag.p, c->hasBC,
+                                                        c->feBuf.p);
+    PXG_HIP(hipGetLastError());
+    k_rhs_scatter<<<gridFor(c->ndofs), kBlock, 0, s>>>(c->ndofs, c->dofSeg.p, c->dofContrib.p, c->feBuf.p, c->b.p);
+    PXG_HIP(hipGetLastError());
+    PXG_HIP(hipStreamSynchronize(s));
+    return;
+  }
   RhsParams prm{{0., 0., 0., 0.}};
   if (params)
     for (int k = 0; k < 4; ++k)

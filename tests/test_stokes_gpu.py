@@ -215,6 +215,44 @@ def test_stokes3d_known_answer_gpu(gpu, pair):
         assert abs(e - r) < 1e-12
 
 
+@pytest.mark.parametrize("pair,n", [("quad_q2q1", (5, 4)), ("tet_p2p1", (2, 2, 3)), ("hex_q2q1", (3, 2, 2))],
+                         ids=["quad_q2q1", "tet_p2p1", "hex_q2q1"])
+def test_stokes_velocity_forcing_matches_component_spaces(gpu, pair, n):
+    """buildRhs(AssemblyAnalyticRhs) on the vector velocity space: component d
+    of the pair's b equals, bit for bit, the scalar velocity space's b with
+    the forcing of component d (the scalar path is pinned against the oracle);
+    the pressure rows get nothing.  Constant and per-point forcing, with
+    Dirichlet rows (zeroed in both)."""
+    prob = W.StokesProblem(n, pair=pair)
+    dim = prob.dim
+    vfe = VFE[pair][0]
+    dofs, vals, flag, full = prob.dirichlet([("bottom", tuple(0.0 for _ in range(dim)))])
+    a = prob.assembler()
+    a.set_dirichlet(dofs, vals)
+    a.build_pattern()
+    scal = W.Problem(vfe, n, mesh=VFE[pair][1], length=prob.vel.length).assembler()
+    sdofs = prob.vel.bc_dofs(("bottom",))
+    scal.set_dirichlet(sdofs, np.zeros(len(sdofs)))
+    scal.build_pattern()
+    nq = {"quad_q2q1": 9, "tet_p2p1": 11, "hex_q2q1": 27}[pair]
+    rng = np.random.default_rng(4)
+    qv = rng.uniform(-1, 1, (a.nelem, nq, dim))
+    consts = tuple(1.5 - 0.75 * d for d in range(dim))
+    for kind, params, qvalues in (("const", consts, None), ("qvalues", (), qv)):
+        a.clear()
+        a.assemble_rhs(kind, params, qvalues)
+        bp = a.rhs_host()
+        for d in range(dim):
+            scal.clear()
+            if kind == "const":
+                scal.assemble_rhs("const", (consts[d],))
+            else:
+                scal.assemble_rhs("qvalues", (), np.ascontiguousarray(qv[:, :, d]).ravel())
+            bs = scal.rhs_host()
+            assert np.array_equal(bp[prob.vel_id(np.arange(prob.nU), d)].view(np.uint64), bs.view(np.uint64))
+        assert not bp[dim * prob.nU:].any()
+
+
 def test_stokes_rejects_unsupported(gpu):
     prob = W.StokesProblem((3, 2))
     a = prob.assembler()
@@ -222,7 +260,7 @@ def test_stokes_rejects_unsupported(gpu):
     with pytest.raises(N.PxgError):
         a.assemble([("mass_fe", 1.0)])
     with pytest.raises(N.PxgError):
-        a.assemble_rhs("const", (1.0,))
+        a.assemble_rhs("sinsin", (1.0, 1.0, 1.0))
     scalar = W.Problem("quad_q2", (3, 2)).assembler()
     scalar.build_pattern()
     for kind in ("grad", "div", "tensor_stiffness"):
