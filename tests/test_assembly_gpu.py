@@ -64,6 +64,12 @@ CASES = [
     ("tet_p2", "cube_tet6", (3, 3, 3), 0.2, [("stiffness_fe", 1.0), ("mass", 1.0)], "sinsinsin",
      (1.0, PI, PI, PI), None),
     ("tet_p2", "cube_tet5", (2, 3, 2), 0.0, [("stiffness", 1.0)], "const", (1.0,), ("left", "back")),
+    # medium sizes: the persistent matrix-core kernels take many element batches
+    # per wave from their work counters (Q2 hex, P2 tet, Q2 quad shapes)
+    ("hex_q2", "cube_hex", (16, 16, 16), 0.2, [("stiffness_fe", 1.0), ("mass_fe", 0.5)], "sinsinsin",
+     (1.0, PI, PI, PI), ("left", "front")),
+    ("tet_p2", "cube_tet6", (10, 10, 10), 0.2, [("stiffness_fe", 1.0), ("mass", 1.0)], "const", (1.0,), None),
+    ("quad_q2", "square", (64, 64), 0.2, [("stiffness_fe", 1.0), ("mass", 2.0)], "sinsin", (1.0, PI, PI), None),
 ]
 
 
