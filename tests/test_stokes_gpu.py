@@ -61,6 +61,10 @@ CASES = [
     ("hex_q2q1", (3, 2, 2), False, 0.15, {"tensor": 0.1, "grad": -1.0, "div": -1.0}, S3D_BC, ()),
     ("hex_q2q1", (2, 2, 3), True, 0.1, {"stiffness": 0.5, "tensor": 0.2, "mass": 1.0, "grad": 1.0, "div": -0.5},
      MIX3_BC, ((7, -0.5),)),
+    # medium sizes: many element blocks and stage blocks
+    ("quad_q2q1", (48, 40), True, 0.2, {"tensor": 0.1, "mass": 1.0, "grad": -1.0, "div": -1.0}, S2D_BC, ()),
+    ("tet_p2p1", (6, 6, 5), False, 0.15, {"tensor": 0.1, "grad": -1.0, "div": -1.0}, S3D_BC, ()),
+    ("hex_q2q1", (5, 4, 4), False, 0.15, {"stiffness": 0.3, "tensor": 0.1, "grad": -1.0, "div": -1.0}, S3D_BC, ()),
 ]
 VFE = {"quad_q2q1": ("quad_q2", "square"), "tet_p2p1": ("tet_p2", "cube_tet5"), "hex_q2q1": ("hex_q2", "cube_hex")}
 
