@@ -75,7 +75,7 @@ def kappa_apply_batch(args, dev, a, kappa_host, nb=8):
     torch.cuda.synchronize()
     t_fb = (time.perf_counter() - t0) / reps
     E = a.nelem
-    # SURVEY-style algorithmic work of the forward apply: Q [1140 geometry + 2n kappa + nb (12 n + 5)]
+    # algorithmic work of the forward apply (reference formulation): Q [1140 geometry + 2n kappa + nb (12 n + 5)]
     fl = E * 27 * (1140 + 2 * 27 + nb * (12 * 27 + 5))
     return {"batch": nb, "forward_ms": 1e3 * t_fwd, "forward_backward_ms": 1e3 * t_fb,
             "forward_gflops": fl / t_fwd / 1e9}
