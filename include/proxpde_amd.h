@@ -107,6 +107,11 @@ const char * pxg_last_error(const pxg_ctx * ctx);
  * stream); the context creates a non-blocking stream of its own at create  */
 int pxg_set_stream(pxg_ctx * ctx, void * hip_stream);
 int pxg_synchronize(pxg_ctx * ctx);
+/* sizes of the context (any pointer may be NULL): dofs of the dof map (0
+ * before pxg_set_dofmap), mesh elements, local dofs per element, HIP device.
+ * No counterpart in the reference (a Builder is sized by its FESpace); it is
+ * for bindings that check the lengths of the buffers they pass.           */
+int pxg_get_sizes(pxg_ctx * ctx, int64_t * n_dofs, int64_t * n_elem, int * n_local, int * device);
 /* per-kernel timing with HIP events on the context stream: enable resets the
  * totals; names: "geometry", "element", "element32", "scatter", "scatter32",
  * "rhs_scatter", "close", "mf_apply", "mf_gradk", "spmv", "jacobi"            */
@@ -166,7 +171,11 @@ int pxg_assemble_rhs(pxg_ctx * ctx, int rhs_kind, const double * params, const d
  * evaluation of arbitrary right-hand sides (CurFE::qpoint, curfe.hpp:80)  */
 int pxg_get_qpoints(pxg_ctx * ctx, double * out);
 
-/* Builder::closeMatrix (include/builder.hpp:366-371): drop |v| <= eps      */
+/* Builder::closeMatrix (include/builder.hpp:366-371): sum the builds since
+ * the last close or clear, drop |v| <= eps.  Like the reference, which clears
+ * its triplet list here, close uses the builds up: the next pxg_assemble
+ * starts a new matrix (b keeps accumulating until pxg_clear), and a close
+ * with no build since the last one gives the empty matrix.                  */
 int pxg_close(pxg_ctx * ctx, double prune_eps, int64_t * nnz);
 
 /* Builder::clear (include/builder.hpp:381-385): zero values and b          */

@@ -1697,7 +1697,12 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
   if (!rowsOnly && !mixed)
   {
     ProfScope ps(c, precision == PXG_FP64 ? "element" : "element32");
-    fused = elementMfma(c, flags, cS, cSF, cM, cMF, precision);
+    // FP64 linear elements: the reference's evaluation order, so the
+    // rounding residues that closeMatrix prunes come out the same
+    if (precision == PXG_FP64)
+      fused = elementExact(c, nterms, terms);
+    if (!fused)
+      fused = elementMfma(c, flags, cS, cSF, cM, cMF, precision);
   }
   if (!fused && !mixed)
   {
@@ -1954,6 +1959,17 @@ void geometryOut(pxg_ctx * c, double * detJ, double * JxW)
 
 void buildStageMaps(pxg_ctx * c);
 
+// closeMatrix uses up the triplets (reference include/builder.hpp:366-371:
+// setFromTriplets, prune, _triplets.clear()): the next build starts from
+// zero values -- and may write straight into the layout just made -- while b
+// keeps accumulating.  A close with no build since the last one closes an
+// empty triplet list: every entry pruned.
+void consumeValues(pxg_ctx * c)
+{
+  c->valsZero = true;
+  c->valsDirty = false;
+}
+
 void closeMatrix(pxg_ctx * c, double eps)
 {
   PXG_REQUIRE(c->nnzS > 0, PXG_E_STATE, "close needs build_pattern");
@@ -1961,7 +1977,8 @@ void closeMatrix(pxg_ctx * c, double eps)
   int64_t const nr = c->ndofs, ns = c->nnzS;
   bool const f32 = c->valPrecision == PXG_FP32;
   ProfScope ps(c, "close");
-  if (c->valsSplit && eps == kDefaultPrune && c->layoutValid && c->layoutPrecision == c->valPrecision)
+  if (c->valsSplit && !c->valsZero && eps == kDefaultPrune && c->layoutValid &&
+      c->layoutPrecision == c->valPrecision)
   {
     // if every prune flag matches the layout, rowptr and col stand and the
     // kept values are in place: only the inverse diagonal is new.  It is
@@ -1994,6 +2011,7 @@ void closeMatrix(pxg_ctx * c, double eps)
       c->val64Valid = !f32;
       c->closed = true;
       ++c->stamp;
+      consumeValues(c);
       return;
     }
   }
@@ -2071,6 +2089,7 @@ void closeMatrix(pxg_ctx * c, double eps)
   }
   else
     c->stageValid = false;
+  consumeValues(c);
 }
 
 // stage blocks and maps of the pruned layout (k_scatter_staged); left

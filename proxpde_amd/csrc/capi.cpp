@@ -136,6 +136,80 @@ void buildTables(pxg_ctx * c, int feType, int nq, double const * nodes, double c
   T.afragKey64 = T.afragKey32 = 0;
   PXG_HIP(hipStreamSynchronize(s));
 }
+// Everything derived from the dof map (pattern, element->nnz and stage maps,
+// dof adjacency, values, the closed matrix and its packed copy) is dropped
+// when the dof map, the mesh or the FE changes: buffers sized for the old
+// element or dof counts must not meet kernels launched for the new ones.
+void dropPattern(pxg_ctx * c)
+{
+  PXG_HIP(hipStreamSynchronize(c->stream));
+  c->rowptrS.release();
+  c->colS.release();
+  c->segPtr.release();
+  c->contrib.release();
+  c->emapNnz.release();
+  c->diagPos.release();
+  c->diagC.release();
+  c->keep.release();
+  c->pos.release();
+  c->dofSeg.release();
+  c->dofContrib.release();
+  c->stageR0.release();
+  c->stageBlk.release();
+  c->stageInfo.release();
+  c->stageSid.release();
+  c->valS.release();
+  c->valS32.release();
+  c->b.release();
+  c->rowptr.release();
+  c->col.release();
+  c->val.release();
+  c->val32.release();
+  c->dinv.release();
+  c->dinv32.release();
+  c->blockRow.release();
+  c->packBase.release();
+  c->packLo.release();
+  c->packHi.release();
+  c->packVal.release();
+  c->packVal32.release();
+  c->keBuf.release();
+  c->keBuf32.release();
+  c->feBuf.release();
+  c->geoBuf.release();
+  c->mfBuf.release();
+  c->nnzS = 0;
+  c->nnz = 0;
+  c->nStage = 0;
+  c->nStreamBlocks = 0;
+  c->closed = false;
+  c->valsDirty = false;
+  c->valsZero = false;
+  c->keepFresh = false;
+  c->val32Valid = false;
+  c->val64Valid = false;
+  c->layoutValid = false;
+  c->valsSplit = false;
+  c->stageValid = false;
+  c->stagedFused = false;
+  c->packOk = false;
+  c->packStamp = c->packValStamp = c->packVal32Stamp = 0;
+  c->blockStamp = 0;
+  ++c->patStamp;
+  ++c->stamp;
+}
+
+void dropDofs(pxg_ctx * c)
+{
+  dropPattern(c);
+  c->ndofs = 0;
+  c->elemMap.release();
+  c->bcFlag.release();
+  c->bcVal.release();
+  c->kappa.release();
+  c->hasBC = false;
+  c->hasKappa = false;
+}
 } // namespace
 
 extern "C" {
@@ -207,6 +281,22 @@ int pxg_synchronize(pxg_ctx * c)
   return guarded(c, [&] { PXG_HIP(hipStreamSynchronize(c->stream)); });
 }
 
+int pxg_get_sizes(pxg_ctx * c, int64_t * ndofs, int64_t * nelem, int * nlocal, int * device)
+{
+  return guarded(c,
+                 [&]
+                 {
+                   if (ndofs)
+                     *ndofs = c->ndofs;
+                   if (nelem)
+                     *nelem = c->nelem;
+                   if (nlocal)
+                     *nlocal = c->fe.n;
+                   if (device)
+                     *device = c->device;
+                 });
+}
+
 int pxg_profile(pxg_ctx * c, int enable)
 {
   return guarded(c,
@@ -243,7 +333,20 @@ int pxg_set_fe(pxg_ctx * c, int feType, int nq, const double * nodes, const doub
                    PXG_REQUIRE(feType >= PXG_FE_LINE_P1 && feType <= PXG_FE_TET_P2P1, PXG_E_INVALID,
                                "unknown FE type");
                    PXG_REQUIRE(nq >= 0 && nq <= 64, PXG_E_INVALID, "quadrature size must be 0..64");
+                   int const oldN = c->fe.n, oldNpe = c->fe.npe, oldType = c->fe.feType;
                    buildTables(c, feType, nq, nodes, weights);
+                   // another element shape invalidates the mesh, other local
+                   // dofs the dof map; another quadrature rule keeps both
+                   if (oldType && c->fe.npe != oldNpe)
+                   {
+                     dropDofs(c);
+                     c->npts = c->nelem = 0;
+                     c->npe = 0;
+                     c->xyz.release();
+                     c->conn.release();
+                   }
+                   else if (oldType && (c->fe.n != oldN || c->fe.feType != oldType))
+                     dropDofs(c);
                  });
 }
 
@@ -258,6 +361,8 @@ int pxg_set_mesh(pxg_ctx * c, int64_t npts, const double * coords, int64_t nelem
                    PXG_REQUIRE(npts < (int64_t)0xffffffffLL, PXG_E_INVALID, "too many points");
                    for (int64_t k = 0; k < nelem * npe; ++k)
                      PXG_REQUIRE(conn[k] < (uint64_t)npts, PXG_E_INVALID, "connectivity out of range");
+                   if (c->ndofs && nelem != c->nelem)
+                     dropDofs(c); // the dof map has one row per element of the old mesh
                    c->npts = npts;
                    c->nelem = nelem;
                    c->npe = npe;
@@ -277,19 +382,13 @@ int pxg_set_dofmap(pxg_ctx * c, int64_t ndofs, const uint32_t * emap)
                    int64_t const m = c->nelem * c->fe.n;
                    for (int64_t k = 0; k < m; ++k)
                      PXG_REQUIRE(emap[k] < (uint64_t)ndofs, PXG_E_INVALID, "dof id out of range");
+                   dropDofs(c);
                    c->ndofs = ndofs;
                    c->elemMap.upload(emap, m, c->stream);
                    c->bcFlag.resize(ndofs);
                    c->bcVal.resize(ndofs);
                    PXG_HIP(hipMemsetAsync(c->bcFlag.p, 0, ndofs, c->stream));
                    PXG_HIP(hipMemsetAsync(c->bcVal.p, 0, ndofs * sizeof(double), c->stream));
-                   c->hasBC = false;
-                   c->hasKappa = false;
-                   c->nnzS = 0;
-                   c->closed = false;
-                   c->layoutValid = false;
-                   c->valsSplit = false;
-                   ++c->stamp;
                    PXG_HIP(hipStreamSynchronize(c->stream));
                  });
 }

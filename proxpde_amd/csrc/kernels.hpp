@@ -23,6 +23,9 @@ void ensureVal32(pxg_ctx * c);
 void ensureVal64(pxg_ctx * c);
 // element_mfma.hip: fused geometry + MFMA element kernel; false if not applicable
 bool elementMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, double cMF, int precision);
+// element_exact.hip: FP64 linear elements (P1 line, Q1 quad, Q1 hex, P1 tet)
+// in the reference's evaluation order, geometry fused; false if not applicable
+bool elementExact(pxg_ctx * c, int nterms, pxg_term const * terms);
 
 // Row stride of the element-matrix buffers (keBuf, keBuf32): rows of 16 or
 // more entries are padded to a multiple of 16 entries (128 B in FP64, 64 B in

@@ -77,6 +77,14 @@ struct pxg_mg
   DevBuf<int32_t> freeDofs; // free index -> coarse dof
   DevBuf<double> inv, invTmp;
   DevBuf<int> flag;
+  // orders the cycle stream after work issued on the level contexts' own
+  // streams (value casts, packed copies made on first use)
+  hipEvent_t join = nullptr;
+  ~pxg_mg()
+  {
+    if (join)
+      (void)hipEventDestroy(join);
+  }
 };
 
 namespace
@@ -99,6 +107,20 @@ struct Buf<float>
   static float * t(pxg_mg::Level & L) { return L.t32.p; }
   static float * r(pxg_mg::Level & L) { return L.r32.p; }
 };
+
+// work of the levels issued on their own streams (ensureOperator,
+// ensureVal64) completes before the cycle stream reads it
+void joinLevels(pxg_mg * m)
+{
+  for (auto * c: m->ctx)
+    if (c->stream != m->s)
+    {
+      if (!m->join)
+        PXG_HIP(hipEventCreateWithFlags(&m->join, hipEventDisableTiming));
+      PXG_HIP(hipEventRecord(m->join, c->stream));
+      PXG_HIP(hipStreamWaitEvent(m->s, m->join, 0));
+    }
+}
 
 // largest free-dof block inverted densely on the coarsest level (8 n^2 bytes,
 // n Gauss-Jordan steps of n^2 work each)
@@ -391,6 +413,7 @@ void factorCoarse(pxg_mg * m)
   pxg_ctx * c = m->ctx.back();
   hipStream_t s = m->s;
   ensureVal64(c);
+  joinLevels(m);
   if (m->denseStamp == c->stamp && c->stamp != 0)
     return;
   int64_t const n = c->ndofs;
@@ -790,12 +813,14 @@ int pxg_mg_solve(pxg_mg * m, int mode, const double * b, double * x, double rtol
     {
       for (auto * c: m->ctx)
         ensureOperator(c, PXG_FP32);
+      joinLevels(m);
       it = solveIR(m, b, x, rtol, maxit, rr);
     }
     else
     {
       for (auto * c: m->ctx)
         ensureOperator(c, PXG_FP64);
+      joinLevels(m);
       it = pcg<double>(m, b, x, rtol, maxit, rr, false);
     }
     PXG_HIP(hipStreamSynchronize(m->s));
@@ -832,6 +857,7 @@ int pxg_mg_vcycle(pxg_mg * m, const double * b, double * x)
       ensureOperator(c, PXG_FP64);
     }
     factorCoarse(m);
+    joinLevels(m);
     vcycle<double>(m, 0, b, x);
     PXG_HIP(hipStreamSynchronize(m->s));
     return PXG_OK;

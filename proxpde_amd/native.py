@@ -70,6 +70,7 @@ def lib():
         "pxg_destroy": [vp],
         "pxg_set_stream": [vp, vp],
         "pxg_synchronize": [vp],
+        "pxg_get_sizes": [vp, i64p, i64p, C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "pxg_profile": [vp, i32],
         "pxg_profile_get": [vp, C.c_char_p, dp, i64p],
         "pxg_set_fe": [vp, i32, i32, dp, dp],
@@ -360,6 +361,13 @@ class Assembler:
 
     def synchronize(self):
         self._ck(lib().pxg_synchronize(self.h))
+
+    def sizes(self):
+        """(n_dofs, n_local, n_elem, device) of the context (pxg_get_sizes)."""
+        nd, ne = C.c_int64(), C.c_int64()
+        nl, dev = C.c_int(), C.c_int()
+        self._ck(lib().pxg_get_sizes(self.h, C.byref(nd), C.byref(ne), C.byref(nl), C.byref(dev)))
+        return nd.value, nl.value, ne.value, dev.value
 
     def profile(self, enable=True):
         self._ck(lib().pxg_profile(self.h, int(enable)))

@@ -47,8 +47,49 @@ void checkMg(int rc, pxg_mg * m, char const * what)
     throw std::runtime_error(std::string(what) + " failed (" + std::to_string(rc) + "): " + pxg_mg_last_error(m));
 }
 
+// sizes of a context (dofs, device)
+struct Sizes
+{
+  int64_t ndofs = 0;
+  int device = 0;
+};
+
+Sizes sizesOf(pxg_ctx * c)
+{
+  Sizes z;
+  check(pxg_get_sizes(c, &z.ndofs, nullptr, nullptr, &z.device), c, "pxg_get_sizes");
+  return z;
+}
+
+// a float64 GPU tensor: [n] (vectors) or [n] / [n, nb] with 1 <= nb <= 16
+// (right-hand sides); checked before the context is touched
+void checkKind(Tensor const & t, char const * name, bool batched)
+{
+  TORCH_CHECK(t.defined(), name, " is undefined");
+  TORCH_CHECK(t.scalar_type() == torch::kFloat64, name, " must be float64");
+  TORCH_CHECK(t.is_cuda(), name, " must be a GPU tensor");
+  TORCH_CHECK(batched ? (t.dim() == 1 || t.dim() == 2) : t.dim() == 1, name, " has the wrong number of dimensions");
+  if (t.dim() == 2)
+    TORCH_CHECK(t.size(1) >= 1 && t.size(1) <= 16, name, ": 1 to 16 right-hand sides");
+}
+
+// ... on the context's device with one row per dof, so no kernel reads past
+// the end of the buffer or dereferences memory of another device
+void checkDofTensor(Tensor const & t, Sizes const & z, char const * name, bool batched)
+{
+  checkKind(t, name, batched);
+  TORCH_CHECK(t.get_device() == z.device, name, " is on device ", t.get_device(), ", the context on ", z.device);
+  TORCH_CHECK(t.size(0) == z.ndofs, name, " has ", t.size(0), " rows, the context ", z.ndofs, " dofs");
+}
+
 Tensor applyK(pxg_ctx * c, void * stream, Tensor const & kappa, Tensor const & x, bool useBC)
 {
+  checkKind(kappa, "kappa", false);
+  checkKind(x, "x", true);
+  Sizes const z = sizesOf(c);
+  checkDofTensor(kappa, z, "kappa", false);
+  checkDofTensor(x, z, "x", true);
+  TORCH_CHECK(kappa.is_contiguous(), "kappa must be contiguous");
   auto xc = x.contiguous();
   int const nb = xc.dim() == 1 ? 1 : (int)xc.size(1);
   auto y = torch::empty_like(xc);
@@ -60,6 +101,12 @@ Tensor applyK(pxg_ctx * c, void * stream, Tensor const & kappa, Tensor const & x
 
 Tensor gradKappa(pxg_ctx * c, void * stream, Tensor const & x, Tensor const & ybar, bool useBC)
 {
+  checkKind(x, "x", true);
+  checkKind(ybar, "ybar", true);
+  Sizes const z = sizesOf(c);
+  checkDofTensor(x, z, "x", true);
+  checkDofTensor(ybar, z, "ybar", true);
+  TORCH_CHECK(x.sizes() == ybar.sizes(), "x and the output gradient must have the same shape");
   auto xc = x.contiguous();
   auto yc = ybar.contiguous();
   int const nb = xc.dim() == 1 ? 1 : (int)xc.size(1);
@@ -183,9 +230,23 @@ public:
   {
     TORCH_CHECK(kappa.scalar_type() == torch::kFloat64 && b.scalar_type() == torch::kFloat64, "float64 tensors");
     TORCH_CHECK(!levels.empty() && inject.size() + 1 == levels.size(), "one injection map per coarse level");
+    checkKind(kappa, "kappa", false);
+    checkKind(b, "b", false);
     SolveSetup S;
     for (auto v: levels)
       S.levels.push_back(reinterpret_cast<pxg_ctx *>(v));
+    Sizes const z = sizesOf(S.levels[0]);
+    checkDofTensor(kappa, z, "kappa", false);
+    checkDofTensor(b, z, "b", false);
+    for (size_t l = 0; l < inject.size(); ++l)
+    {
+      Sizes const zl = sizesOf(S.levels[l + 1]);
+      auto const & m = inject[l];
+      TORCH_CHECK(m.scalar_type() == torch::kInt64 && m.dim() == 1 && m.is_cuda() && m.get_device() == z.device,
+                  "injection maps: int64 vectors on the context's device");
+      TORCH_CHECK(m.size(0) == zl.ndofs, "injection map ", l, " has ", m.size(0), " entries, level ", l + 1, " ",
+                  zl.ndofs, " dofs");
+    }
     S.inject = inject;
     S.mg = reinterpret_cast<pxg_mg *>(mg);
     S.stream = reinterpret_cast<void *>(stream);
@@ -220,6 +281,7 @@ public:
     // cleared since the forward pass (another solve, a right-hand side)
     if (stampsOf(S) != ctx->saved_data["stamps"].toIntVector())
       assembleLevels(S, kappa);
+    checkDofTensor(grads[0], sizesOf(S.levels[0]), "grad_u", false);
     Tensor lambda = solveWith(S, grads[0]);
     Tensor gk;
     if (ctx->needs_input_grad(0))

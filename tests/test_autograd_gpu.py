@@ -125,3 +125,25 @@ def test_kappa_solve_backward_reassembles_only_when_needed(ag, gpu):
             assert [c.stamp() for c in solver.ctx] == stamps
         grads.append(k.grad.clone())
     assert float((grads[0] - grads[1]).abs().max()) <= 1e-12 * float(grads[0].abs().max())
+
+
+def test_operators_check_lengths_and_batch(ag, gpu):
+    """Tensors whose length is not the context's dof count, or with more than
+    16 right-hand sides, are rejected before any kernel runs."""
+    prob = W.Problem("hex_q1", (2, 2, 2))
+    a = prob.assembler()
+    op = ag.KappaOperator(a)
+    n = prob.ndofs
+    k = torch.ones(n, dtype=torch.float64, device=gpu)
+    with pytest.raises(RuntimeError, match="rows"):
+        op(torch.ones(n - 1, dtype=torch.float64, device=gpu), torch.ones(n, 2, dtype=torch.float64, device=gpu))
+    with pytest.raises(RuntimeError, match="rows"):
+        op(k, torch.ones(n + 3, 2, dtype=torch.float64, device=gpu))
+    with pytest.raises(RuntimeError, match="right-hand sides"):
+        op(k, torch.ones(n, 17, dtype=torch.float64, device=gpu))
+    y = op(k, torch.ones(n, 16, dtype=torch.float64, device=gpu))
+    assert y.shape == (n, 16)
+    solver = ag.KappaSolver("hex_q1", [4, 2])
+    ks = torch.ones(solver.ndofs, dtype=torch.float64, device=gpu)
+    with pytest.raises(RuntimeError, match="rows"):
+        solver(ks, torch.ones(solver.ndofs + 1, dtype=torch.float64, device=gpu))

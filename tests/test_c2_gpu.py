@@ -1,0 +1,116 @@
+"""The Q1 hex workload with kappa = 1 and no Dirichlet data (stiffness into
+CSR, then 100 SpMV applications on x ~ U(-1, 1), seed 0).
+
+On this uniform mesh about a fifth of the structural entries are zero in
+exact arithmetic and come out of the element sums as rounding residues around
+1e-16, the threshold of Builder::closeMatrix's prune(1, 1e-16) (reference
+include/builder.hpp:366-371).  The FP64 element kernel of the linear elements
+evaluates CurFE::reinit and AssemblyStiffness in the oracle's order without
+contraction, and the scatter sums in element order, so the pruned pattern and
+the values must equal the oracle's bit for bit -- at 32^3 against the oracle
+run here, at 128^3 against the digests the oracle wrote into
+tests/golden/c2_q1hex128.json (tests/golden/make_c2_golden.py).
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from proxpde_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def _build(n, precision=0):
+    prob = W.Problem("hex_q1", (n, n, n))
+    a = prob.assembler()
+    a.build_pattern()
+    a.assemble([("stiffness", 1.0)], precision=precision)
+    a.close()
+    return prob, a
+
+
+def _spmv100(a, x, precision=0):
+    """100 applications y = A x through pxg_spmv; all results must agree bitwise."""
+    import torch
+    dt = torch.float64 if precision == 0 else torch.float32
+    xd = torch.from_numpy(x).to("cuda:0", dt)
+    ys = [torch.empty_like(xd) for _ in range(2)]
+    torch.cuda.synchronize()
+    first = None
+    for k in range(100):
+        y = ys[k % 2]
+        a.spmv_dev(xd.data_ptr(), y.data_ptr(), precision)
+        if k == 0:
+            a.synchronize()
+            first = y.clone()
+    a.synchronize()
+    assert torch.equal(ys[1], first), "SpMV results differ between applications"
+    return first.cpu().numpy()
+
+
+def test_c2_q1hex32_bit_exact(oracle, gpu):
+    n = 32
+    print(f"case C2 Q1 hex {n}^3 FP64", flush=True)
+    prob, a = _build(n)
+    o = oracle.Space("hex_q1", "cube_hex", (n, n, n))
+    rp_o, col_o, v_o, _ = o.assemble([("stiffness", 1.0)])
+    rp, col, v, b = a.csr()
+    assert len(v) == len(v_o), f"nnz {len(v)} vs oracle {len(v_o)}"
+    assert np.array_equal(rp, rp_o) and np.array_equal(col, col_o.astype(np.int32))
+    assert np.array_equal(v.view(np.uint64), v_o.view(np.uint64)), "values not bitwise equal"
+    assert not b.any()
+    x = np.random.default_rng(0).uniform(-1.0, 1.0, prob.ndofs)
+    y = _spmv100(a, x)
+    y_o = o.spmv(x)
+    scale = np.abs(v_o).max() * np.abs(x).max()
+    assert np.abs(y - y_o).max() <= 1e-13 * scale
+
+
+def test_c2_q1hex128_matches_oracle_digest(gpu):
+    with open(os.path.join(HERE, "golden", "c2_q1hex128.json")) as f:
+        g = json.load(f)
+    n = g["n"]
+    print(f"case C2 Q1 hex {n}^3 FP64 against the oracle digest", flush=True)
+    prob, a = _build(n)
+    rp, col, v, _ = a.csr()
+    assert prob.ndofs == g["ndofs"]
+    assert len(v) == g["nnz"], f"nnz {len(v)} vs oracle {g['nnz']}"
+    assert _digest(rp.astype(np.int64)) == g["rowptr_int64_sha256"]
+    assert _digest(col.astype(np.int32)) == g["col_int32_sha256"]
+    assert _digest(v.astype("<f8")) == g["val_float64_sha256"], "values not bitwise the oracle's"
+    # 100 SpMVs against the same CSR product on the host; stiffness without
+    # Dirichlet data annihilates constants (row sums of a uniform kappa = 1)
+    x = np.random.default_rng(0).uniform(-1.0, 1.0, prob.ndofs)
+    y = _spmv100(a, x)
+    A = sp.csr_matrix((v, col, rp), shape=(prob.ndofs, prob.ndofs))
+    scale = np.abs(v).max()
+    assert np.abs(y - A @ x).max() <= 1e-13 * scale
+    one = _spmv100(a, np.ones(prob.ndofs))
+    assert np.abs(one).max() <= 1e-13 * scale
+    # FP32 element kernels: the exact-arithmetic pattern (a subset of the
+    # FP64 one, whose extra entries are rounding residues), values within
+    # FP32 rounding of FP64
+    print(f"case C2 Q1 hex {n}^3 FP32", flush=True)
+    _, a32 = _build(n, precision=1)
+    rp32, col32, v32, _ = a32.csr(fp32=True)
+    A32 = sp.csr_matrix((v32.astype(np.float64), col32, rp32), shape=A.shape)
+    D = (A - A32).tocsr()
+    assert len(v32) <= len(v)
+    assert abs(D).max() <= 2e-7 * scale
+    sig = A.copy()
+    sig.data[np.abs(sig.data) <= 1e-15 * scale] = 0.0
+    sig.eliminate_zeros()
+    assert np.array_equal(sig.indptr, rp32) and np.array_equal(sig.indices, col32), \
+        "FP32 pattern is not the FP64 one without its rounding residues"
+    y32 = _spmv100(a32, x.astype(np.float32), precision=1)
+    assert np.abs(y32 - A32 @ x.astype(np.float32)).max() <= 1e-5 * scale
+    print(f"  nnz FP64 {len(v)} (oracle {g['nnz']}), FP32 {len(v32)}")

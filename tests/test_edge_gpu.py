@@ -12,6 +12,7 @@
 * SpMV and Jacobi on the closed one-element matrices against the oracle.
 """
 import numpy as np
+from _tol import entrywise
 import pytest
 import scipy.sparse as sp
 
@@ -34,9 +35,9 @@ ALL = {1: ("left", "right"), 2: ("bottom", "right", "top", "left"),
        3: ("bottom", "right", "top", "left", "back", "front")}
 
 
-def _rel_ok(a, b, tol=1e-12):
-    scale = np.maximum(np.abs(b), 1e-4 * np.abs(b).max())
-    return np.abs(a - b).max() <= tol * scale.max()
+def _rel_ok(a, b, rowptr=None, tol=1e-12):
+    """Entry-wise bound of tests/_tol.py (per matrix row when rowptr is given)."""
+    return entrywise(a, b, rowptr, tol)[0]
 
 
 def _setup(oracle, fe, mesh, n, sides):
@@ -70,7 +71,7 @@ def test_single_cell_meshes(oracle, gpu, fe, mesh, n, bc):
     first = a.csr()
     rp, col, v, b = first
     assert np.array_equal(rp, rp_o) and np.array_equal(col, col_o.astype(np.int32))
-    assert _rel_ok(v, v_o) and _rel_ok(b, b_o)
+    assert _rel_ok(v, v_o, rp_o) and _rel_ok(b, b_o)
     # rebuild after clear: the staged path into the closed layout
     for _ in range(2):
         a.clear()
@@ -173,3 +174,57 @@ def test_spmv_and_jacobi_single_cell(oracle, gpu, fe, mesh, n):
     a.jacobi_dev(bt.data_ptr(), zt.data_ptr(), 2.0 / 3.0, 2)
     a.synchronize()
     assert np.abs(zt.cpu().numpy() - yj).max() <= 1e-13
+
+
+def test_new_mesh_fe_or_dofmap_drops_derived_state(gpu):
+    """A mesh with another element count drops the dof map and everything
+    built from it; a new dof map drops the pattern and the dof adjacency of
+    the matrix-free operator; another element drops the mesh.  The next calls
+    report PXG_E_STATE (-5) or rebuild, instead of launching over buffers
+    sized for the old mesh."""
+    import torch
+    small, big = W.Problem("hex_q1", (3, 3, 3)), W.Problem("hex_q1", (4, 3, 3))
+    a = small.assembler()
+    a.build_pattern()
+    a.assemble([("stiffness", 1.0)])
+    a.close()
+    a.set_mesh(big.xyz, big.conn)
+    with pytest.raises(N.PxgError) as err:
+        a.assemble([("stiffness", 1.0)])
+    assert err.value.code == -5
+    a.set_dofmap(big.ndofs, big.emap)
+    a.build_pattern()
+    a.assemble([("stiffness", 1.0)])
+    a.close()
+    fresh = big.assembler()
+    fresh.build_pattern()
+    fresh.assemble([("stiffness", 1.0)])
+    fresh.close()
+    for x, y in zip(a.csr(), fresh.csr()):
+        assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
+    # matrix-free: first on the small mesh, then on the big one through the
+    # same context (its dof adjacency must be rebuilt for the new map)
+    m = small.assembler()
+    m.set_stream(torch.cuda.current_stream().cuda_stream)
+    for prob, first in ((small, True), (big, False)):
+        if not first:
+            m.set_mesh(prob.xyz, prob.conn)
+            m.set_dofmap(prob.ndofs, prob.emap)
+        k = torch.ones(prob.ndofs, dtype=torch.float64, device=gpu)
+        x = torch.linspace(-1.0, 1.0, 2 * prob.ndofs, dtype=torch.float64, device=gpu).reshape(prob.ndofs, 2)
+        y = torch.empty_like(x)
+        m.mf_apply_dev(k.data_ptr(), 2, x.data_ptr(), y.data_ptr())
+        m.synchronize()
+        ref = prob.assembler()
+        ref.set_stream(torch.cuda.current_stream().cuda_stream)
+        yr = torch.empty_like(x)
+        ref.mf_apply_dev(k.data_ptr(), 2, x.data_ptr(), yr.data_ptr())
+        ref.synchronize()
+        assert torch.equal(y, yr)
+    # another element shape: the mesh goes too
+    a.set_fe("tet_p1")
+    with pytest.raises(N.PxgError) as err:
+        a.assemble([("stiffness", 1.0)])
+    assert err.value.code == -5
+    nd, nl, ne, dev = a.sizes()
+    assert nd == 0 and ne == 0 and nl == 4

@@ -21,9 +21,9 @@ def _torch_stream(a):
     return a
 
 
-def _hex_level(fe, n, kappa_fn, sides=True):
+def _hex_level(fe, n, kappa_fn, sides=True, own_stream=False):
     prob = W.Problem(fe, (n, n, n))
-    a = _torch_stream(prob.assembler())
+    a = prob.assembler() if own_stream else _torch_stream(prob.assembler())
     dofs = prob.all_sides() if sides else np.zeros(0, np.uint32)
     a.set_dirichlet(dofs, np.zeros(len(dofs)))
     kap = kappa_fn(prob.dof_coords())
@@ -317,3 +317,26 @@ def test_gradient_central_differences(gpu):
     fd = (L(kap + h * d) - L(kap - h * d)) / (2 * h)
     an = float(gt.cpu().numpy() @ d)
     assert abs(fd - an) <= 1e-6 * max(1.0, abs(an))
+
+
+def test_mg_levels_on_their_own_streams(native, gpu):
+    """Every level context keeps the non-blocking stream it was created with:
+    the values each level casts and packs on its own stream on first use must
+    be complete before the cycle (on the finest level's stream) reads them.
+    The solves equal, bit for bit, those of levels sharing one stream."""
+    import torch
+    out = []
+    for own in (False, True):
+        gl = [_hex_level("hex_q1", n, _kappa, own_stream=own) for n in (16, 8, 4, 2)]
+        mg = native.Multigrid([a for _, a, _, _ in gl], [(n, n, n) for n in (16, 8, 4, 2)])
+        a0 = gl[0][1]
+        res = []
+        for mode in (0, 1, 0):
+            xt = torch.zeros(gl[0][0].ndofs, dtype=torch.float64, device=gpu)
+            torch.cuda.synchronize()
+            it, rr = mg.solve_dev(a0.rhs_ptr(), xt.data_ptr(), rtol=1e-8, maxit=200, mode=mode)
+            assert rr <= 1e-8
+            res.append((it, xt.cpu().numpy()))
+        out.append(res)
+    for (it0, x0), (it1, x1) in zip(out[0], out[1]):
+        assert it0 == it1 and np.array_equal(x0, x1)
