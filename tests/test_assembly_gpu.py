@@ -1,7 +1,7 @@
 """Parity of the HIP assembly path (through the C ABI) with the oracle.
 
 Bar: sparsity pattern, element->nnz map and dof maps bit-exact; FP64 values of
-A and b within 1e-12 relative entry by entry (|d| <= 1e-12 * max(|ref|, 1e-3 *
+A and b within 1e-12 relative entry by entry (|d| <= 1e-12 * max(|ref|, 0.1 *
 the largest |ref| of the row; of the vector for b), tests/_tol.py);
 the reference's known-answer error norms reproduced at abs. 1e-12 with the
 GPU-assembled system; FP32 element kernels reported against the FP64 oracle.

@@ -106,8 +106,13 @@ def test_c2_q1hex128_matches_oracle_digest(gpu):
     D = (A - A32).tocsr()
     assert len(v32) <= len(v)
     assert abs(D).max() <= 2e-7 * scale
+    # the entries of the exact-arithmetic pattern are at least h / 12 (1/32 of
+    # the largest); FP64 rounding residues kept by the prune are far below
     sig = A.copy()
-    sig.data[np.abs(sig.data) <= 1e-15 * scale] = 0.0
+    resid = np.abs(sig.data) <= 1e-10 * scale
+    big = np.abs(sig.data[resid]).max() if resid.any() else 0.0
+    print(f"  FP64 rounding residues kept: {int(resid.sum())}, largest {big:.2e}")
+    sig.data[resid] = 0.0
     sig.eliminate_zeros()
     assert np.array_equal(sig.indptr, rp32) and np.array_equal(sig.indices, col32), \
         "FP32 pattern is not the FP64 one without its rounding residues"
