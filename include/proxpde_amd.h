@@ -197,6 +197,19 @@ int pxg_get_geometry(pxg_ctx * ctx, double * detJ, double * JxW);
 int pxg_spmv(pxg_ctx * ctx, int precision, const void * x_dev, void * y_dev);
 /* sweeps of x <- x + omega D^-1 (b - A x); returns when they are complete */
 int pxg_jacobi(pxg_ctx * ctx, int precision, double omega, int sweeps, const void * b_dev, void * x_dev);
+/* pxg_jacobi that allocates nothing, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * other vector of the sweeps is a device array of 2^22 doubles (32 MiB) per
+ * GPU, made when the library's code object loads on that GPU and held until
+ * the process ends; a matrix with more than 4,194,304 rows is rejected
+ * (PXG_E_INVALID).  The closed matrix is applied as it is: its row blocks
+ * and packed copy only if an earlier pxg_spmv / pxg_jacobi made them
+ * current, the plain CSR kernels otherwise (same results).  Returns without
+ * waiting for the GPU.  Every call on one GPU shares the array and leaves
+ * its last contents there: calls that overlap in time on one GPU (other
+ * streams, other threads) race.                                            */
+int pxg_jacobi_noalloc(pxg_ctx * ctx, int precision, double omega, int sweeps, const void * b_dev, void * x_dev);
 /* device pointers to the closed CSR arrays (valid until the next close)    */
 int pxg_csr_device(pxg_ctx * ctx, const int64_t ** rowptr, const int32_t ** col, const double ** val,
                    const float ** val32, int64_t * nnz);

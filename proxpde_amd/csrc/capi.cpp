@@ -4,16 +4,50 @@
 #include "kernels.hpp"
 
 #include <cstring>
+#include <mutex>
+#include <new>
 #include <type_traits>
 #include <utility>
-#include <new>
+#include <vector>
 
 using namespace pxg;
+
+// Second vector of the sweeps of pxg_jacobi_noalloc: one copy per GPU, made
+// when this code object is loaded on the device and held until the process
+// ends.  In host code the name is a host-side stand-in, not a device
+// address: host code uses only the address resolved per device at
+// pxg_create (noAllocScratch).
+constexpr int64_t kNoAllocMaxRows = int64_t(1) << 22; // 4,194,304 entries, 32 MiB of doubles
+static __device__ __align__(256) double scratch[kNoAllocMaxRows];
 
 namespace
 {
 // message of the last failure that had no context to hold it (pxg_create)
 thread_local std::string g_lastError;
+
+// device address of `scratch` per device, resolved once per device by the
+// first pxg_create on it (with that device current)
+std::mutex g_scratchMu;
+std::vector<void *> g_scratch;
+
+void resolveNoAllocScratch(int device)
+{
+  std::lock_guard<std::mutex> lk(g_scratchMu);
+  if ((int)g_scratch.size() > device && g_scratch[device])
+    return;
+  void * p = nullptr;
+  PXG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  PXG_REQUIRE(p, PXG_E_HIP, "no device address for the scratch vector");
+  if ((int)g_scratch.size() <= device)
+    g_scratch.resize(device + 1, nullptr);
+  g_scratch[device] = p;
+}
+
+void * noAllocScratch(int device)
+{
+  std::lock_guard<std::mutex> lk(g_scratchMu);
+  return device >= 0 && device < (int)g_scratch.size() ? g_scratch[device] : nullptr;
+}
 
 template <typename F>
 int guarded(pxg_ctx * c, F && f)
@@ -235,6 +269,7 @@ int pxg_create(pxg_ctx ** out, int device)
                      PXG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
                      c->ownStream = true;
                      c->status.resize(1);
+                     resolveNoAllocScratch(device);
                    });
   if (rc != PXG_OK)
   {
@@ -592,6 +627,46 @@ int pxg_jacobi(pxg_ctx * c, int precision, double omega, int sweeps, const void 
                    else
                      run((double *)x, (double const *)b, c->jacTmp);
                    PXG_HIP(hipStreamSynchronize(c->stream));
+                 });
+}
+
+int pxg_jacobi_noalloc(pxg_ctx * c, int precision, double omega, int sweeps, const void * b, void * x)
+{
+  return guarded(c,
+                 [&]
+                 {
+                   PXG_REQUIRE(c->closed, PXG_E_STATE, "close before jacobi");
+                   PXG_REQUIRE(b && x && b != x && sweeps >= 0, PXG_E_INVALID, "jacobi arguments");
+                   int64_t const n = c->ndofs;
+                   PXG_REQUIRE(n <= kNoAllocMaxRows, PXG_E_INVALID,
+                               "pxg_jacobi_noalloc takes at most 4194304 rows (its scratch vector)");
+                   void * scr = noAllocScratch(c->device);
+                   PXG_REQUIRE(scr, PXG_E_STATE, "scratch vector not resolved on this device");
+                   // values in the asked precision (a cast into the closed
+                   // buffers, no allocation); row blocks and the packed copy
+                   // are used only if current, never built here
+                   if (precision == PXG_FP32)
+                     ensureVal32(c);
+                   else
+                     ensureVal64(c);
+                   CsrView const A = viewOf(c);
+                   auto run = [&](auto * xs, auto const * bs)
+                   {
+                     using T = std::remove_pointer_t<decltype(xs)>;
+                     T * cur = xs;
+                     T * nxt = static_cast<T *>(scr);
+                     for (int k = 0; k < sweeps; ++k)
+                     {
+                       jacobiStep<T>(A, bs, cur, nxt, omega, c->stream);
+                       std::swap(cur, nxt);
+                     }
+                     if (cur != xs)
+                       PXG_HIP(hipMemcpyAsync(xs, cur, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+                   };
+                   if (precision == PXG_FP32)
+                     run((float *)x, (float const *)b);
+                   else
+                     run((double *)x, (double const *)b);
                  });
 }
 

@@ -89,6 +89,7 @@ def lib():
         "pxg_get_geometry": [vp, dp, dp],
         "pxg_spmv": [vp, i32, vp, vp],
         "pxg_jacobi": [vp, i32, C.c_double, i32, vp, vp],
+        "pxg_jacobi_noalloc": [vp, i32, C.c_double, i32, vp, vp],
         "pxg_csr_device": [vp, C.POINTER(i64p), C.POINTER(i32p), C.POINTER(dp), C.POINTER(fp), i64p],
         "pxg_rhs_device": [vp, C.POINTER(dp)],
         "pxg_matrix_stamp": [vp, C.POINTER(C.c_uint64)],
@@ -347,6 +348,10 @@ class Assembler:
 
     def jacobi_dev(self, b_ptr, x_ptr, omega, sweeps, precision=FP64):
         self._ck(lib().pxg_jacobi(self.h, precision, omega, sweeps, b_ptr, x_ptr))
+
+    def jacobi_noalloc_dev(self, b_ptr, x_ptr, omega, sweeps, precision=FP64):
+        """pxg_jacobi_noalloc: asynchronous on the context stream, no allocation."""
+        self._ck(lib().pxg_jacobi_noalloc(self.h, precision, omega, sweeps, b_ptr, x_ptr))
 
     def mf_apply_dev(self, kappa_ptr, nb, x_ptr, y_ptr, use_bc=False):
         self._ck(lib().pxg_mf_apply(self.h, kappa_ptr, nb, x_ptr, y_ptr, int(use_bc)))
