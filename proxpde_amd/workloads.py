@@ -23,7 +23,8 @@ MESH_OF_FE = {"line_p1": "line", "quad_q1": "square", "quad_q2": "square", "hex_
 
 
 def perturb_interior(xyz, h, seed, amount=0.2, origin=(0.0, 0.0, 0.0), length=(1.0, 1.0, 1.0)):
-    """Move vertices off the boundary by U(-amount*h, amount*h) per coordinate."""
+    """Move vertices off the boundary by U(-amount*h, amount*h) per coordinate
+    (coordinates whose length is 0, the normal of a planar mesh, stay put)."""
     rng = np.random.default_rng(seed)
     d = rng.uniform(-amount * h, amount * h, size=xyz.shape)
     o = np.asarray(origin)
@@ -34,6 +35,7 @@ def perturb_interior(xyz, h, seed, amount=0.2, origin=(0.0, 0.0, 0.0), length=(1
         if ln[k] > 0:
             on_bd |= np.abs(xyz[:, k] - o[k]) < tol
             on_bd |= np.abs(xyz[:, k] - o[k] - ln[k]) < tol
+    d[:, ln == 0] = 0.0  # a planar (or line) mesh stays in its plane
     out = xyz.copy()
     out[~on_bd] += d[~on_bd]
     return out

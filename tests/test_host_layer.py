@@ -123,3 +123,33 @@ def test_stokes_dofmap_matches_oracle(oracle, native, interleaved, pair, vfe, pf
     assert np.all(full[prob.vel_id(top_only, 0)] == 1.0) and not flag[prob.vel_id(top_only, 1)].any()
     assert flag[dim * nU + 2] == 1 and full[dim * nU + 2] == 0.5
     assert np.array_equal(dofs, np.nonzero(flag)[0]) and np.array_equal(vals, full[dofs])
+
+
+def _hand_cases():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dofmap_hand.json")) as f:
+        d = json.load(f)
+    return [(k, v) for k, v in d.items() if isinstance(v, dict)]
+
+
+@pytest.mark.parametrize("name,case", _hand_cases(), ids=[c[0] for c in _hand_cases()])
+def test_dofmap_matches_hand_derived_numbering(oracle, native, name, case):
+    """The first-visit numbering of the product's host layer and of the oracle
+    equal elemMap tables derived by hand from the reference's
+    DOF::setupElemMap (tests/golden/dofmap_hand.json: the visit order of
+    points, edges, faces and interiors is spelled out there step by step)."""
+    n = tuple(case["n"])
+    ln = LEN.get(case["mesh"], (1.0, 1.0, 1.0))
+    m = native.HostMesh(case["mesh"], n, (0.0, 0.0, 0.0), ln)
+    _, conn = m.arrays()
+    assert np.array_equal(conn, np.array(case["conn"], dtype=np.uint32))
+    nd, emap = m.dofmap(case["fe"])
+    want = np.array(case["elem_map"], dtype=np.uint32)
+    assert nd == case["ndofs"]
+    assert np.array_equal(emap, want)
+    o = oracle.Space(case["fe"], case["mesh"], n, length=ln)
+    _, co = o.mesh()
+    assert np.array_equal(co, np.array(case["conn"], dtype=np.uint32))
+    assert o.ndofs == case["ndofs"]
+    assert np.array_equal(o.elem_map(), want)

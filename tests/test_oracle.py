@@ -189,3 +189,86 @@ def test_patch_quadratic_exact(oracle, fe, mesh, pert):
     rp, col, v, b = s.assemble([("stiffness", 1.0)], bc=(flag, u), rhs="const", rhs_params=(-2.0 * dim,))
     uh = _solve(rp, col, v, b)
     assert np.abs(uh - u).max() < 1e-11
+
+
+def _lagrange_1d(nodes, a):
+    """1D Lagrange polynomial of node a over the given nodes (numpy Polynomial)."""
+    from numpy.polynomial import Polynomial as P
+    p = P([1.0])
+    for k, x in enumerate(nodes):
+        if k != a:
+            p = p * P([-x, 1.0]) / (nodes[a] - x)
+    return p
+
+
+def _mass_1d(nodes, weight):
+    """int_0^1 weight(x) l_a(x) l_b(x) dx, exact polynomial integration."""
+    n = len(nodes)
+    out = np.empty((n, n))
+    for a in range(n):
+        for b in range(n):
+            q = (weight * _lagrange_1d(nodes, a) * _lagrange_1d(nodes, b)).integ()
+            out[a, b] = q(1.0) - q(0.0)
+    return out
+
+
+@pytest.mark.parametrize("fe,mesh,nodes", [("hex_q1", "cube_hex", [0.0, 1.0]),
+                                           ("hex_q2", "cube_hex", [0.0, 1.0, 0.5]),
+                                           ("quad_q1", "square", [0.0, 1.0]),
+                                           ("quad_q2", "square", [0.0, 1.0, 0.5])])
+def test_mass_fe_single_cell_closed_form(oracle, fe, mesh, nodes):
+    """AssemblyMassFE (reference include/assembly_lhs.hpp:233-283) on the unit
+    cell with kappa = 2 + x (its own interpolant) and coefficient 1.5:
+    M_ij = 1.5 int (2 + x) phi_i phi_j is a tensor product of 1D integrals,
+    the x factor weighted by 2 + x, each integrated exactly (the Gauss rules
+    are exact for these degrees, up to the reference's 3-point abscissa
+    truncated to 0.774596669241483, src/qr.cpp:38, worth a few 1e-15).  Every
+    entry is pinned, so a dropped coefficient, a swapped index or a kappa
+    taken at the wrong node fails."""
+    from numpy.polynomial import Polynomial as P
+    dim = 3 if mesh == "cube_hex" else 2
+    n = (1,) * dim
+    o = oracle.Space(fe, mesh, n, length=(1.0, 1.0, 1.0) if dim == 3 else (1.0, 1.0, 0.0))
+    xd = o.dof_coords()
+    kappa = 2.0 + xd[:, 0]
+    rp, col, v, _ = o.assemble([("mass_fe", 1.5)], kappa=kappa, prune=-1.0)
+    M = sp.csr_matrix((v, col.astype(np.int64), rp), shape=(o.ndofs, o.ndofs)).toarray()
+    Ix = _mass_1d(nodes, P([2.0, 1.0]))
+    I1 = _mass_1d(nodes, P([1.0]))
+    role = {x: k for k, x in enumerate(nodes)}
+    r = [[role[round(float(c), 12)] for c in p[:dim]] for p in xd]
+    want = np.empty_like(M)
+    for i in range(o.ndofs):
+        for j in range(o.ndofs):
+            w = 1.5 * Ix[r[i][0], r[j][0]]
+            for d in range(1, dim):
+                w *= I1[r[i][d], r[j][d]]
+            want[i, j] = w
+    assert np.abs(M - want).max() <= 5e-15 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("fe,mesh,n,pert", [("hex_q2", "cube_hex", (3, 2, 2), 0.2),
+                                            ("hex_q1", "cube_hex", (4, 3, 2), 0.2),
+                                            ("tet_p2", "cube_tet6", (2, 2, 2), 0.2),
+                                            ("quad_q2", "square", (4, 3), 0.2)])
+def test_mass_fe_integrates_linear_kappa(oracle, fe, mesh, n, pert):
+    """1^T M_kappa 1 = int kappa (partition of unity) for kappa = 0.5 + x - 2y
+    on perturbed meshes of the unit cube / square: the interpolant reproduces
+    kappa, and the Gauss rules integrate kappa detJ exactly, so the sum of
+    all entries is c (3.5 + 1/2 - 1) for kappa = 3.5 + x - 2y and coefficient
+    c.  Also M_{c 1} = c M."""
+    from proxpde_amd import workloads as W
+    dim = len(n)
+    ln = (1.0, 1.0, 1.0) if dim == 3 else (1.0, 1.0, 0.0)
+    o = oracle.Space(fe, mesh, n, length=ln)
+    xyz, _ = o.mesh()
+    o.set_coords(W.perturb_interior(xyz, 1.0 / max(n), seed=1, amount=pert, length=ln))
+    xd = o.dof_coords()
+    kappa = 3.5 + xd[:, 0] - 2.0 * xd[:, 1]
+    c = 0.75
+    _, _, v, _ = o.assemble([("mass_fe", c)], kappa=kappa, prune=-1.0)
+    assert abs(v.sum() - c * (3.5 + 0.5 - 1.0)) <= 1e-13
+    rp1, col1, v1, _ = o.assemble([("mass_fe", c)], kappa=np.ones(o.ndofs), prune=-1.0)
+    rp0, col0, v0, _ = o.assemble([("mass", 1.0)], prune=-1.0)
+    assert np.array_equal(rp1, rp0) and np.array_equal(col1, col0)
+    assert np.abs(v1 - c * v0).max() <= 4e-16 * np.abs(v0).max()
