@@ -106,8 +106,10 @@ def solve_and_adjoint(args, dev, N, W):
     # fp32_ir: FP32 element matrices on all but the coarsest level, FP64
     # refinement residual by the matrix-free operator; fp32_ir_fine64: the
     # finest level assembled in FP64 (FP32 copy rounded), residual by that matrix
+    # fp32_ir_mf: the finest level not assembled, applied matrix-free in FP32
+    # (per-point metrics once per kappa), FP64 residual by the FP64 operator
     for name, mode, fine64 in (("fp64", N.FP64, False), ("fp32_ir", N.FP32, False),
-                               ("fp32_ir_fine64", N.FP32, True)):
+                               ("fp32_ir_fine64", N.FP32, True), ("fp32_ir_mf", N.FP32_MF, False)):
         progress(f"solve+adjoint {name}")
         times = []
         for rep in range(args.solve_reps + 1):
@@ -139,7 +141,7 @@ def solve_and_adjoint(args, dev, N, W):
             r = b - op(kap, u.unsqueeze(1)).squeeze(1)
         res[name]["relres_fp64"] = float(r.norm() / b.norm())
     u64, g64, J64 = sol["fp64"]
-    for name in ("fp32_ir", "fp32_ir_fine64"):
+    for name in ("fp32_ir", "fp32_ir_fine64", "fp32_ir_mf"):
         u32, g32, J32 = sol[name]
         res[name]["u_rel_err_vs_fp64"] = float((u32 - u64).norm() / u64.norm())
         res[name]["grad_rel_err_vs_fp64"] = float((g32 - g64).norm() / g64.norm())

@@ -243,6 +243,7 @@ void dropDofs(pxg_ctx * c)
   c->kappa.release();
   c->hasBC = false;
   c->hasKappa = false;
+  ++c->kappaStamp;
 }
 } // namespace
 
@@ -459,6 +460,7 @@ int pxg_set_coefficient(pxg_ctx * c, const double * kappa)
                    PXG_REQUIRE(kappa, PXG_E_INVALID, "null coefficient");
                    c->kappa.upload(kappa, c->ndofs, c->stream);
                    c->hasKappa = true;
+                   ++c->kappaStamp;
                    PXG_HIP(hipStreamSynchronize(c->stream));
                  });
 }
@@ -474,6 +476,7 @@ int pxg_set_coefficient_device(pxg_ctx * c, const double * kappa)
                    PXG_HIP(hipMemcpyAsync(c->kappa.p, kappa, c->ndofs * sizeof(double), hipMemcpyDeviceToDevice,
                                           c->stream));
                    c->hasKappa = true;
+                   ++c->kappaStamp;
                  });
 }
 

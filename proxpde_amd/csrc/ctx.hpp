@@ -135,6 +135,7 @@ struct pxg_ctx
   // coefficient field
   bool hasKappa = false;
   pxg::DevBuf<double> kappa;
+  uint64_t kappaStamp = 0; // counts the times kappa was set (state derived from it checks it)
   // structural pattern and maps
   int64_t nnzS = 0;
   pxg::DevBuf<int64_t> rowptrS;

@@ -27,6 +27,29 @@ bool elementMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, doubl
 // in the reference's evaluation order, geometry fused; false if not applicable
 bool elementExact(pxg_ctx * c, int nterms, pxg_term const * terms);
 
+// matfree_sf.hip: sum-factorised matrix-free kappa-stiffness operator of the
+// Q1 / Q2 hexahedra with their tensor Gauss rules (element buffer [E][n][nb],
+// reduced per dof in element order by the gather)
+bool sfAvailable(pxg_ctx const * c);
+void sfPrepare(pxg_ctx * c);
+template <typename T>
+void sfElementApply(pxg_ctx * c, double const * kappa, int nb, T const * x, T * buf, bool useBC, hipStream_t s);
+void sfElementGrad(pxg_ctx * c, int nb, double const * x, double const * ybar, double * buf, bool useBC,
+                   hipStream_t s);
+template <typename T>
+void sfElementDiag(pxg_ctx * c, double const * kappa, T * buf, hipStream_t s);
+// per-point metrics (kappa, geometry, weights) of every element, [E][6][n],
+// and the apply of one vector that reads them instead of recomputing them
+template <typename T>
+void sfMetric(pxg_ctx * c, double const * kappa, T * met, hipStream_t s);
+template <typename T>
+void sfApplyPA(pxg_ctx * c, T const * met, T const * x, T * buf, bool useBC, hipStream_t s);
+// per-dof gather of an [E][n] element buffer; mode 0: y = Ax, 1: y = b - Ax,
+// 2: y = x + omega dinv (b - Ax), 3: y = 1 / Ax
+template <typename T>
+void sfGather(pxg_ctx * c, int mode, T const * buf, T const * x, T const * b, T const * dinv, double omega, T * y,
+              hipStream_t s);
+
 // Row stride of the element-matrix buffers (keBuf, keBuf32): rows of 16 or
 // more entries are padded to a multiple of 16 entries (128 B in FP64, 64 B in
 // FP32), so an element-matrix row fills whole cache lines and no line holds

@@ -492,7 +492,10 @@ void mfApply(pxg_ctx * c, double const * kappa, int nb, double const * x, double
   uint8_t const * cons = (useBC && c->hasBC) ? c->bcFlag.p : nullptr;
   ProfScope ps(c, "mf_apply");
   PXG_HIP(hipMemsetAsync(c->status.p, 0, sizeof(int), s));
-  launchElem<false>(c, kappa, cons, nb, x, nullptr, c->mfBuf.p);
+  if (sfAvailable(c)) // Q1 / Q2 hexahedra: sum-factorised element kernel
+    sfElementApply<double>(c, kappa, nb, x, c->mfBuf.p, useBC, s);
+  else
+    launchElem<false>(c, kappa, cons, nb, x, nullptr, c->mfBuf.p);
   int64_t const m = c->ndofs * nb;
   k_gather<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(c->ndofs, nb, c->dofSeg.p, c->dofContrib.p, c->mfBuf.p, y);
   PXG_HIP(hipGetLastError());
@@ -512,7 +515,10 @@ void mfGradKappa(pxg_ctx * c, int nb, double const * x, double const * ybar, dou
   uint8_t const * cons = (useBC && c->hasBC) ? c->bcFlag.p : nullptr;
   ProfScope ps(c, "mf_gradk");
   PXG_HIP(hipMemsetAsync(c->status.p, 0, sizeof(int), s));
-  launchElem<true>(c, nullptr, cons, nb, x, ybar, c->mfBuf.p);
+  if (sfAvailable(c))
+    sfElementGrad(c, nb, x, ybar, c->mfBuf.p, useBC, s);
+  else
+    launchElem<true>(c, nullptr, cons, nb, x, ybar, c->mfBuf.p);
   k_gather<<<(unsigned)((c->ndofs + 255) / 256), 256, 0, s>>>(c->ndofs, 1, c->dofSeg.p, c->dofContrib.p, c->mfBuf.p,
                                                               gk);
   PXG_HIP(hipGetLastError());

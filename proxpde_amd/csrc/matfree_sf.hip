@@ -1,0 +1,1056 @@
+// Sum-factorised matrix-free kappa-stiffness operator for the tensor-product
+// hexahedra: Q1 with the 2x2x2 and Q2 with the 3x3x3 Gauss rule (the
+// reference's RecommendedQR, include/fe.hpp).  Same bilinear form as
+// matfree.hip (AssemblyStiffnessFE, include/assembly_lhs.hpp:93-109, with the
+// Dirichlet form C K C + D_c of Builder::buildLhs), same element buffer and
+// per-dof gather, but the contractions run one direction at a time:
+//
+//   basis phi_(i,j,k) = l_i(xi) l_j(eta) l_k(zeta), points q = (qx, qy, qz),
+//   B[q][t] = l_t(g_q), D[q][t] = l_t'(g_q) (P x P, P = 2 or 3)
+//   reference gradient of u at the points:
+//     u_xi = (D x B x B) u, u_eta = (B x D x B) u, u_zeta = (B x B x D) u
+//   v_q = M_q grad u_q,  M_q = kappa_q w_q |det J_q| J_q^-1 J_q^-T
+//   y = (D^T x B^T x B^T) v_xi + (B^T x D^T x B^T) v_eta + (B^T x B^T x D^T) v_zeta
+//
+// 6 P^4 multiply-adds per element and vector against 2 P^6 x 3 for the
+// dense contraction (Q2: 1,458 against 4,374, plus the metric).
+//
+// Lanes: P^2 lanes per element hold one pencil each (P values along one
+// direction); a contraction along the pencil's direction runs in registers,
+// the others after a transpose through the element's LDS region (the lanes
+// of an element are in one wavefront: a wave barrier orders the LDS accesses).
+// Pencil roles: stage x (j, k), stage y (qx, k), stage z (qx, qy).  Q2 packs
+// 7 elements (63 lanes) per wavefront, Q1 16.
+//
+// Geometry: the trilinear vertex map, which is the reference's isoparametric
+// map (the Q2 mapping points are vertex averages, include/reffe.hpp:949-980);
+// computed per point by the lane that owns the point's z pencil and kept in
+// registers across the batch.
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace pxg
+{
+
+namespace
+{
+constexpr int kSfWaves = 4;
+
+__device__ inline void sfSync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// 1D tables of the tensor-product element (lexicographic nodes t = 0..P-1
+// from xi = -1 to +1), Gauss points and weights, and the reference local
+// index of every lexicographic node (i + P (j + P k))
+template <int P>
+struct SfTab
+{
+  double B[P][P];
+  double D[P][P];
+  float Bf[P][P]; // the same in FP32 for the FP32 contractions
+  float Df[P][P];
+  double g[P];
+  double w[P];
+  int local[P * P * P];
+};
+
+// table entries in the precision of the contraction
+template <typename T, int P>
+__device__ inline T tB(SfTab<P> const & t, int q, int i)
+{
+  if constexpr (sizeof(T) == 4)
+    return t.Bf[q][i];
+  else
+    return t.B[q][i];
+}
+template <typename T, int P>
+__device__ inline T tD(SfTab<P> const & t, int q, int i)
+{
+  if constexpr (sizeof(T) == 4)
+    return t.Df[q][i];
+  else
+    return t.D[q][i];
+}
+
+// LDS of one element: two regions of 3 arrays of P^3 values (stage buffers;
+// before the batch loop region 1 holds the vertex coordinates)
+template <int P>
+struct SfLds
+{
+  double r[2][3][P * P * P];
+};
+
+enum : int
+{
+  kSfApply = 0, // y_e = K_e x (element buffer [E][n][nb])
+  kSfGrad = 1,  // g_e = d/dkappa (ybar . K x) per node (element buffer [E][n])
+  kSfDiag = 2,  // diagonal of K_e (element buffer [E][n])
+  kSfMetric = 3, // per-point metrics M_q of every element ([E][6][n], see k_sf_apply_pa)
+};
+
+// Data of one element gathered by a lane of its x pencil (j, k): dof ids,
+// constraint flags and coefficient values of the pencil's P nodes, and the
+// vertex coordinates this lane stages (lanes take the 24 values in turn)
+template <int P>
+struct SfGather
+{
+  static constexpr int CPL = (24 + P * P - 1) / (P * P); // coordinates per lane
+  uint32_t id[P];
+  uint32_t pid[CPL];
+  bool fl[P];
+  double kn[P];
+  double xc[CPL];
+};
+
+// Persistent element kernel: wave w takes the batches w, w + W, ... of EPW
+// consecutive elements (W = waves in the grid) and loads the dof ids of the
+// next batch, then its coordinates, coefficients and flags (nb = 1: also its
+// vector values), while the current batch computes.  NBC vectors of the
+// batch go through the stages one after the other, with their values loaded
+// together and their results written together per node ([E][n][nb], nb
+// contiguous values).
+template <typename T, int P, int MODE, int NBC>
+__global__ void __launch_bounds__(64 * kSfWaves)
+    k_sf_elem(int64_t E, int nb, SfTab<P> tab, double const * __restrict__ xyz, uint32_t const * __restrict__ conn,
+              uint32_t const * __restrict__ emap, double const * __restrict__ kappa,
+              uint8_t const * __restrict__ cons, T const * __restrict__ x, T const * __restrict__ ybar,
+              T * __restrict__ out, int * __restrict__ status)
+{
+  constexpr int P2 = P * P, N = P * P * P, EPW = 64 / P2;
+  using G = SfGather<P>;
+  constexpr int CPL = G::CPL;
+  // a lane past the last whole element of the wave (Q2: lane 63) works on a
+  // region of its own and stores nothing
+  constexpr int SLOTS = (64 + P2 - 1) / P2;
+  __shared__ SfLds<P> lds[kSfWaves * SLOTS];
+  int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int const slot = lane / P2, pl = lane - slot * P2;
+  int const pa = pl % P, pc = pl / P; // the two pencil indices
+  SfLds<P> & L = lds[wave * SLOTS + slot];
+  auto at = [](int i, int j, int k) { return i + P * (j + P * k); };
+  int loc[P]; // reference local index of the lane's x-pencil nodes
+#pragma unroll
+  for (int i = 0; i < P; ++i)
+    loc[i] = tab.local[at(i, pa, pc)];
+  int64_t const nbatch = (E + EPW - 1) / EPW;
+  int64_t const stride = (int64_t)gridDim.x * kSfWaves;
+  int64_t bt = (int64_t)blockIdx.x * kSfWaves + wave;
+
+  auto elemOf = [&](int64_t b) -> int64_t { return b * EPW + slot; };
+  auto liveOf = [&](int64_t b) -> bool { return b < nbatch && slot < EPW && elemOf(b) < E; };
+  // all loads are unconditional, from indices clamped into the arrays (a
+  // lane without an element reads the last one and stores nothing), so the
+  // loads of a batch are in flight together with no branch around them
+  auto loadIds = [&](G & g, int64_t b)
+  {
+    int64_t const e = std::min(elemOf(b), E - 1);
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+      g.id[i] = emap[e * N + loc[i]];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c)
+      g.pid[c] = conn[e * 8 + std::min(pl + c * P2, 23) / 3];
+  };
+  auto loadVals = [&](G & g, int64_t b)
+  {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c)
+      g.xc[c] = xyz[3ull * g.pid[c] + std::min(pl + c * P2, 23) % 3];
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+    {
+      g.fl[i] = cons ? cons[g.id[i]] != 0 : false;
+      if constexpr (MODE != kSfGrad)
+        g.kn[i] = kappa[g.id[i]];
+    }
+  };
+  // vector values of a chunk of NBC vectors starting at b0 (zero past nb)
+  auto loadVec = [&](T const * v, G const & g, int b0, T (&out)[P][NBC])
+  {
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+#pragma unroll
+      for (int c = 0; c < NBC; ++c)
+      {
+        T const t = v[(size_t)g.id[i] * nb + std::min(b0 + c, nb - 1)];
+        out[i][c] = b0 + c < nb ? t : T(0);
+      }
+  };
+
+  G cur, nxt;
+  loadIds(cur, bt);
+  loadVals(cur, bt);
+  T xcur[P][NBC]; // nb = 1: the vector values travel with the prefetch
+  if constexpr (NBC == 1 && MODE == kSfApply)
+    loadVec(x, cur, 0, xcur);
+
+  while (bt < nbatch)
+  {
+    bool const live = liveOf(bt);
+    int64_t const e = elemOf(bt);
+    int64_t const bn = bt + stride;
+    loadIds(nxt, bn);
+    // vertex coordinates to region 1
+#pragma unroll
+    for (int c = 0; c < CPL; ++c)
+    {
+      int const v = pl + c * P2;
+      if (v < 24)
+        L.r[1][0][v] = cur.xc[c];
+    }
+
+    // ---- kappa at the points (apply, diag): interpolation B x B x B ----
+    double kq[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+      kq[q] = 1.;
+    if constexpr (MODE != kSfGrad)
+    {
+#pragma unroll
+      for (int q = 0; q < P; ++q) // x: lane (j, k)
+      {
+        double s = 0.;
+#pragma unroll
+        for (int i = 0; i < P; ++i)
+          s += tab.B[q][i] * cur.kn[i];
+        L.r[0][0][at(q, pa, pc)] = s;
+      }
+      sfSync();
+#pragma unroll
+      for (int q = 0; q < P; ++q) // y: lane (qx, k)
+      {
+        double s = 0.;
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+          s += tab.B[q][j] * L.r[0][0][at(pa, j, pc)];
+        L.r[0][1][at(pa, q, pc)] = s;
+      }
+      sfSync();
+#pragma unroll
+      for (int q = 0; q < P; ++q) // z: lane (qx, qy)
+      {
+        double s = 0.;
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+          s += tab.B[q][k] * L.r[0][1][at(pa, pc, k)];
+        kq[q] = s;
+      }
+    }
+    sfSync();
+
+    // ---- metric of the lane's z pencil: points (qx, qy, qz) = (pa, pc, qz) ----
+    // M = s (J^T J)^-1 = s J^-1 J^-T, s = kappa_q w_q |det J| (grad: without kappa)
+    double M[P][6];
+    {
+      double const * X = L.r[1][0];
+      // vertex v of the reference hexahedron at (vx, vy, vz)
+      constexpr int vid[2][2][2] = {{{0, 1}, {3, 2}}, {{4, 5}, {7, 6}}}; // [vz][vy][vx]
+      double const xi = tab.g[pa], eta = tab.g[pc];
+      double const lx[2] = {0.5 * (1. - xi), 0.5 * (1. + xi)};
+      double const ly[2] = {0.5 * (1. - eta), 0.5 * (1. + eta)};
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+      {
+        double const zeta = tab.g[q];
+        double const lz[2] = {0.5 * (1. - zeta), 0.5 * (1. + zeta)};
+        double J[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+        {
+          double d0 = 0., d1 = 0., d2 = 0.;
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+            {
+              d0 += ly[a] * lz[b] * (X[3 * vid[b][a][1] + r] - X[3 * vid[b][a][0] + r]);
+              d1 += lx[a] * lz[b] * (X[3 * vid[b][1][a] + r] - X[3 * vid[b][0][a] + r]);
+              d2 += lx[a] * ly[b] * (X[3 * vid[1][b][a] + r] - X[3 * vid[0][b][a] + r]);
+            }
+          J[r][0] = 0.5 * d0;
+          J[r][1] = 0.5 * d1;
+          J[r][2] = 0.5 * d2;
+        }
+        double const c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        double const c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        double const c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        double const det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+        if (live && !(det != 0. && std::isfinite(det)))
+          atomicOr(status, 1);
+        double const id_ = 1. / det;
+        // J^-1[a][r] = cofactor(r, a) / det
+        double Ji[3][3];
+        Ji[0][0] = c00 * id_;
+        Ji[1][0] = c01 * id_;
+        Ji[2][0] = c02 * id_;
+        Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id_;
+        Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id_;
+        Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id_;
+        Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id_;
+        Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id_;
+        Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id_;
+        double const s = kq[q] * (tab.w[pa] * tab.w[pc] * tab.w[q]) * fabs(det);
+        int m = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int b = a; b < 3; ++b)
+            M[q][m++] = s * (Ji[a][0] * Ji[b][0] + Ji[a][1] * Ji[b][1] + Ji[a][2] * Ji[b][2]);
+      }
+    }
+    sfSync(); // region 1 (coordinates) is a stage buffer from here on
+    // the next batch's coordinates, coefficients and flags are in flight
+    // during the contractions (nb = 1: its vector values too)
+    loadVals(nxt, bn);
+    T xnxt[P][NBC];
+    if constexpr (NBC == 1 && MODE == kSfApply)
+      loadVec(x, nxt, 0, xnxt);
+
+    T * A0 = reinterpret_cast<T *>(L.r[0][0]); // stage buffers: 3 arrays of N values each
+    T * A1 = A0 + N;
+    T * A2 = A1 + N;
+    T * B0 = reinterpret_cast<T *>(L.r[1][0]);
+    T * B1 = B0 + N;
+    T * B2 = B1 + N;
+
+    // forward stages of one vector (node values u of the lane's x pencil)
+    // to the reference gradient at the lane's points (g[q][0..2], q = qz)
+    auto forward = [&](T const (&u)[P], T (&g)[P][3])
+    {
+#pragma unroll
+      for (int q = 0; q < P; ++q) // x: lane (j, k)
+      {
+        T sb = T(0), sd = T(0);
+#pragma unroll
+        for (int i = 0; i < P; ++i)
+        {
+          sb += tB<T>(tab, q, i) * u[i];
+          sd += tD<T>(tab, q, i) * u[i];
+        }
+        A0[at(q, pa, pc)] = sb;
+        A1[at(q, pa, pc)] = sd;
+      }
+      sfSync();
+#pragma unroll
+      for (int q = 0; q < P; ++q) // y: lane (qx, k)
+      {
+        T bb = T(0), bd = T(0), db = T(0);
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+        {
+          T const ub = A0[at(pa, j, pc)], ud = A1[at(pa, j, pc)];
+          bb += tB<T>(tab, q, j) * ub;
+          bd += tD<T>(tab, q, j) * ub;
+          db += tB<T>(tab, q, j) * ud;
+        }
+        B0[at(pa, q, pc)] = bb;
+        B1[at(pa, q, pc)] = bd;
+        B2[at(pa, q, pc)] = db;
+      }
+      sfSync();
+#pragma unroll
+      for (int q = 0; q < P; ++q) // z: lane (qx, qy)
+      {
+        T gx = T(0), gy = T(0), gz = T(0);
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+        {
+          gx += tB<T>(tab, q, k) * B2[at(pa, pc, k)];
+          gy += tB<T>(tab, q, k) * B1[at(pa, pc, k)];
+          gz += tD<T>(tab, q, k) * B0[at(pa, pc, k)];
+        }
+        g[q][0] = gx;
+        g[q][1] = gy;
+        g[q][2] = gz;
+      }
+    };
+
+    if constexpr (MODE == kSfApply)
+    {
+      T Mt[P][6];
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+#pragma unroll
+        for (int m = 0; m < 6; ++m)
+          Mt[q][m] = T(M[q][m]);
+      for (int b0 = 0; b0 < nb; b0 += NBC)
+      {
+        T xv[P][NBC], yv[P][NBC];
+        if constexpr (NBC == 1)
+        {
+#pragma unroll
+          for (int i = 0; i < P; ++i)
+            xv[i][0] = xcur[i][0];
+        }
+        else
+          loadVec(x, cur, b0, xv);
+#pragma unroll
+        for (int c = 0; c < NBC; ++c)
+        {
+          T u[P];
+#pragma unroll
+          for (int i = 0; i < P; ++i)
+            u[i] = cur.fl[i] ? T(0) : xv[i][c];
+          T g[P][3];
+          forward(u, g); // ends after reading B0..B2; the A buffers are free
+          T v[P][3];
+#pragma unroll
+          for (int q = 0; q < P; ++q)
+          {
+            v[q][0] = Mt[q][0] * g[q][0] + Mt[q][1] * g[q][1] + Mt[q][2] * g[q][2];
+            v[q][1] = Mt[q][1] * g[q][0] + Mt[q][3] * g[q][1] + Mt[q][4] * g[q][2];
+            v[q][2] = Mt[q][2] * g[q][0] + Mt[q][4] * g[q][1] + Mt[q][5] * g[q][2];
+          }
+#pragma unroll
+          for (int k = 0; k < P; ++k) // z^T: lane (qx, qy)
+          {
+            T s1 = T(0), s2 = T(0), s3 = T(0);
+#pragma unroll
+            for (int q = 0; q < P; ++q)
+            {
+              s1 += tB<T>(tab, q, k) * v[q][0];
+              s2 += tB<T>(tab, q, k) * v[q][1];
+              s3 += tD<T>(tab, q, k) * v[q][2];
+            }
+            A0[at(pa, pc, k)] = s1;
+            A1[at(pa, pc, k)] = s2;
+            A2[at(pa, pc, k)] = s3;
+          }
+          sfSync();
+#pragma unroll
+          for (int j = 0; j < P; ++j) // y^T: lane (qx, k)
+          {
+            T sd = T(0), sb = T(0);
+#pragma unroll
+            for (int q = 0; q < P; ++q)
+            {
+              sd += tB<T>(tab, q, j) * A0[at(pa, q, pc)];
+              sb += tD<T>(tab, q, j) * A1[at(pa, q, pc)] + tB<T>(tab, q, j) * A2[at(pa, q, pc)];
+            }
+            B0[at(pa, j, pc)] = sd;
+            B1[at(pa, j, pc)] = sb;
+          }
+          sfSync();
+#pragma unroll
+          for (int i = 0; i < P; ++i) // x^T: lane (j, k)
+          {
+            T y = T(0);
+#pragma unroll
+            for (int q = 0; q < P; ++q)
+              y += tD<T>(tab, q, i) * B0[at(q, pa, pc)] + tB<T>(tab, q, i) * B1[at(q, pa, pc)];
+            // unit diagonal per element on constrained rows
+            yv[i][c] = cur.fl[i] ? xv[i][c] : y;
+          }
+          sfSync(); // the B buffers are rewritten by the next vector
+        }
+        if (live)
+        {
+#pragma unroll
+          for (int i = 0; i < P; ++i)
+          {
+            T * o = out + ((size_t)e * N + loc[i]) * nb + b0;
+            if (nb == NBC)
+            {
+#pragma unroll
+              for (int c = 0; c < NBC; ++c)
+                o[c] = yv[i][c];
+            }
+            else
+            {
+#pragma unroll
+              for (int c = 0; c < NBC; ++c)
+                if (b0 + c < nb)
+                  o[c] = yv[i][c];
+            }
+          }
+        }
+      }
+    }
+    else if constexpr (MODE == kSfGrad)
+    {
+      // c_q = sum_b grad x_b . M grad ybar_b (M without kappa), then the
+      // transposed interpolation B^T x B^T x B^T of c
+      T cq[P];
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+        cq[q] = T(0);
+      for (int b0 = 0; b0 < nb; b0 += NBC)
+      {
+        T xv[P][NBC], wv[P][NBC];
+        loadVec(x, cur, b0, xv);
+        loadVec(ybar, cur, b0, wv);
+#pragma unroll
+        for (int c = 0; c < NBC; ++c)
+        {
+          T u[P], w[P];
+#pragma unroll
+          for (int i = 0; i < P; ++i)
+          {
+            u[i] = cur.fl[i] ? T(0) : xv[i][c];
+            w[i] = cur.fl[i] ? T(0) : wv[i][c];
+          }
+          T gu[P][3], gw[P][3];
+          forward(u, gu);
+          sfSync();
+          forward(w, gw);
+          sfSync();
+#pragma unroll
+          for (int q = 0; q < P; ++q)
+          {
+            T const t0 = T(M[q][0]) * gw[q][0] + T(M[q][1]) * gw[q][1] + T(M[q][2]) * gw[q][2];
+            T const t1 = T(M[q][1]) * gw[q][0] + T(M[q][3]) * gw[q][1] + T(M[q][4]) * gw[q][2];
+            T const t2 = T(M[q][2]) * gw[q][0] + T(M[q][4]) * gw[q][1] + T(M[q][5]) * gw[q][2];
+            cq[q] += gu[q][0] * t0 + gu[q][1] * t1 + gu[q][2] * t2;
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < P; ++k) // z^T: lane (qx, qy)
+      {
+        T s = T(0);
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+          s += tB<T>(tab, q, k) * cq[q];
+        A0[at(pa, pc, k)] = s;
+      }
+      sfSync();
+#pragma unroll
+      for (int j = 0; j < P; ++j) // y^T: lane (qx, k)
+      {
+        T s = T(0);
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+          s += tB<T>(tab, q, j) * A0[at(pa, q, pc)];
+        B0[at(pa, j, pc)] = s;
+      }
+      sfSync();
+#pragma unroll
+      for (int i = 0; i < P; ++i) // x^T: lane (j, k)
+      {
+        T s = T(0);
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+          s += tB<T>(tab, q, i) * B0[at(q, pa, pc)];
+        if (live)
+          out[(size_t)e * N + loc[i]] = s;
+      }
+      sfSync();
+    }
+    else if constexpr (MODE == kSfMetric)
+    {
+      if (live)
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+#pragma unroll
+          for (int m = 0; m < 6; ++m)
+            out[((size_t)e * 6 + m) * N + at(pa, pc, q)] = T(M[q][m]);
+    }
+    else // kSfDiag: (K_e)_nn = sum_q grad phi_n(q) . M_q grad phi_n(q)
+    {
+      // all points' metrics to LDS (6 arrays of N values over both regions)
+      double * Mall = L.r[0][0];
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+#pragma unroll
+        for (int m = 0; m < 6; ++m)
+          Mall[m * N + at(pa, pc, q)] = M[q][m];
+      sfSync();
+#pragma unroll 1
+      for (int i = 0; i < P; ++i) // node (i, j, k) = (i, pa, pc)
+      {
+        double s = 0.;
+#pragma unroll 1
+        for (int qz = 0; qz < P; ++qz)
+#pragma unroll 1
+          for (int qy = 0; qy < P; ++qy)
+#pragma unroll
+            for (int qx = 0; qx < P; ++qx)
+            {
+              double const gx = tab.D[qx][i] * tab.B[qy][pa] * tab.B[qz][pc];
+              double const gy = tab.B[qx][i] * tab.D[qy][pa] * tab.B[qz][pc];
+              double const gz = tab.B[qx][i] * tab.B[qy][pa] * tab.D[qz][pc];
+              int const q = at(qx, qy, qz);
+              s += Mall[q] * gx * gx + Mall[3 * N + q] * gy * gy + Mall[5 * N + q] * gz * gz +
+                   2. * (Mall[N + q] * gx * gy + Mall[2 * N + q] * gx * gz + Mall[4 * N + q] * gy * gz);
+            }
+        if (cur.fl[i])
+          s = 1.; // unit diagonal per element on constrained rows
+        if (live)
+          out[(size_t)e * N + loc[i]] = T(s);
+      }
+      sfSync();
+    }
+    cur = nxt;
+    if constexpr (NBC == 1 && MODE == kSfApply)
+    {
+#pragma unroll
+      for (int i = 0; i < P; ++i)
+        xcur[i][0] = xnxt[i][0];
+    }
+    bt = bn;
+  }
+}
+
+// Apply with the per-point metrics computed once per coefficient field
+// (kSfMetric, [E][6][n]: kappa, geometry and weights folded in; the
+// "partially assembled" operator): per element only the dof ids, the
+// constraint flags, P^3 vector values and 6 P^3 metric values are read, and
+// only the tensor contractions are computed.  One vector; waves prefetch the
+// next batch's data while the current one computes.
+template <typename T, int P>
+__global__ void __launch_bounds__(64 * kSfWaves)
+    k_sf_apply_pa(int64_t E, SfTab<P> tab, uint32_t const * __restrict__ emap, uint8_t const * __restrict__ cons,
+                  T const * __restrict__ met, T const * __restrict__ x, T * __restrict__ out)
+{
+  constexpr int P2 = P * P, N = P * P * P, EPW = 64 / P2;
+  constexpr int SLOTS = (64 + P2 - 1) / P2;
+  // three arrays of N values per stage buffer, in T
+  struct Lds
+  {
+    T a[3][N];
+    T b[3][N];
+  };
+  __shared__ Lds lds[kSfWaves * SLOTS];
+  int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int const slot = lane / P2, pl = lane - slot * P2;
+  int const pa = pl % P, pc = pl / P;
+  Lds & L = lds[wave * SLOTS + slot];
+  auto at = [](int i, int j, int k) { return i + P * (j + P * k); };
+  int loc[P];
+#pragma unroll
+  for (int i = 0; i < P; ++i)
+    loc[i] = tab.local[at(i, pa, pc)];
+  int64_t const nbatch = (E + EPW - 1) / EPW;
+  int64_t const stride = (int64_t)gridDim.x * kSfWaves;
+  int64_t bt = (int64_t)blockIdx.x * kSfWaves + wave;
+  auto elemOf = [&](int64_t b) -> int64_t { return b * EPW + slot; };
+  auto clampE = [&](int64_t b) -> int64_t { return std::min(elemOf(b), E - 1); };
+  struct D
+  {
+    uint32_t id[P];
+    T xv[P];
+    bool fl[P];
+    T m[P][6]; // metric at the lane's z-pencil points (pa, pc, q)
+  };
+  auto loadIds = [&](D & d, int64_t b)
+  {
+    int64_t const e = clampE(b);
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+      d.id[i] = emap[e * N + loc[i]];
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+#pragma unroll
+      for (int m = 0; m < 6; ++m)
+        d.m[q][m] = met[((size_t)e * 6 + m) * N + at(pa, pc, q)];
+  };
+  auto loadVals = [&](D & d)
+  {
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+    {
+      d.xv[i] = x[d.id[i]];
+      d.fl[i] = cons ? cons[d.id[i]] != 0 : false;
+    }
+  };
+  D cur, nxt;
+  loadIds(cur, bt);
+  loadVals(cur);
+  T * A0 = L.a[0];
+  T * A1 = L.a[1];
+  T * A2 = L.a[2];
+  T * B0 = L.b[0];
+  T * B1 = L.b[1];
+  T * B2 = L.b[2];
+  while (bt < nbatch)
+  {
+    bool const live = slot < EPW && elemOf(bt) < E;
+    int64_t const e = elemOf(bt);
+    int64_t const bn = bt + stride;
+    loadIds(nxt, bn);
+    T u[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+      u[i] = cur.fl[i] ? T(0) : cur.xv[i];
+#pragma unroll
+    for (int q = 0; q < P; ++q) // x: lane (j, k)
+    {
+      T sb = T(0), sd = T(0);
+#pragma unroll
+      for (int i = 0; i < P; ++i)
+      {
+        sb += tB<T>(tab, q, i) * u[i];
+        sd += tD<T>(tab, q, i) * u[i];
+      }
+      A0[at(q, pa, pc)] = sb;
+      A1[at(q, pa, pc)] = sd;
+    }
+    sfSync();
+    loadVals(nxt);
+#pragma unroll
+    for (int q = 0; q < P; ++q) // y: lane (qx, k)
+    {
+      T bb = T(0), bd = T(0), db = T(0);
+#pragma unroll
+      for (int j = 0; j < P; ++j)
+      {
+        T const ub = A0[at(pa, j, pc)], ud = A1[at(pa, j, pc)];
+        bb += tB<T>(tab, q, j) * ub;
+        bd += tD<T>(tab, q, j) * ub;
+        db += tB<T>(tab, q, j) * ud;
+      }
+      B0[at(pa, q, pc)] = bb;
+      B1[at(pa, q, pc)] = bd;
+      B2[at(pa, q, pc)] = db;
+    }
+    sfSync();
+    T v[P][3];
+#pragma unroll
+    for (int q = 0; q < P; ++q) // z: lane (qx, qy), then the metric
+    {
+      T gx = T(0), gy = T(0), gz = T(0);
+#pragma unroll
+      for (int k = 0; k < P; ++k)
+      {
+        gx += tB<T>(tab, q, k) * B2[at(pa, pc, k)];
+        gy += tB<T>(tab, q, k) * B1[at(pa, pc, k)];
+        gz += tD<T>(tab, q, k) * B0[at(pa, pc, k)];
+      }
+      T const * M = cur.m[q];
+      v[q][0] = M[0] * gx + M[1] * gy + M[2] * gz;
+      v[q][1] = M[1] * gx + M[3] * gy + M[4] * gz;
+      v[q][2] = M[2] * gx + M[4] * gy + M[5] * gz;
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) // z^T
+    {
+      T s1 = T(0), s2 = T(0), s3 = T(0);
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+      {
+        s1 += tB<T>(tab, q, k) * v[q][0];
+        s2 += tB<T>(tab, q, k) * v[q][1];
+        s3 += tD<T>(tab, q, k) * v[q][2];
+      }
+      A0[at(pa, pc, k)] = s1;
+      A1[at(pa, pc, k)] = s2;
+      A2[at(pa, pc, k)] = s3;
+    }
+    sfSync();
+#pragma unroll
+    for (int j = 0; j < P; ++j) // y^T: lane (qx, k)
+    {
+      T sd = T(0), sb = T(0);
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+      {
+        sd += tB<T>(tab, q, j) * A0[at(pa, q, pc)];
+        sb += tD<T>(tab, q, j) * A1[at(pa, q, pc)] + tB<T>(tab, q, j) * A2[at(pa, q, pc)];
+      }
+      B0[at(pa, j, pc)] = sd;
+      B1[at(pa, j, pc)] = sb;
+    }
+    sfSync();
+#pragma unroll
+    for (int i = 0; i < P; ++i) // x^T: lane (j, k)
+    {
+      T y = T(0);
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+        y += tD<T>(tab, q, i) * B0[at(q, pa, pc)] + tB<T>(tab, q, i) * B1[at(q, pa, pc)];
+      if (cur.fl[i])
+        y = cur.xv[i]; // unit diagonal per element on constrained rows
+      if (live)
+        out[(size_t)e * N + loc[i]] = y;
+    }
+    sfSync();
+    cur = nxt;
+    bt = bn;
+  }
+}
+
+// per-dof sum of the element buffer in element order (nb = 1), with the
+// epilogue of the operator use: apply y = Ax, residual y = b - Ax, Jacobi
+// y = x + omega dinv (b - Ax), inverse y = 1 / Ax (the diagonal)
+enum : int
+{
+  kGApply = 0,
+  kGResidual = 1,
+  kGJacobi = 2,
+  kGInverse = 3,
+};
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256)
+    k_sf_gather(int64_t ndofs, uint32_t const * __restrict__ seg, uint32_t const * __restrict__ ids,
+                T const * __restrict__ buf, T const * __restrict__ x, T const * __restrict__ b,
+                T const * __restrict__ dinv, T omega, T * __restrict__ y)
+{
+  int64_t const d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndofs)
+    return;
+  uint32_t const k0 = seg[d], k1 = seg[d + 1];
+  T acc = T(0);
+  if (k1 > k0)
+  {
+    constexpr int kMaxC = 8;
+    uint32_t id[kMaxC];
+#pragma unroll
+    for (int j = 0; j < kMaxC; ++j)
+      id[j] = ids[min(k0 + j, k1 - 1)];
+    T v[kMaxC];
+#pragma unroll
+    for (int j = 0; j < kMaxC; ++j)
+      v[j] = buf[id[j]];
+#pragma unroll
+    for (int j = 0; j < kMaxC; ++j)
+      if (k0 + j < k1)
+        acc += v[j];
+    for (uint32_t k = k0 + kMaxC; k < k1; ++k)
+      acc += buf[ids[k]];
+  }
+  if constexpr (MODE == kGApply)
+    y[d] = acc;
+  else if constexpr (MODE == kGResidual)
+    y[d] = b[d] - acc;
+  else if constexpr (MODE == kGJacobi)
+    y[d] = x[d] + omega * dinv[d] * (b[d] - acc);
+  else
+    y[d] = T(1) / acc;
+}
+
+template <int P>
+SfTab<P> makeTab(int feType)
+{
+  SfTab<P> t{};
+  // 1D Gauss rule of the reference (src/qr.cpp:27,38; the 3-point abscissa
+  // as truncated there) and the 1D Lagrange nodes -1 (, 0), +1
+  double const a3 = 0.774596669241483, a2 = 0.5773502691896258;
+  double const nodes3[3] = {-1., 0., 1.}, nodes2[2] = {-1., 1.};
+  double const * nd = P == 3 ? nodes3 : nodes2;
+  for (int q = 0; q < P; ++q)
+  {
+    t.g[q] = P == 3 ? (q == 0 ? -a3 : (q == 1 ? 0. : a3)) : (q == 0 ? -a2 : a2);
+    t.w[q] = P == 3 ? (q == 1 ? 8. / 9. : 5. / 9.) : 1.;
+    for (int a = 0; a < P; ++a)
+    {
+      double v = 1., dv = 0.;
+      for (int m = 0; m < P; ++m)
+      {
+        if (m == a)
+          continue;
+        double const f = (t.g[q] - nd[m]) / (nd[a] - nd[m]);
+        double dp = 1. / (nd[a] - nd[m]);
+        for (int r = 0; r < P; ++r)
+          if (r != a && r != m)
+            dp *= (t.g[q] - nd[r]) / (nd[a] - nd[r]);
+        dv += dp;
+        v *= f;
+      }
+      t.B[q][a] = v;
+      t.D[q][a] = dv;
+      t.Bf[q][a] = (float)v;
+      t.Df[q][a] = (float)dv;
+    }
+  }
+  // reference local index of each lexicographic node: the node whose
+  // dof point (in reference coordinates) is (nd[i], nd[j], nd[k])
+  FeInfo const & fe = feInfo(feType);
+  double vtx[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                      {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+  double pts[27 * 3];
+  fe.dofPoints(&vtx[0][0], pts);
+  for (int k = 0; k < P; ++k)
+    for (int j = 0; j < P; ++j)
+      for (int i = 0; i < P; ++i)
+      {
+        int found = -1;
+        for (int L = 0; L < fe.n; ++L)
+          if (pts[3 * L] == nd[i] && pts[3 * L + 1] == nd[j] && pts[3 * L + 2] == nd[k])
+            found = L;
+        if (found < 0)
+          throw Error(PXG_E_INVALID, "tensor-product node table");
+        t.local[i + P * (j + P * k)] = found;
+      }
+  return t;
+}
+
+template <int P>
+SfTab<P> const & tabFor(int feType)
+{
+  static SfTab<P> const t = makeTab<P>(feType);
+  return t;
+}
+
+// the context holds a Q1 / Q2 hexahedron with the recommended tensor Gauss rule
+int sfOrder(pxg_ctx const * c)
+{
+  FeTables const & T = c->fe;
+  int P = 0;
+  if (T.feType == PXG_FE_HEX_Q1)
+    P = 2;
+  else if (T.feType == PXG_FE_HEX_Q2)
+    P = 3;
+  else
+    return 0;
+  Quadrature const q = recommendedQuadrature(T.feType);
+  if (T.nq != q.nq || T.xi != q.xi || T.w != q.w)
+    return 0;
+  return P;
+}
+
+template <typename T, int P, int MODE, int NBC>
+void launchSfP(pxg_ctx * c, int nb, double const * kappa, uint8_t const * cons, T const * x, T const * ybar, T * out,
+               hipStream_t s)
+{
+  int64_t const E = c->nelem;
+  constexpr int EPW = 64 / (P * P);
+  auto kern = k_sf_elem<T, P, MODE, NBC>;
+  int dev = 0, ncu = 0, perCu = 0;
+  PXG_HIP(hipGetDevice(&dev));
+  PXG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  PXG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kern, 64 * kSfWaves, 0));
+  int64_t const nbatch = (E + EPW - 1) / EPW;
+  int64_t const need = (nbatch + kSfWaves - 1) / kSfWaves;
+  // one resident wave of blocks; each wave loops over its batches
+  int64_t const grid = std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)ncu * std::max(perCu, 1)));
+  kern<<<(unsigned)grid, 64 * kSfWaves, 0, s>>>(E, nb, tabFor<P>(P == 3 ? PXG_FE_HEX_Q2 : PXG_FE_HEX_Q1), c->xyz.p,
+                                                c->conn.p, c->elemMap.p, kappa, cons, x, ybar, out, c->status.p);
+  PXG_HIP(hipGetLastError());
+}
+
+template <typename T, int MODE>
+void launchSf(pxg_ctx * c, int P, int nb, double const * kappa, uint8_t const * cons, T const * x, T const * ybar,
+              T * out, hipStream_t s)
+{
+  // vectors per chunk: 1 for a single vector (its values ride with the
+  // prefetch), 8 for batches of the apply, 4 for the gradient (two vectors
+  // per chunk entry)
+  if (P == 3)
+  {
+    if constexpr (MODE == kSfDiag || MODE == kSfMetric)
+      launchSfP<T, 3, MODE, 1>(c, nb, kappa, cons, x, ybar, out, s);
+    else if (nb > 1)
+      launchSfP<T, 3, MODE, MODE == kSfApply ? 8 : 4>(c, nb, kappa, cons, x, ybar, out, s);
+    else
+      launchSfP<T, 3, MODE, 1>(c, nb, kappa, cons, x, ybar, out, s);
+  }
+  else
+  {
+    if constexpr (MODE == kSfDiag || MODE == kSfMetric)
+      launchSfP<T, 2, MODE, 1>(c, nb, kappa, cons, x, ybar, out, s);
+    else if (nb > 1)
+      launchSfP<T, 2, MODE, MODE == kSfApply ? 8 : 4>(c, nb, kappa, cons, x, ybar, out, s);
+    else
+      launchSfP<T, 2, MODE, 1>(c, nb, kappa, cons, x, ybar, out, s);
+  }
+}
+
+template <typename T, int MODE>
+void launchGather(pxg_ctx * c, T const * buf, T const * x, T const * b, T const * dinv, double omega, T * y,
+                  hipStream_t s)
+{
+  int64_t const n = c->ndofs;
+  k_sf_gather<T, MODE><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, c->dofSeg.p, c->dofContrib.p, buf, x, b, dinv,
+                                                                     T(omega), y);
+  PXG_HIP(hipGetLastError());
+}
+} // namespace
+
+bool sfAvailable(pxg_ctx const * c) { return sfOrder(c) != 0; }
+
+void sfPrepare(pxg_ctx * c)
+{
+  PXG_REQUIRE(c->fe.feType && c->nelem && c->ndofs, PXG_E_STATE, "matrix-free needs set_fe, set_mesh, set_dofmap");
+  PXG_REQUIRE(sfOrder(c), PXG_E_INVALID, "sum-factorised operator needs Q1 / Q2 hexahedra with the Gauss rule");
+  if (!c->dofSeg.p)
+    buildDofAdjacency(c);
+}
+
+template <typename T>
+void sfElementApply(pxg_ctx * c, double const * kappa, int nb, T const * x, T * buf, bool useBC, hipStream_t s)
+{
+  uint8_t const * cons = (useBC && c->hasBC) ? c->bcFlag.p : nullptr;
+  launchSf<T, kSfApply>(c, sfOrder(c), nb, kappa, cons, x, nullptr, buf, s);
+}
+
+void sfElementGrad(pxg_ctx * c, int nb, double const * x, double const * ybar, double * buf, bool useBC,
+                   hipStream_t s)
+{
+  uint8_t const * cons = (useBC && c->hasBC) ? c->bcFlag.p : nullptr;
+  launchSf<double, kSfGrad>(c, sfOrder(c), nb, nullptr, cons, x, ybar, buf, s);
+}
+
+template <typename T>
+void sfElementDiag(pxg_ctx * c, double const * kappa, T * buf, hipStream_t s)
+{
+  uint8_t const * cons = c->hasBC ? c->bcFlag.p : nullptr;
+  launchSf<T, kSfDiag>(c, sfOrder(c), 1, kappa, cons, nullptr, nullptr, buf, s);
+}
+
+template <typename T>
+void sfMetric(pxg_ctx * c, double const * kappa, T * met, hipStream_t s)
+{
+  launchSf<T, kSfMetric>(c, sfOrder(c), 1, kappa, nullptr, nullptr, nullptr, met, s);
+}
+
+template <typename T>
+void sfApplyPA(pxg_ctx * c, T const * met, T const * x, T * buf, bool useBC, hipStream_t s)
+{
+  uint8_t const * cons = (useBC && c->hasBC) ? c->bcFlag.p : nullptr;
+  int64_t const E = c->nelem;
+  int dev = 0, ncu = 0, perCu = 0;
+  PXG_HIP(hipGetDevice(&dev));
+  PXG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  auto go = [&](auto pC)
+  {
+    constexpr int P = decltype(pC)::value;
+    constexpr int EPW = 64 / (P * P);
+    auto kern = k_sf_apply_pa<T, P>;
+    PXG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kern, 64 * kSfWaves, 0));
+    int64_t const nbatch = (E + EPW - 1) / EPW;
+    int64_t const need = (nbatch + kSfWaves - 1) / kSfWaves;
+    int64_t const grid = std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)ncu * std::max(perCu, 1)));
+    kern<<<(unsigned)grid, 64 * kSfWaves, 0, s>>>(E, tabFor<P>(P == 3 ? PXG_FE_HEX_Q2 : PXG_FE_HEX_Q1),
+                                                  c->elemMap.p, cons, met, x, buf);
+  };
+  if (sfOrder(c) == 3)
+    go(std::integral_constant<int, 3>{});
+  else
+    go(std::integral_constant<int, 2>{});
+  PXG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void sfGather(pxg_ctx * c, int mode, T const * buf, T const * x, T const * b, T const * dinv, double omega, T * y,
+              hipStream_t s)
+{
+  switch (mode)
+  {
+  case kGApply: launchGather<T, kGApply>(c, buf, x, b, dinv, omega, y, s); break;
+  case kGResidual: launchGather<T, kGResidual>(c, buf, x, b, dinv, omega, y, s); break;
+  case kGJacobi: launchGather<T, kGJacobi>(c, buf, x, b, dinv, omega, y, s); break;
+  default: launchGather<T, kGInverse>(c, buf, x, b, dinv, omega, y, s); break;
+  }
+}
+
+template void sfElementApply<double>(pxg_ctx *, double const *, int, double const *, double *, bool, hipStream_t);
+template void sfElementApply<float>(pxg_ctx *, double const *, int, float const *, float *, bool, hipStream_t);
+template void sfElementDiag<double>(pxg_ctx *, double const *, double *, hipStream_t);
+template void sfElementDiag<float>(pxg_ctx *, double const *, float *, hipStream_t);
+template void sfMetric<double>(pxg_ctx *, double const *, double *, hipStream_t);
+template void sfMetric<float>(pxg_ctx *, double const *, float *, hipStream_t);
+template void sfApplyPA<double>(pxg_ctx *, double const *, double const *, double *, bool, hipStream_t);
+template void sfApplyPA<float>(pxg_ctx *, float const *, float const *, float *, bool, hipStream_t);
+template void sfGather<double>(pxg_ctx *, int, double const *, double const *, double const *, double const *,
+                               double, double *, hipStream_t);
+template void sfGather<float>(pxg_ctx *, int, float const *, float const *, float const *, float const *, double,
+                              float *, hipStream_t);
+
+} // namespace pxg

@@ -77,6 +77,12 @@ struct pxg_mg
   DevBuf<int32_t> freeDofs; // free index -> coarse dof
   DevBuf<double> inv, invTmp;
   DevBuf<int> flag;
+  // level 0 applied matrix-free (mode PXG_FP32_MF): FP32 per-point metrics
+  // of the current kappa field, element buffer, inverse diagonal
+  bool mf0 = false;
+  uint64_t mfKappaStamp = 0;
+  DevBuf<float> met32, mfBuf32, dinvMf32;
+  DevBuf<double> mfBuf64;
   // orders the cycle stream after work issued on the level contexts' own
   // streams (value casts, packed copies made on first use)
   hipEvent_t join = nullptr;
@@ -376,14 +382,66 @@ void buildProlongation(pxg_mg * m, int l)
   L.R.blockRow = L.R.nblocks > 0 ? L.Rblk.p : nullptr;
 }
 
+// operator of level l in the cycle: the assembled CSR matrix, or on level 0
+// in mode PXG_FP32_MF the partially assembled matrix-free operator (FP32)
+template <typename T>
+struct LevelOp
+{
+  pxg_mg * m;
+  int l;
+  CsrView A;
+  bool mf;
+  LevelOp(pxg_mg * mg, int lev): m(mg), l(lev), A(viewOf(mg->ctx[lev])), mf(lev == 0 && mg->mf0)
+  {
+    if (mf && sizeof(T) != 4)
+      throw Error(PXG_E_INVALID, "the matrix-free level 0 is FP32");
+  }
+  int64_t n() const { return A.nrows; }
+  T const * dinv() const
+  {
+    if (mf)
+      return (T const *)m->dinvMf32.p;
+    return sizeof(T) == 8 ? (T const *)A.dinv : (T const *)A.dinv32;
+  }
+  // element kernel into the element buffer, then the gather with its epilogue
+  void mfRun(int mode, T const * x, T const * b, double omega, T * y, hipStream_t s) const
+  {
+    pxg_ctx * c = m->ctx[0];
+    sfApplyPA<float>(c, m->met32.p, (float const *)x, m->mfBuf32.p, true, s);
+    sfGather<float>(c, mode, m->mfBuf32.p, (float const *)x, (float const *)b, m->dinvMf32.p, omega, (float *)y, s);
+  }
+  void apply(T const * x, T * y, hipStream_t s) const
+  {
+    if (mf)
+      mfRun(0, x, nullptr, 0., y, s);
+    else
+      spmv<T>(A, x, y, s);
+  }
+  void residual(T const * b, T const * x, T * r, hipStream_t s) const
+  {
+    if (mf)
+      mfRun(1, x, b, 0., r, s);
+    else
+      ::pxg::residual<T>(A, b, x, r, s);
+  }
+  void jacobi(T const * b, T const * x, T * y, double omega, hipStream_t s) const
+  {
+    if (mf)
+      mfRun(2, x, b, omega, y, s);
+    else
+      jacobiStep<T>(A, b, x, y, omega, s);
+  }
+};
+
 // smoothing sweeps, ping-ponging between x and t, result left in x
 template <typename T>
-void smooth(CsrView const & A, T const * b, T * x, T * t, int sweeps, double omega, bool fromZero, hipStream_t s)
+void smooth(LevelOp<T> const & A, T const * b, T * x, T * t, int sweeps, double omega, bool fromZero, hipStream_t s)
 {
+  int64_t const n = A.n();
   if (sweeps <= 0)
   {
     if (fromZero)
-      fill<T>(A.nrows, x, T(0), s);
+      fill<T>(n, x, T(0), s);
     return;
   }
   T * cur = x;
@@ -394,16 +452,38 @@ void smooth(CsrView const & A, T const * b, T * x, T * t, int sweeps, double ome
     // remaining sweeps end in x
     steps = sweeps - 1;
     cur = (steps % 2 == 0) ? x : t;
-    jacobiDiag<T>(A.nrows, sizeof(T) == 8 ? (T const *)A.dinv : (T const *)A.dinv32, b, cur, omega, s);
+    jacobiDiag<T>(n, A.dinv(), b, cur, omega, s);
   }
   for (int k = 0; k < steps; ++k)
   {
     T * oth = (cur == x) ? t : x;
-    jacobiStep<T>(A, b, cur, oth, omega, s);
+    A.jacobi(b, cur, oth, omega, s);
     cur = oth;
   }
   if (cur != x)
-    PXG_HIP(hipMemcpyAsync(x, cur, A.nrows * sizeof(T), hipMemcpyDeviceToDevice, s));
+    PXG_HIP(hipMemcpyAsync(x, cur, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+}
+
+// metrics and inverse diagonal of the matrix-free level 0 for the current
+// kappa field of the level-0 context (recomputed when kappa was set again)
+void prepareMf0(pxg_mg * m)
+{
+  pxg_ctx * c = m->ctx[0];
+  PXG_REQUIRE(c->hasKappa, PXG_E_STATE, "the matrix-free level 0 needs the kappa field (pxg_set_coefficient)");
+  PXG_REQUIRE(sfAvailable(c), PXG_E_INVALID,
+              "the matrix-free level 0 needs Q1 / Q2 hexahedra with their Gauss rule");
+  sfPrepare(c);
+  if (m->mfKappaStamp == c->kappaStamp && m->met32.p)
+    return;
+  hipStream_t s = m->s;
+  int64_t const E = c->nelem, n = c->fe.n;
+  m->met32.resize((size_t)E * 6 * n);
+  m->mfBuf32.resize((size_t)E * n);
+  m->dinvMf32.resize(c->ndofs);
+  sfMetric<float>(c, c->kappa.p, m->met32.p, s);
+  sfElementDiag<float>(c, c->kappa.p, m->mfBuf32.p, s);
+  sfGather<float>(c, 3, m->mfBuf32.p, nullptr, nullptr, nullptr, 0., m->dinvMf32.p, s);
+  m->mfKappaStamp = c->kappaStamp;
 }
 
 // (re)builds the dense inverse of the coarsest free-dof block when the coarse
@@ -535,11 +615,11 @@ void vcycle(pxg_mg * m, int l, T const * b, T * x)
   }
   pxg_mg::Level & L = m->lv[l];
   pxg_mg::Level & Lc = m->lv[l + 1];
-  CsrView const A = viewOf(m->ctx[l]);
+  LevelOp<T> const A(m, l);
   T * t = Buf<T>::t(L);
   T * r = Buf<T>::r(L);
   smooth<T>(A, b, x, t, m->nu1, m->omega, true, s);
-  residual<T>(A, b, x, r, s);
+  A.residual(b, x, r, s);
   T * bc = Buf<T>::b(Lc);
   T * xc = Buf<T>::x(Lc);
   spmv<T>(L.R, r, bc, s);
@@ -553,8 +633,8 @@ template <typename T>
 int pcg(pxg_mg * m, T const * b, T * x, double rtol, int maxit, double & relres, bool zeroStart)
 {
   hipStream_t s = m->s;
-  CsrView const A = viewOf(m->ctx[0]);
-  int64_t const n = A.nrows;
+  LevelOp<T> const A(m, 0);
+  int64_t const n = A.n();
   T * r = sizeof(T) == 8 ? (T *)m->r.p : (T *)m->r32.p;
   T * z = sizeof(T) == 8 ? (T *)m->z.p : (T *)m->z32.p;
   T * p = sizeof(T) == 8 ? (T *)m->p.p : (T *)m->p32.p;
@@ -566,7 +646,7 @@ int pcg(pxg_mg * m, T const * b, T * x, double rtol, int maxit, double & relres,
     PXG_HIP(hipMemcpyAsync(r, b, n * sizeof(T), hipMemcpyDeviceToDevice, s));
   }
   else
-    residual<T>(A, b, x, r, s);
+    A.residual(b, x, r, s);
   dot<T>(n, b, b, sc + 0, m->partial.p, s);
   dot<T>(n, r, r, sc + 1, m->partial.p, s);
   double const bn = std::sqrt(readScalar(m, 0));
@@ -580,7 +660,7 @@ int pcg(pxg_mg * m, T const * b, T * x, double rtol, int maxit, double & relres,
   int it = 0;
   while (it < maxit)
   {
-    spmv<T>(A, p, q, s);
+    A.apply(p, q, s);
     dot<T>(n, p, q, sc + 3, m->partial.p, s);
     cgUpdate<T>(n, x, p, r, q, sc, 2, 3, s);
     ++it;
@@ -603,7 +683,14 @@ int pcg(pxg_mg * m, T const * b, T * x, double rtol, int maxit, double & relres,
 void residual64(pxg_mg * m, double const * b, double const * x, double * r)
 {
   pxg_ctx * c = m->ctx[0];
-  if (c->valPrecision == PXG_FP64)
+  if (m->mf0)
+  {
+    // the FP64 sum-factorised operator (geometry recomputed), same Dirichlet form
+    m->mfBuf64.resize((size_t)c->nelem * c->fe.n);
+    sfElementApply<double>(c, c->kappa.p, 1, x, m->mfBuf64.p, true, m->s);
+    sfGather<double>(c, 1, m->mfBuf64.p, x, b, nullptr, 0., r, m->s);
+  }
+  else if (c->valPrecision == PXG_FP64 && c->closed)
     residual<double>(viewOf(c), b, x, r, m->s);
   else
   {
@@ -632,7 +719,7 @@ int solveIR(pxg_mg * m, double const * b, double * x, double rtol, int maxit, do
   constexpr double kReplace = 1e-2;
   hipStream_t s = m->s;
   int64_t const n = m->lv[0].n;
-  CsrView const A = viewOf(m->ctx[0]);
+  LevelOp<float> const A(m, 0);
   double * r64 = m->r.p;
   float * r = m->r32.p, * z = m->z32.p, * p = m->p32.p, * q = m->q32.p;
   double * sc = m->sc.p;
@@ -665,7 +752,7 @@ int solveIR(pxg_mg * m, double const * b, double * x, double rtol, int maxit, do
   int it = 0;
   while (it < maxit)
   {
-    spmv<float>(A, p, q, s);
+    A.apply(p, q, s);
     dot<float>(n, p, q, sc + 3, m->partial.p, s);
     cgUpdateMixed(n, x, p, r, q, sc, 2, 3, s);
     ++it;
@@ -804,16 +891,27 @@ int pxg_mg_solve(pxg_mg * m, int mode, const double * b, double * x, double rtol
   {
     PXG_HIP(hipSetDevice(m->ctx[0]->device));
     m->s = m->ctx[0]->stream; // follow the finest context (its stream may have been replaced)
-    for (auto * c: m->ctx)
-      PXG_REQUIRE(c->closed, PXG_E_STATE, "levels must be closed");
+    PXG_REQUIRE(mode == PXG_FP64 || mode == PXG_FP32 || mode == PXG_FP32_MF, PXG_E_INVALID, "solve mode");
+    bool const mf = mode == PXG_FP32_MF;
+    PXG_REQUIRE(!mf || m->nlev >= 2, PXG_E_INVALID, "the matrix-free level 0 needs a coarser level");
+    for (int l = mf ? 1 : 0; l < m->nlev; ++l)
+      PXG_REQUIRE(m->ctx[l]->closed, PXG_E_STATE, "levels must be closed");
     factorCoarse(m);
     double rr = 0.;
     int it = 0;
-    if (mode == PXG_FP32)
+    if (mode != PXG_FP64)
     {
-      for (auto * c: m->ctx)
-        ensureOperator(c, PXG_FP32);
+      for (int l = mf ? 1 : 0; l < m->nlev; ++l)
+        ensureOperator(m->ctx[l], PXG_FP32);
       joinLevels(m);
+      struct Mf0 // the mode holds for this solve only
+      {
+        pxg_mg * m;
+        ~Mf0() { m->mf0 = false; }
+      } guard{m};
+      m->mf0 = mf;
+      if (mf)
+        prepareMf0(m);
       it = solveIR(m, b, x, rtol, maxit, rr);
     }
     else

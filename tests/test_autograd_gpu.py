@@ -92,17 +92,27 @@ def test_kappa_solve_adjoint_gradient(ag, oracle, gpu, fe, ns):
     assert np.abs(g - g_ref).max() <= 1e-7 * np.abs(g_ref).max()
 
 
-@pytest.mark.parametrize("fine_fp64", [False, True])
-def test_kappa_solve_fp32_refined_matches_fp64(ag, gpu, fine_fp64):
-    solver = ag.KappaSolver("hex_q1", (16, 8, 4, 2))
+@pytest.mark.parametrize("fe,ns", [("hex_q1", (16, 8, 4, 2)), ("hex_q2", (8, 4, 2))])
+@pytest.mark.parametrize("variant", ["fp32", "fp32_fine64", "fp32_mf"])
+def test_kappa_solve_fp32_refined_matches_fp64(ag, gpu, fe, ns, variant):
+    """The refined forms (FP32 levels with the matrix-free FP64 residual; the
+    finest level in FP64; the finest level matrix-free in FP32) against the
+    FP64 solve, solution and adjoint gradient."""
+    solver = ag.KappaSolver(fe, ns)
     x = solver.probs[0].dof_coords()
-    kt = torch.from_numpy(np.exp(np.sin(4 * x[:, 0]) * np.cos(3 * x[:, 1]))).to(gpu)
+    kap = torch.from_numpy(np.exp(np.sin(4 * x[:, 0]) * np.cos(3 * x[:, 1]))).to(gpu)
     b = torch.from_numpy(solver.rhs(1.0)).to(gpu)
-    u64 = solver(kt, b, rtol=1e-8, mode=N.FP64)
-    u32 = solver(kt, b, rtol=1e-8, mode=N.FP32, fine_fp64=fine_fp64)
-    err = float((u32 - u64).norm() / u64.norm())
-    print(f"  FP32+IR vs FP64 relative difference {err:.2e}")
-    assert err < 1e-6
+    mode = N.FP32_MF if variant == "fp32_mf" else N.FP32
+    out = {}
+    for name, md, f64 in (("fp64", N.FP64, False), (variant, mode, variant == "fp32_fine64")):
+        k = kap.clone().requires_grad_()
+        u = solver(k, b, rtol=1e-8, mode=md, fine_fp64=f64)
+        (0.5 * (u ** 2).sum()).backward()
+        out[name] = (u.detach(), k.grad.detach())
+    err = float((out[variant][0] - out["fp64"][0]).norm() / out["fp64"][0].norm())
+    gerr = float((out[variant][1] - out["fp64"][1]).norm() / out["fp64"][1].norm())
+    print(f"  {fe} {variant} vs FP64: solution {err:.2e}, gradient {gerr:.2e}")
+    assert err < 1e-6 and gerr < 1e-6
 
 
 def test_kappa_solve_backward_reassembles_only_when_needed(ag, gpu):

@@ -232,13 +232,15 @@ def test_mg_pcg_fp64_and_fp32_refined(oracle, native, gpu, fe, ns):
     omg = oracle.Multigrid(ol)
     u_o, it_o, rr_o = omg.pcg(b, rtol=1e-8)
     bt = torch.from_numpy(b).to(gpu)
-    for mode in (0, 1):
+    # mode 2: FP32-refined with level 0 applied matrix-free from kappa (its
+    # operator is the assembled matrix A up to rounding)
+    for mode in (0, 1, 2):
         xt = torch.zeros_like(bt)
         it, rr = mg.solve_dev(a0.rhs_ptr(), xt.data_ptr(), rtol=1e-8, maxit=500, mode=mode)
         u = xt.cpu().numpy()
         res = np.linalg.norm(b - A @ u) / np.linalg.norm(b)
         err = np.linalg.norm(u - u_exact) / np.linalg.norm(u_exact)
-        print(f"  {fe} mode={'fp64' if mode == 0 else 'fp32-ir'} iters={it} (oracle {it_o}) "
+        print(f"  {fe} mode={('fp64', 'fp32-ir', 'fp32-ir-mf')[mode]} iters={it} (oracle {it_o}) "
               f"relres={res:.2e} err vs exact={err:.2e}")
         assert res <= 1.05e-8
         assert err <= 1e-6
