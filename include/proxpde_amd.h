@@ -259,7 +259,8 @@ int pxg_mg_create(pxg_mg ** mg, int n_lev, pxg_ctx ** levels, const int * dims, 
 int pxg_mg_destroy(pxg_mg * mg);
 /* refresh level matrices after a re-assembly (same patterns): refactors
  * the coarsest level now (solve and vcycle also do it when they find the
- * coarsest matrix closed again since the last factorisation)               */
+ * coarsest matrix closed again since the last factorisation); needs the
+ * coarsest level closed, the others are checked by the solve              */
 int pxg_mg_update(pxg_mg * mg);
 /* solve A x = b (device pointers, FP64); mode PXG_FP64: MG-PCG in FP64;
  * PXG_FP32: MG-PCG on FP32 matrices and vectors with an FP64 iterate,

@@ -167,7 +167,6 @@ void buildTables(pxg_ctx * c, int feType, int nq, double const * nodes, double c
   std::vector<float> p32(T.phi.begin(), T.phi.end()), g32(T.grad.begin(), T.grad.end());
   T.d_phi32.upload(p32.data(), p32.size(), s);
   T.d_grad32.upload(g32.data(), g32.size(), s);
-  T.afragKey64 = T.afragKey32 = 0;
   PXG_HIP(hipStreamSynchronize(s));
 }
 // Everything derived from the dof map (pattern, element->nnz and stage maps,

@@ -104,10 +104,6 @@ struct FeTables
   std::vector<double> w;       // nq
   DevBuf<double> d_phi, d_grad, d_geoGrad, d_xi, d_w;
   DevBuf<float> d_phi32, d_grad32;
-  // matrix-core fragments of the constant element operand (built on first use)
-  DevBuf<double> d_afrag;
-  DevBuf<float> d_afrag32;
-  int afragKey64 = 0, afragKey32 = 0; // which variant the fragments hold (0: none)
 };
 
 } // namespace pxg

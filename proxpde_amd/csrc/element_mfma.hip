@@ -2,13 +2,16 @@
 //
 // For an element with local basis gradients g_j(q) (reference coordinates),
 // per-point metric M_q = s_q (J_q^T J_q)^{-1} and mass weight m_q:
-//     Ke = A * C,   A[i][k] = g_i(q,a) | phi_i(q)                     (N x K, constant)
+//     Ke = A * C,   A[i][k] = g_i(q,a) | phi_i(q)                     (N x K)
 //                   C[k][j] = sum_b M_q^{ab} g_j(q,b) | m_q phi_j(q)  (K x N, per element)
-// with k running over the (q, a) pairs, then over q for the mass rows.  A is
-// the same for every element, so its MFMA fragments are built once and kept
-// in LDS for the whole kernel; each C operand value is formed in registers right before its
-// matrix instruction from the per-point metric (LDS) and the gradient table
-// (LDS), so no K x N staging buffer is needed and more waves fit per CU.
+// with k running over the (q, a) pairs (and the mass rows).  Every operand
+// value is formed in registers right before its matrix instruction from the
+// per-point metric (LDS) and the gradient table (LDS), so no K x N staging
+// buffer is needed and more waves fit per CU.  Single-tile elements (n <= 16:
+// P2 tet, Q2 quad) step through the points one at a time (rows a = the lane's
+// row group); larger ones (Q2 hex) through blocks of four points and one
+// component a per step, where a lane's gradient values are both its A and
+// the input of its C operands.
 // The N x N product runs as v_mfma_f64_16x16x4f64 (FP64) or
 // v_mfma_f32_16x16x4f32 (FP32) tiles; only the upper tiles are multiplied
 // (Ke is symmetric) and stored to both triangles of the row-major Ke, so the
@@ -105,14 +108,14 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
                    uint8_t const * __restrict__ bcFlag, double const * __restrict__ bcVal, int hasBC,
                    double const * __restrict__ geoGradG, double const * __restrict__ wG,
                    double const * __restrict__ phiG, double const * __restrict__ gradG,
-                   T const * __restrict__ afrag, double cS, double cSF, double cM, double cMF, int flags,
+                   double cS, double cSF, double cM, double cMF, int flags,
                    T * __restrict__ ke, double * __restrict__ fe, int * __restrict__ status,
                    int * __restrict__ work)
 {
   using S = Shape<N, DIM, NQ, MASS>;
   using L = WaveLds<T, N, NPE, NQ, Shape<N, DIM, NQ, MASS>::MROWS>;
   using Acc = typename Mfma<T>::Acc;
-  constexpr int NT = S::NT, KS = S::KS, STEPS = S::STEPS;
+  constexpr int NT = S::NT;
   constexpr bool QM = S::QM;
   bool const needK = (flags & 10) != 0;
 
@@ -126,8 +129,6 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
   // for q-major elements
   __shared__ double sGrad[NQ * N * DIM];
   __shared__ double sW[NQ];
-  __shared__ T sA[QM ? 1 : NT * STEPS * 64]; // constant operand fragments
-  __shared__ int sKA[QM ? 1 : STEPS * 4];     // row k = 4 s + kr of C: (q << 2 | a), -1 for padding
   __shared__ L lds[kWaves];
 
   int const tid = threadIdx.x;
@@ -158,14 +159,6 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
   int const wave = tid >> 6, lane = tid & 63;
   L & W = lds[wave];
 
-  // constant operand fragments: sA[(ti STEPS + s) 64 + lane] = A[16 ti + lane%16][4 s + lane/16]
-  if constexpr (!QM)
-  {
-    for (int k = tid; k < NT * STEPS * 64; k += blockDim.x)
-      sA[k] = afrag[k];
-    for (int k = tid; k < STEPS * 4; k += blockDim.x)
-      sKA[k] = k < KS ? (((k / DIM) << 2) | (k % DIM)) : ((MASS && k < KS + NQ) ? (((k - KS) << 2) | DIM) : -1);
-  }
   __syncthreads();
 
   int const kr = lane >> 4, jc = lane & 15;
@@ -393,58 +386,78 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
     }
     else
     {
-      // C[4s + kr][16 tj + jc] for row code (q, a) of step s: the metric row
-      // (or the mass row) times the reference gradient (or phi) of dof j at q;
-      // indices are clamped and the value masked, so there are no branches
-      auto cval = [&](int s, int tj) -> T
+      // K ordered by (block of 4 points, component): step (qb, a) takes the
+      // rows k = (q, a) for q = 4 qb + kr.  The lane's reference gradients
+      // g_{16 t + jc}(q, 0..2) of its two column tiles are at once its A
+      // operands (component a) and the input of its C operands
+      //   C[(q, a)][16 t + jc] = sum_b M_q[a][b] g_{16 t + jc}(q, b)
+      // so per block of points a lane reads 6 gradient values and the point's
+      // metric (a broadcast within its 16-lane row group) for 3 (mass: 4)
+      // steps of NT (NT + 1) / 2 matrix instructions; padded points (q >=
+      // NQ) and columns (j >= N) give zero C values.
+      constexpr int QB = (NQ + 3) / 4;
+      auto load = [&](int qb, T (&g)[NT][DIM], T (&cv)[NT][DIM], T (&ph)[NT], T (&pm)[NT])
       {
-        int const code = sKA[4 * s + kr];
-        int const j = 16 * tj + jc;
-        bool const valid = code >= 0 && j < N;
-        int const c = code < 0 ? 0 : code;
-        int const q = c >> 2, a = c & 3;
-        int const jj = j < N ? j : N - 1;
-        double const * m = W.M[q][a];
-        double const * g = sGrad + (q * N + jj) * DIM;
-        double g0 = g[0];
-        if constexpr (MASS)
-          g0 = a == DIM ? sPhi[q * N + jj] : g0;
-        double v = m[0] * g0 + m[1] * g[1];
-        if constexpr (DIM == 3)
-          v += m[2] * g[2];
-        return valid ? (T)v : T(0);
-      };
-      // software-pipelined: the operands of step s+1 are read from LDS while the
-      // matrix instructions of step s run.  Not unrolled: it keeps the kernel
-      // at or below 128 VGPRs, i.e. two blocks (16 waves) per CU
-      T aC[NT], bC[NT];
-  #pragma unroll
-      for (int t = 0; t < NT; ++t)
-      {
-        aC[t] = sA[(t * STEPS) * 64 + lane];
-        bC[t] = cval(0, t);
-      }
-  #pragma unroll 1
-      for (int s = 0; s < STEPS; ++s)
-      {
-        int const sn = s + 1 < STEPS ? s + 1 : s;
-        T aN[NT], bN[NT];
-  #pragma unroll
+        int const q = 4 * qb + kr;
+        bool const qv = q < NQ;
+        int const qq = qv ? q : NQ - 1;
+        double const * m = &W.M[qq][0][0];
+        double Mq[DIM][DIM];
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+#pragma unroll
+          for (int b = 0; b < DIM; ++b)
+            Mq[a][b] = m[3 * a + b];
+        double const mm = MASS ? W.M[qq][DIM][0] : 0.;
+#pragma unroll
         for (int t = 0; t < NT; ++t)
         {
-          aN[t] = sA[(t * STEPS + sn) * 64 + lane];
-          bN[t] = cval(sn, t);
+          int const j = 16 * t + jc;
+          bool const v = qv && j < N;
+          int const jj = j < N ? j : N - 1;
+          double const * gp = sGrad + (qq * N + jj) * DIM;
+          double gd[DIM];
+#pragma unroll
+          for (int b = 0; b < DIM; ++b)
+            gd[b] = gp[b];
+#pragma unroll
+          for (int a = 0; a < DIM; ++a)
+          {
+            double c = 0.;
+#pragma unroll
+            for (int b = 0; b < DIM; ++b)
+              c += Mq[a][b] * gd[b];
+            g[t][a] = (T)gd[a];
+            cv[t][a] = v ? (T)c : T(0);
+          }
+          if constexpr (MASS)
+          {
+            double const p = sPhi[qq * N + jj];
+            ph[t] = (T)p;
+            pm[t] = v ? (T)(mm * p) : T(0);
+          }
         }
-  #pragma unroll
-        for (int tj = 0; tj < NT; ++tj)
-  #pragma unroll
-          for (int ti = 0; ti <= tj; ++ti)
-            acc[ti][tj] = Mfma<T>::mma(aC[ti], bC[tj], acc[ti][tj]);
-  #pragma unroll
-        for (int t = 0; t < NT; ++t)
+      };
+#pragma unroll 1
+      for (int qb = 0; qb < QB; ++qb)
+      {
+        // (other waves of the SIMD cover the LDS latency of the operands)
+        T gC[NT][DIM], cC[NT][DIM], pC[NT], qC[NT];
+        load(qb, gC, cC, pC, qC);
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+#pragma unroll
+          for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+            for (int ti = 0; ti <= tj; ++ti)
+              acc[ti][tj] = Mfma<T>::mma(gC[ti][a], cC[tj][a], acc[ti][tj]);
+        if constexpr (MASS)
         {
-          aC[t] = aN[t];
-          bC[t] = bN[t];
+#pragma unroll
+          for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+            for (int ti = 0; ti <= tj; ++ti)
+              acc[ti][tj] = Mfma<T>::mma(pC[ti], qC[tj], acc[ti][tj]);
         }
       }
     }
@@ -560,61 +573,10 @@ __global__ void k_reset_counters(int * c)
     c[threadIdx.x * kCounterStride] = 0;
 }
 
-// host: MFMA fragments of the constant operand A (N x K)
-template <typename T, int N, int DIM, int NQ, bool MASS>
-std::vector<T> buildFragments(FeTables const & tab)
-{
-  using S = Shape<N, DIM, NQ, MASS>;
-  std::vector<T> out((size_t)S::NT * S::STEPS * 64, T(0));
-  for (int ti = 0; ti < S::NT; ++ti)
-    for (int s = 0; s < S::STEPS; ++s)
-      for (int l = 0; l < 64; ++l)
-      {
-        int const i = 16 * ti + (l & 15), k = 4 * s + (l >> 4);
-        double v = 0.;
-        if (i < N)
-        {
-          if (k < S::KS)
-            v = tab.grad[((size_t)(k / DIM) * N + i) * DIM + k % DIM];
-          else if (MASS && k < S::KS + NQ)
-            v = tab.phi[(size_t)(k - S::KS) * N + i];
-        }
-        out[((size_t)ti * S::STEPS + s) * 64 + l] = (T)v;
-      }
-  return out;
-}
-
 template <typename T, int N, int DIM, int NPE, int NQ, bool MASS>
 void launchMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, double cMF, T * ke)
 {
   FeTables & tab = c->fe;
-  // cache key: precision and mass rows
-  int const key = (sizeof(T) == 8 ? 1 : 2) | (MASS ? 4 : 0);
-  T const * dfrag = nullptr;
-  if constexpr (Shape<N, DIM, NQ, MASS>::QM)
-    ; // q-major elements read the gradient table directly
-  else if constexpr (sizeof(T) == 8)
-  {
-    if (tab.afragKey64 != key)
-    {
-      auto frag = buildFragments<T, N, DIM, NQ, MASS>(tab);
-      tab.d_afrag.upload(frag.data(), frag.size(), c->stream);
-      PXG_HIP(hipStreamSynchronize(c->stream));
-      tab.afragKey64 = key;
-    }
-    dfrag = tab.d_afrag.p;
-  }
-  else
-  {
-    if (tab.afragKey32 != key)
-    {
-      auto frag = buildFragments<T, N, DIM, NQ, MASS>(tab);
-      tab.d_afrag32.upload(frag.data(), frag.size(), c->stream);
-      PXG_HIP(hipStreamSynchronize(c->stream));
-      tab.afragKey32 = key;
-    }
-    dfrag = tab.d_afrag32.p;
-  }
   int dev = 0, ncu = 0;
   PXG_HIP(hipGetDevice(&dev));
   PXG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -631,7 +593,7 @@ void launchMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, double
   k_reset_counters<<<1, 64, 0, c->stream>>>(c->workCounter.p);
   k_element_mfma<T, N, DIM, NPE, NQ, MASS><<<grid, 64 * kWaves, 0, c->stream>>>(
       c->nelem, c->xyz.p, c->conn.p, c->elemMap.p, c->kappa.p, c->bcFlag.p, c->bcVal.p, c->hasBC ? 1 : 0,
-      tab.d_geoGrad.p, tab.d_w.p, tab.d_phi.p, tab.d_grad.p, dfrag, cS, cSF, cM, cMF, flags, ke, c->feBuf.p,
+      tab.d_geoGrad.p, tab.d_w.p, tab.d_phi.p, tab.d_grad.p, cS, cSF, cM, cMF, flags, ke, c->feBuf.p,
       c->status.p, c->workCounter.p);
   PXG_HIP(hipGetLastError());
 }

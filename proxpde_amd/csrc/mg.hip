@@ -869,8 +869,9 @@ int pxg_mg_update(pxg_mg * m)
   {
     PXG_HIP(hipSetDevice(m->ctx[0]->device));
     m->s = m->ctx[0]->stream;
-    for (auto * c: m->ctx)
-      PXG_REQUIRE(c->closed, PXG_E_STATE, "levels must be closed");
+    // the coarsest matrix is factored here; the others are checked by the
+    // solve (level 0 need not be assembled for PXG_FP32_MF)
+    PXG_REQUIRE(m->ctx.back()->closed, PXG_E_STATE, "the coarsest level must be closed");
     factorCoarse(m);
     m->err.clear();
     return PXG_OK;

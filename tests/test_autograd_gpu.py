@@ -104,7 +104,9 @@ def test_kappa_solve_fp32_refined_matches_fp64(ag, gpu, fe, ns, variant):
     b = torch.from_numpy(solver.rhs(1.0)).to(gpu)
     mode = N.FP32_MF if variant == "fp32_mf" else N.FP32
     out = {}
-    for name, md, f64 in (("fp64", N.FP64, False), (variant, mode, variant == "fp32_fine64")):
+    # the refined form first: straight after solver.rhs() (which clears the
+    # finest level), before any FP64 solve has closed it again
+    for name, md, f64 in ((variant, mode, variant == "fp32_fine64"), ("fp64", N.FP64, False)):
         k = kap.clone().requires_grad_()
         u = solver(k, b, rtol=1e-8, mode=md, fine_fp64=f64)
         (0.5 * (u ** 2).sum()).backward()
