@@ -56,6 +56,7 @@ def lib():
         L.orc_mesh_gmsh.argtypes = [vp, C.c_char_p, i32]
         L.orc_set_coords.argtypes = [vp, dp]
         L.orc_set_qr.argtypes = [vp, i32, dp, dp]
+        L.orc_set_threads.argtypes = [vp, i32]
         L.orc_info.restype = i64
         L.orc_info.argtypes = [vp, i32]
         L.orc_get_mesh.argtypes = [vp, dp, u32p]
@@ -146,6 +147,11 @@ class Space:
         xyz = _f64(xyz)
         assert xyz.shape == (self.npts, 3)
         self.L.orc_set_coords(self.h, _p(xyz))
+
+    def set_threads(self, n: int):
+        """Threads of the Builder's element loop and close (1: serial, as the
+        reference); the results are bitwise the same for any count."""
+        self.L.orc_set_threads(self.h, int(n))
 
     def bfacets(self):
         nb = self.L.orc_info(self.h, 7)

@@ -126,6 +126,13 @@ int orc_set_qr(void * p, int nq, double const * nodes, double const * weights)
   return 0;
 }
 
+// threads of the Builder's element loop and close (1: the reference's serial loop)
+int orc_set_threads(void * p, int n)
+{
+  static_cast<Ctx *>(p)->builder.threads = n < 1 ? 1 : n;
+  return 0;
+}
+
 // 0 npts, 1 nelem, 2 points/elem, 3 ndofs, 4 local dofs, 5 quadrature points,
 // 6 nnz (closed matrix), 7 boundary facets
 int64_t orc_info(void * p, int what)

@@ -1001,103 +1001,151 @@ static void gatherVtx(Mesh const & m, i64 e, std::vector<double> & vtx)
       vtx[3 * p + k] = m.coords[3ull * m.conn[e * m.npe + p] + k];
 }
 
+// element matrix and right-hand side of element e (buildLhs loop body,
+// include/builder.hpp:53-128): terms in call order, per-element Dirichlet
+// elimination; Ke is n x n row-major, Fe n values
+static void elementLhs(
+    Mesh const & m, CurFE & cfe, DofMap const & d, std::vector<Term> const & terms,
+    std::vector<uint8_t> const & bcFlag, std::vector<double> const & bcVal, i64 e, double * Ke, double * Fe,
+    std::vector<double> & h, std::vector<double> & vtx, std::vector<double> & kloc, std::vector<int> & cons)
+{
+  int const n = cfe.n, nq = cfe.nq;
+  bool const hasBC = !bcFlag.empty();
+  gatherVtx(m, e, vtx);
+  cfe.reinit(vtx.data());
+  u32 const * ids = &d.elemMap[e * n];
+  std::fill(Ke, Ke + n * n, 0.);
+  std::fill(Fe, Fe + n, 0.);
+  for (auto const & t: terms)
+  {
+    bool const fe = t.kind == TERM_STIFFNESS_FE || t.kind == TERM_MASS_FE;
+    if (fe)
+      for (int i = 0; i < n; ++i)
+        kloc[i] = t.kappa[ids[i]];
+    for (int q = 0; q < nq; ++q)
+    {
+      double kq = 0.;
+      if (fe)
+        for (int i = 0; i < n; ++i)
+          kq += cfe.phiRef[q * n + i] * kloc[i];
+      double s = 0.;
+      switch (t.kind)
+      {
+      case TERM_STIFFNESS:
+      case TERM_MASS: s = t.coef * cfe.JxW[q]; break;
+      case TERM_STIFFNESS_FE: s = kq * cfe.JxW[q]; break;
+      case TERM_MASS_FE: s = (t.coef * kq) * cfe.JxW[q]; break;
+      }
+      bool const stiff = t.kind == TERM_STIFFNESS || t.kind == TERM_STIFFNESS_FE;
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+        {
+          double v;
+          if (stiff)
+          {
+            double const * gi = &cfe.dphi[(q * n + i) * 3];
+            double const * gj = &cfe.dphi[(q * n + j) * 3];
+            v = gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2];
+          }
+          else
+            v = cfe.phiRef[q * n + i] * cfe.phiRef[q * n + j];
+          Ke[i * n + j] += s * v;
+        }
+    }
+  }
+  // Dirichlet elimination, include/builder.hpp:83-124
+  if (hasBC)
+  {
+    bool any = false;
+    for (int i = 0; i < n; ++i)
+    {
+      cons[i] = bcFlag[ids[i]];
+      h[i] = cons[i] ? bcVal[ids[i]] : 0.;
+      any = any || cons[i];
+    }
+    if (any)
+    {
+      for (int i = 0; i < n; ++i)
+      {
+        double s = 0.;
+        for (int j = 0; j < n; ++j)
+          s += Ke[i * n + j] * h[j];
+        Fe[i] = cons[i] ? 0. : (Fe[i] - s);
+      }
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+          if (cons[i] || cons[j])
+            Ke[i * n + j] = 0.;
+      for (int i = 0; i < n; ++i)
+        if (cons[i])
+        {
+          Ke[i * n + i] = 1.0; // BCEss::diag, include/bc.hpp:301
+          Fe[i] = h[i];
+        }
+    }
+  }
+}
+
 void Builder::buildLhs(
     Mesh const & m, CurFE & cfe, DofMap const & d, std::vector<Term> const & terms,
     std::vector<uint8_t> const & bcFlag, std::vector<double> const & bcVal)
 {
-  int const n = cfe.n, nq = cfe.nq;
-  std::vector<double> Ke(n * n), Fe(n), h(n), vtx, kloc(n);
-  std::vector<int> cons(n);
-  bool const hasBC = !bcFlag.empty();
-  trow.reserve(trow.size() + (size_t)m.nelem() * n * n);
-  tcol.reserve(tcol.size() + (size_t)m.nelem() * n * n);
-  tval.reserve(tval.size() + (size_t)m.nelem() * n * n);
-  for (i64 e = 0; e < m.nelem(); ++e)
+  int const n = cfe.n;
+  i64 const E = m.nelem();
+  size_t const nn = (size_t)n * n, base = trow.size();
+  trow.resize(base + (size_t)E * nn);
+  tcol.resize(base + (size_t)E * nn);
+  tval.resize(base + (size_t)E * nn);
+  if (threads <= 1)
   {
-    gatherVtx(m, e, vtx);
-    cfe.reinit(vtx.data());
-    u32 const * ids = &d.elemMap[e * n];
-    std::fill(Ke.begin(), Ke.end(), 0.);
-    std::fill(Fe.begin(), Fe.end(), 0.);
-    for (auto const & t: terms)
+    // the reference's serial loop: triplets and b in element order
+    std::vector<double> Ke(nn), Fe(n), h(n), vtx, kloc(n);
+    std::vector<int> cons(n);
+    for (i64 e = 0; e < E; ++e)
     {
-      bool const fe = t.kind == TERM_STIFFNESS_FE || t.kind == TERM_MASS_FE;
-      if (fe)
-        for (int i = 0; i < n; ++i)
-          kloc[i] = t.kappa[ids[i]];
-      for (int q = 0; q < nq; ++q)
-      {
-        double kq = 0.;
-        if (fe)
-          for (int i = 0; i < n; ++i)
-            kq += cfe.phiRef[q * n + i] * kloc[i];
-        double s = 0.;
-        switch (t.kind)
-        {
-        case TERM_STIFFNESS:
-        case TERM_MASS: s = t.coef * cfe.JxW[q]; break;
-        case TERM_STIFFNESS_FE: s = kq * cfe.JxW[q]; break;
-        case TERM_MASS_FE: s = (t.coef * kq) * cfe.JxW[q]; break;
-        }
-        bool const stiff = t.kind == TERM_STIFFNESS || t.kind == TERM_STIFFNESS_FE;
-        for (int i = 0; i < n; ++i)
-          for (int j = 0; j < n; ++j)
-          {
-            double v;
-            if (stiff)
-            {
-              double const * gi = &cfe.dphi[(q * n + i) * 3];
-              double const * gj = &cfe.dphi[(q * n + j) * 3];
-              v = gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2];
-            }
-            else
-              v = cfe.phiRef[q * n + i] * cfe.phiRef[q * n + j];
-            Ke[i * n + j] += s * v;
-          }
-      }
-    }
-    // Dirichlet elimination, include/builder.hpp:83-124
-    if (hasBC)
-    {
-      bool any = false;
+      elementLhs(m, cfe, d, terms, bcFlag, bcVal, e, Ke.data(), Fe.data(), h, vtx, kloc, cons);
+      u32 const * ids = &d.elemMap[e * n];
       for (int i = 0; i < n; ++i)
       {
-        cons[i] = bcFlag[ids[i]];
-        h[i] = cons[i] ? bcVal[ids[i]] : 0.;
-        any = any || cons[i];
-      }
-      if (any)
-      {
-        for (int i = 0; i < n; ++i)
+        b[ids[i]] += Fe[i];
+        for (int j = 0; j < n; ++j)
         {
-          double s = 0.;
-          for (int j = 0; j < n; ++j)
-            s += Ke[i * n + j] * h[j];
-          Fe[i] = cons[i] ? 0. : (Fe[i] - s);
+          size_t const k = base + e * nn + (size_t)i * n + j;
+          trow[k] = ids[i];
+          tcol[k] = ids[j];
+          tval[k] = Ke[i * n + j];
         }
-        for (int i = 0; i < n; ++i)
-          for (int j = 0; j < n; ++j)
-            if (cons[i] || cons[j])
-              Ke[i * n + j] = 0.;
-        for (int i = 0; i < n; ++i)
-          if (cons[i])
-          {
-            Ke[i * n + i] = 1.0; // BCEss::diag, include/bc.hpp:301
-            Fe[i] = h[i];
-          }
       }
     }
-    for (int i = 0; i < n; ++i)
+    return;
+  }
+  // threaded element loop (the CPU baseline on all host cores): every
+  // element writes its own triplet range; b is summed afterwards in element
+  // order, so the result is bitwise the serial one
+  std::vector<double> FeAll((size_t)E * n);
+#pragma omp parallel num_threads(threads)
+  {
+    CurFE local = cfe;
+    std::vector<double> Ke(nn), h(n), vtx, kloc(n);
+    std::vector<int> cons(n);
+#pragma omp for schedule(static)
+    for (i64 e = 0; e < E; ++e)
     {
-      b[ids[i]] += Fe[i];
-      for (int j = 0; j < n; ++j)
-      {
-        trow.push_back(ids[i]);
-        tcol.push_back(ids[j]);
-        tval.push_back(Ke[i * n + j]);
-      }
+      elementLhs(m, local, d, terms, bcFlag, bcVal, e, Ke.data(), &FeAll[(size_t)e * n], h, vtx, kloc, cons);
+      u32 const * ids = &d.elemMap[e * n];
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+        {
+          size_t const k = base + e * nn + (size_t)i * n + j;
+          trow[k] = ids[i];
+          tcol[k] = ids[j];
+          tval[k] = Ke[i * n + j];
+        }
     }
   }
+  for (i64 e = 0; e < E; ++e)
+    for (int i = 0; i < n; ++i)
+      b[d.elemMap[e * n + i]] += FeAll[(size_t)e * n + i];
 }
 
 DofMap vectorDofMap(DofMap const & s, int dim, bool interleaved, u32 offset)
@@ -1349,6 +1397,8 @@ void Builder::buildRhs(
 
 Csr Builder::close(double pruneEps) const
 {
+  if (threads > 1)
+    return closeThreaded(pruneEps);
   size_t const nt = trow.size();
   // counting sort by row (stable), then stable sort by column inside each row:
   // duplicates end up adjacent in insertion order and are summed left to right
@@ -1388,6 +1438,68 @@ Csr Builder::close(double pruneEps) const
     }
     A.rowptr[r + 1] = (i64)A.col.size();
   }
+  return A;
+}
+
+// close() with the rows sorted and summed in parallel: the same per-row
+// stable sort and left-to-right sums, so the same CSR bit for bit
+Csr Builder::closeThreaded(double pruneEps) const
+{
+  size_t const nt = trow.size();
+  std::vector<i64> cnt(ndofs + 1, 0);
+  for (size_t k = 0; k < nt; ++k)
+    cnt[trow[k] + 1]++;
+  for (i64 r = 0; r < ndofs; ++r)
+    cnt[r + 1] += cnt[r];
+  std::vector<size_t> ord(nt);
+  {
+    std::vector<i64> pos(cnt.begin(), cnt.end() - 1);
+    for (size_t k = 0; k < nt; ++k)
+      ord[pos[trow[k]]++] = k;
+  }
+  // per row: the kept (column, sum) pairs at the start of its segment
+  std::vector<u32> ucol(nt);
+  std::vector<double> uval(nt);
+  std::vector<i64> kept(ndofs + 1, 0);
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1024)
+  for (i64 r = 0; r < ndofs; ++r)
+  {
+    auto b0 = ord.begin() + cnt[r], b1 = ord.begin() + cnt[r + 1];
+    std::stable_sort(b0, b1, [&](size_t a, size_t c) { return tcol[a] < tcol[c]; });
+    i64 u = cnt[r];
+    for (auto it = b0; it != b1;)
+    {
+      u32 const c = tcol[*it];
+      double s = tval[*it];
+      ++it;
+      while (it != b1 && tcol[*it] == c)
+      {
+        s += tval[*it];
+        ++it;
+      }
+      if (!(std::fabs(s) <= pruneEps))
+      {
+        ucol[u] = c;
+        uval[u] = s;
+        ++u;
+      }
+    }
+    kept[r + 1] = u - cnt[r];
+  }
+  Csr A;
+  A.nrows = ndofs;
+  A.rowptr.assign(ndofs + 1, 0);
+  for (i64 r = 0; r < ndofs; ++r)
+    A.rowptr[r + 1] = A.rowptr[r] + kept[r + 1];
+  A.col.resize(A.rowptr[ndofs]);
+  A.val.resize(A.rowptr[ndofs]);
+#pragma omp parallel for num_threads(threads) schedule(static)
+  for (i64 r = 0; r < ndofs; ++r)
+    for (i64 k = 0; k < kept[r + 1]; ++k)
+    {
+      A.col[A.rowptr[r] + k] = ucol[cnt[r] + k];
+      A.val[A.rowptr[r] + k] = uval[cnt[r] + k];
+    }
   return A;
 }
 

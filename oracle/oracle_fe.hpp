@@ -231,6 +231,7 @@ struct Csr
 struct Builder
 {
   i64 ndofs = 0;
+  int threads = 1; // > 1: element loop and close in parallel (same results)
   std::vector<double> b;
   std::vector<u32> trow, tcol; // triplets, reference order
   std::vector<double> tval;
@@ -250,6 +251,7 @@ struct Builder
       double const * qvalues, std::vector<uint8_t> const & bcFlag);
   // setFromTriplets (sum duplicates in insertion order, sorted columns) + prune
   Csr close(double pruneEps) const;
+  Csr closeThreaded(double pruneEps) const;
   // buildLhs({AssemblyStiffness(cS), AssemblyTensorStiffness(cT),
   // AssemblyScalarMass(cM)}) on a space of `dim` components (reference
   // builder.hpp:31-154, assembly_lhs.hpp:36-75, 115-173, 190-229; a zero

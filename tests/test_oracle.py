@@ -272,3 +272,25 @@ def test_mass_fe_integrates_linear_kappa(oracle, fe, mesh, n, pert):
     rp0, col0, v0, _ = o.assemble([("mass", 1.0)], prune=-1.0)
     assert np.array_equal(rp1, rp0) and np.array_equal(col1, col0)
     assert np.abs(v1 - c * v0).max() <= 4e-16 * np.abs(v0).max()
+
+
+@pytest.mark.parametrize("fe,mesh,n,terms", [("hex_q2", "cube_hex", (4, 3, 3), [("stiffness_fe", 1.0), ("mass", 1.0)]),
+                                             ("tet_p2", "cube_tet6", (3, 3, 2), [("stiffness_fe", 1.0)]),
+                                             ("quad_q1", "square", (9, 7), [("stiffness", 1.0), ("mass_fe", 2.0)])])
+def test_threaded_builder_is_bitwise_serial(oracle, fe, mesh, n, terms):
+    """The threaded Builder (the CPU baseline on all host cores) gives the
+    serial one's CSR and b bit for bit (own triplet range per element, b and
+    the per-row sums in the serial order)."""
+    ln = (1.0, 1.0, 1.0) if len(n) == 3 else (1.0, 1.0, 0.0)
+    out = []
+    for threads in (1, 4):
+        o = oracle.Space(fe, mesh, n, length=ln)
+        o.set_threads(threads)
+        xd = o.dof_coords()
+        kap = np.exp(np.sin(3 * xd[:, 0]) * np.cos(2 * xd[:, 1]))
+        flag = np.zeros(o.ndofs, np.uint8)
+        flag[o.bc_dofs("left")] = 1
+        val = np.where(flag == 1, 1.0 + xd[:, 1], 0.0)
+        out.append(o.assemble(terms, kappa=kap, bc=(flag, val), rhs="const", rhs_params=(1.0,)))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
