@@ -49,6 +49,7 @@ def kappa_apply_batch(args, dev, a, kappa_host, nb=8):
     from proxpde_amd import autograd as A
 
     op = A.KappaOperator(a)
+    a.set_stream(torch.cuda.current_stream().cuda_stream)
     g = torch.Generator(device=dev)
     g.manual_seed(3)
     kap = torch.from_numpy(kappa_host).to(dev)
@@ -74,11 +75,33 @@ def kappa_apply_batch(args, dev, a, kappa_host, nb=8):
         (y * w).sum().backward()
     torch.cuda.synchronize()
     t_fb = (time.perf_counter() - t0) / reps
+    # the operator's own launches (element kernel + per-dof gather), HIP events
+    # on the context stream, without the autograd wrapper and its host wait
+    kap_c = kap.contiguous()
+    y = torch.empty_like(x)
+    gk = torch.empty_like(kap)
+    a.profile(True)
+    for _ in range(reps):
+        a.mf_apply_dev(kap_c.data_ptr(), nb, x.data_ptr(), y.data_ptr())
+        a.mf_grad_kappa_dev(nb, x.data_ptr(), w.data_ptr(), gk.data_ptr())
+    ap_ms, ap_n = a.profile_get("mf_apply")
+    gr_ms, gr_n = a.profile_get("mf_gradk")
+    a.profile(False)
     E = a.nelem
+    N_ = a.ndofs
     # algorithmic work of the forward apply (reference formulation): Q [1140 geometry + 2n kappa + nb (12 n + 5)]
     fl = E * 27 * (1140 + 2 * 27 + nb * (12 * 27 + 5))
+    # compulsory bytes of the apply: x in and y out (nb vectors), kappa, the
+    # vertices (65^3) and the vertex and dof ids of every element
+    nv = (round(E ** (1 / 3)) + 1) ** 3
+    by = 2 * nb * N_ * 8 + 8 * N_ + 24 * nv + 4 * E * (8 + 27)
+    t_ap = ap_ms / max(ap_n, 1) * 1e-3
     return {"batch": nb, "forward_ms": 1e3 * t_fwd, "forward_backward_ms": 1e3 * t_fb,
-            "forward_gflops": fl / t_fwd / 1e9}
+            "forward_gflops": fl / t_fwd / 1e9, "apply_kernels_ms": 1e3 * t_ap,
+            "gradk_kernels_ms": gr_ms / max(gr_n, 1), "apply_bytes": by,
+            "roofline": {"kernel": "k_sf_elem<double,3,0,8> + k_gather", "bound": "hbm",
+                         "achieved": by / t_ap / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": by / t_ap / 1e9 / HBM_PEAK_GBS, "traffic": None}}
 
 
 def solve_and_adjoint(args, dev, N, W):
@@ -521,16 +544,18 @@ def main():
     pk_active = a.packed()
 
     # one weighted-Jacobi sweep (the multigrid smoother, omega 2/3) on the packed
-    # copy, in place on x: matrix + x, b, the inverse diagonal read and x written
+    # copy: matrix + x, b, the inverse diagonal read and x written.  Two sweeps
+    # per call of pxg_jacobi_noalloc (x -> its scratch vector -> x, no copy and
+    # no host wait between calls), the time per sweep reported
     def time_jacobi(xv, bv, prec):
         for _ in range(3):
-            a.jacobi_dev(bv.data_ptr(), xv.data_ptr(), 2.0 / 3.0, 1, precision=prec)
+            a.jacobi_noalloc_dev(bv.data_ptr(), xv.data_ptr(), 2.0 / 3.0, 2, precision=prec)
         ev0.record(stream)
         for _ in range(args.spmv):
-            a.jacobi_dev(bv.data_ptr(), xv.data_ptr(), 2.0 / 3.0, 1, precision=prec)
+            a.jacobi_noalloc_dev(bv.data_ptr(), xv.data_ptr(), 2.0 / 3.0, 2, precision=prec)
         ev1.record(stream)
         torch.cuda.synchronize()
-        return ev0.elapsed_time(ev1) / args.spmv
+        return ev0.elapsed_time(ev1) / (2 * args.spmv)
 
     bj = torch.rand(prob.ndofs, dtype=torch.float64, device=dev, generator=gen) * 2 - 1
     jx, jx32 = y.clone(), y32.clone()

@@ -24,11 +24,13 @@
 // (include/builder.hpp:83-124).  The global sums (scatter in element order)
 // then give the same bits, so the pruned pattern is the same entry for entry.
 //
-// One block of 128 threads takes EPB elements: phase 0 stages vertex
-// coordinates, coefficient values and Dirichlet data in LDS; phase 1 has one
-// thread per (element, point) building the geometry and all physical
-// gradients of that point into LDS; phase 2 has one thread per (element,
-// row) contracting its row.  The geometry never goes to HBM.
+// Each wavefront takes its own EPW elements (64 / max(n, nq)): phase 0
+// stages vertex coordinates, coefficient values and Dirichlet data in LDS;
+// phase 1 has one lane per (element, point) building the geometry and all
+// physical gradients of that point into LDS; phase 2 computes every entry of
+// the upper triangle once (lane per (element, row), cyclic pairs) into the
+// element matrix in LDS; phase 3 has one lane per (element, row) for the
+// Dirichlet elimination and the row store.  The geometry never goes to HBM.
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -41,7 +43,7 @@ namespace pxg
 
 namespace
 {
-constexpr int kExactThreads = 128;
+constexpr int kExactWaves = 2; // wavefronts per block; each works on its own elements
 constexpr int kExactMaxTerms = 8;
 
 struct ExactTerms
@@ -51,32 +53,49 @@ struct ExactTerms
   double coef[kExactMaxTerms];
 };
 
+__device__ inline void exactSync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS doubles per element of the kernel below
+__host__ __device__ constexpr int exactLdsPerElem(int N, int nq, int nterms)
+{
+  return N * 3 + 2 * N + nq * nterms + (nq * N * 3 > N * N ? nq * N * 3 : N * N) + N; // + flags
+}
+
 template <int N, int DIM>
-__global__ void __launch_bounds__(kExactThreads)
-    k_element_exact(int64_t E, int nq, int epb, double const * __restrict__ xyz, uint32_t const * __restrict__ conn,
+__global__ void __launch_bounds__(64 * kExactWaves)
+    k_element_exact(int64_t E, int nq, int epw, double const * __restrict__ xyz, uint32_t const * __restrict__ conn,
                     double const * __restrict__ grad, double const * __restrict__ phi, double const * __restrict__ w,
                     uint32_t const * __restrict__ emap, double const * __restrict__ kappa, ExactTerms terms,
                     uint8_t const * __restrict__ bcFlag, double const * __restrict__ bcVal, int hasBC,
                     double * __restrict__ ke, double * __restrict__ fe, int * __restrict__ status)
 {
-  // LDS: vertex coordinates, coefficient and Dirichlet data per element,
-  // term weights per (element, point, term), physical gradients per
-  // (element, point, basis)
+  // LDS of a wave: vertex coordinates, coefficient and Dirichlet data per
+  // element, term weights per (element, point, term), physical gradients
+  // per (element, point, basis) -- reused for the element matrices once
+  // the contractions are done -- and the constrained flags
   extern __shared__ double lds[];
-  double * sX = lds;                       // epb x N x 3
-  double * sK = sX + epb * N * 3;          // epb x N
-  double * sH = sK + epb * N;              // epb x N Dirichlet values
-  double * sS = sH + epb * N;              // epb x nq x terms.n
-  double * sD = sS + epb * nq * terms.n;   // epb x nq x N x 3
-  int * sC = reinterpret_cast<int *>(sD + epb * nq * N * 3); // epb x N constrained flags
-  int64_t const e0 = (int64_t)blockIdx.x * epb;
-  int const ne = (int)std::min<int64_t>(epb, E - e0);
+  int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int const per = exactLdsPerElem(N, nq, terms.n);
+  double * sX = lds + (size_t)wave * epw * per; // epw x N x 3
+  double * sK = sX + epw * N * 3;               // epw x N
+  double * sH = sK + epw * N;                   // epw x N Dirichlet values
+  double * sS = sH + epw * N;                   // epw x nq x terms.n
+  double * sD = sS + epw * nq * terms.n;        // epw x nq x N x 3, then epw x N x N
+  double * sC = sD + epw * (nq * N * 3 > N * N ? nq * N * 3 : N * N); // epw x N flags (0 / 1)
+  double * sKe = sD;
+  int64_t const e0 = ((int64_t)blockIdx.x * kExactWaves + wave) * epw;
+  int const ne = (int)(e0 < E ? std::min<int64_t>(epw, E - e0) : 0);
   bool needK = false;
   for (int t = 0; t < terms.n; ++t)
     needK = needK || terms.kind[t] == PXG_TERM_STIFFNESS_FE || terms.kind[t] == PXG_TERM_MASS_FE;
 
   // phase 0: element data
-  for (int k = threadIdx.x; k < ne * N; k += kExactThreads)
+  for (int k = lane; k < ne * N; k += 64)
   {
     int64_t const ge = e0 * N + k;
     uint32_t const pid = conn[ge];
@@ -86,13 +105,13 @@ __global__ void __launch_bounds__(kExactThreads)
     uint32_t const id = emap[ge];
     sK[k] = needK ? kappa[id] : 0.;
     bool const con = hasBC && bcFlag[id];
-    sC[k] = con;
+    sC[k] = con ? 1. : 0.;
     sH[k] = con ? bcVal[id] : 0.;
   }
-  __syncthreads();
+  exactSync();
 
   // phase 1: geometry of one (element, point), gradients of every basis
-  for (int k = threadIdx.x; k < ne * nq; k += kExactThreads)
+  for (int k = lane; k < ne * nq; k += 64)
   {
     int const el = k / nq, q = k - el * nq;
     double const * x = sX + el * N * 3;
@@ -200,79 +219,113 @@ __global__ void __launch_bounds__(kExactThreads)
         d[i * 3 + r] = s;
       }
   }
-  __syncthreads();
+  exactSync();
 
-  // phase 2: row i of one element
-  for (int k = threadIdx.x; k < ne * N; k += kExactThreads)
-  {
-    int const el = k / N, i = k - el * N;
-    double acc[N];
+  // phase 2: every entry of the upper triangle once.  Lane (element, i)
+  // takes the pairs (i, (i + d) mod N) for d = 0 .. N/2 (d = N/2 only for
+  // i < N/2 when N is even): each unordered pair exactly once.  The value of
+  // (i, j) and of (j, i) are the same bits (each product is commutative and
+  // the sums run in the same order), so both triangles take it.
+  constexpr int ND = N / 2 + 1;
+  double acc[ND];
+  int const el2 = lane / N, i2 = lane - el2 * N;
+  bool const live2 = lane < 64 - 64 % N && el2 < ne;
 #pragma unroll
-    for (int j = 0; j < N; ++j)
-      acc[j] = 0.;
+  for (int dd = 0; dd < ND; ++dd)
+    acc[dd] = 0.;
+  if (live2)
+  {
     for (int t = 0; t < terms.n; ++t)
     {
       bool const stiff = terms.kind[t] == PXG_TERM_STIFFNESS || terms.kind[t] == PXG_TERM_STIFFNESS_FE;
       for (int q = 0; q < nq; ++q)
       {
-        double const s = sS[(el * nq + q) * terms.n + t];
+        double const s = sS[(el2 * nq + q) * terms.n + t];
         if (stiff)
         {
-          double const * d = sD + (el * nq + q) * N * 3;
-          double const g0 = d[i * 3], g1 = d[i * 3 + 1], g2 = d[i * 3 + 2];
+          double const * d = sD + (el2 * nq + q) * N * 3;
+          double const g0 = d[i2 * 3], g1 = d[i2 * 3 + 1], g2 = d[i2 * 3 + 2];
 #pragma unroll
-          for (int j = 0; j < N; ++j)
+          for (int dd = 0; dd < ND; ++dd)
           {
+            int const j = (i2 + dd) % N;
             double const v = g0 * d[j * 3] + g1 * d[j * 3 + 1] + g2 * d[j * 3 + 2];
-            acc[j] += s * v;
+            acc[dd] += s * v;
           }
         }
         else
         {
-          double const pi = phi[q * N + i];
+          double const pi = phi[q * N + i2];
 #pragma unroll
-          for (int j = 0; j < N; ++j)
+          for (int dd = 0; dd < ND; ++dd)
           {
+            int const j = (i2 + dd) % N;
             double const v = pi * phi[q * N + j];
-            acc[j] += s * v;
+            acc[dd] += s * v;
           }
         }
       }
     }
-    // Dirichlet elimination: Fe_i = -sum_j Ke_ij h_j on free rows (h_j = 0
-    // on free columns adds exact zeros, skipped), unit row and Fe_i = h_i on
-    // constrained ones, constrained columns cleared
+  }
+  exactSync(); // the gradients are read: their LDS takes the element matrices
+  if (live2)
+  {
+    double * K = sKe + el2 * N * N;
+#pragma unroll
+    for (int dd = 0; dd < ND; ++dd)
+    {
+      if (N % 2 == 0 && dd == N / 2 && i2 >= N / 2)
+        continue;
+      int const j = (i2 + dd) % N;
+      K[i2 * N + j] = acc[dd];
+      K[j * N + i2] = acc[dd];
+    }
+  }
+  exactSync();
+
+  // phase 3: row i of one element, Dirichlet elimination, store.  Fe_i =
+  // -sum_j Ke_ij h_j on free rows (h_j = 0 on free columns adds exact
+  // zeros, skipped), unit row and Fe_i = h_i on constrained ones,
+  // constrained columns cleared (include/builder.hpp:83-124)
+  for (int k = lane; k < ne * N; k += 64)
+  {
+    int const el = k / N, i = k - el * N;
+    double row[N];
+    double const * K = sKe + el * N * N + i * N;
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      row[j] = K[j];
     double f = 0.;
     if (hasBC)
     {
       double const * h = sH + el * N;
-      int const * con = sC + el * N;
-      if (con[i])
+      double const * con = sC + el * N;
+      if (con[i] != 0.)
       {
         f = h[i];
 #pragma unroll
         for (int j = 0; j < N; ++j)
-          acc[j] = j == i ? 1. : 0.;
+          row[j] = j == i ? 1. : 0.;
       }
       else
       {
         double s = 0.;
 #pragma unroll
         for (int j = 0; j < N; ++j)
-          if (con[j])
+          if (con[j] != 0.)
           {
-            s += acc[j] * h[j];
-            acc[j] = 0.;
+            s += row[j] * h[j];
+            row[j] = 0.;
           }
         f = 0. - s;
       }
     }
-    int64_t const row = e0 * N + k;
-    double * out = ke + row * keRowStride(N);
+    int64_t const r = e0 * N + k;
+    double * out = ke + r * keRowStride(N);
 #pragma unroll
     for (int j = 0; j < N; ++j)
-      out[j] = acc[j];
-    fe[row] = f;
+      out[j] = row[j];
+    fe[r] = f;
   }
 }
 
@@ -281,15 +334,17 @@ void launchExact(pxg_ctx * c, ExactTerms const & terms)
 {
   FeTables const & T = c->fe;
   int const nq = T.nq;
-  int const epb = std::max(1, kExactThreads / std::max(N, nq));
-  size_t const lds =
-      sizeof(double) * (size_t)epb * (N * 3 + 2 * N + nq * terms.n + nq * N * 3) + sizeof(int) * (size_t)epb * N;
+  // elements per wave: every lane takes a point in the geometry phase and
+  // a row in the contraction
+  int const epw = std::max(1, 64 / std::max(N, nq));
+  size_t const lds = sizeof(double) * (size_t)kExactWaves * epw * exactLdsPerElem(N, nq, terms.n);
   PXG_REQUIRE(lds <= 65536, PXG_E_INVALID, "quadrature rule too large for the linear element kernel");
   int64_t const E = c->nelem;
-  unsigned const grid = (unsigned)((E + epb - 1) / epb);
+  int64_t const per = (int64_t)kExactWaves * epw;
+  unsigned const grid = (unsigned)((E + per - 1) / per);
   c->keBuf.resize((size_t)E * N * keRowStride(N));
-  k_element_exact<N, DIM><<<grid, kExactThreads, lds, c->stream>>>(
-      E, nq, epb, c->xyz.p, c->conn.p, T.d_grad.p, T.d_phi.p, T.d_w.p, c->elemMap.p, c->kappa.p, terms, c->bcFlag.p,
+  k_element_exact<N, DIM><<<grid, 64 * kExactWaves, lds, c->stream>>>(
+      E, nq, epw, c->xyz.p, c->conn.p, T.d_grad.p, T.d_phi.p, T.d_w.p, c->elemMap.p, c->kappa.p, terms, c->bcFlag.p,
       c->bcVal.p, c->hasBC, c->keBuf.p, c->feBuf.p, c->status.p);
   PXG_HIP(hipGetLastError());
 }
