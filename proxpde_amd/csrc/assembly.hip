@@ -1697,10 +1697,9 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
   if (!rowsOnly && !mixed)
   {
     ProfScope ps(c, precision == PXG_FP64 ? "element" : "element32");
-    // FP64 linear elements: the reference's evaluation order, so the
+    // linear elements: FP64 in the reference's evaluation order, so the
     // rounding residues that closeMatrix prunes come out the same
-    if (precision == PXG_FP64)
-      fused = elementExact(c, nterms, terms);
+    fused = elementExact(c, nterms, terms, precision);
     if (!fused)
       fused = elementMfma(c, flags, cS, cSF, cM, cMF, precision);
   }

@@ -23,9 +23,11 @@ void ensureVal32(pxg_ctx * c);
 void ensureVal64(pxg_ctx * c);
 // element_mfma.hip: fused geometry + MFMA element kernel; false if not applicable
 bool elementMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, double cMF, int precision);
-// element_exact.hip: FP64 linear elements (P1 line, Q1 quad, Q1 hex, P1 tet)
-// in the reference's evaluation order, geometry fused; false if not applicable
-bool elementExact(pxg_ctx * c, int nterms, pxg_term const * terms);
+// element_exact.hip: linear elements (P1 line, Q1 quad, Q1 hex, P1 tet)
+// computed in FP64 in the reference's evaluation order, geometry fused, into
+// keBuf (PXG_FP64) or rounded into keBuf32 (PXG_FP32, exact-arithmetic zeros
+// flushed); false if not applicable
+bool elementExact(pxg_ctx * c, int nterms, pxg_term const * terms, int precision);
 
 // matfree_sf.hip: sum-factorised matrix-free kappa-stiffness operator of the
 // Q1 / Q2 hexahedra with their tensor Gauss rules (element buffer [E][n][nb],
