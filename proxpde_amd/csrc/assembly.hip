@@ -975,7 +975,7 @@ __global__ void k_stage_maps(int n, int ns, StageBlock const * __restrict__ bloc
 // diagonal of the closed matrix (dinv in T, dinvO in the other precision, as
 // k_diag_inverse_split and the cast of the close would) and adds the Dirichlet
 // lifting of the elements to b (as k_rhs_scatter would, same order).
-template <typename T, int N, int NS, int P>
+template <typename T, int N, int NS, int P, bool PO>
 __global__ void __launch_bounds__(kBlock)
     k_scatter_staged(StageBlock const * __restrict__ blocks, uint32_t const * __restrict__ dofContrib,
                      T const * __restrict__ ke, uint16_t const * __restrict__ info, uint16_t const * __restrict__ sid,
@@ -1034,7 +1034,9 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
   __syncthreads();
-  // 2. element-matrix rows and lifting values of the pairs -> LDS
+  // 2. element-matrix rows and lifting values of the pairs -> LDS (PO: the
+  //    element kernel wrote them in pair order, rows of N values: one
+  //    contiguous run per block)
   {
     int const nst = np * N;
     T rv[kPer];
@@ -1042,10 +1044,15 @@ __global__ void __launch_bounds__(kBlock)
     for (int m = 0; m < kPer; ++m)
     {
       int const i = min(t + m * kBlock, nst - 1);
-      int const p = i / N, lj = i - p * N;
-      rv[m] = ke[(size_t)lpc[p] * NS + lj];
+      if constexpr (PO)
+        rv[m] = ke[(size_t)B.p0 * N + i];
+      else
+      {
+        int const p = i / N, lj = i - p * N;
+        rv[m] = ke[(size_t)lpc[p] * NS + lj];
+      }
     }
-    double const fv = t < np ? fe[lpc[t]] : 0.;
+    double const fv = t < np ? fe[PO ? B.p0 + t : lpc[t]] : 0.;
 #pragma unroll
     for (int m = 0; m < kPer; ++m)
       if (t + m * kBlock < nst)
@@ -1187,9 +1194,15 @@ void launchRebuild(pxg_ctx * c, T const * ke, T * valC, T * valS, hipStream_t s)
           dinv = c->dinv32.p;
           dinvO = c->dinv.p;
         }
-        k_scatter_staged<T, N, keRowStride(N), P><<<(unsigned)c->nStage, kBlock, 0, s>>>(
-            reinterpret_cast<StageBlock const *>(c->stageBlk.p), c->dofContrib.p, ke, c->stageInfo.p,
-            c->stageSid.p, valC, valS, c->changed.p, c->dofSeg.p, c->feBuf.p, c->b.p, dinv, dinvO);
+        auto const * blk = reinterpret_cast<StageBlock const *>(c->stageBlk.p);
+        if (c->kePairOrder)
+          k_scatter_staged<T, N, keRowStride(N), P, true><<<(unsigned)c->nStage, kBlock, 0, s>>>(
+              blk, c->dofContrib.p, ke, c->stageInfo.p, c->stageSid.p, valC, valS, c->changed.p, c->dofSeg.p,
+              c->feBuf.p, c->b.p, dinv, dinvO);
+        else
+          k_scatter_staged<T, N, keRowStride(N), P, false><<<(unsigned)c->nStage, kBlock, 0, s>>>(
+              blk, c->dofContrib.p, ke, c->stageInfo.p, c->stageSid.p, valC, valS, c->changed.p, c->dofSeg.p,
+              c->feBuf.p, c->b.p, dinv, dinvO);
       }
     };
     if (c->stagePairs == 32)
@@ -1693,6 +1706,13 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
     char const * v = std::getenv("PXG_ELEMENT_PATH");
     return v && std::string(v) == "rows";
   }();
+  // values are logically zero after clear / pattern build: overwrite them
+  bool const overwrite = c->valsZero;
+  // a rebuild after clear writes straight into the pruned layout of the last close
+  bool const direct = overwrite && c->layoutValid && c->layoutPrecision == precision;
+  // ... through the stage blocks: the fused element kernels write their rows
+  // in pair order
+  c->kePairOrder = direct && c->stageValid && c->nStage > 0 && c->pairPos.n == (size_t)(E * T.n);
   bool fused = false;
   if (!rowsOnly && !mixed)
   {
@@ -1703,6 +1723,8 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
     if (!fused)
       fused = elementMfma(c, flags, cS, cSF, cM, cMF, precision);
   }
+  if (!fused)
+    c->kePairOrder = false; // the geometry + row and Taylor-Hood kernels write element order
   if (!fused && !mixed)
   {
     c->geoBuf.resize(E * T.nq * 8);
@@ -1717,10 +1739,6 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                 });
     PXG_HIP(hipGetLastError());
   }
-  // values are logically zero after clear / pattern build: overwrite them
-  bool const overwrite = c->valsZero;
-  // a rebuild after clear writes straight into the pruned layout of the last close
-  bool const direct = overwrite && c->layoutValid && c->layoutPrecision == precision;
   c->stagedFused = false;
   if (!direct && !overwrite)
     unsplitValues(c); // accumulating: structural values first
@@ -2123,7 +2141,7 @@ void buildStageMaps(pxg_ctx * c)
   auto * blocks = reinterpret_cast<StageBlock *>(c->stageBlk.p);
   k_stage_blocks<<<gridFor(nb), kBlock, 0, s>>>(nb, c->stageR0.p, c->rowptrS.p, c->dofSeg.p, c->segPtr.p, c->pos.p,
                                                 blocks);
-  DevBuf<uint32_t> pairPos;
+  DevBuf<uint32_t> & pairPos = c->pairPos;
   pairPos.resize(npairs);
   k_pair_pos<<<gridFor(npairs), kBlock, 0, s>>>(npairs, c->dofContrib.p, pairPos.p);
   c->stageInfo.resize(std::max<int64_t>(c->nnzS, 1));

@@ -191,6 +191,7 @@ void dropPattern(pxg_ctx * c)
   c->stageBlk.release();
   c->stageInfo.release();
   c->stageSid.release();
+  c->pairPos.release();
   c->valS.release();
   c->valS32.release();
   c->b.release();

@@ -171,6 +171,12 @@ struct pxg_ctx
   pxg::DevBuf<int32_t> stageR0;   // nStage + 1 first rows
   pxg::DevBuf<uint32_t> stageBlk; // 8 words per block
   pxg::DevBuf<uint16_t> stageInfo, stageSid;
+  // position of every element-matrix row (e, i) in dofContrib order; a build
+  // that rebuilds through the stage blocks has its element kernel write row
+  // (e, i) at that position with stride n (kePairOrder), so a stage block
+  // reads its rows as one contiguous run
+  pxg::DevBuf<uint32_t> pairPos;
+  bool kePairOrder = false;
   uint64_t stamp = 0;      // counts closes: changes whenever the closed matrix may have changed
   pxg::DevBuf<double> valS;
   pxg::DevBuf<float> valS32;

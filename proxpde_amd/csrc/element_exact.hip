@@ -137,7 +137,8 @@ __global__ void __launch_bounds__(kExactThreads) __attribute__((amdgpu_waves_per
                     double const * __restrict__ grad, double const * __restrict__ phi, double const * __restrict__ w,
                     uint32_t const * __restrict__ emap, double const * __restrict__ kappa, ExactTerms terms,
                     uint8_t const * __restrict__ bcFlag, double const * __restrict__ bcVal, int hasBC,
-                    TO * __restrict__ ke, double * __restrict__ fe, int * __restrict__ status)
+                    TO * __restrict__ ke, double * __restrict__ fe, int * __restrict__ status,
+                    uint32_t const * __restrict__ rowPos)
 {
   constexpr int NP = N * (N + 1) / 2; // upper-triangle entries, row-major
   int64_t const e = (int64_t)blockIdx.x * kExactThreads + threadIdx.x;
@@ -300,8 +301,10 @@ __global__ void __launch_bounds__(kExactThreads) __attribute__((amdgpu_waves_per
         f = 0. - sum;
       }
     }
-    int64_t const r = e * N + i;
-    TO * out = ke + r * keRowStride(N);
+    // row (e, i) in element order, or at its pair position (rows of N values
+    // either way: the linear elements have no row padding)
+    int64_t const r = rowPos ? (int64_t)rowPos[e * N + i] : e * N + i;
+    TO * out = ke + r * N;
     if constexpr (sizeof(TO) == 8 && N % 2 == 0)
     {
 #pragma unroll
@@ -338,14 +341,16 @@ void launchExact(pxg_ctx * c, ExactTerms const & terms, int precision)
     c->keBuf32.resize(sz);
   if (E == 0)
     return;
+  static_assert(keRowStride(N) == N);
+  uint32_t const * rowPos = c->kePairOrder ? c->pairPos.p : nullptr;
   if (precision == PXG_FP64)
     k_element_exact<double, N, DIM><<<grid, kExactThreads, 0, c->stream>>>(
         E, T.nq, c->xyz.p, c->conn.p, T.d_grad.p, T.d_phi.p, T.d_w.p, c->elemMap.p, c->kappa.p, terms, c->bcFlag.p,
-        c->bcVal.p, c->hasBC, c->keBuf.p, c->feBuf.p, c->status.p);
+        c->bcVal.p, c->hasBC, c->keBuf.p, c->feBuf.p, c->status.p, rowPos);
   else
     k_element_exact<float, N, DIM><<<grid, kExactThreads, 0, c->stream>>>(
         E, T.nq, c->xyz.p, c->conn.p, T.d_grad.p, T.d_phi.p, T.d_w.p, c->elemMap.p, c->kappa.p, terms, c->bcFlag.p,
-        c->bcVal.p, c->hasBC, c->keBuf32.p, c->feBuf.p, c->status.p);
+        c->bcVal.p, c->hasBC, c->keBuf32.p, c->feBuf.p, c->status.p, rowPos);
   PXG_HIP(hipGetLastError());
 }
 } // namespace

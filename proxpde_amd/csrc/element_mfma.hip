@@ -99,6 +99,7 @@ struct WaveLds
   double f[N];  // lifting accumulator of the Dirichlet elimination
   double dg[sizeof(T) == 4 ? N : 1]; // FP32: diagonal of Ke, the flush threshold of noise entries
   int flag[N];
+  uint32_t rp[N]; // output row of each element row (element or pair order)
 };
 
 template <typename T, int N, int DIM, int NPE, int NQ, bool MASS>
@@ -110,7 +111,7 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
                    double const * __restrict__ phiG, double const * __restrict__ gradG,
                    double cS, double cSF, double cM, double cMF, int flags,
                    T * __restrict__ ke, double * __restrict__ fe, int * __restrict__ status,
-                   int * __restrict__ work)
+                   int * __restrict__ work, uint32_t const * __restrict__ rowPos)
 {
   using S = Shape<N, DIM, NQ, MASS>;
   using L = WaveLds<T, N, NPE, NQ, Shape<N, DIM, NQ, MASS>::MROWS>;
@@ -194,6 +195,7 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
   // gathered data of the element being processed, loaded one element ahead
   double rx = 0., rk = 0., rb = 0.;
   int rf = 0;
+  uint32_t rr = 0; // output row of element row `lane`
   auto gatherIds = [&](int64_t el, uint32_t & pid, uint32_t & id)
   {
     if (el < E)
@@ -201,7 +203,10 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
       if (lane < NPE * 3)
         pid = conn[el * NPE + lane / 3];
       if (lane < N)
+      {
         id = emap[el * N + lane];
+        rr = rowPos ? rowPos[el * N + lane] : (uint32_t)(el * N + lane);
+      }
     }
   };
   auto gatherVals = [&](int64_t el, uint32_t pid, uint32_t id)
@@ -235,6 +240,7 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
       W.flag[lane] = rf;
       W.bcv[lane] = rb;
       W.f[lane] = 0.;
+      W.rp[lane] = rr;
     }
     waveSync();
     uint32_t npid = 0, nid = 0;
@@ -469,8 +475,9 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
     //         unit rows; the lifting f = -Ke bcv uses the unmodified Ke, its
     //         lower part through symmetry.
     bool const anyBC = __any(lane < N && W.flag[lane] != 0);
-    constexpr int NS = keRowStride(N);
-    T * out = ke + e * (N * NS);
+    // row-major rows of NS values in element order, or of N values at the
+    // rows' pair positions
+    size_t const ns = rowPos ? (size_t)N : (size_t)keRowStride(N);
     if constexpr (sizeof(T) == 4)
     {
       // FP32: entries below 2^-19 sqrt(Ke_ii Ke_jj) are rounding noise of
@@ -539,9 +546,9 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
           }
           if (valid)
           {
-            out[i * NS + j] = v;
+            ke[(size_t)W.rp[i] * ns + j] = v;
             if (j > i)
-              out[j * NS + i] = v; // the lower triangle repeats the upper one exactly
+              ke[(size_t)W.rp[j] * ns + i] = v; // the lower triangle repeats the upper one exactly
           }
         }
     }
@@ -560,7 +567,7 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
       waveSync();
     }
     if (lane < N)
-      fe[e * N + lane] = anyBC ? (W.flag[lane] ? W.bcv[lane] : 0. - W.f[lane]) : 0.;
+      fe[W.rp[lane]] = anyBC ? (W.flag[lane] ? W.bcv[lane] : 0. - W.f[lane]) : 0.;
     waveSync(); // LDS of this element is reused by the next one
     e = en;
     en = elem(ennReq);
@@ -589,12 +596,13 @@ void launchMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, double
       (int)std::max<int64_t>(1, std::min<int64_t>(blocksNeeded, (int64_t)ncu * std::max(perCu, 1)));
   PXG_REQUIRE(c->nelem < ((int64_t)1 << 31) - (int64_t)grid * kWaves * 2, PXG_E_INVALID,
               "element count exceeds the 32-bit work counters");
+  PXG_REQUIRE(c->nelem * N < ((int64_t)1 << 32), PXG_E_INVALID, "element rows exceed 32-bit row positions");
   c->workCounter.resize(kCounters * kCounterStride);
   k_reset_counters<<<1, 64, 0, c->stream>>>(c->workCounter.p);
   k_element_mfma<T, N, DIM, NPE, NQ, MASS><<<grid, 64 * kWaves, 0, c->stream>>>(
       c->nelem, c->xyz.p, c->conn.p, c->elemMap.p, c->kappa.p, c->bcFlag.p, c->bcVal.p, c->hasBC ? 1 : 0,
       tab.d_geoGrad.p, tab.d_w.p, tab.d_phi.p, tab.d_grad.p, cS, cSF, cM, cMF, flags, ke, c->feBuf.p,
-      c->status.p, c->workCounter.p);
+      c->status.p, c->workCounter.p, c->kePairOrder ? c->pairPos.p : nullptr);
   PXG_HIP(hipGetLastError());
 }
 

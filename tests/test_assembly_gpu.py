@@ -183,12 +183,18 @@ def _bitwise_equal(x, y):
     return all(np.array_equal(np.asarray(p).view(np.uint8), np.asarray(q).view(np.uint8)) for p, q in zip(x, y))
 
 
-def test_rebuild_into_pruned_layout(oracle, gpu):
-    """A rebuild after clear writes into the pruned layout of the last close;
-    every outcome equals a fresh build bit for bit: same prune set (fast close),
-    changed prune set (all-zero stiffness, then back), accumulation onto a
-    direct rebuild, and FP32 rebuilds."""
-    prob, o, kappa, dofs, vals, bc = build_pair(oracle, "hex_q2", "cube_hex", (3, 3, 2))
+REBUILD = [("hex_q2", "cube_hex", (3, 3, 2)), ("tet_p2", "cube_tet5", (2, 2, 2)), ("hex_q1", "cube_hex", (5, 4, 3))]
+
+
+@pytest.mark.parametrize("fe,mesh,n", REBUILD, ids=[r[0] for r in REBUILD])
+def test_rebuild_into_pruned_layout(oracle, gpu, fe, mesh, n):
+    """A rebuild after clear writes into the pruned layout of the last close
+    (the element kernels then write their rows in the stage blocks' pair
+    order); every outcome equals a fresh build bit for bit: same prune set
+    (fast close), changed prune set (all-zero stiffness, then back),
+    accumulation onto a direct rebuild, and FP32 rebuilds."""
+    print(f"case rebuild {fe} {n}", flush=True)
+    prob, o, kappa, dofs, vals, bc = build_pair(oracle, fe, mesh, n)
     kappa2 = kappa[::-1].copy() * 1.5
 
     def fresh(steps, kap, precision=0):
