@@ -14,8 +14,8 @@
 // the input of its C operands.
 // The N x N product runs as v_mfma_f64_16x16x4f64 (FP64) or
 // v_mfma_f32_16x16x4f32 (FP32) tiles; only the upper tiles are multiplied
-// (Ke is symmetric) and stored to both triangles of the row-major Ke, so the
-// scatter reads whole element rows.  The epilogue applies the per-element
+// (Ke is symmetric): the diagonal tiles are stored whole, the off-diagonal
+// tile also as its transpose, so the scatter reads whole element rows.  The epilogue applies the per-element
 // Dirichlet elimination of Builder::buildLhs (reference
 // include/builder.hpp:83-124) on the accumulators and writes Ke plus the
 // lifting Fe.
@@ -467,8 +467,8 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
         }
       }
     }
-    // ---- 4. the upper triangle goes from the accumulators straight to the
-    //         row-major store, mirrored into the lower one: tile (ti, tj),
+    // ---- 4. the upper tiles go from the accumulators straight to the
+    //         row-major store (the off-diagonal one mirrored): tile (ti, tj),
     //         value r of a lane is D[i][j], i = 16 ti + row(r, lane/16),
     //         j = 16 tj + lane%16 (16 consecutive j per row are contiguous).
     //         Dirichlet elimination: rows/columns of constrained dofs become
@@ -544,11 +544,13 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
             if (W.flag[ii] | fj)
               v = (i == j) ? T(1) : T(0);
           }
-          if (valid)
+          // a diagonal tile holds both triangles (each product computed on
+          // its own); the off-diagonal tile is stored again as its transpose
+          if (i < N && j < N)
           {
             ke[(size_t)W.rp[i] * ns + j] = v;
-            if (j > i)
-              ke[(size_t)W.rp[j] * ns + i] = v; // the lower triangle repeats the upper one exactly
+            if (ti != tj)
+              ke[(size_t)W.rp[j] * ns + i] = v;
           }
         }
     }
