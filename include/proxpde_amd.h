@@ -89,7 +89,8 @@ extern "C" {
 
 #define PXG_FP64 0
 #define PXG_FP32 1
-/* pxg_mg_solve only: PXG_FP32 with level 0 applied matrix-free (below)  */
+/* pxg_mg_solve only: PXG_FP32 with the levels above the coarsest applied
+ * matrix-free (below)                                                     */
 #define PXG_FP32_MF 2
 
 typedef struct pxg_ctx pxg_ctx;
@@ -267,13 +268,14 @@ int pxg_mg_update(pxg_mg * mg);
  * refined by replacing the CG residual with the FP64 residual b - A x each
  * time it has fallen a hundredfold (FP64 inner products, FP64 coarsest
  * solve; convergence judged on the FP64 residual).
- * PXG_FP32_MF: as PXG_FP32, but level 0 needs no assembled matrix: its
- * operator is the kappa-stiffness form of AssemblyStiffnessFE with the
- * Dirichlet rows of the assembled form (C K C + D_c), applied matrix-free
- * from the kappa field of the level-0 context -- FP32 contractions with the
- * per-point metrics computed once per kappa (6 FP32 values per point), FP32
- * inverse diagonal for the smoother, FP64 sum-factorised operator for the
- * refinement residual.  Q1 / Q2 hexahedra with their Gauss rule.          */
+ * PXG_FP32_MF: as PXG_FP32, but only the coarsest level needs an
+ * assembled matrix: every other level's operator is the kappa-stiffness
+ * form of AssemblyStiffnessFE with the Dirichlet rows of the assembled form
+ * (C K C + D_c), applied matrix-free from the kappa field of that level's
+ * context -- FP32 contractions with the per-point metrics computed once per
+ * kappa (6 FP32 values per point), FP32 inverse diagonal for the smoother;
+ * on level 0 the FP64 sum-factorised operator gives the refinement
+ * residual.  Q1 / Q2 hexahedra with their Gauss rule.                      */
 int pxg_mg_solve(pxg_mg * mg, int mode, const double * b_dev, double * x_dev, double rtol, int maxit,
                  int * iters, double * relres);
 /* one V-cycle x = M^{-1} b (FP64), for testing                            */

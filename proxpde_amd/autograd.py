@@ -104,9 +104,9 @@ class KappaSolver:
         and the FP64 refinement residual through the matrix-free operator; with
         fine_fp64 the finest level is assembled in FP64, its FP32 copy rounded from
         it, and the residual uses that matrix (cheaper for low-order elements).
-        mode FP32_MF: as FP32, with the finest level not assembled at all but
-        applied matrix-free in FP32 from kappa (per-point metrics computed once
-        per kappa)."""
+        mode FP32_MF: as FP32, with only the coarsest level assembled and the
+        others applied matrix-free in FP32 from kappa (per-point metrics
+        computed once per kappa)."""
         if fp32_levels is None:
             fp32_levels = mode in (N.FP32, N.FP32_MF)
         return torch.ops.proxpde.kappa_solve(kappa, b, [c.h.value for c in self.ctx], self.inject,

@@ -62,6 +62,10 @@ struct pxg_mg
     CsrView P, R;
     DevBuf<double> b, x, t, r;
     DevBuf<float> b32, x32, t32, r32;
+    // applied matrix-free (mode PXG_FP32_MF): FP32 per-point metrics of the
+    // level's current kappa field, element buffer, inverse diagonal
+    uint64_t mfKappaStamp = 0;
+    DevBuf<float> met32, mfBuf32, dinvMf32;
   };
   std::vector<Level> lv;
   DevBuf<double> sc, partial;
@@ -76,12 +80,12 @@ struct pxg_mg
   DevBuf<int32_t> freeIdx;  // coarse dof -> free index, -1 for constrained dofs
   DevBuf<int32_t> freeDofs; // free index -> coarse dof
   DevBuf<double> inv, invTmp;
+  DevBuf<double> gjP, gjR; // pivot-block inverse and scaled pivot rows of the blocked inversion
   DevBuf<int> flag;
-  // level 0 applied matrix-free (mode PXG_FP32_MF): FP32 per-point metrics
-  // of the current kappa field, element buffer, inverse diagonal
-  bool mf0 = false;
-  uint64_t mfKappaStamp = 0;
-  DevBuf<float> met32, mfBuf32, dinvMf32;
+  // levels [0, mfLevels) applied matrix-free in this solve (mode
+  // PXG_FP32_MF: all but the coarsest); element buffer of the FP64
+  // matrix-free residual on level 0
+  int mfLevels = 0;
   DevBuf<double> mfBuf64;
   // orders the cycle stream after work issued on the level contexts' own
   // streams (value casts, packed copies made on first use)
@@ -129,7 +133,7 @@ void joinLevels(pxg_mg * m)
 }
 
 // largest free-dof block inverted densely on the coarsest level (8 n^2 bytes,
-// n Gauss-Jordan steps of n^2 work each)
+// n^3 work in n / 32 blocked Gauss-Jordan steps)
 constexpr int64_t kDenseMax = 4096;
 
 __global__ void k_dense_scatter(int64_t n, int32_t const * __restrict__ freeIdx, int64_t const * __restrict__ rowptr,
@@ -154,20 +158,89 @@ __global__ void k_dense_scatter(int64_t n, int32_t const * __restrict__ freeIdx,
 // reads the pivot row and column as they were before the step).  After steps
 // 0..n-1 an SPD matrix has become its inverse; the pivots of an SPD matrix
 // stay positive, so no pivoting is needed.
-__global__ void k_gauss_jordan(int64_t nf, int64_t k, double const * __restrict__ A, double * __restrict__ B)
+// Blocked Gauss-Jordan inversion in place (the sweep operator; no pivoting,
+// the free block of an SPD coarse matrix), pivots in blocks K of kGjBlock:
+//   P = A_KK^-1,  R = P A_K,  then out of place
+//   B_KK = P,  B_Kj = R_j,  B_iK = -A_iK P,  B_ij = A_ij - A_iK R_j  (i, j not in K)
+// Three launches per block of pivots instead of one per pivot.
+constexpr int kGjBlock = 32;
+
+// P = A_KK^-1 of the pivot block [k0, k0 + nb): one block, unblocked
+// Gauss-Jordan in LDS
+__global__ void __launch_bounds__(1024) k_gj_pivot(int64_t nf, int64_t k0, int nb, double const * __restrict__ A,
+                                                  double * __restrict__ P)
+{
+  __shared__ double d[kGjBlock][kGjBlock + 1];
+  __shared__ double prow[kGjBlock], pcol[kGjBlock];
+  int const t = threadIdx.x;
+  for (int e = t; e < nb * nb; e += blockDim.x)
+    d[e / nb][e % nb] = A[(k0 + e / nb) * nf + k0 + e % nb];
+  __syncthreads();
+  for (int k = 0; k < nb; ++k)
+  {
+    double const rp = 1. / d[k][k];
+    if (t < nb)
+    {
+      prow[t] = d[k][t];
+      pcol[t] = d[t][k];
+    }
+    __syncthreads();
+    for (int e = t; e < nb * nb; e += blockDim.x)
+    {
+      int const i = e / nb, j = e % nb;
+      double v;
+      if (i == k)
+        v = (j == k) ? rp : prow[j] * rp;
+      else if (j == k)
+        v = -pcol[i] * rp;
+      else
+        v = d[i][j] - pcol[i] * (prow[j] * rp);
+      d[i][j] = v;
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < nb * nb; e += blockDim.x)
+    P[e] = d[e / nb][e % nb];
+}
+
+// R = P A_K (nb x nf; columns in K unused)
+__global__ void k_gj_rows(int64_t nf, int64_t k0, int nb, double const * __restrict__ A,
+                          double const * __restrict__ P, double * __restrict__ R)
+{
+  int64_t const t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nb * nf)
+    return;
+  int64_t const m = t / nf, j = t - m * nf;
+  double v = 0.;
+  for (int l = 0; l < nb; ++l)
+    v += P[m * nb + l] * A[(k0 + l) * nf + j];
+  R[t] = v;
+}
+
+// B from A, P and R (see above)
+__global__ void k_gj_update(int64_t nf, int64_t k0, int nb, double const * __restrict__ A,
+                            double const * __restrict__ P, double const * __restrict__ R, double * __restrict__ B)
 {
   int64_t const t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= nf * nf)
     return;
   int64_t const i = t / nf, j = t - i * nf;
-  double const piv = A[k * nf + k];
+  bool const ik = i >= k0 && i < k0 + nb, jk = j >= k0 && j < k0 + nb;
   double v;
-  if (i == k)
-    v = (j == k) ? 1. / piv : A[k * nf + j] / piv;
-  else if (j == k)
-    v = -A[i * nf + k] / piv;
+  if (ik)
+    v = jk ? P[(i - k0) * nb + (j - k0)] : R[(i - k0) * nf + j];
+  else if (jk)
+  {
+    v = 0.;
+    for (int l = 0; l < nb; ++l)
+      v -= A[i * nf + k0 + l] * P[l * nb + (j - k0)];
+  }
   else
-    v = A[t] - A[i * nf + k] * (A[k * nf + j] / piv);
+  {
+    v = A[t];
+    for (int l = 0; l < nb; ++l)
+      v -= A[i * nf + k0 + l] * R[l * nf + j];
+  }
   B[t] = v;
 }
 
@@ -382,8 +455,9 @@ void buildProlongation(pxg_mg * m, int l)
   L.R.blockRow = L.R.nblocks > 0 ? L.Rblk.p : nullptr;
 }
 
-// operator of level l in the cycle: the assembled CSR matrix, or on level 0
-// in mode PXG_FP32_MF the partially assembled matrix-free operator (FP32)
+// operator of level l in the cycle: the assembled CSR matrix, or in mode
+// PXG_FP32_MF (every level but the coarsest) the partially assembled
+// matrix-free operator (FP32)
 template <typename T>
 struct LevelOp
 {
@@ -391,24 +465,25 @@ struct LevelOp
   int l;
   CsrView A;
   bool mf;
-  LevelOp(pxg_mg * mg, int lev): m(mg), l(lev), A(viewOf(mg->ctx[lev])), mf(lev == 0 && mg->mf0)
+  LevelOp(pxg_mg * mg, int lev): m(mg), l(lev), A(viewOf(mg->ctx[lev])), mf(lev < mg->mfLevels)
   {
     if (mf && sizeof(T) != 4)
-      throw Error(PXG_E_INVALID, "the matrix-free level 0 is FP32");
+      throw Error(PXG_E_INVALID, "the matrix-free levels are FP32");
   }
   int64_t n() const { return A.nrows; }
   T const * dinv() const
   {
     if (mf)
-      return (T const *)m->dinvMf32.p;
+      return (T const *)m->lv[l].dinvMf32.p;
     return sizeof(T) == 8 ? (T const *)A.dinv : (T const *)A.dinv32;
   }
   // element kernel into the element buffer, then the gather with its epilogue
   void mfRun(int mode, T const * x, T const * b, double omega, T * y, hipStream_t s) const
   {
-    pxg_ctx * c = m->ctx[0];
-    sfApplyPA<float>(c, m->met32.p, (float const *)x, m->mfBuf32.p, true, s);
-    sfGather<float>(c, mode, m->mfBuf32.p, (float const *)x, (float const *)b, m->dinvMf32.p, omega, (float *)y, s);
+    pxg_ctx * c = m->ctx[l];
+    auto & L = m->lv[l];
+    sfApplyPA<float>(c, L.met32.p, (float const *)x, L.mfBuf32.p, true, s);
+    sfGather<float>(c, mode, L.mfBuf32.p, (float const *)x, (float const *)b, L.dinvMf32.p, omega, (float *)y, s);
   }
   void apply(T const * x, T * y, hipStream_t s) const
   {
@@ -464,26 +539,27 @@ void smooth(LevelOp<T> const & A, T const * b, T * x, T * t, int sweeps, double 
     PXG_HIP(hipMemcpyAsync(x, cur, n * sizeof(T), hipMemcpyDeviceToDevice, s));
 }
 
-// metrics and inverse diagonal of the matrix-free level 0 for the current
-// kappa field of the level-0 context (recomputed when kappa was set again)
-void prepareMf0(pxg_mg * m)
+// metrics and inverse diagonal of matrix-free level l for the current kappa
+// field of its context (recomputed when kappa was set again)
+void prepareMf(pxg_mg * m, int l)
 {
-  pxg_ctx * c = m->ctx[0];
-  PXG_REQUIRE(c->hasKappa, PXG_E_STATE, "the matrix-free level 0 needs the kappa field (pxg_set_coefficient)");
+  pxg_ctx * c = m->ctx[l];
+  auto & L = m->lv[l];
+  PXG_REQUIRE(c->hasKappa, PXG_E_STATE, "the matrix-free levels need their kappa field (pxg_set_coefficient)");
   PXG_REQUIRE(sfAvailable(c), PXG_E_INVALID,
-              "the matrix-free level 0 needs Q1 / Q2 hexahedra with their Gauss rule");
+              "the matrix-free levels need Q1 / Q2 hexahedra with their Gauss rule");
   sfPrepare(c);
-  if (m->mfKappaStamp == c->kappaStamp && m->met32.p)
+  if (L.mfKappaStamp == c->kappaStamp && L.met32.p)
     return;
   hipStream_t s = m->s;
   int64_t const E = c->nelem, n = c->fe.n;
-  m->met32.resize((size_t)E * 6 * n);
-  m->mfBuf32.resize((size_t)E * n);
-  m->dinvMf32.resize(c->ndofs);
-  sfMetric<float>(c, c->kappa.p, m->met32.p, s);
-  sfElementDiag<float>(c, c->kappa.p, m->mfBuf32.p, s);
-  sfGather<float>(c, 3, m->mfBuf32.p, nullptr, nullptr, nullptr, 0., m->dinvMf32.p, s);
-  m->mfKappaStamp = c->kappaStamp;
+  L.met32.resize((size_t)E * 6 * n);
+  L.mfBuf32.resize((size_t)E * n);
+  L.dinvMf32.resize(c->ndofs);
+  sfMetric<float>(c, c->kappa.p, L.met32.p, s);
+  sfElementDiag<float>(c, c->kappa.p, L.mfBuf32.p, s);
+  sfGather<float>(c, 3, L.mfBuf32.p, nullptr, nullptr, nullptr, 0., L.dinvMf32.p, s);
+  L.mfKappaStamp = c->kappaStamp;
 }
 
 // (re)builds the dense inverse of the coarsest free-dof block when the coarse
@@ -533,9 +609,14 @@ void factorCoarse(pxg_mg * m)
   fill<double>(nf * nf, m->inv.p, 0., s);
   k_dense_scatter<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(n, m->freeIdx.p, A.rowptr, A.col, A.val, nf, m->inv.p);
   double * a = m->inv.p, * b = m->invTmp.p;
-  for (int64_t k = 0; k < nf; ++k)
+  m->gjP.resize(kGjBlock * kGjBlock);
+  m->gjR.resize(kGjBlock * nf);
+  for (int64_t k0 = 0; k0 < nf; k0 += kGjBlock)
   {
-    k_gauss_jordan<<<g2, bs, 0, s>>>(nf, k, a, b);
+    int const nb = (int)std::min<int64_t>(kGjBlock, nf - k0);
+    k_gj_pivot<<<1, 1024, 0, s>>>(nf, k0, nb, a, m->gjP.p);
+    k_gj_rows<<<(unsigned)((nb * nf + bs - 1) / bs), bs, 0, s>>>(nf, k0, nb, a, m->gjP.p, m->gjR.p);
+    k_gj_update<<<g2, bs, 0, s>>>(nf, k0, nb, a, m->gjP.p, m->gjR.p, b);
     std::swap(a, b);
   }
   if (a != m->inv.p)
@@ -683,7 +764,7 @@ int pcg(pxg_mg * m, T const * b, T * x, double rtol, int maxit, double & relres,
 void residual64(pxg_mg * m, double const * b, double const * x, double * r)
 {
   pxg_ctx * c = m->ctx[0];
-  if (m->mf0)
+  if (m->mfLevels > 0)
   {
     // the FP64 sum-factorised operator (geometry recomputed), same Dirichlet form
     m->mfBuf64.resize((size_t)c->nelem * c->fe.n);
@@ -870,7 +951,7 @@ int pxg_mg_update(pxg_mg * m)
     PXG_HIP(hipSetDevice(m->ctx[0]->device));
     m->s = m->ctx[0]->stream;
     // the coarsest matrix is factored here; the others are checked by the
-    // solve (level 0 need not be assembled for PXG_FP32_MF)
+    // solve (only the coarsest need be assembled for PXG_FP32_MF)
     PXG_REQUIRE(m->ctx.back()->closed, PXG_E_STATE, "the coarsest level must be closed");
     factorCoarse(m);
     m->err.clear();
@@ -894,25 +975,26 @@ int pxg_mg_solve(pxg_mg * m, int mode, const double * b, double * x, double rtol
     m->s = m->ctx[0]->stream; // follow the finest context (its stream may have been replaced)
     PXG_REQUIRE(mode == PXG_FP64 || mode == PXG_FP32 || mode == PXG_FP32_MF, PXG_E_INVALID, "solve mode");
     bool const mf = mode == PXG_FP32_MF;
-    PXG_REQUIRE(!mf || m->nlev >= 2, PXG_E_INVALID, "the matrix-free level 0 needs a coarser level");
-    for (int l = mf ? 1 : 0; l < m->nlev; ++l)
+    PXG_REQUIRE(!mf || m->nlev >= 2, PXG_E_INVALID, "the matrix-free levels need an assembled coarsest level");
+    int const nmf = mf ? m->nlev - 1 : 0;
+    for (int l = nmf; l < m->nlev; ++l)
       PXG_REQUIRE(m->ctx[l]->closed, PXG_E_STATE, "levels must be closed");
     factorCoarse(m);
     double rr = 0.;
     int it = 0;
     if (mode != PXG_FP64)
     {
-      for (int l = mf ? 1 : 0; l < m->nlev; ++l)
+      for (int l = nmf; l < m->nlev; ++l)
         ensureOperator(m->ctx[l], PXG_FP32);
       joinLevels(m);
-      struct Mf0 // the mode holds for this solve only
+      struct Mf // the mode holds for this solve only
       {
         pxg_mg * m;
-        ~Mf0() { m->mf0 = false; }
+        ~Mf() { m->mfLevels = 0; }
       } guard{m};
-      m->mf0 = mf;
-      if (mf)
-        prepareMf0(m);
+      m->mfLevels = nmf;
+      for (int l = 0; l < nmf; ++l)
+        prepareMf(m, l);
       it = solveIR(m, b, x, rtol, maxit, rr);
     }
     else

@@ -22,7 +22,7 @@ SIDE = {"bottom": 1, "right": 2, "top": 3, "left": 4, "back": 5, "front": 6}
 TERM = {"stiffness": 1, "stiffness_fe": 2, "mass": 3, "mass_fe": 4, "grad": 5, "div": 6, "tensor_stiffness": 7}
 RHS = {"none": 0, "const": 1, "sinsin": 2, "sinsinsin": 3, "fevar1d": 4, "qvalues": 5, "projection": 6}
 FP64, FP32 = 0, 1
-FP32_MF = 2  # pxg_mg_solve: FP32 refined, level 0 matrix-free
+FP32_MF = 2  # pxg_mg_solve: FP32 refined, all levels but the coarsest matrix-free
 
 _lib = None
 

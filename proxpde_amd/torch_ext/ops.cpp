@@ -10,7 +10,8 @@
 //       forward : re-assembles K(kappa) on every multigrid level (kappa injected to
 //                 the coarse dofs through `inject`), closes, and solves K u = b with
 //                 MG-PCG (mode 0: FP64; mode 1: FP32 CG refined by FP64 residuals;
-//                 mode 2: as 1 with level 0 applied matrix-free, not assembled).
+//                 mode 2: as 1 with all levels but the coarsest applied
+//                 matrix-free, not assembled).
 //                 fp32_levels: element matrices of all but the coarsest level in FP32;
 //                 the refinement residual then uses the FP64 matrix-free operator.
 //                 fine_fp64: the finest level is still assembled in FP64 (its FP32
@@ -184,8 +185,8 @@ void assembleLevels(SolveSetup const & S, Tensor const & kappa)
     Tensor kl = l == 0 ? kappa : kappa.index_select(0, S.inject[l - 1]).contiguous();
     check(pxg_set_stream(c, S.stream), c, "pxg_set_stream");
     check(pxg_set_coefficient_device(c, kl.data_ptr<double>()), c, "pxg_set_coefficient_device");
-    if (l == 0 && S.mode == PXG_FP32_MF)
-      continue; // level 0 is applied matrix-free from kappa
+    if (l + 1 < nl && S.mode == PXG_FP32_MF)
+      continue; // applied matrix-free from kappa (only the coarsest is assembled)
     check(pxg_clear(c), c, "pxg_clear");
     // the coarsest level stays FP64 (FP64 coarse solve)
     int const prec = (S.fp32Levels && l + 1 < nl && !(l == 0 && S.fineFp64)) ? PXG_FP32 : PXG_FP64;

@@ -96,8 +96,8 @@ def test_kappa_solve_adjoint_gradient(ag, oracle, gpu, fe, ns):
 @pytest.mark.parametrize("variant", ["fp32", "fp32_fine64", "fp32_mf"])
 def test_kappa_solve_fp32_refined_matches_fp64(ag, gpu, fe, ns, variant):
     """The refined forms (FP32 levels with the matrix-free FP64 residual; the
-    finest level in FP64; the finest level matrix-free in FP32) against the
-    FP64 solve, solution and adjoint gradient."""
+    finest level in FP64; all levels but the coarsest matrix-free in FP32)
+    against the FP64 solve, solution and adjoint gradient."""
     solver = ag.KappaSolver(fe, ns)
     x = solver.probs[0].dof_coords()
     kap = torch.from_numpy(np.exp(np.sin(4 * x[:, 0]) * np.cos(3 * x[:, 1]))).to(gpu)

@@ -232,8 +232,9 @@ def test_mg_pcg_fp64_and_fp32_refined(oracle, native, gpu, fe, ns):
     omg = oracle.Multigrid(ol)
     u_o, it_o, rr_o = omg.pcg(b, rtol=1e-8)
     bt = torch.from_numpy(b).to(gpu)
-    # mode 2: FP32-refined with level 0 applied matrix-free from kappa (its
-    # operator is the assembled matrix A up to rounding)
+    # mode 2: FP32-refined with every level but the coarsest applied
+    # matrix-free from kappa (their operators are the assembled matrices up to
+    # rounding)
     for mode in (0, 1, 2):
         xt = torch.zeros_like(bt)
         it, rr = mg.solve_dev(a0.rhs_ptr(), xt.data_ptr(), rtol=1e-8, maxit=500, mode=mode)
