@@ -645,14 +645,15 @@ def main():
     rf_element = {"kernel": "k_element_mfma<double,27,3,8,27,false>", "bound": "mfma", "achieved": el_ach,
                   "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                   "frac": el_ach / FP64_PEAK_TFLOPS if el_ach else None, "traffic": tr("element")}
-    # rebuild into the pruned layout (staged by row blocks): per structural
-    # entry its 16-bit info word and the value written (8 B, kept or pruned);
-    # per element-matrix entry its 16-bit LDS slot and its value (8 B); per
-    # element-matrix row its pair id and lifting value (4 + 8 B); per row its
-    # pair offset, b read and written, both inverse diagonals (4 + 16 + 12 B)
-    sc_by = nnz_s * (2 + 8) + E * 27 * 27 * (2 + 8) + E * 27 * (4 + 8) + prob.ndofs * (4 + 16 + 12)
+    # rebuild into the pruned layout (staged by row blocks, element rows in
+    # pair order): per structural entry its 16-bit info word and the value
+    # written (8 B, kept or pruned); per element-matrix entry its 16-bit LDS
+    # slot and its value (8 B); per element-matrix row its lifting value
+    # (8 B); per row its pair offset, b read and written, both inverse
+    # diagonals (4 + 16 + 12 B)
+    sc_by = nnz_s * (2 + 8) + E * 27 * 27 * (2 + 8) + E * 27 * 8 + prob.ndofs * (4 + 16 + 12)
     sc_ach = sc_by / (per["scatter"] * 1e-3) / 1e9 if per["scatter"] else None
-    rf_scatter = {"kernel": "k_scatter_staged<double,27,32,64>", "bound": "hbm", "achieved": sc_ach,
+    rf_scatter = {"kernel": "k_scatter_staged<double,27,32,64,true>", "bound": "hbm", "achieved": sc_ach,
                   "peak": HBM_PEAK_GBS,
                   "unit": "GB/s", "frac": sc_ach / HBM_PEAK_GBS if sc_ach else None, "traffic": tr("scatter")}
     rf_spmv = {"kernel": "k_csr_stream<double,0>", "bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS,
