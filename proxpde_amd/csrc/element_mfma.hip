@@ -25,6 +25,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace pxg
@@ -102,7 +103,7 @@ struct WaveLds
   uint32_t rp[N]; // output row of each element row (element or pair order)
 };
 
-template <typename T, int N, int DIM, int NPE, int NQ, bool MASS>
+template <typename T, int N, int DIM, int NPE, int NQ, bool MASS, bool PO>
 __global__ void __launch_bounds__(64 * kWaves, 4)
     k_element_mfma(int64_t E, double const * __restrict__ xyz, uint32_t const * __restrict__ conn,
                    uint32_t const * __restrict__ emap, double const * __restrict__ kappa,
@@ -195,7 +196,7 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
   // gathered data of the element being processed, loaded one element ahead
   double rx = 0., rk = 0., rb = 0.;
   int rf = 0;
-  uint32_t rr = 0; // output row of element row `lane`
+  uint32_t rr = 0; // output row of element row `lane` (PO: its pair position)
   auto gatherIds = [&](int64_t el, uint32_t & pid, uint32_t & id)
   {
     if (el < E)
@@ -205,7 +206,8 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
       if (lane < N)
       {
         id = emap[el * N + lane];
-        rr = rowPos ? rowPos[el * N + lane] : (uint32_t)(el * N + lane);
+        if constexpr (PO)
+          rr = rowPos[el * N + lane];
       }
     }
   };
@@ -240,7 +242,8 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
       W.flag[lane] = rf;
       W.bcv[lane] = rb;
       W.f[lane] = 0.;
-      W.rp[lane] = rr;
+      if constexpr (PO)
+        W.rp[lane] = rr;
     }
     waveSync();
     uint32_t npid = 0, nid = 0;
@@ -475,9 +478,16 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
     //         unit rows; the lifting f = -Ke bcv uses the unmodified Ke, its
     //         lower part through symmetry.
     bool const anyBC = __any(lane < N && W.flag[lane] != 0);
-    // row-major rows of NS values in element order, or of N values at the
-    // rows' pair positions
-    size_t const ns = rowPos ? (size_t)N : (size_t)keRowStride(N);
+    // row-major rows of NS values in element order, or (PO) of N values at
+    // the rows' pair positions
+    constexpr size_t ns = PO ? (size_t)N : (size_t)keRowStride(N);
+    auto rowOf = [&](int i) -> size_t
+    {
+      if constexpr (PO)
+        return (size_t)W.rp[i];
+      else
+        return (size_t)(e * N + i);
+    };
     if constexpr (sizeof(T) == 4)
     {
       // FP32: entries below 2^-19 sqrt(Ke_ii Ke_jj) are rounding noise of
@@ -548,9 +558,9 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
           // its own); the off-diagonal tile is stored again as its transpose
           if (i < N && j < N)
           {
-            ke[(size_t)W.rp[i] * ns + j] = v;
+            ke[rowOf(i) * ns + j] = v;
             if (ti != tj)
-              ke[(size_t)W.rp[j] * ns + i] = v;
+              ke[rowOf(j) * ns + i] = v;
           }
         }
     }
@@ -569,7 +579,7 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
       waveSync();
     }
     if (lane < N)
-      fe[W.rp[lane]] = anyBC ? (W.flag[lane] ? W.bcv[lane] : 0. - W.f[lane]) : 0.;
+      fe[rowOf(lane)] = anyBC ? (W.flag[lane] ? W.bcv[lane] : 0. - W.f[lane]) : 0.;
     waveSync(); // LDS of this element is reused by the next one
     e = en;
     en = elem(ennReq);
@@ -589,22 +599,40 @@ void launchMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, double
   int dev = 0, ncu = 0;
   PXG_HIP(hipGetDevice(&dev));
   PXG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  // one resident wave of blocks, each striding over the elements
-  int perCu = 0;
-  PXG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_element_mfma<T, N, DIM, NPE, NQ, MASS>,
-                                                       64 * kWaves, 0));
-  int64_t const blocksNeeded = (c->nelem + kWaves - 1) / kWaves;
-  int const grid =
-      (int)std::max<int64_t>(1, std::min<int64_t>(blocksNeeded, (int64_t)ncu * std::max(perCu, 1)));
-  PXG_REQUIRE(c->nelem < ((int64_t)1 << 31) - (int64_t)grid * kWaves * 2, PXG_E_INVALID,
-              "element count exceeds the 32-bit work counters");
-  PXG_REQUIRE(c->nelem * N < ((int64_t)1 << 32), PXG_E_INVALID, "element rows exceed 32-bit row positions");
-  c->workCounter.resize(kCounters * kCounterStride);
-  k_reset_counters<<<1, 64, 0, c->stream>>>(c->workCounter.p);
-  k_element_mfma<T, N, DIM, NPE, NQ, MASS><<<grid, 64 * kWaves, 0, c->stream>>>(
-      c->nelem, c->xyz.p, c->conn.p, c->elemMap.p, c->kappa.p, c->bcFlag.p, c->bcVal.p, c->hasBC ? 1 : 0,
-      tab.d_geoGrad.p, tab.d_w.p, tab.d_phi.p, tab.d_grad.p, cS, cSF, cM, cMF, flags, ke, c->feBuf.p,
-      c->status.p, c->workCounter.p, c->kePairOrder ? c->pairPos.p : nullptr);
+  // one resident wave of blocks, each striding over the elements; the
+  // pair-order variant exists for the Q2 hexahedron (the rows whose element
+  // matrices the rebuild writes in pair order)
+  auto go = [&](auto po)
+  {
+    constexpr bool PO = decltype(po)::value;
+    auto * kern = k_element_mfma<T, N, DIM, NPE, NQ, MASS, PO>;
+    int perCu = 0;
+    PXG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kern, 64 * kWaves, 0));
+    int64_t const blocksNeeded = (c->nelem + kWaves - 1) / kWaves;
+    int const grid =
+        (int)std::max<int64_t>(1, std::min<int64_t>(blocksNeeded, (int64_t)ncu * std::max(perCu, 1)));
+    PXG_REQUIRE(c->nelem < ((int64_t)1 << 31) - (int64_t)grid * kWaves * 2, PXG_E_INVALID,
+                "element count exceeds the 32-bit work counters");
+    PXG_REQUIRE(c->nelem * N < ((int64_t)1 << 32), PXG_E_INVALID, "element rows exceed 32-bit row positions");
+    c->workCounter.resize(kCounters * kCounterStride);
+    k_reset_counters<<<1, 64, 0, c->stream>>>(c->workCounter.p);
+    kern<<<grid, 64 * kWaves, 0, c->stream>>>(c->nelem, c->xyz.p, c->conn.p, c->elemMap.p, c->kappa.p, c->bcFlag.p,
+                                              c->bcVal.p, c->hasBC ? 1 : 0, tab.d_geoGrad.p, tab.d_w.p, tab.d_phi.p,
+                                              tab.d_grad.p, cS, cSF, cM, cMF, flags, ke, c->feBuf.p, c->status.p,
+                                              c->workCounter.p, PO ? c->pairPos.p : nullptr);
+  };
+  if constexpr (N == 27)
+  {
+    if (c->kePairOrder)
+      go(std::true_type{});
+    else
+      go(std::false_type{});
+  }
+  else
+  {
+    PXG_REQUIRE(!c->kePairOrder, PXG_E_STATE, "pair-order element output is for the hexahedra");
+    go(std::false_type{});
+  }
   PXG_HIP(hipGetLastError());
 }
 
