@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 namespace pxg
 {
@@ -93,6 +94,58 @@ enum : int
   kSfDiag = 2,  // diagonal of K_e (element buffer [E][n])
   kSfMetric = 3, // per-point metrics M_q of every element ([E][6][n], see k_sf_apply_pa)
 };
+
+// metric M_q = kappa_q w_q |det J_q| J_q^-1 J_q^-T of point (qx, qy, qz)
+// from the vertex coordinates X (trilinear map); false for a degenerate J
+__device__ inline bool sfPointMetric(double const * X, double xi, double eta, double zeta, double wq, double kq,
+                                     double (&M)[6])
+{
+  constexpr int vid[2][2][2] = {{{0, 1}, {3, 2}}, {{4, 5}, {7, 6}}}; // [vz][vy][vx]
+  double const lx[2] = {0.5 * (1. - xi), 0.5 * (1. + xi)};
+  double const ly[2] = {0.5 * (1. - eta), 0.5 * (1. + eta)};
+  double const lz[2] = {0.5 * (1. - zeta), 0.5 * (1. + zeta)};
+  double J[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+  {
+    double d0 = 0., d1 = 0., d2 = 0.;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+      {
+        d0 += ly[a] * lz[b] * (X[3 * vid[b][a][1] + r] - X[3 * vid[b][a][0] + r]);
+        d1 += lx[a] * lz[b] * (X[3 * vid[b][1][a] + r] - X[3 * vid[b][0][a] + r]);
+        d2 += lx[a] * ly[b] * (X[3 * vid[1][b][a] + r] - X[3 * vid[0][b][a] + r]);
+      }
+    J[r][0] = 0.5 * d0;
+    J[r][1] = 0.5 * d1;
+    J[r][2] = 0.5 * d2;
+  }
+  double const c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  double const c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  double const c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  double const det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  double const id_ = 1. / det;
+  double Ji[3][3]; // J^-1[a][r] = cofactor(r, a) / det
+  Ji[0][0] = c00 * id_;
+  Ji[1][0] = c01 * id_;
+  Ji[2][0] = c02 * id_;
+  Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id_;
+  Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id_;
+  Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id_;
+  Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id_;
+  Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id_;
+  Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id_;
+  double const s = kq * wq * fabs(det);
+  int m = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = a; b < 3; ++b)
+      M[m++] = s * (Ji[a][0] * Ji[b][0] + Ji[a][1] * Ji[b][1] + Ji[a][2] * Ji[b][2]);
+  return det != 0. && std::isfinite(det);
+}
 
 // Data of one element gathered by a lane of its x pencil (j, k): dof ids,
 // constraint flags and coefficient values of the pencil's P nodes, and the
@@ -247,63 +300,11 @@ __global__ void __launch_bounds__(64 * kSfWaves)
     // ---- metric of the lane's z pencil: points (qx, qy, qz) = (pa, pc, qz) ----
     // M = s (J^T J)^-1 = s J^-1 J^-T, s = kappa_q w_q |det J| (grad: without kappa)
     double M[P][6];
-    {
-      double const * X = L.r[1][0];
-      // vertex v of the reference hexahedron at (vx, vy, vz)
-      constexpr int vid[2][2][2] = {{{0, 1}, {3, 2}}, {{4, 5}, {7, 6}}}; // [vz][vy][vx]
-      double const xi = tab.g[pa], eta = tab.g[pc];
-      double const lx[2] = {0.5 * (1. - xi), 0.5 * (1. + xi)};
-      double const ly[2] = {0.5 * (1. - eta), 0.5 * (1. + eta)};
 #pragma unroll
-      for (int q = 0; q < P; ++q)
-      {
-        double const zeta = tab.g[q];
-        double const lz[2] = {0.5 * (1. - zeta), 0.5 * (1. + zeta)};
-        double J[3][3];
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-        {
-          double d0 = 0., d1 = 0., d2 = 0.;
-#pragma unroll
-          for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-            {
-              d0 += ly[a] * lz[b] * (X[3 * vid[b][a][1] + r] - X[3 * vid[b][a][0] + r]);
-              d1 += lx[a] * lz[b] * (X[3 * vid[b][1][a] + r] - X[3 * vid[b][0][a] + r]);
-              d2 += lx[a] * ly[b] * (X[3 * vid[1][b][a] + r] - X[3 * vid[0][b][a] + r]);
-            }
-          J[r][0] = 0.5 * d0;
-          J[r][1] = 0.5 * d1;
-          J[r][2] = 0.5 * d2;
-        }
-        double const c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-        double const c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-        double const c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-        double const det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-        if (live && !(det != 0. && std::isfinite(det)))
-          atomicOr(status, 1);
-        double const id_ = 1. / det;
-        // J^-1[a][r] = cofactor(r, a) / det
-        double Ji[3][3];
-        Ji[0][0] = c00 * id_;
-        Ji[1][0] = c01 * id_;
-        Ji[2][0] = c02 * id_;
-        Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id_;
-        Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id_;
-        Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id_;
-        Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id_;
-        Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id_;
-        Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id_;
-        double const s = kq[q] * (tab.w[pa] * tab.w[pc] * tab.w[q]) * fabs(det);
-        int m = 0;
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-          for (int b = a; b < 3; ++b)
-            M[q][m++] = s * (Ji[a][0] * Ji[b][0] + Ji[a][1] * Ji[b][1] + Ji[a][2] * Ji[b][2]);
-      }
-    }
+    for (int q = 0; q < P; ++q)
+      if (!sfPointMetric(L.r[1][0], tab.g[pa], tab.g[pc], tab.g[q], tab.w[pa] * tab.w[pc] * tab.w[q], kq[q], M[q]) &&
+          live)
+        atomicOr(status, 1);
     sfSync(); // region 1 (coordinates) is a stage buffer from here on
     // the next batch's coordinates, coefficients and flags are in flight
     // during the contractions (nb = 1: its vector values too)
@@ -594,6 +595,278 @@ __global__ void __launch_bounds__(64 * kSfWaves)
         xcur[i][0] = xnxt[i][0];
     }
     bt = bn;
+  }
+}
+
+// Batched apply, one lane per (element, vector): LPE lanes of a wavefront
+// share an element and lane c applies the element operator to the vectors
+// c, c + LPE, ... of the batch.  The element's point metrics are computed
+// once (the lanes take the points in turn) and kept in LDS; the tensor
+// contractions of a vector run in the lane's registers, one qx slab of
+// points at a time, and the element rows go to the element buffer
+// ([E][n][nb], summed per dof by the gather).
+template <int P>
+struct SfBatchSlot
+{
+  double M[6][P * P * P]; // metrics of the points (upper triangle, row-major)
+  double X[24];           // vertex coordinates
+  double kn[P * P * P];   // nodal coefficient values (lexicographic)
+  uint32_t id[P * P * P]; // dof ids of the nodes (lexicographic)
+};
+
+// a[i] for a lane-dependent i, by selects over constant indices (a table in
+// the kernel arguments indexed at run time would be copied to scratch)
+template <int P>
+__device__ inline double pick(double const (&a)[P], int i)
+{
+  double r = a[0];
+#pragma unroll
+  for (int t = 1; t < P; ++t)
+    r = i == t ? a[t] : r;
+  return r;
+}
+template <int P>
+__device__ inline double pickRow(double const (&a)[P][P], int q, int i)
+{
+  double r = a[0][i];
+#pragma unroll
+  for (int t = 1; t < P; ++t)
+    r = q == t ? a[t][i] : r;
+  return r;
+}
+
+// waves per block of the batch kernel: the LDS of a block (EPW element
+// slots per wave) stays at that of 4 waves of 8 elements
+template <int LPE>
+constexpr int sfBatchWaves()
+{
+  return LPE >= 8 ? kSfWaves : (LPE == 4 ? kSfWaves / 2 : 1);
+}
+
+template <int P, int LPE>
+__global__ void __launch_bounds__(64 * sfBatchWaves<LPE>())
+    k_sf_batch(int64_t E, int nb, SfTab<P> tab, double const * __restrict__ xyz, uint32_t const * __restrict__ conn,
+               uint32_t const * __restrict__ emap, double const * __restrict__ kappa,
+               uint8_t const * __restrict__ cons, double const * __restrict__ x, double * __restrict__ out,
+               int * __restrict__ status)
+{
+  constexpr int N = P * P * P, EPW = 64 / LPE, WPB = sfBatchWaves<LPE>();
+  __shared__ SfBatchSlot<P> lds[WPB * EPW];
+  int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int const slot = lane / LPE, c = lane % LPE;
+  SfBatchSlot<P> & L = lds[wave * EPW + slot];
+  auto at = [](int i, int j, int k) { return i + P * (j + P * k); };
+  constexpr int NV = (24 + LPE - 1) / LPE; // vertex coordinates staged per lane
+  int64_t const ngroup = (E + EPW - 1) / EPW;
+  int64_t const stride = (int64_t)gridDim.x * WPB;
+  for (int64_t grp = (int64_t)blockIdx.x * WPB + wave; grp < ngroup; grp += stride)
+  {
+    int64_t const e = grp * EPW + slot;
+    bool const live = e < E;
+    // (an opaque copy of the lane's index keeps the compiler from hoisting
+    // the lane's point tables out of the loop and holding them in registers)
+    int cl = c;
+    asm volatile("" : "+v"(cl));
+    // ---- 1. dof ids, constraint flags, vertex coordinates and coefficient
+    //         values (the last two to LDS, the lanes taking them in turn)
+    int64_t const ee = live ? e : E - 1; // a lane past the end reads the last element and stores nothing
+    uint32_t id[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      id[i] = emap[ee * N + tab.local[i]];
+#pragma unroll
+    for (int m = 0; m < NV; ++m)
+      if (m * LPE + c < 24)
+        L.X[m * LPE + c] = xyz[3ull * conn[ee * 8 + (m * LPE + c) / 3] + (m * LPE + c) % 3];
+#pragma unroll
+    for (int i0 = 0; i0 < N; i0 += LPE)
+      if (i0 + c < N)
+      {
+        uint32_t d = id[0];
+#pragma unroll
+        for (int i = 0; i < N; ++i) // (constant register indices)
+          d = i == i0 + c ? id[i] : d;
+        L.kn[i0 + c] = kappa[d];
+        L.id[i0 + c] = d;
+      }
+    uint32_t fl = 0;
+    if (cons)
+    {
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+        fl |= cons[id[i]] ? (1u << i) : 0u;
+    }
+    sfSync();
+    // ---- 2. metrics of the points c, c + LPE, ... -> LDS
+    bool bad = false;
+#pragma unroll
+    for (int q0 = 0; q0 < N; q0 += LPE)
+    {
+      int const q = q0 + cl;
+      if (q < N)
+      {
+        int const qx = q % P, qy = (q / P) % P, qz = q / (P * P);
+        double bx[P], by[P], bz[P]; // B[qx][.], B[qy][.], B[qz][.]
+#pragma unroll
+        for (int i = 0; i < P; ++i)
+        {
+          bx[i] = pickRow<P>(tab.B, qx, i);
+          by[i] = pickRow<P>(tab.B, qy, i);
+          bz[i] = pickRow<P>(tab.B, qz, i);
+        }
+        double kq = 0.; // B x B x B interpolation of the coefficient, x first
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+        {
+          double t = 0.;
+#pragma unroll
+          for (int j = 0; j < P; ++j)
+          {
+            double s = 0.;
+#pragma unroll
+            for (int i = 0; i < P; ++i)
+              s += bx[i] * L.kn[at(i, j, k)];
+            t += by[j] * s;
+          }
+          kq += bz[k] * t;
+        }
+        double const wq = pick<P>(tab.w, qx) * pick<P>(tab.w, qy) * pick<P>(tab.w, qz);
+        double Mq[6];
+        bad |= !sfPointMetric(L.X, pick<P>(tab.g, qx), pick<P>(tab.g, qy), pick<P>(tab.g, qz), wq, kq, Mq);
+#pragma unroll
+        for (int m = 0; m < 6; ++m)
+          L.M[m][q] = Mq[m];
+      }
+    }
+    if (live && bad)
+      atomicOr(status, 1);
+    sfSync();
+    // ---- 3. the vectors of the lane
+    for (int b0 = 0; b0 < nb; b0 += LPE)
+    {
+      // the metrics are read from LDS where they are used, not hoisted
+      // out of this loop (162 values would not fit in registers)
+      asm volatile("" ::: "memory");
+      int const v = b0 + c;
+      int const vc = v < nb ? v : nb - 1;
+      double u[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+      {
+        double const t = x[L.id[i] * (uint32_t)nb + (uint32_t)vc];
+        u[i] = (fl >> i) & 1u ? 0. : t;
+      }
+      // one qx slab of points at a time: x stage of the slab (a = B_x u,
+      // d = D_x u at (qx, j, k)), then per z pencil (qx, qy) the y and z
+      // stages, the metric, z^T, and its share of y^T (accumulated in td,
+      // the D_x term of the transposed x stage, and tb, the B_x terms);
+      // the slab's share of x^T goes to ye
+      double ye[N];
+#pragma unroll
+      for (int qx = 0; qx < P; ++qx)
+      {
+        double a[P][P], d[P][P]; // [j][k]
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+#pragma unroll
+          for (int j = 0; j < P; ++j)
+          {
+            double sb = 0., sd = 0.;
+#pragma unroll
+            for (int i = 0; i < P; ++i)
+            {
+              sb += tab.B[qx][i] * u[at(i, j, k)];
+              sd += tab.D[qx][i] * u[at(i, j, k)];
+            }
+            a[j][k] = sb;
+            d[j][k] = sd;
+          }
+        double td[P][P], tb[P][P]; // [j][k]
+#pragma unroll
+        for (int qy = 0; qy < P; ++qy)
+        {
+          double bb[P], bd[P], db[P]; // [k]
+#pragma unroll
+          for (int k = 0; k < P; ++k)
+          {
+            double s0 = 0., s1 = 0., s2 = 0.;
+#pragma unroll
+            for (int j = 0; j < P; ++j)
+            {
+              s0 += tab.B[qy][j] * a[j][k];
+              s1 += tab.D[qy][j] * a[j][k];
+              s2 += tab.B[qy][j] * d[j][k];
+            }
+            bb[k] = s0;
+            bd[k] = s1;
+            db[k] = s2;
+          }
+          double vx[P], vy[P], vz[P];
+#pragma unroll
+          for (int qz = 0; qz < P; ++qz)
+          {
+            double gx = 0., gy = 0., gz = 0.;
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+            {
+              gx += tab.B[qz][k] * db[k];
+              gy += tab.B[qz][k] * bd[k];
+              gz += tab.D[qz][k] * bb[k];
+            }
+            int const q = at(qx, qy, qz);
+            double const m0 = L.M[0][q], m1 = L.M[1][q], m2 = L.M[2][q];
+            double const m3 = L.M[3][q], m4 = L.M[4][q], m5 = L.M[5][q];
+            vx[qz] = m0 * gx + m1 * gy + m2 * gz;
+            vy[qz] = m1 * gx + m3 * gy + m4 * gz;
+            vz[qz] = m2 * gx + m4 * gy + m5 * gz;
+          }
+#pragma unroll
+          for (int k = 0; k < P; ++k)
+          {
+            double r1 = 0., r2 = 0., r3 = 0.;
+#pragma unroll
+            for (int qz = 0; qz < P; ++qz)
+            {
+              r1 += tab.B[qz][k] * vx[qz];
+              r2 += tab.B[qz][k] * vy[qz];
+              r3 += tab.D[qz][k] * vz[qz];
+            }
+#pragma unroll
+            for (int j = 0; j < P; ++j)
+            {
+              double const dd = tab.B[qy][j] * r1;
+              double const bt = tab.D[qy][j] * r2 + tab.B[qy][j] * r3;
+              td[j][k] = qy == 0 ? dd : td[j][k] + dd;
+              tb[j][k] = qy == 0 ? bt : tb[j][k] + bt;
+            }
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+#pragma unroll
+          for (int j = 0; j < P; ++j)
+#pragma unroll
+            for (int i = 0; i < P; ++i)
+            {
+              double const t = tab.D[qx][i] * td[j][k] + tab.B[qx][i] * tb[j][k];
+              ye[at(i, j, k)] = qx == 0 ? t : ye[at(i, j, k)] + t;
+            }
+      }
+      // the unit diagonal per element on constrained rows
+      if (live && v < nb)
+      {
+        double * const ob = out + (size_t)e * N * nb + v;
+#pragma unroll
+        for (int n = 0; n < N; ++n)
+        {
+          double r = ye[n];
+          if ((fl >> n) & 1u)
+            r = x[L.id[n] * (uint32_t)nb + (uint32_t)v];
+          ob[(uint32_t)tab.local[n] * (uint32_t)nb] = r;
+        }
+      }
+    }
+    sfSync(); // the slot is rewritten by the next group
   }
 }
 
@@ -905,6 +1178,17 @@ int sfOrder(pxg_ctx const * c)
   return P;
 }
 
+// PXG_SF_BATCH=0 keeps FP64 batches on the pencil kernel (for comparisons)
+bool sfBatchOn()
+{
+  static bool const on = []
+  {
+    char const * e = std::getenv("PXG_SF_BATCH");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 template <typename T, int P, int MODE, int NBC>
 void launchSfP(pxg_ctx * c, int nb, double const * kappa, uint8_t const * cons, T const * x, T const * ybar, T * out,
                hipStream_t s)
@@ -925,10 +1209,54 @@ void launchSfP(pxg_ctx * c, int nb, double const * kappa, uint8_t const * cons, 
   PXG_HIP(hipGetLastError());
 }
 
+template <int P, int LPE>
+void launchSfBatchP(pxg_ctx * c, int nb, double const * kappa, uint8_t const * cons, double const * x, double * out,
+                    hipStream_t s)
+{
+  int64_t const E = c->nelem;
+  constexpr int EPW = 64 / LPE, WPB = sfBatchWaves<LPE>();
+  auto kern = k_sf_batch<P, LPE>;
+  int dev = 0, ncu = 0, perCu = 0;
+  PXG_HIP(hipGetDevice(&dev));
+  PXG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  PXG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kern, 64 * WPB, 0));
+  int64_t const ngroup = (E + EPW - 1) / EPW;
+  int64_t const need = (ngroup + WPB - 1) / WPB;
+  int64_t const grid = std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)ncu * std::max(perCu, 1)));
+  kern<<<(unsigned)grid, 64 * WPB, 0, s>>>(E, nb, tabFor<P>(P == 3 ? PXG_FE_HEX_Q2 : PXG_FE_HEX_Q1), c->xyz.p,
+                                           c->conn.p, c->elemMap.p, kappa, cons, x, out, c->status.p);
+  PXG_HIP(hipGetLastError());
+}
+
+// FP64 batches: lanes per element = the batch rounded up to 2, 4 or 8
+template <int P>
+void launchSfBatch(pxg_ctx * c, int nb, double const * kappa, uint8_t const * cons, double const * x, double * out,
+                   hipStream_t s)
+{
+  if (nb > 4)
+    launchSfBatchP<P, 8>(c, nb, kappa, cons, x, out, s);
+  else if (nb > 2)
+    launchSfBatchP<P, 4>(c, nb, kappa, cons, x, out, s);
+  else
+    launchSfBatchP<P, 2>(c, nb, kappa, cons, x, out, s);
+}
+
 template <typename T, int MODE>
 void launchSf(pxg_ctx * c, int P, int nb, double const * kappa, uint8_t const * cons, T const * x, T const * ybar,
               T * out, hipStream_t s)
 {
+  if constexpr (MODE == kSfApply && sizeof(T) == 8)
+  {
+    if (nb > 1 && sfBatchOn())
+    {
+      PXG_REQUIRE((double)c->ndofs * nb < 4294967295., PXG_E_INVALID, "batch too large for 32-bit vector offsets");
+      if (P == 3)
+        launchSfBatch<3>(c, nb, kappa, cons, x, out, s);
+      else
+        launchSfBatch<2>(c, nb, kappa, cons, x, out, s);
+      return;
+    }
+  }
   // vectors per chunk: 1 for a single vector (its values ride with the
   // prefetch), 8 for batches of the apply, 4 for the gradient (two vectors
   // per chunk entry)

@@ -293,6 +293,38 @@ def test_matrix_free_apply_and_gradient(oracle, gpu, fe, mesh, n, pert, use_bc):
         assert np.abs(A @ x - yt.cpu().numpy()).max() <= 1e-12 * np.abs(y_ref).max()
 
 
+@pytest.mark.parametrize("fe,n", [("hex_q2", (3, 2, 2)), ("hex_q1", (5, 3, 2))])
+@pytest.mark.parametrize("nb", [1, 2, 3, 4, 5, 11])
+def test_matrix_free_batch_sizes(oracle, gpu, fe, n, nb):
+    """The apply at every batch size: one vector, batches filling 2, 4 or 8
+    lanes per element, lanes without a vector (3, 5, 11) and a second chunk
+    of vectors (11); element counts that leave a wavefront's last element
+    slots empty.  Against the oracle, with the Dirichlet form."""
+    import torch
+    prob = W.Problem(fe, n)
+    o = oracle.Space(fe, "cube_hex", n)
+    xyz = W.perturb_interior(prob.xyz, 1.0 / max(n), seed=5, amount=0.2)
+    prob.set_coords(xyz)
+    o.set_coords(xyz)
+    a = _torch_stream(prob.assembler())
+    dofs = prob.all_sides()
+    a.set_dirichlet(dofs, np.zeros(len(dofs)))
+    cons = np.zeros(prob.ndofs, np.uint8)
+    cons[dofs] = 1
+    kap = np.exp(W.grf(prob.dof_coords(), seed=6, ell=0.3, features=64))
+    x = np.random.default_rng(7).normal(size=(prob.ndofs, nb))
+    kt = torch.from_numpy(kap).to(gpu)
+    xt = torch.from_numpy(x).to(gpu)
+    yt = torch.full_like(xt, np.nan)
+    a.mf_apply_dev(kt.data_ptr(), nb, xt.data_ptr(), yt.data_ptr(), True)
+    y_ref = o.mf_apply(kap, x, cons)
+    assert np.abs(yt.cpu().numpy() - y_ref).max() <= 1e-12 * np.abs(y_ref).max()
+    # every entry written (the output starts as NaN), the same bits again
+    y2 = torch.full_like(xt, np.nan)
+    a.mf_apply_dev(kt.data_ptr(), nb, xt.data_ptr(), y2.data_ptr(), True)
+    assert torch.equal(yt, y2)
+
+
 def test_gradient_central_differences(gpu):
     """d/dkappa (ybar . K(kappa) x) against central differences, step 1e-6."""
     import torch
