@@ -187,6 +187,14 @@ void dropPattern(pxg_ctx * c)
   c->pos.release();
   c->dofSeg.release();
   c->dofContrib.release();
+  c->cubeState = 0;
+  c->nCube = c->nPartDof = c->nPart = 0;
+  c->cubeElem.release();
+  c->cubeOut.release();
+  c->partDof.release();
+  c->partSeg.release();
+  c->partIds.release();
+  c->partBuf.release();
   c->stageR0.release();
   c->stageBlk.release();
   c->stageInfo.release();

@@ -212,6 +212,13 @@ struct pxg_ctx
   pxg::DevBuf<double> feBuf;  // E*n
   pxg::DevBuf<double> qwork;  // E*nq scratch
   pxg::DevBuf<double> mfBuf;  // matrix-free element results [E][n][nb]
+  // batches of the sum-factorised apply in 2x2x2 element cubes (matfree_sf.hip)
+  int cubeState = 0; // 0: not built, 1: built, -1: the mesh keeps the element buffer
+  int64_t nCube = 0, nPartDof = 0, nPart = 0;
+  pxg::DevBuf<uint32_t> cubeElem; // [G][8] element of each corner, ~0u empty
+  pxg::DevBuf<uint32_t> cubeOut;  // [G][n][8] where each corner's node goes
+  pxg::DevBuf<uint32_t> partDof, partSeg, partIds; // dofs left partial, their slots
+  pxg::DevBuf<double> partBuf;    // [nPart][nb]
   pxg::DevBuf<int> status;    // device error flag
   bool statusPending = false; // status of the last build not read yet (read by close)
   pxg::DevBuf<int> workCounter; // element counter of the persistent element kernels

@@ -36,6 +36,10 @@ bool sfAvailable(pxg_ctx const * c);
 void sfPrepare(pxg_ctx * c);
 template <typename T>
 void sfElementApply(pxg_ctx * c, double const * kappa, int nb, T const * x, T * buf, bool useBC, hipStream_t s);
+// batches of two or more vectors straight into y (element cubes + the sum of
+// the partial values per dof); false if the mesh has no cube structure
+bool sfApplyCubes(pxg_ctx * c, double const * kappa, int nb, double const * x, double * y, bool useBC,
+                  hipStream_t s);
 void sfElementGrad(pxg_ctx * c, int nb, double const * x, double const * ybar, double * buf, bool useBC,
                    hipStream_t s);
 template <typename T>

@@ -487,18 +487,21 @@ void mfApply(pxg_ctx * c, double const * kappa, int nb, double const * x, double
   prepare(c, nb);
   FeTables & T = c->fe;
   int64_t const E = c->nelem;
-  c->mfBuf.resize(std::max<size_t>(c->mfBuf.n, (size_t)E * T.n * nb));
   hipStream_t s = c->stream;
   uint8_t const * cons = (useBC && c->hasBC) ? c->bcFlag.p : nullptr;
   ProfScope ps(c, "mf_apply");
   PXG_HIP(hipMemsetAsync(c->status.p, 0, sizeof(int), s));
-  if (sfAvailable(c)) // Q1 / Q2 hexahedra: sum-factorised element kernel
-    sfElementApply<double>(c, kappa, nb, x, c->mfBuf.p, useBC, s);
-  else
-    launchElem<false>(c, kappa, cons, nb, x, nullptr, c->mfBuf.p);
-  int64_t const m = c->ndofs * nb;
-  k_gather<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(c->ndofs, nb, c->dofSeg.p, c->dofContrib.p, c->mfBuf.p, y);
-  PXG_HIP(hipGetLastError());
+  if (!(sfAvailable(c) && sfApplyCubes(c, kappa, nb, x, y, useBC, s)))
+  {
+    c->mfBuf.resize(std::max<size_t>(c->mfBuf.n, (size_t)E * T.n * nb));
+    if (sfAvailable(c)) // Q1 / Q2 hexahedra: sum-factorised element kernel
+      sfElementApply<double>(c, kappa, nb, x, c->mfBuf.p, useBC, s);
+    else
+      launchElem<false>(c, kappa, cons, nb, x, nullptr, c->mfBuf.p);
+    int64_t const m = c->ndofs * nb;
+    k_gather<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(c->ndofs, nb, c->dofSeg.p, c->dofContrib.p, c->mfBuf.p, y);
+    PXG_HIP(hipGetLastError());
+  }
   int h = 0;
   PXG_HIP(hipMemcpyAsync(&h, c->status.p, sizeof(int), hipMemcpyDeviceToHost, s));
   PXG_HIP(hipStreamSynchronize(s));

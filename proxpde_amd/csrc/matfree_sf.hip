@@ -29,8 +29,11 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
+#include <unordered_map>
+#include <vector>
 
 namespace pxg
 {
@@ -598,13 +601,187 @@ __global__ void __launch_bounds__(64 * kSfWaves)
   }
 }
 
+// 1D tables of the tensor-product element as compile-time functions (the
+// batch kernel's contractions unroll over constant indices, so zero and unit
+// entries fold away): Lagrange basis at the nodes -1 (, 0), +1 evaluated at
+// the Gauss points (the reference's abscissae, src/qr.cpp:27,38, the 3-point
+// one as truncated there), and the collocation derivative Dc[q][p] = L_p'(g_q)
+// of the Lagrange polynomials through the Gauss points: for polynomials of
+// degree < P, D u = Dc (B u), so the reference gradient at the points follows
+// from the values at the points.
+__host__ __device__ constexpr double sfGauss(int P, int q)
+{
+  return P == 3 ? (q == 0 ? -0.774596669241483 : (q == 1 ? 0. : 0.774596669241483))
+                : (q == 0 ? -0.5773502691896258 : 0.5773502691896258);
+}
+__host__ __device__ constexpr double sfNode(int P, int i) { return P == 3 ? (double)(i - 1) : (i == 0 ? -1. : 1.); }
+// weight of Gauss point q (src/qr.cpp)
+__host__ __device__ constexpr double sfWeight(int P, int q) { return P == 3 ? (q == 1 ? 8. / 9. : 5. / 9.) : 1.; }
+// reference local index (include/reffe.hpp:877-992 node orders) of the
+// lexicographic node i + P (j + P k); makeTab checks it against the dof points
+template <int P>
+__host__ __device__ constexpr int sfLocal(int n)
+{
+  constexpr int q1[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+  constexpr int q2[27] = {0,  8,  1,  11, 20, 9,  3,  10, 2,  12, 22, 13, 21, 26,
+                          23, 15, 24, 14, 4,  16, 5,  19, 25, 17, 7,  18, 6};
+  return P == 3 ? q2[n] : q1[n];
+}
+template <int P>
+__host__ __device__ constexpr double sfB(int q, int i)
+{
+  double const t = sfGauss(P, q);
+  double v = 1.;
+  for (int m = 0; m < P; ++m)
+    if (m != i)
+      v *= (t - sfNode(P, m)) / (sfNode(P, i) - sfNode(P, m));
+  return v;
+}
+template <int P>
+__host__ __device__ constexpr double sfDc(int q, int p)
+{
+  double const t = sfGauss(P, q);
+  double s = 0.;
+  for (int m = 0; m < P; ++m)
+  {
+    if (m == p)
+      continue;
+    double d = 1. / (sfGauss(P, p) - sfGauss(P, m));
+    for (int r = 0; r < P; ++r)
+      if (r != p && r != m)
+        d *= (t - sfGauss(P, r)) / (sfGauss(P, p) - sfGauss(P, r));
+    s += d;
+  }
+  return s;
+}
+// s + a u with a constant a (zero and one fold away after unrolling)
+__device__ inline double sfMad(double a, double u, double s)
+{
+  if (a == 0.)
+    return s;
+  if (a == 1.)
+    return s + u;
+  return fma(a, u, s);
+}
+// table values at a lane-dependent point index q (selects over constants)
+template <int P>
+__device__ inline double sfBsel(int q, int i)
+{
+  double r = sfB<P>(0, i);
+#pragma unroll
+  for (int t = 1; t < P; ++t)
+    r = q == t ? sfB<P>(t, i) : r;
+  return r;
+}
+template <int P>
+__device__ inline double sfGsel(int q)
+{
+  double r = sfGauss(P, 0);
+#pragma unroll
+  for (int t = 1; t < P; ++t)
+    r = q == t ? sfGauss(P, t) : r;
+  return r;
+}
+template <int P>
+__device__ inline double sfWsel(int q)
+{
+  double r = sfWeight(P, 0);
+#pragma unroll
+  for (int t = 1; t < P; ++t)
+    r = q == t ? sfWeight(P, t) : r;
+  return r;
+}
+// values the compiler must have formed here, and memory accesses that stay
+// on their side of this point (an empty asm with them as operands)
+template <int P>
+__device__ inline void sfFence(double (&a)[P], double (&b)[P], double (&c)[P])
+{
+#pragma unroll
+  for (int i = 0; i < P; ++i)
+    asm volatile("" : "+v"(a[i]), "+v"(b[i]), "+v"(c[i])::"memory");
+}
+// in place along direction DIR of a P^3 lexicographic array: w <- A w per
+// line, A = B (DC = false) or Dc (DC = true), transposed for TR
+template <int P, int DIR, bool TR, bool DC>
+__device__ inline void sfLines(double (&w)[P * P * P])
+{
+  auto idx = [](int t, int a, int b)
+  {
+    if constexpr (DIR == 0)
+      return t + P * (a + P * b);
+    else if constexpr (DIR == 1)
+      return a + P * (t + P * b);
+    else
+      return a + P * (b + P * t);
+  };
+#pragma unroll
+  for (int b = 0; b < P; ++b)
+#pragma unroll
+    for (int a = 0; a < P; ++a)
+    {
+      double in[P], r[P];
+#pragma unroll
+      for (int t = 0; t < P; ++t)
+        in[t] = w[idx(t, a, b)];
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+      {
+        double s = 0.;
+#pragma unroll
+        for (int t = 0; t < P; ++t)
+        {
+          double const m = DC ? (TR ? sfDc<P>(t, q) : sfDc<P>(q, t)) : (TR ? sfB<P>(t, q) : sfB<P>(q, t));
+          s = sfMad(m, in[t], s);
+        }
+        r[q] = s;
+      }
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+        w[idx(q, a, b)] = r[q];
+    }
+}
+
 // Batched apply, one lane per (element, vector): LPE lanes of a wavefront
 // share an element and lane c applies the element operator to the vectors
 // c, c + LPE, ... of the batch.  The element's point metrics are computed
 // once (the lanes take the points in turn) and kept in LDS; the tensor
-// contractions of a vector run in the lane's registers, one qx slab of
-// points at a time, and the element rows go to the element buffer
-// ([E][n][nb], summed per dof by the gather).
+// contractions of a vector run in the lane's registers (collocation form,
+// 5 x 3 P^4 multiply-adds less the folded zeros of the tables), and the
+// element rows go to the element buffer ([E][n][nb], summed per dof by the
+// gather).
+// sum across the two corners of a cube pair in direction DIR (lanes LPE <<
+// DIR apart): the nodes of the shared face, low corner's value first (a
+// corner without an element holds zeros).  After the three directions every
+// corner holds, at each node, the sum over the corners sharing it.
+template <int P, int DIR, int LPE>
+__device__ inline void sfCubeSum(double (&o)[P * P * P], int bit)
+{
+  auto idx = [](int t, int a, int b)
+  {
+    if constexpr (DIR == 0)
+      return t + P * (a + P * b);
+    else if constexpr (DIR == 1)
+      return a + P * (t + P * b);
+    else
+      return a + P * (b + P * t);
+  };
+#pragma unroll
+  for (int b = 0; b < P; ++b)
+#pragma unroll
+    for (int a = 0; a < P; ++a)
+    {
+      // my face node: t = P - 1 on the low corner, t = 0 on the high one
+      double const lo = o[idx(P - 1, a, b)], hi = o[idx(0, a, b)];
+      double const mine = bit ? hi : lo;
+      double const theirs = __shfl_xor(mine, LPE << DIR);
+      double const sum = bit ? theirs + mine : mine + theirs;
+      if (bit)
+        o[idx(0, a, b)] = sum;
+      else
+        o[idx(P - 1, a, b)] = sum;
+    }
+}
+
 template <int P>
 struct SfBatchSlot
 {
@@ -643,25 +820,39 @@ constexpr int sfBatchWaves()
   return LPE >= 8 ? kSfWaves : (LPE == 4 ? kSfWaves / 2 : 1);
 }
 
-template <int P, int LPE>
+// CUBE: the wave's element slots are the corners (a, b, c) = bits of the
+// slot index of 2 x 2 x 2 element cubes (cubeElem[G][8], ~0u for an empty
+// corner; cube faces match node for node in the same order).  The nodes a
+// cube's elements share are summed across the lanes (x, then y, then z: the
+// low corner's value first, so both lanes hold the same bits), and each
+// node is written once, by the first corner holding it, as cubeOut[G][n][8]
+// says: to y (bit 31, the dof id: every contribution of the dof is in the
+// cube) or to the partial buffer [slot][nb], which k_sf_part sums per dof.
+// Without CUBE: element e = slot index, element buffer out [E][n][nb].
+template <int P, int LPE, bool CUBE>
 __global__ void __launch_bounds__(64 * sfBatchWaves<LPE>())
-    k_sf_batch(int64_t E, int nb, SfTab<P> tab, double const * __restrict__ xyz, uint32_t const * __restrict__ conn,
-               uint32_t const * __restrict__ emap, double const * __restrict__ kappa,
+    k_sf_batch(int64_t E, int64_t nslot, int nb, double const * __restrict__ xyz, uint32_t const * __restrict__ conn,
+               uint32_t const * __restrict__ emap, uint32_t const * __restrict__ cubeElem,
+               uint32_t const * __restrict__ cubeOut, double const * __restrict__ kappa,
                uint8_t const * __restrict__ cons, double const * __restrict__ x, double * __restrict__ out,
-               int * __restrict__ status)
+               double * __restrict__ part, int * __restrict__ status)
 {
   constexpr int N = P * P * P, EPW = 64 / LPE, WPB = sfBatchWaves<LPE>();
+  static_assert(!CUBE || EPW % 8 == 0, "whole cubes per wavefront");
   __shared__ SfBatchSlot<P> lds[WPB * EPW];
   int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int const slot = lane / LPE, c = lane % LPE;
   SfBatchSlot<P> & L = lds[wave * EPW + slot];
   auto at = [](int i, int j, int k) { return i + P * (j + P * k); };
   constexpr int NV = (24 + LPE - 1) / LPE; // vertex coordinates staged per lane
-  int64_t const ngroup = (E + EPW - 1) / EPW;
+  int64_t const ngroup = (nslot + EPW - 1) / EPW;
   int64_t const stride = (int64_t)gridDim.x * WPB;
   for (int64_t grp = (int64_t)blockIdx.x * WPB + wave; grp < ngroup; grp += stride)
   {
-    int64_t const e = grp * EPW + slot;
+    int64_t const gs = grp * EPW + slot; // element slot
+    int64_t e = gs;
+    if constexpr (CUBE)
+      e = gs < nslot ? (int64_t)cubeElem[gs] : (int64_t)0xffffffffu;
     bool const live = e < E;
     // (an opaque copy of the lane's index keeps the compiler from hoisting
     // the lane's point tables out of the loop and holding them in registers)
@@ -673,7 +864,7 @@ __global__ void __launch_bounds__(64 * sfBatchWaves<LPE>())
     uint32_t id[N];
 #pragma unroll
     for (int i = 0; i < N; ++i)
-      id[i] = emap[ee * N + tab.local[i]];
+      id[i] = emap[ee * N + sfLocal<P>(i)];
 #pragma unroll
     for (int m = 0; m < NV; ++m)
       if (m * LPE + c < 24)
@@ -710,9 +901,9 @@ __global__ void __launch_bounds__(64 * sfBatchWaves<LPE>())
 #pragma unroll
         for (int i = 0; i < P; ++i)
         {
-          bx[i] = pickRow<P>(tab.B, qx, i);
-          by[i] = pickRow<P>(tab.B, qy, i);
-          bz[i] = pickRow<P>(tab.B, qz, i);
+          bx[i] = sfBsel<P>(qx, i);
+          by[i] = sfBsel<P>(qy, i);
+          bz[i] = sfBsel<P>(qz, i);
         }
         double kq = 0.; // B x B x B interpolation of the coefficient, x first
 #pragma unroll
@@ -730,9 +921,9 @@ __global__ void __launch_bounds__(64 * sfBatchWaves<LPE>())
           }
           kq += bz[k] * t;
         }
-        double const wq = pick<P>(tab.w, qx) * pick<P>(tab.w, qy) * pick<P>(tab.w, qz);
+        double const wq = sfWsel<P>(qx) * sfWsel<P>(qy) * sfWsel<P>(qz);
         double Mq[6];
-        bad |= !sfPointMetric(L.X, pick<P>(tab.g, qx), pick<P>(tab.g, qy), pick<P>(tab.g, qz), wq, kq, Mq);
+        bad |= !sfPointMetric(L.X, sfGsel<P>(qx), sfGsel<P>(qy), sfGsel<P>(qz), wq, kq, Mq);
 #pragma unroll
         for (int m = 0; m < 6; ++m)
           L.M[m][q] = Mq[m];
@@ -741,7 +932,10 @@ __global__ void __launch_bounds__(64 * sfBatchWaves<LPE>())
     if (live && bad)
       atomicOr(status, 1);
     sfSync();
-    // ---- 3. the vectors of the lane
+    // ---- 3. the vectors of the lane (collocation form: values at the
+    //         points, gradients by the point-to-point derivative matrix,
+    //         its transpose, then the transposed interpolation; all in the
+    //         lane's registers with the 1D tables as constants)
     for (int b0 = 0; b0 < nb; b0 += LPE)
     {
       // the metrics are read from LDS where they are used, not hoisted
@@ -749,121 +943,108 @@ __global__ void __launch_bounds__(64 * sfBatchWaves<LPE>())
       asm volatile("" ::: "memory");
       int const v = b0 + c;
       int const vc = v < nb ? v : nb - 1;
-      double u[N];
+      // (opaque copies of the constraint bits: their per-node masks are made
+      // where they are used, not hoisted out of the loop into scalar registers)
+      uint32_t fl1 = fl;
+      asm volatile("" : "+v"(fl1));
+      double w[N];
 #pragma unroll
       for (int i = 0; i < N; ++i)
       {
         double const t = x[L.id[i] * (uint32_t)nb + (uint32_t)vc];
-        u[i] = (fl >> i) & 1u ? 0. : t;
+        w[i] = (fl1 >> i) & 1u ? 0. : t;
       }
-      // one qx slab of points at a time: x stage of the slab (a = B_x u,
-      // d = D_x u at (qx, j, k)), then per z pencil (qx, qy) the y and z
-      // stages, the metric, z^T, and its share of y^T (accumulated in td,
-      // the D_x term of the transposed x stage, and tb, the B_x terms);
-      // the slab's share of x^T goes to ye
-      double ye[N];
+      sfLines<P, 0, false, false>(w);
+      sfLines<P, 1, false, false>(w);
+      sfLines<P, 2, false, false>(w);
+      double o[N];
 #pragma unroll
-      for (int qx = 0; qx < P; ++qx)
-      {
-        double a[P][P], d[P][P]; // [j][k]
+      for (int n = 0; n < N; ++n)
+        o[n] = 0.;
 #pragma unroll
-        for (int k = 0; k < P; ++k)
-#pragma unroll
-          for (int j = 0; j < P; ++j)
-          {
-            double sb = 0., sd = 0.;
-#pragma unroll
-            for (int i = 0; i < P; ++i)
-            {
-              sb += tab.B[qx][i] * u[at(i, j, k)];
-              sd += tab.D[qx][i] * u[at(i, j, k)];
-            }
-            a[j][k] = sb;
-            d[j][k] = sd;
-          }
-        double td[P][P], tb[P][P]; // [j][k]
+      for (int qz = 0; qz < P; ++qz)
 #pragma unroll
         for (int qy = 0; qy < P; ++qy)
         {
-          double bb[P], bd[P], db[P]; // [k]
-#pragma unroll
-          for (int k = 0; k < P; ++k)
-          {
-            double s0 = 0., s1 = 0., s2 = 0.;
-#pragma unroll
-            for (int j = 0; j < P; ++j)
-            {
-              s0 += tab.B[qy][j] * a[j][k];
-              s1 += tab.D[qy][j] * a[j][k];
-              s2 += tab.B[qy][j] * d[j][k];
-            }
-            bb[k] = s0;
-            bd[k] = s1;
-            db[k] = s2;
-          }
           double vx[P], vy[P], vz[P];
 #pragma unroll
-          for (int qz = 0; qz < P; ++qz)
+          for (int qx = 0; qx < P; ++qx)
           {
             double gx = 0., gy = 0., gz = 0.;
 #pragma unroll
-            for (int k = 0; k < P; ++k)
+            for (int t = 0; t < P; ++t)
             {
-              gx += tab.B[qz][k] * db[k];
-              gy += tab.B[qz][k] * bd[k];
-              gz += tab.D[qz][k] * bb[k];
+              gx = sfMad(sfDc<P>(qx, t), w[at(t, qy, qz)], gx);
+              gy = sfMad(sfDc<P>(qy, t), w[at(qx, t, qz)], gy);
+              gz = sfMad(sfDc<P>(qz, t), w[at(qx, qy, t)], gz);
             }
             int const q = at(qx, qy, qz);
             double const m0 = L.M[0][q], m1 = L.M[1][q], m2 = L.M[2][q];
             double const m3 = L.M[3][q], m4 = L.M[4][q], m5 = L.M[5][q];
-            vx[qz] = m0 * gx + m1 * gy + m2 * gz;
-            vy[qz] = m1 * gx + m3 * gy + m4 * gz;
-            vz[qz] = m2 * gx + m4 * gy + m5 * gz;
+            vx[qx] = m0 * gx + m1 * gy + m2 * gz;
+            vy[qx] = m1 * gx + m3 * gy + m4 * gz;
+            vz[qx] = m2 * gx + m4 * gy + m5 * gz;
           }
+          // a line of points at a time: the line's products are formed before
+          // the next line's metric values are read (the scheduler would
+          // otherwise load all 6 P^3 of them at once and run out of registers)
+          sfFence<P>(vx, vy, vz);
 #pragma unroll
-          for (int k = 0; k < P; ++k)
+          for (int qx = 0; qx < P; ++qx)
+#pragma unroll
+            for (int t = 0; t < P; ++t)
+            {
+              o[at(t, qy, qz)] = sfMad(sfDc<P>(qx, t), vx[qx], o[at(t, qy, qz)]);
+              o[at(qx, t, qz)] = sfMad(sfDc<P>(qy, t), vy[qx], o[at(qx, t, qz)]);
+              o[at(qx, qy, t)] = sfMad(sfDc<P>(qz, t), vz[qx], o[at(qx, qy, t)]);
+            }
+        }
+      sfLines<P, 0, true, false>(o);
+      sfLines<P, 1, true, false>(o);
+      sfLines<P, 2, true, false>(o);
+      // the unit diagonal per element on constrained rows (the dof ids are
+      // read again here rather than kept in registers across the loop)
+      asm volatile("" ::: "memory");
+      uint32_t fl2 = fl;
+      asm volatile("" : "+v"(fl2));
+#pragma unroll
+      for (int n = 0; n < N; ++n)
+        if ((fl2 >> n) & 1u)
+          o[n] = x[L.id[n] * (uint32_t)nb + (uint32_t)vc];
+      if constexpr (CUBE)
+      {
+        int const corner = slot & 7;
+        if (!live)
+        {
+#pragma unroll
+          for (int n = 0; n < N; ++n)
+            o[n] = 0.;
+        }
+        sfCubeSum<P, 0, LPE>(o, corner & 1);
+        sfCubeSum<P, 1, LPE>(o, (corner >> 1) & 1);
+        sfCubeSum<P, 2, LPE>(o, (corner >> 2) & 1);
+        if (live && v < nb)
+        {
+          uint32_t const * const co = cubeOut + (gs >> 3) * (N * 8) + corner;
+#pragma unroll
+          for (int n = 0; n < N; ++n)
           {
-            double r1 = 0., r2 = 0., r3 = 0.;
-#pragma unroll
-            for (int qz = 0; qz < P; ++qz)
-            {
-              r1 += tab.B[qz][k] * vx[qz];
-              r2 += tab.B[qz][k] * vy[qz];
-              r3 += tab.D[qz][k] * vz[qz];
-            }
-#pragma unroll
-            for (int j = 0; j < P; ++j)
-            {
-              double const dd = tab.B[qy][j] * r1;
-              double const bt = tab.D[qy][j] * r2 + tab.B[qy][j] * r3;
-              td[j][k] = qy == 0 ? dd : td[j][k] + dd;
-              tb[j][k] = qy == 0 ? bt : tb[j][k] + bt;
-            }
+            uint32_t const code = co[n * 8];
+            if (code == 0xffffffffu)
+              continue;
+            if (code & 0x80000000u)
+              out[(size_t)(code & 0x7fffffffu) * nb + v] = o[n];
+            else
+              part[(size_t)code * nb + v] = o[n];
           }
         }
-#pragma unroll
-        for (int k = 0; k < P; ++k)
-#pragma unroll
-          for (int j = 0; j < P; ++j)
-#pragma unroll
-            for (int i = 0; i < P; ++i)
-            {
-              double const t = tab.D[qx][i] * td[j][k] + tab.B[qx][i] * tb[j][k];
-              ye[at(i, j, k)] = qx == 0 ? t : ye[at(i, j, k)] + t;
-            }
       }
-      // the unit diagonal per element on constrained rows
-      if (live && v < nb)
+      else if (live && v < nb)
       {
         double * const ob = out + (size_t)e * N * nb + v;
 #pragma unroll
         for (int n = 0; n < N; ++n)
-        {
-          double r = ye[n];
-          if ((fl >> n) & 1u)
-            r = x[L.id[n] * (uint32_t)nb + (uint32_t)v];
-          ob[(uint32_t)tab.local[n] * (uint32_t)nb] = r;
-        }
+          ob[(uint32_t)sfLocal<P>(n) * (uint32_t)nb] = o[n];
       }
     }
     sfSync(); // the slot is rewritten by the next group
@@ -1098,6 +1279,37 @@ __global__ void __launch_bounds__(256)
     y[d] = T(1) / acc;
 }
 
+// y[d][b] of the dofs the cubes leave partial: the sum of their partial
+// values in cube order
+__global__ void k_sf_part(int64_t npd, int nb, uint32_t const * __restrict__ dofs, uint32_t const * __restrict__ seg,
+                          uint32_t const * __restrict__ ids, double const * __restrict__ part,
+                          double * __restrict__ y)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npd * nb)
+    return;
+  int64_t const k = t / nb;
+  int const b = (int)(t - k * nb);
+  uint32_t const k0 = seg[k], k1 = seg[k + 1];
+  constexpr int kMaxC = 8; // a cube-face dof has 2 parts, an edge dof 4, a vertex 8
+  uint32_t id[kMaxC];
+#pragma unroll
+  for (int j = 0; j < kMaxC; ++j)
+    id[j] = ids[min(k0 + j, k1 - 1)];
+  double v[kMaxC];
+#pragma unroll
+  for (int j = 0; j < kMaxC; ++j)
+    v[j] = part[(size_t)id[j] * nb + b];
+  double acc = v[0];
+#pragma unroll
+  for (int j = 1; j < kMaxC; ++j)
+    if (k0 + j < k1)
+      acc += v[j];
+  for (uint32_t j = k0 + kMaxC; j < k1; ++j)
+    acc += part[(size_t)ids[j] * nb + b];
+  y[(size_t)dofs[k] * nb + b] = acc;
+}
+
 template <int P>
 SfTab<P> makeTab(int feType)
 {
@@ -1147,7 +1359,7 @@ SfTab<P> makeTab(int feType)
         for (int L = 0; L < fe.n; ++L)
           if (pts[3 * L] == nd[i] && pts[3 * L + 1] == nd[j] && pts[3 * L + 2] == nd[k])
             found = L;
-        if (found < 0)
+        if (found < 0 || found != sfLocal<P>(i + P * (j + P * k)))
           throw Error(PXG_E_INVALID, "tensor-product node table");
         t.local[i + P * (j + P * k)] = found;
       }
@@ -1209,36 +1421,238 @@ void launchSfP(pxg_ctx * c, int nb, double const * kappa, uint8_t const * cons, 
   PXG_HIP(hipGetLastError());
 }
 
-template <int P, int LPE>
+// CUBE: out = y, part = the partial buffer; otherwise out = element buffer
+template <int P, int LPE, bool CUBE>
 void launchSfBatchP(pxg_ctx * c, int nb, double const * kappa, uint8_t const * cons, double const * x, double * out,
-                    hipStream_t s)
+                    double * part, hipStream_t s)
 {
   int64_t const E = c->nelem;
+  int64_t const nslot = CUBE ? c->nCube * 8 : E;
   constexpr int EPW = 64 / LPE, WPB = sfBatchWaves<LPE>();
-  auto kern = k_sf_batch<P, LPE>;
+  auto kern = k_sf_batch<P, LPE, CUBE>;
   int dev = 0, ncu = 0, perCu = 0;
   PXG_HIP(hipGetDevice(&dev));
   PXG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   PXG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kern, 64 * WPB, 0));
-  int64_t const ngroup = (E + EPW - 1) / EPW;
+  int64_t const ngroup = (nslot + EPW - 1) / EPW;
   int64_t const need = (ngroup + WPB - 1) / WPB;
   int64_t const grid = std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)ncu * std::max(perCu, 1)));
-  kern<<<(unsigned)grid, 64 * WPB, 0, s>>>(E, nb, tabFor<P>(P == 3 ? PXG_FE_HEX_Q2 : PXG_FE_HEX_Q1), c->xyz.p,
-                                           c->conn.p, c->elemMap.p, kappa, cons, x, out, c->status.p);
+  kern<<<(unsigned)grid, 64 * WPB, 0, s>>>(E, nslot, nb, c->xyz.p, c->conn.p, c->elemMap.p, c->cubeElem.p,
+                                           c->cubeOut.p, kappa, cons, x, out, part, c->status.p);
   PXG_HIP(hipGetLastError());
 }
 
 // FP64 batches: lanes per element = the batch rounded up to 2, 4 or 8
-template <int P>
+template <int P, bool CUBE>
 void launchSfBatch(pxg_ctx * c, int nb, double const * kappa, uint8_t const * cons, double const * x, double * out,
-                   hipStream_t s)
+                   double * part, hipStream_t s)
 {
   if (nb > 4)
-    launchSfBatchP<P, 8>(c, nb, kappa, cons, x, out, s);
+    launchSfBatchP<P, 8, CUBE>(c, nb, kappa, cons, x, out, part, s);
   else if (nb > 2)
-    launchSfBatchP<P, 4>(c, nb, kappa, cons, x, out, s);
+    launchSfBatchP<P, 4, CUBE>(c, nb, kappa, cons, x, out, part, s);
   else
-    launchSfBatchP<P, 2>(c, nb, kappa, cons, x, out, s);
+    launchSfBatchP<P, 2, CUBE>(c, nb, kappa, cons, x, out, part, s);
+}
+
+// ---------------------------------------------------------------------------
+// Element cubes of the batch apply, built once per mesh and dof map on the
+// host.  Face neighbours: element f is the +d neighbour of e when f's low
+// face in direction d holds e's high face's dofs in the same order.  Cubes
+// are formed greedily in element order, e at corner (0, 0, 0), the other
+// corners from the neighbour relation; a corner that is missing, taken or
+// inconsistent stays empty (every filled pair of corners must be linked, and
+// distinct lattice nodes of the cube must hold distinct dofs, else the cube
+// keeps e alone).  Each lattice node is written by the first corner holding
+// it: to y when the cube holds every contribution of its dof (the dof's
+// element count), else to a partial slot; per partial dof its slots in cube
+// order.  Returns false when the cubes would leave most wavefront slots
+// empty (meshes without the structure).
+template <int P>
+bool buildCubes(pxg_ctx * c)
+{
+  constexpr int N = P * P * P, F = P * P, LD = 2 * P - 1;
+  int64_t const E = c->nelem, nd = c->ndofs;
+  if (nd >= ((int64_t)1 << 31) || E >= (int64_t)0xffffffffLL)
+    return false;
+  std::vector<uint32_t> em((size_t)E * N);
+  c->elemMap.download(em.data(), em.size(), c->stream);
+  PXG_HIP(hipStreamSynchronize(c->stream));
+  auto lex = [&](int64_t e, int n) { return em[(size_t)e * N + sfLocal<P>(n)]; };
+  auto node = [](int d, int t, int a, int b)
+  { return d == 0 ? t + P * (a + P * b) : (d == 1 ? a + P * (t + P * b) : a + P * (b + P * t)); };
+  using Face = std::array<uint32_t, F>;
+  struct FaceHash
+  {
+    size_t operator()(Face const & f) const
+    {
+      uint64_t h = 1469598103934665603ull;
+      for (uint32_t v : f)
+        h = (h ^ v) * 1099511628211ull;
+      return (size_t)h;
+    }
+  };
+  auto face = [&](int64_t e, int d, int t)
+  {
+    Face f;
+    for (int b = 0; b < P; ++b)
+      for (int a = 0; a < P; ++a)
+        f[a + P * b] = lex(e, node(d, t, a, b));
+    return f;
+  };
+  std::vector<uint32_t> nbr((size_t)3 * E, 0xffffffffu);
+  for (int d = 0; d < 3; ++d)
+  {
+    std::unordered_map<Face, uint32_t, FaceHash> low;
+    low.reserve((size_t)E * 2);
+    for (int64_t e = 0; e < E; ++e)
+      low.emplace(face(e, d, 0), (uint32_t)e);
+    for (int64_t e = 0; e < E; ++e)
+    {
+      auto it = low.find(face(e, d, P - 1));
+      if (it != low.end() && it->second != (uint32_t)e)
+        nbr[(size_t)d * E + e] = it->second;
+    }
+  }
+  std::vector<uint32_t> count((size_t)nd, 0);
+  for (uint32_t v : em)
+    count[v]++;
+  for (uint32_t k : count)
+    if (!k) // a dof of no element: the element-buffer gather writes its zero
+      return false;
+  std::vector<uint8_t> used((size_t)E, 0);
+  std::vector<uint32_t> ce, co;
+  std::vector<std::pair<uint32_t, uint32_t>> parts; // (dof, slot)
+  uint32_t nslot = 0;
+  int64_t full = 0;
+  std::vector<int> posOf(LD * LD * LD);
+  std::unordered_map<uint32_t, int> dofPos;
+  for (int64_t e0 = 0; e0 < E; ++e0)
+  {
+    if (used[e0])
+      continue;
+    uint32_t sl[8];
+    auto nx = [&](uint32_t e, int d) { return e == 0xffffffffu ? e : nbr[(size_t)d * E + e]; };
+    sl[0] = (uint32_t)e0;
+    sl[1] = nx(sl[0], 0);
+    sl[2] = nx(sl[0], 1);
+    sl[3] = nx(sl[1], 1);
+    sl[4] = nx(sl[0], 2);
+    sl[5] = nx(sl[1], 2);
+    sl[6] = nx(sl[2], 2);
+    sl[7] = nx(sl[3], 2);
+    for (int t = 1; t < 8; ++t)
+    {
+      bool bad = sl[t] != 0xffffffffu && used[sl[t]];
+      for (int u = 0; u < t && !bad; ++u)
+        bad = sl[u] == sl[t];
+      if (bad)
+        sl[t] = 0xffffffffu;
+    }
+    bool ok = true;
+    for (int t = 0; t < 8 && ok; ++t)
+      for (int d = 0; d < 3 && ok; ++d)
+        if (!((t >> d) & 1) && sl[t] != 0xffffffffu && sl[t | (1 << d)] != 0xffffffffu)
+          ok = nbr[(size_t)d * E + sl[t]] == sl[t | (1 << d)];
+    // lattice consistency: one dof per node, one node per dof
+    std::fill(posOf.begin(), posOf.end(), -1);
+    dofPos.clear();
+    for (int t = 0; t < 8 && ok; ++t)
+    {
+      if (sl[t] == 0xffffffffu)
+        continue;
+      for (int n = 0; n < N && ok; ++n)
+      {
+        int const i = n % P, j = (n / P) % P, k = n / F;
+        int const X = (t & 1) * (P - 1) + i, Y = ((t >> 1) & 1) * (P - 1) + j, Z = ((t >> 2) & 1) * (P - 1) + k;
+        int const pos = X + LD * (Y + LD * Z);
+        uint32_t const dof = lex(sl[t], n);
+        if (posOf[pos] < 0)
+        {
+          auto r = dofPos.emplace(dof, pos);
+          ok = r.second;
+          posOf[pos] = (int)dof;
+        }
+        else
+          ok = (uint32_t)posOf[pos] == dof;
+      }
+    }
+    if (!ok)
+      for (int t = 1; t < 8; ++t)
+        sl[t] = 0xffffffffu;
+    int filled = 0;
+    for (int t = 0; t < 8; ++t)
+      if (sl[t] != 0xffffffffu)
+      {
+        used[sl[t]] = 1;
+        ++filled;
+      }
+    full += filled == 8;
+    size_t const g = ce.size() / 8;
+    ce.insert(ce.end(), sl, sl + 8);
+    co.resize((g + 1) * N * 8, 0xffffffffu);
+    // owner and completeness of each lattice node
+    for (int Z = 0; Z < LD; ++Z)
+      for (int Y = 0; Y < LD; ++Y)
+        for (int X = 0; X < LD; ++X)
+        {
+          int owner = -1, owned = 0, onode = 0;
+          for (int t = 0; t < 8; ++t)
+          {
+            if (sl[t] == 0xffffffffu)
+              continue;
+            int const i = X - (t & 1) * (P - 1), j = Y - ((t >> 1) & 1) * (P - 1), k = Z - ((t >> 2) & 1) * (P - 1);
+            if (i < 0 || i >= P || j < 0 || j >= P || k < 0 || k >= P)
+              continue;
+            ++owned;
+            if (owner < 0)
+            {
+              owner = t;
+              onode = i + P * (j + P * k);
+            }
+          }
+          if (owner < 0)
+            continue;
+          uint32_t const dof = lex(sl[owner], onode);
+          uint32_t code;
+          if ((uint32_t)owned == count[dof])
+            code = dof | 0x80000000u;
+          else
+          {
+            code = nslot++;
+            parts.emplace_back(dof, code);
+          }
+          co[(g * N + onode) * 8 + owner] = code;
+        }
+  }
+  int64_t const G = (int64_t)(ce.size() / 8);
+  if (8 * G > 2 * E && full * 8 < E) // mostly lone elements: keep the element buffer
+    return false;
+  std::stable_sort(parts.begin(), parts.end(),
+                   [](auto const & a, auto const & b) { return a.first < b.first; });
+  std::vector<uint32_t> pd, ps, pi;
+  pi.reserve(parts.size());
+  for (size_t k = 0; k < parts.size(); ++k)
+  {
+    if (k == 0 || parts[k].first != parts[k - 1].first)
+    {
+      pd.push_back(parts[k].first);
+      ps.push_back((uint32_t)k);
+    }
+    pi.push_back(parts[k].second);
+  }
+  ps.push_back((uint32_t)parts.size());
+  hipStream_t s = c->stream;
+  c->cubeElem.upload(ce.data(), ce.size(), s);
+  c->cubeOut.upload(co.data(), co.size(), s);
+  c->partDof.upload(pd.data(), pd.size(), s);
+  c->partSeg.upload(ps.data(), ps.size(), s);
+  c->partIds.upload(pi.data(), pi.size(), s);
+  PXG_HIP(hipStreamSynchronize(s));
+  c->nCube = G;
+  c->nPartDof = (int64_t)pd.size();
+  c->nPart = nslot;
+  return true;
 }
 
 template <typename T, int MODE>
@@ -1251,9 +1665,9 @@ void launchSf(pxg_ctx * c, int P, int nb, double const * kappa, uint8_t const * 
     {
       PXG_REQUIRE((double)c->ndofs * nb < 4294967295., PXG_E_INVALID, "batch too large for 32-bit vector offsets");
       if (P == 3)
-        launchSfBatch<3>(c, nb, kappa, cons, x, out, s);
+        launchSfBatch<3, false>(c, nb, kappa, cons, x, out, nullptr, s);
       else
-        launchSfBatch<2>(c, nb, kappa, cons, x, out, s);
+        launchSfBatch<2, false>(c, nb, kappa, cons, x, out, nullptr, s);
       return;
     }
   }
@@ -1299,6 +1713,44 @@ void sfPrepare(pxg_ctx * c)
   PXG_REQUIRE(sfOrder(c), PXG_E_INVALID, "sum-factorised operator needs Q1 / Q2 hexahedra with the Gauss rule");
   if (!c->dofSeg.p)
     buildDofAdjacency(c);
+}
+
+// PXG_SF_CUBES=0 keeps batches on the element buffer and the per-dof gather
+bool sfCubesOn()
+{
+  static bool const on = []
+  {
+    char const * e = std::getenv("PXG_SF_CUBES");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+bool sfApplyCubes(pxg_ctx * c, double const * kappa, int nb, double const * x, double * y, bool useBC,
+                  hipStream_t s)
+{
+  int const P = sfOrder(c);
+  if (nb < 2 || !P || !sfBatchOn() || !sfCubesOn())
+    return false;
+  if (c->cubeState == 0)
+    c->cubeState = (P == 3 ? buildCubes<3>(c) : buildCubes<2>(c)) ? 1 : -1;
+  if (c->cubeState < 0)
+    return false;
+  PXG_REQUIRE((double)c->ndofs * nb < 4294967295., PXG_E_INVALID, "batch too large for 32-bit vector offsets");
+  c->partBuf.resize(std::max<size_t>(c->partBuf.n, (size_t)c->nPart * nb));
+  uint8_t const * cons = (useBC && c->hasBC) ? c->bcFlag.p : nullptr;
+  if (P == 3)
+    launchSfBatch<3, true>(c, nb, kappa, cons, x, y, c->partBuf.p, s);
+  else
+    launchSfBatch<2, true>(c, nb, kappa, cons, x, y, c->partBuf.p, s);
+  int64_t const m = c->nPartDof * nb;
+  if (m)
+  {
+    k_sf_part<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(c->nPartDof, nb, c->partDof.p, c->partSeg.p, c->partIds.p,
+                                                          c->partBuf.p, y);
+    PXG_HIP(hipGetLastError());
+  }
+  return true;
 }
 
 template <typename T>

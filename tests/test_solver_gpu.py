@@ -325,6 +325,40 @@ def test_matrix_free_batch_sizes(oracle, gpu, fe, n, nb):
     assert torch.equal(yt, y2)
 
 
+@pytest.mark.parametrize("fe,n", [("hex_q2", (4, 4, 4)), ("hex_q2", (6, 4, 2)), ("hex_q2", (5, 4, 3)),
+                                  ("hex_q1", (8, 6, 4)), ("hex_q1", (7, 3, 2))])
+@pytest.mark.parametrize("nb", [8, 4, 2, 11])
+def test_matrix_free_element_cubes(oracle, gpu, fe, n, nb):
+    """Batches go through 2 x 2 x 2 element cubes (shared nodes summed across
+    lanes, cube-interior dofs written directly, the others from partial
+    values): whole cubes (even sizes), cubes cut at odd sizes, perturbed
+    interior nodes, with and without the Dirichlet form; against the oracle
+    and bitwise repeatable."""
+    import torch
+    prob = W.Problem(fe, n)
+    o = oracle.Space(fe, "cube_hex", n)
+    xyz = W.perturb_interior(prob.xyz, 1.0 / max(n), seed=11, amount=0.25)
+    prob.set_coords(xyz)
+    o.set_coords(xyz)
+    a = _torch_stream(prob.assembler())
+    dofs = prob.all_sides()
+    a.set_dirichlet(dofs, np.zeros(len(dofs)))
+    cons = np.zeros(prob.ndofs, np.uint8)
+    cons[dofs] = 1
+    kap = np.exp(W.grf(prob.dof_coords(), seed=12, ell=0.3, features=64))
+    x = np.random.default_rng(13).normal(size=(prob.ndofs, nb))
+    kt = torch.from_numpy(kap).to(gpu)
+    xt = torch.from_numpy(x).to(gpu)
+    for use_bc in (True, False):
+        yt = torch.full_like(xt, np.nan)
+        a.mf_apply_dev(kt.data_ptr(), nb, xt.data_ptr(), yt.data_ptr(), use_bc)
+        y_ref = o.mf_apply(kap, x, cons if use_bc else None)
+        assert np.abs(yt.cpu().numpy() - y_ref).max() <= 1e-12 * np.abs(y_ref).max()
+        y2 = torch.full_like(xt, np.nan)
+        a.mf_apply_dev(kt.data_ptr(), nb, xt.data_ptr(), y2.data_ptr(), use_bc)
+        assert torch.equal(yt, y2)
+
+
 def test_gradient_central_differences(gpu):
     """d/dkappa (ybar . K(kappa) x) against central differences, step 1e-6."""
     import torch
