@@ -84,6 +84,11 @@ struct Shape
   // step with rows a = lane/16 (see k_element_mfma)
   static constexpr bool QM = NT == 1;
   static constexpr int MROWS = QM ? 4 : DIM + (MASS ? 1 : 0); // rows of the C factor per point
+  // multi-tile elements: gradient / basis tables and per-point factors padded
+  // with zeros to whole tiles of columns and whole blocks of four points, so
+  // the operand loop needs no clamps or selects
+  static constexpr int NP = QM ? N : 16 * NT;
+  static constexpr int NQP = QM ? NQ : 4 * ((NQ + 3) / 4);
 };
 
 // per-wave LDS (kept small: two blocks of kWaves waves must fit one CU's LDS)
@@ -96,7 +101,7 @@ struct WaveLds
   // rows of the per-point C operand factor: a < DIM the metric row
   // s_q (J^T J)^{-1}[a][:], a = DIM the mass row (m_q, 0, 0) when mass
   // terms are built (always, and zero rows after it, for q-major elements)
-  double M[NQ][MROWS][3];
+  double M[NQ][MROWS][3]; // (NQ: points padded to blocks of four, the padding zero)
   double f[N];  // lifting accumulator of the Dirichlet elimination
   double dg[sizeof(T) == 4 ? N : 1]; // FP32: diagonal of Ke, the flush threshold of noise entries
   int flag[N];
@@ -115,21 +120,22 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
                    int * __restrict__ work, uint32_t const * __restrict__ rowPos)
 {
   using S = Shape<N, DIM, NQ, MASS>;
-  using L = WaveLds<T, N, NPE, NQ, Shape<N, DIM, NQ, MASS>::MROWS>;
+  using L = WaveLds<T, N, NPE, Shape<N, DIM, NQ, MASS>::NQP, Shape<N, DIM, NQ, MASS>::MROWS>;
   using Acc = typename Mfma<T>::Acc;
   constexpr int NT = S::NT;
   constexpr bool QM = S::QM;
+  constexpr int NP = S::NP, NQP = S::NQP;
   bool const needK = (flags & 10) != 0;
 
   // mapping gradients as [vertex][component][q] and basis values as [node][q]
   // (sPhiT, for the coefficient at the points): the geometry phase has lane q
   // read them, consecutive words across lanes and so no bank conflicts
   __shared__ double sGeo[NQ * NPE * DIM];
-  __shared__ double sPhi[NQ * N];
+  __shared__ double sPhi[NQP * NP]; // phi_j(q) at q NP + j (zero padding)
   __shared__ double sPhiT[N * NQ];
-  // reference gradients: g_j(q, b) at (q N + j) DIM + b, or at (q DIM + b) N + j
-  // for q-major elements
-  __shared__ double sGrad[NQ * N * DIM];
+  // reference gradients: g_j(q, b) at (q NP + j) DIM + b (zero padding), or
+  // at (q DIM + b) N + j for q-major elements
+  __shared__ double sGrad[NQP * NP * DIM];
   __shared__ double sW[NQ];
   __shared__ L lds[kWaves];
 
@@ -141,25 +147,38 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
   }
   for (int k = tid; k < NQ * N; k += blockDim.x)
   {
-    sPhi[k] = phiG[k];
     int const q = k / N, i = k - q * N;
     sPhiT[i * NQ + q] = phiG[k];
   }
-  for (int k = tid; k < NQ * N * DIM; k += blockDim.x)
+  for (int k = tid; k < NQP * NP; k += blockDim.x)
   {
-    if constexpr (QM)
+    int const q = k / NP, j = k - q * NP;
+    sPhi[k] = q < NQ && j < N ? phiG[q * N + j] : 0.;
+  }
+  if constexpr (QM)
+  {
+    for (int k = tid; k < NQ * N * DIM; k += blockDim.x)
     {
       int const q = k / (N * DIM), j = (k / DIM) % N, b = k % DIM;
       sGrad[(q * DIM + b) * N + j] = gradG[k];
     }
-    else
-      sGrad[k] = gradG[k];
+  }
+  else
+  {
+    for (int k = tid; k < NQP * NP * DIM; k += blockDim.x)
+    {
+      int const q = k / (NP * DIM), j = (k / DIM) % NP, b = k % DIM;
+      sGrad[k] = q < NQ && j < N ? gradG[(q * N + j) * DIM + b] : 0.;
+    }
   }
   for (int k = tid; k < NQ; k += blockDim.x)
     sW[k] = wG[k];
 
   int const wave = tid >> 6, lane = tid & 63;
   L & W = lds[wave];
+  // the padded points' factors stay zero (the geometry writes q < NQ)
+  for (int k = NQ * S::MROWS * 3 + lane; k < NQP * S::MROWS * 3; k += 64)
+    (&W.M[0][0][0])[k] = 0.;
 
   __syncthreads();
 
@@ -407,24 +426,22 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
       constexpr int QB = (NQ + 3) / 4;
       auto load = [&](int qb, T (&g)[NT][DIM], T (&cv)[NT][DIM], T (&ph)[NT], T (&pm)[NT])
       {
+        // padded points (q >= NQ) have zero factors, padded columns (j >=
+        // N) zero tables: their C values come out zero
         int const q = 4 * qb + kr;
-        bool const qv = q < NQ;
-        int const qq = qv ? q : NQ - 1;
-        double const * m = &W.M[qq][0][0];
+        double const * m = &W.M[q][0][0];
         double Mq[DIM][DIM];
 #pragma unroll
         for (int a = 0; a < DIM; ++a)
 #pragma unroll
           for (int b = 0; b < DIM; ++b)
             Mq[a][b] = m[3 * a + b];
-        double const mm = MASS ? W.M[qq][DIM][0] : 0.;
+        double const mm = MASS ? W.M[q][DIM][0] : 0.;
 #pragma unroll
         for (int t = 0; t < NT; ++t)
         {
           int const j = 16 * t + jc;
-          bool const v = qv && j < N;
-          int const jj = j < N ? j : N - 1;
-          double const * gp = sGrad + (qq * N + jj) * DIM;
+          double const * gp = sGrad + (q * NP + j) * DIM;
           double gd[DIM];
 #pragma unroll
           for (int b = 0; b < DIM; ++b)
@@ -437,13 +454,13 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
             for (int b = 0; b < DIM; ++b)
               c += Mq[a][b] * gd[b];
             g[t][a] = (T)gd[a];
-            cv[t][a] = v ? (T)c : T(0);
+            cv[t][a] = (T)c;
           }
           if constexpr (MASS)
           {
-            double const p = sPhi[qq * N + jj];
+            double const p = sPhi[q * NP + j];
             ph[t] = (T)p;
-            pm[t] = v ? (T)(mm * p) : T(0);
+            pm[t] = (T)(mm * p);
           }
         }
       };
