@@ -195,6 +195,13 @@ void dropPattern(pxg_ctx * c)
   c->partSeg.release();
   c->partIds.release();
   c->partBuf.release();
+  c->nChunk = c->nPartDofF = 0;
+  c->cubeIdC.release();
+  c->cubeCmC.release();
+  c->cubeSlotOf.release();
+  c->partDofF.release();
+  c->partSegF.release();
+  c->partIdsF.release();
   c->stageR0.release();
   c->stageBlk.release();
   c->stageInfo.release();

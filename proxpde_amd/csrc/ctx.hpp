@@ -219,6 +219,12 @@ struct pxg_ctx
   pxg::DevBuf<uint32_t> cubeOut;  // [G][n][8] where each corner's node goes
   pxg::DevBuf<uint32_t> partDof, partSeg, partIds; // dofs left partial, their slots
   pxg::DevBuf<double> partBuf;    // [nPart][nb]
+  // the same cubes for the one-vector FP32 operator: slots in chunks of 64
+  int64_t nChunk = 0, nPartDofF = 0;
+  pxg::DevBuf<uint32_t> cubeIdC;    // [chunk][n][64] dof of each slot's node, bit 31: written by the slot
+  pxg::DevBuf<uint32_t> cubeCmC;    // [chunk][64] complete nodes of the slot, bit 31: slot has an element
+  pxg::DevBuf<uint32_t> cubeSlotOf; // slot of each element
+  pxg::DevBuf<uint32_t> partDofF, partSegF, partIdsF; // partial values at cube * (2P-1)^3 + node
   pxg::DevBuf<int> status;    // device error flag
   bool statusPending = false; // status of the last build not read yet (read by close)
   pxg::DevBuf<int> workCounter; // element counter of the persistent element kernels

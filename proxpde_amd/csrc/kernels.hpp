@@ -40,6 +40,15 @@ void sfElementApply(pxg_ctx * c, double const * kappa, int nb, T const * x, T * 
 // the partial values per dof); false if the mesh has no cube structure
 bool sfApplyCubes(pxg_ctx * c, double const * kappa, int nb, double const * x, double * y, bool useBC,
                   hipStream_t s);
+// element cubes of the context's mesh (built on first use); false if the
+// mesh has no cube structure
+bool sfCubesReady(pxg_ctx * c);
+// [E][6][n] FP32 metrics to the slot-major layout of the cubes ([chunk][6][n][64])
+void sfMetricSlots(pxg_ctx * c, float const * met, float * metC, hipStream_t s);
+// one vector through the FP32 operator of slot-major metrics, straight into
+// y with the epilogue of sfGather (modes 0 to 2); part: nCube (2P-1)^3 floats
+void sfApplyPaCubes(pxg_ctx * c, int mode, float const * metC, float const * x, float const * b, float const * dinv,
+                    double omega, float * y, float * part, bool useBC, hipStream_t s);
 void sfElementGrad(pxg_ctx * c, int nb, double const * x, double const * ybar, double * buf, bool useBC,
                    hipStream_t s);
 template <typename T>

@@ -654,14 +654,16 @@ __host__ __device__ constexpr double sfDc(int q, int p)
   }
   return s;
 }
-// s + a u with a constant a (zero and one fold away after unrolling)
-__device__ inline double sfMad(double a, double u, double s)
+// s + a u with a constant a (zero and one fold away after unrolling; in
+// FP32 the table value rounded to FP32)
+template <typename T>
+__device__ inline T sfMad(double a, T u, T s)
 {
   if (a == 0.)
     return s;
   if (a == 1.)
     return s + u;
-  return fma(a, u, s);
+  return fma(T(a), u, s);
 }
 // table values at a lane-dependent point index q (selects over constants)
 template <int P>
@@ -702,8 +704,8 @@ __device__ inline void sfFence(double (&a)[P], double (&b)[P], double (&c)[P])
 }
 // in place along direction DIR of a P^3 lexicographic array: w <- A w per
 // line, A = B (DC = false) or Dc (DC = true), transposed for TR
-template <int P, int DIR, bool TR, bool DC>
-__device__ inline void sfLines(double (&w)[P * P * P])
+template <int P, int DIR, bool TR, bool DC, typename T>
+__device__ inline void sfLines(T (&w)[P * P * P])
 {
   auto idx = [](int t, int a, int b)
   {
@@ -719,14 +721,14 @@ __device__ inline void sfLines(double (&w)[P * P * P])
 #pragma unroll
     for (int a = 0; a < P; ++a)
     {
-      double in[P], r[P];
+      T in[P], r[P];
 #pragma unroll
       for (int t = 0; t < P; ++t)
         in[t] = w[idx(t, a, b)];
 #pragma unroll
       for (int q = 0; q < P; ++q)
       {
-        double s = 0.;
+        T s = T(0);
 #pragma unroll
         for (int t = 0; t < P; ++t)
         {
@@ -753,8 +755,8 @@ __device__ inline void sfLines(double (&w)[P * P * P])
 // DIR apart): the nodes of the shared face, low corner's value first (a
 // corner without an element holds zeros).  After the three directions every
 // corner holds, at each node, the sum over the corners sharing it.
-template <int P, int DIR, int LPE>
-__device__ inline void sfCubeSum(double (&o)[P * P * P], int bit)
+template <int P, int DIR, int LPE, typename T>
+__device__ inline void sfCubeSum(T (&o)[P * P * P], int bit)
 {
   auto idx = [](int t, int a, int b)
   {
@@ -771,10 +773,10 @@ __device__ inline void sfCubeSum(double (&o)[P * P * P], int bit)
     for (int a = 0; a < P; ++a)
     {
       // my face node: t = P - 1 on the low corner, t = 0 on the high one
-      double const lo = o[idx(P - 1, a, b)], hi = o[idx(0, a, b)];
-      double const mine = bit ? hi : lo;
-      double const theirs = __shfl_xor(mine, LPE << DIR);
-      double const sum = bit ? theirs + mine : mine + theirs;
+      T const lo = o[idx(P - 1, a, b)], hi = o[idx(0, a, b)];
+      T const mine = bit ? hi : lo;
+      T const theirs = __shfl_xor(mine, LPE << DIR);
+      T const sum = bit ? theirs + mine : mine + theirs;
       if (bit)
         o[idx(0, a, b)] = sum;
       else
@@ -1310,6 +1312,211 @@ __global__ void k_sf_part(int64_t npd, int nb, uint32_t const * __restrict__ dof
   y[(size_t)dofs[k] * nb + b] = acc;
 }
 
+// metrics of k_sf_elem's [E][6][n] layout to the slots of the element
+// cubes, [chunk][6][n][64] (element slot innermost)
+template <typename T>
+__global__ void k_met_slots(int64_t E, int n, uint32_t const * __restrict__ slotOf, T const * __restrict__ met,
+                            T * __restrict__ metC)
+{
+  int64_t const t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t const per = 6 * (int64_t)n;
+  if (t >= E * per)
+    return;
+  int64_t const e = t / per;
+  int64_t const r = t - e * per;
+  uint32_t const so = slotOf[e];
+  metC[((int64_t)(so >> 6) * per + r) * 64 + (so & 63)] = met[t];
+}
+
+// the epilogue of a summed operator row d: y = A x (0), b - A x (1),
+// x + omega dinv (b - A x) (2)
+template <int MODE>
+__device__ inline float sfEpilogue(float acc, uint32_t d, float const * __restrict__ x, float const * __restrict__ b,
+                                   float const * __restrict__ dinv, float omega)
+{
+  if constexpr (MODE == kGApply)
+    return acc;
+  else if constexpr (MODE == kGResidual)
+    return b[d] - acc;
+  else
+    return x[d] + omega * dinv[d] * (b[d] - acc);
+}
+
+// One vector through the FP32 operator of stored metrics (the smoother,
+// residual and apply of a matrix-free multigrid level), one lane per
+// element slot of the element cubes: the slot's dof ids, complete-node bits
+// and metrics are read lane-contiguous (chunks of 64 slots), the tensor
+// contractions run in registers (collocation form), the cube's shared nodes
+// are summed across lanes, and the slot writes the nodes it owns: the
+// epilogue's value to y where the cube holds every contribution, else the
+// partial value at (cube, lattice node), which k_sf_part_f sums per dof.
+template <int P, int MODE>
+__global__ void __launch_bounds__(256)
+    k_sf_pa_cube(int64_t nChunk, uint32_t const * __restrict__ idC, uint32_t const * __restrict__ cmC,
+                 uint8_t const * __restrict__ cons, float const * __restrict__ metC, float const * __restrict__ x,
+                 float const * __restrict__ b, float const * __restrict__ dinv, float omega, float * __restrict__ y,
+                 float * __restrict__ part)
+{
+  constexpr int N = P * P * P, LD = 2 * P - 1, LATT = LD * LD * LD;
+  int64_t const ck = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (ck >= nChunk) // (whole wavefronts)
+    return;
+  int const lane = threadIdx.x & 63, corner = lane & 7;
+  auto at = [](int i, int j, int k) { return i + P * (j + P * k); };
+  uint32_t const cm = cmC[ck * 64 + lane];
+  bool const live = (cm >> 31) != 0;
+  uint32_t const * const ids = idC + (size_t)ck * N * 64 + lane;
+  float const * const mt = metC + (size_t)ck * 6 * N * 64 + lane;
+  // loads without branches around them (a slot without an element reads
+  // dof 0), so each phase's loads are in flight together
+  float w[N];
+  uint32_t fl = 0;
+  {
+    uint32_t d[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n)
+      d[n] = ids[n * 64] & 0x7fffffffu;
+#pragma unroll
+    for (int n = 0; n < N; ++n)
+      w[n] = x[d[n]];
+    if (cons) // (uniform)
+    {
+      uint8_t cn[N];
+#pragma unroll
+      for (int n = 0; n < N; ++n)
+        cn[n] = cons[d[n]];
+#pragma unroll
+      for (int n = 0; n < N; ++n)
+      {
+        fl |= cn[n] ? 1u << n : 0u;
+        w[n] = cn[n] ? 0.f : w[n];
+      }
+    }
+  }
+  sfLines<P, 0, false, false>(w);
+  sfLines<P, 1, false, false>(w);
+  sfLines<P, 2, false, false>(w);
+  float o[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n)
+    o[n] = 0.f;
+#pragma unroll
+  for (int qz = 0; qz < P; ++qz)
+#pragma unroll
+    for (int qy = 0; qy < P; ++qy)
+#pragma unroll
+      for (int qx = 0; qx < P; ++qx)
+      {
+        float gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+        for (int t = 0; t < P; ++t)
+        {
+          gx = sfMad(sfDc<P>(qx, t), w[at(t, qy, qz)], gx);
+          gy = sfMad(sfDc<P>(qy, t), w[at(qx, t, qz)], gy);
+          gz = sfMad(sfDc<P>(qz, t), w[at(qx, qy, t)], gz);
+        }
+        int const q = at(qx, qy, qz);
+        float const m0 = mt[(0 * N + q) * 64], m1 = mt[(1 * N + q) * 64], m2 = mt[(2 * N + q) * 64];
+        float const m3 = mt[(3 * N + q) * 64], m4 = mt[(4 * N + q) * 64], m5 = mt[(5 * N + q) * 64];
+        float const vx = m0 * gx + m1 * gy + m2 * gz;
+        float const vy = m1 * gx + m3 * gy + m4 * gz;
+        float const vz = m2 * gx + m4 * gy + m5 * gz;
+#pragma unroll
+        for (int t = 0; t < P; ++t)
+        {
+          o[at(t, qy, qz)] = sfMad(sfDc<P>(qx, t), vx, o[at(t, qy, qz)]);
+          o[at(qx, t, qz)] = sfMad(sfDc<P>(qy, t), vy, o[at(qx, t, qz)]);
+          o[at(qx, qy, t)] = sfMad(sfDc<P>(qz, t), vz, o[at(qx, qy, t)]);
+        }
+      }
+  sfLines<P, 0, true, false>(o);
+  sfLines<P, 1, true, false>(o);
+  sfLines<P, 2, true, false>(o);
+  // the unit diagonal per element on constrained rows; a slot without an
+  // element holds zeros
+  if (fl) // (constrained nodes: reload their x values, all at once)
+  {
+    float xr[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n)
+      xr[n] = x[ids[n * 64] & 0x7fffffffu];
+#pragma unroll
+    for (int n = 0; n < N; ++n)
+      o[n] = (fl >> n) & 1u ? xr[n] : o[n];
+  }
+#pragma unroll
+  for (int n = 0; n < N; ++n)
+    o[n] = live ? o[n] : 0.f;
+  sfCubeSum<P, 0, 1>(o, corner & 1);
+  sfCubeSum<P, 1, 1>(o, (corner >> 1) & 1);
+  sfCubeSum<P, 2, 1>(o, (corner >> 2) & 1);
+  int64_t const cube = ck * 8 + (lane >> 3);
+  // the owned nodes, a third of them at a time: epilogue inputs loaded
+  // unconditionally (valid dofs), then predicated stores
+  constexpr int G3 = (N + 2) / 3;
+#pragma unroll
+  for (int n0 = 0; n0 < N; n0 += G3)
+  {
+    uint32_t iv[G3];
+    float ev[G3];
+#pragma unroll
+    for (int k = 0; k < G3; ++k)
+      iv[k] = n0 + k < N ? ids[(n0 + k) * 64] : 0u;
+#pragma unroll
+    for (int k = 0; k < G3; ++k)
+      if (n0 + k < N)
+        ev[k] = sfEpilogue<MODE>(o[n0 + k], iv[k] & 0x7fffffffu, x, b, dinv, omega);
+#pragma unroll
+    for (int k = 0; k < G3; ++k)
+    {
+      int const n = n0 + k;
+      if (n >= N || !(iv[k] & 0x80000000u))
+        continue;
+      if ((cm >> n) & 1u)
+        y[iv[k] & 0x7fffffffu] = ev[k];
+      else
+      {
+        int const X = (corner & 1) * (P - 1) + n % P;
+        int const Y = ((corner >> 1) & 1) * (P - 1) + (n / P) % P;
+        int const Z = ((corner >> 2) & 1) * (P - 1) + n / (P * P);
+        part[cube * LATT + X + LD * (Y + LD * Z)] = o[n];
+      }
+    }
+  }
+}
+
+// rows the cubes leave partial: the sum of their partial values in cube
+// order, then the epilogue
+template <int MODE>
+__global__ void k_sf_part_f(int64_t npd, uint32_t const * __restrict__ dofs, uint32_t const * __restrict__ seg,
+                            uint32_t const * __restrict__ ids, float const * __restrict__ part,
+                            float const * __restrict__ x, float const * __restrict__ b,
+                            float const * __restrict__ dinv, float omega, float * __restrict__ y)
+{
+  int64_t const k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npd)
+    return;
+  uint32_t const k0 = seg[k], k1 = seg[k + 1];
+  constexpr int kMaxC = 8;
+  uint32_t id[kMaxC];
+#pragma unroll
+  for (int j = 0; j < kMaxC; ++j)
+    id[j] = ids[min(k0 + j, k1 - 1)];
+  float v[kMaxC];
+#pragma unroll
+  for (int j = 0; j < kMaxC; ++j)
+    v[j] = part[id[j]];
+  float acc = v[0];
+#pragma unroll
+  for (int j = 1; j < kMaxC; ++j)
+    if (k0 + j < k1)
+      acc += v[j];
+  for (uint32_t j = k0 + kMaxC; j < k1; ++j)
+    acc += part[ids[j]];
+  uint32_t const d = dofs[k];
+  y[d] = sfEpilogue<MODE>(acc, d, x, b, dinv, omega);
+}
+
 template <int P>
 SfTab<P> makeTab(int feType)
 {
@@ -1523,6 +1730,13 @@ bool buildCubes(pxg_ctx * c)
   std::vector<uint8_t> used((size_t)E, 0);
   std::vector<uint32_t> ce, co;
   std::vector<std::pair<uint32_t, uint32_t>> parts; // (dof, slot)
+  // one-vector FP32 operator (k_sf_pa_cube): per slot its nodes' dofs
+  // (bit 31: the slot writes the node), lanes of a chunk of 64 slots
+  // innermost; per slot the complete nodes (bit 31: the slot has an element);
+  // partial values at (cube, lattice node)
+  constexpr int LATT = LD * LD * LD;
+  std::vector<uint32_t> idC, cmC, slotOf((size_t)E, 0);
+  std::vector<std::pair<uint32_t, uint32_t>> partsF;
   uint32_t nslot = 0;
   int64_t full = 0;
   std::vector<int> posOf(LD * LD * LD);
@@ -1591,6 +1805,19 @@ bool buildCubes(pxg_ctx * c)
     size_t const g = ce.size() / 8;
     ce.insert(ce.end(), sl, sl + 8);
     co.resize((g + 1) * N * 8, 0xffffffffu);
+    size_t const nck = (8 * (g + 1) + 63) / 64;
+    idC.resize(nck * N * 64, 0u);
+    cmC.resize(nck * 64, 0u);
+    for (int t = 0; t < 8; ++t)
+    {
+      if (sl[t] == 0xffffffffu)
+        continue;
+      size_t const so = g * 8 + t;
+      slotOf[sl[t]] = (uint32_t)so;
+      cmC[so] = 0x80000000u;
+      for (int n = 0; n < N; ++n)
+        idC[((so / 64) * N + n) * 64 + so % 64] = lex(sl[t], n);
+    }
     // owner and completeness of each lattice node
     for (int Z = 0; Z < LD; ++Z)
       for (int Y = 0; Y < LD; ++Y)
@@ -1615,12 +1842,18 @@ bool buildCubes(pxg_ctx * c)
             continue;
           uint32_t const dof = lex(sl[owner], onode);
           uint32_t code;
+          size_t const so = g * 8 + owner;
+          idC[((so / 64) * N + onode) * 64 + so % 64] |= 0x80000000u;
           if ((uint32_t)owned == count[dof])
+          {
             code = dof | 0x80000000u;
+            cmC[so] |= 1u << onode;
+          }
           else
           {
             code = nslot++;
             parts.emplace_back(dof, code);
+            partsF.emplace_back(dof, (uint32_t)(g * LATT + (X + LD * (Y + LD * Z))));
           }
           co[(g * N + onode) * 8 + owner] = code;
         }
@@ -1628,30 +1861,44 @@ bool buildCubes(pxg_ctx * c)
   int64_t const G = (int64_t)(ce.size() / 8);
   if (8 * G > 2 * E && full * 8 < E) // mostly lone elements: keep the element buffer
     return false;
-  std::stable_sort(parts.begin(), parts.end(),
-                   [](auto const & a, auto const & b) { return a.first < b.first; });
-  std::vector<uint32_t> pd, ps, pi;
-  pi.reserve(parts.size());
-  for (size_t k = 0; k < parts.size(); ++k)
+  // per partial dof its slots in cube order
+  auto lists = [](std::vector<std::pair<uint32_t, uint32_t>> & pr, std::vector<uint32_t> & pd,
+                  std::vector<uint32_t> & ps, std::vector<uint32_t> & pi)
   {
-    if (k == 0 || parts[k].first != parts[k - 1].first)
+    std::stable_sort(pr.begin(), pr.end(), [](auto const & a, auto const & b) { return a.first < b.first; });
+    pi.reserve(pr.size());
+    for (size_t k = 0; k < pr.size(); ++k)
     {
-      pd.push_back(parts[k].first);
-      ps.push_back((uint32_t)k);
+      if (k == 0 || pr[k].first != pr[k - 1].first)
+      {
+        pd.push_back(pr[k].first);
+        ps.push_back((uint32_t)k);
+      }
+      pi.push_back(pr[k].second);
     }
-    pi.push_back(parts[k].second);
-  }
-  ps.push_back((uint32_t)parts.size());
+    ps.push_back((uint32_t)pr.size());
+  };
+  std::vector<uint32_t> pd, ps, pi, pdF, psF, piF;
+  lists(parts, pd, ps, pi);
+  lists(partsF, pdF, psF, piF);
   hipStream_t s = c->stream;
   c->cubeElem.upload(ce.data(), ce.size(), s);
   c->cubeOut.upload(co.data(), co.size(), s);
   c->partDof.upload(pd.data(), pd.size(), s);
   c->partSeg.upload(ps.data(), ps.size(), s);
   c->partIds.upload(pi.data(), pi.size(), s);
+  c->cubeIdC.upload(idC.data(), idC.size(), s);
+  c->cubeCmC.upload(cmC.data(), cmC.size(), s);
+  c->cubeSlotOf.upload(slotOf.data(), slotOf.size(), s);
+  c->partDofF.upload(pdF.data(), pdF.size(), s);
+  c->partSegF.upload(psF.data(), psF.size(), s);
+  c->partIdsF.upload(piF.data(), piF.size(), s);
   PXG_HIP(hipStreamSynchronize(s));
   c->nCube = G;
   c->nPartDof = (int64_t)pd.size();
   c->nPart = nslot;
+  c->nChunk = (int64_t)(cmC.size() / 64);
+  c->nPartDofF = (int64_t)pdF.size();
   return true;
 }
 
@@ -1730,11 +1977,7 @@ bool sfApplyCubes(pxg_ctx * c, double const * kappa, int nb, double const * x, d
                   hipStream_t s)
 {
   int const P = sfOrder(c);
-  if (nb < 2 || !P || !sfBatchOn() || !sfCubesOn())
-    return false;
-  if (c->cubeState == 0)
-    c->cubeState = (P == 3 ? buildCubes<3>(c) : buildCubes<2>(c)) ? 1 : -1;
-  if (c->cubeState < 0)
+  if (nb < 2 || !sfBatchOn() || !sfCubesReady(c))
     return false;
   PXG_REQUIRE((double)c->ndofs * nb < 4294967295., PXG_E_INVALID, "batch too large for 32-bit vector offsets");
   c->partBuf.resize(std::max<size_t>(c->partBuf.n, (size_t)c->nPart * nb));
@@ -1751,6 +1994,61 @@ bool sfApplyCubes(pxg_ctx * c, double const * kappa, int nb, double const * x, d
     PXG_HIP(hipGetLastError());
   }
   return true;
+}
+
+bool sfCubesReady(pxg_ctx * c)
+{
+  int const P = sfOrder(c);
+  if (!P || !sfCubesOn())
+    return false;
+  sfPrepare(c);
+  if (c->cubeState == 0)
+    c->cubeState = (P == 3 ? buildCubes<3>(c) : buildCubes<2>(c)) ? 1 : -1;
+  return c->cubeState > 0;
+}
+
+void sfMetricSlots(pxg_ctx * c, float const * met, float * metC, hipStream_t s)
+{
+  PXG_REQUIRE(c->cubeState > 0, PXG_E_STATE, "element cubes not built");
+  int const n = c->fe.n;
+  PXG_HIP(hipMemsetAsync(metC, 0, sizeof(float) * (size_t)c->nChunk * 6 * n * 64, s));
+  int64_t const m = c->nelem * 6 * n;
+  k_met_slots<float><<<(unsigned)((m + 255) / 256), 256, 0, s>>>(c->nelem, n, c->cubeSlotOf.p, met, metC);
+  PXG_HIP(hipGetLastError());
+}
+
+void sfApplyPaCubes(pxg_ctx * c, int mode, float const * metC, float const * x, float const * b, float const * dinv,
+                    double omega, float * y, float * part, bool useBC, hipStream_t s)
+{
+  PXG_REQUIRE(c->cubeState > 0, PXG_E_STATE, "element cubes not built");
+  uint8_t const * cons = (useBC && c->hasBC) ? c->bcFlag.p : nullptr;
+  unsigned const g = (unsigned)((c->nChunk + 3) / 4);
+  int64_t const npd = c->nPartDofF;
+  unsigned const gp = (unsigned)((npd + 255) / 256);
+  auto go = [&](auto pC, auto mC)
+  {
+    constexpr int P = decltype(pC)::value, MODE = decltype(mC)::value;
+    k_sf_pa_cube<P, MODE><<<g, 256, 0, s>>>(c->nChunk, c->cubeIdC.p, c->cubeCmC.p, cons, metC, x, b, dinv,
+                                            (float)omega, y, part);
+    if (npd)
+      k_sf_part_f<MODE><<<gp, 256, 0, s>>>(npd, c->partDofF.p, c->partSegF.p, c->partIdsF.p, part, x, b, dinv,
+                                           (float)omega, y);
+  };
+  auto pick = [&](auto pC)
+  {
+    switch (mode)
+    {
+    case kGApply: go(pC, std::integral_constant<int, kGApply>{}); break;
+    case kGResidual: go(pC, std::integral_constant<int, kGResidual>{}); break;
+    case kGJacobi: go(pC, std::integral_constant<int, kGJacobi>{}); break;
+    default: throw Error(PXG_E_INVALID, "cube operator mode");
+    }
+  };
+  if (sfOrder(c) == 3)
+    pick(std::integral_constant<int, 3>{});
+  else
+    pick(std::integral_constant<int, 2>{});
+  PXG_HIP(hipGetLastError());
 }
 
 template <typename T>

@@ -25,6 +25,8 @@
 // b - A x (see solveIR).
 #include "kernels.hpp"
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -66,6 +68,10 @@ struct pxg_mg
     // level's current kappa field, element buffer, inverse diagonal
     uint64_t mfKappaStamp = 0;
     DevBuf<float> met32, mfBuf32, dinvMf32;
+    // element cubes of the level's mesh: slot-major metrics and the partial
+    // values of the one-vector operator (met32 and mfBuf32 then released)
+    bool cubes = false;
+    DevBuf<float> metC, partF;
   };
   std::vector<Level> lv;
   DevBuf<double> sc, partial;
@@ -482,6 +488,12 @@ struct LevelOp
   {
     pxg_ctx * c = m->ctx[l];
     auto & L = m->lv[l];
+    if (L.cubes)
+    {
+      sfApplyPaCubes(c, mode, L.metC.p, (float const *)x, (float const *)b, L.dinvMf32.p, omega, (float *)y,
+                     L.partF.p, true, s);
+      return;
+    }
     sfApplyPA<float>(c, L.met32.p, (float const *)x, L.mfBuf32.p, true, s);
     sfGather<float>(c, mode, L.mfBuf32.p, (float const *)x, (float const *)b, L.dinvMf32.p, omega, (float *)y, s);
   }
@@ -539,6 +551,16 @@ void smooth(LevelOp<T> const & A, T const * b, T * x, T * t, int sweeps, double 
     PXG_HIP(hipMemcpyAsync(x, cur, n * sizeof(T), hipMemcpyDeviceToDevice, s));
 }
 
+// smallest level applied through the element cubes (two wavefronts of 64
+// elements per SIMD); PXG_MF_CUBE_MIN_ELEMS sets another bound (tests run
+// small levels through the cubes with it)
+constexpr int64_t kMfCubeMinElems = 131072;
+int64_t mfCubeMinElems()
+{
+  char const * e = std::getenv("PXG_MF_CUBE_MIN_ELEMS");
+  return e && *e ? std::atoll(e) : kMfCubeMinElems;
+}
+
 // metrics and inverse diagonal of matrix-free level l for the current kappa
 // field of its context (recomputed when kappa was set again)
 void prepareMf(pxg_mg * m, int l)
@@ -549,7 +571,7 @@ void prepareMf(pxg_mg * m, int l)
   PXG_REQUIRE(sfAvailable(c), PXG_E_INVALID,
               "the matrix-free levels need Q1 / Q2 hexahedra with their Gauss rule");
   sfPrepare(c);
-  if (L.mfKappaStamp == c->kappaStamp && L.met32.p)
+  if (L.mfKappaStamp == c->kappaStamp && (L.cubes ? L.metC.p : L.met32.p))
     return;
   hipStream_t s = m->s;
   int64_t const E = c->nelem, n = c->fe.n;
@@ -559,6 +581,20 @@ void prepareMf(pxg_mg * m, int l)
   sfMetric<float>(c, c->kappa.p, L.met32.p, s);
   sfElementDiag<float>(c, c->kappa.p, L.mfBuf32.p, s);
   sfGather<float>(c, 3, L.mfBuf32.p, nullptr, nullptr, nullptr, 0., L.dinvMf32.p, s);
+  // (the cube kernel runs one lane per element: on small levels too few
+  // wavefronts to cover its latency, where the pencil kernel's nine lanes per
+  // element do better)
+  L.cubes = E >= mfCubeMinElems() && sfCubesReady(c);
+  if (L.cubes)
+  {
+    int const ld = c->fe.feType == PXG_FE_HEX_Q2 ? 5 : 3; // cube lattice side
+    L.metC.resize((size_t)c->nChunk * 6 * n * 64);
+    L.partF.resize(std::max<size_t>(1, (size_t)c->nCube * ld * ld * ld));
+    sfMetricSlots(c, L.met32.p, L.metC.p, s);
+    PXG_HIP(hipStreamSynchronize(s));
+    L.met32.release();
+    L.mfBuf32.release();
+  }
   L.mfKappaStamp = c->kappaStamp;
 }
 

@@ -220,6 +220,33 @@ def test_restriction_known_answer_and_oracle(oracle, native, gpu):
     assert np.abs(val - oval).max() <= 1e-15
 
 
+@pytest.mark.parametrize("fe,ns", [("hex_q1", (16, 8, 4, 2)), ("hex_q2", (8, 4, 2)), ("hex_q2", (6, 3))])
+def test_mg_matrix_free_levels_through_element_cubes(oracle, native, gpu, fe, ns, monkeypatch):
+    """The FP32 matrix-free levels applied through the element cubes (one
+    lane per element, slot-major metrics, cube nodes summed across lanes,
+    the Jacobi / residual epilogue fused) precondition like the per-element
+    kernel with its gather: same iteration count, solutions within the
+    solve tolerance, FP64 residual at the tolerance.  (Levels of any size go
+    through the cubes here; by default only large ones do.)"""
+    import torch
+    out = {}
+    for cubes in (False, True):
+        monkeypatch.setenv("PXG_MF_CUBE_MIN_ELEMS", "0" if cubes else str(1 << 60))
+        gl, _ = _levels(oracle, fe, ns)
+        prob, a0 = gl[0]
+        mg = native.Multigrid([a for _, a in gl], [(n, n, n) for n in ns])
+        rp, col, val, b = a0.csr()
+        xt = torch.zeros(prob.ndofs, dtype=torch.float64, device=gpu)
+        it, rr = mg.solve_dev(a0.rhs_ptr(), xt.data_ptr(), rtol=1e-8, maxit=500, mode=2)
+        A = sp.csr_matrix((val, col, rp), shape=(prob.ndofs, prob.ndofs))
+        u = xt.cpu().numpy()
+        assert np.linalg.norm(b - A @ u) / np.linalg.norm(b) <= 1.05e-8
+        out[cubes] = (it, u)
+        del mg
+    assert abs(out[True][0] - out[False][0]) <= 1
+    assert np.linalg.norm(out[True][1] - out[False][1]) <= 1e-7 * np.linalg.norm(out[False][1])
+
+
 @pytest.mark.parametrize("fe,ns", [("hex_q1", (16, 8, 4, 2)), ("hex_q2", (8, 4, 2))])
 def test_mg_pcg_fp64_and_fp32_refined(oracle, native, gpu, fe, ns):
     import torch
@@ -235,9 +262,11 @@ def test_mg_pcg_fp64_and_fp32_refined(oracle, native, gpu, fe, ns):
     # mode 2: FP32-refined with every level but the coarsest applied
     # matrix-free from kappa (their operators are the assembled matrices up to
     # rounding)
+    its = {}
     for mode in (0, 1, 2):
         xt = torch.zeros_like(bt)
         it, rr = mg.solve_dev(a0.rhs_ptr(), xt.data_ptr(), rtol=1e-8, maxit=500, mode=mode)
+        its[mode] = it
         u = xt.cpu().numpy()
         res = np.linalg.norm(b - A @ u) / np.linalg.norm(b)
         err = np.linalg.norm(u - u_exact) / np.linalg.norm(u_exact)
@@ -247,6 +276,9 @@ def test_mg_pcg_fp64_and_fp32_refined(oracle, native, gpu, fe, ns):
         assert err <= 1e-6
         if mode == 0:
             assert abs(it - it_o) <= 1
+    # the matrix-free levels (element cubes, FP32) precondition like the
+    # assembled FP32 levels they stand for
+    assert abs(its[2] - its[1]) <= 1
 
 
 @pytest.mark.parametrize("fe,mesh,n,pert", [("hex_q2", "cube_hex", (3, 3, 3), 0.0),
