@@ -115,7 +115,7 @@ def kappa_apply_batch(args, dev, a, kappa_host, nb=8):
     return {"batch": nb, "forward_ms": 1e3 * t_fwd, "forward_backward_ms": 1e3 * t_fb,
             "forward_reference_equiv_gflops": fl / t_fwd / 1e9, "apply_kernels_ms": 1e3 * t_ap,
             "gradk_kernels_ms": gr_ms / max(gr_n, 1), "apply_bytes": by,
-            "roofline": {"kernel": "k_sf_elem<double,3,0,8> + k_gather", "bound": "hbm",
+            "roofline": {"kernel": "k_sf_batch<3,8,true> + k_sf_part", "bound": "hbm",
                          "achieved": by / t_ap / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": by / t_ap / 1e9 / HBM_PEAK_GBS, "traffic": traffic}}
 
