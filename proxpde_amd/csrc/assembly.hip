@@ -1656,6 +1656,17 @@ void buildPattern(pxg_ctx * c)
   ++c->stamp;
 }
 
+// PXG_TH_MFMA=0 keeps the Q2-Q1 hexahedron on the vector kernel (comparisons)
+bool thMfmaOn()
+{
+  static bool const on = []
+  {
+    char const * e = std::getenv("PXG_TH_MFMA");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
 {
   PXG_REQUIRE(c->fe.feType && c->nelem && c->ndofs && c->nnzS, PXG_E_STATE,
@@ -1771,6 +1782,10 @@ void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)
                  int const G = (int)std::min<size_t>(kBlock / N, (65536 - tab) / per);
                  size_t const sh = (size_t)G * per + tab;
                  ProfScope ps(c, sizeof(TT) == 8 ? "element" : "element32");
+                 // the Q2-Q1 hexahedron on the matrix cores (element_th_mfma.hip)
+                 if (thMfmaOn() && elementThMfma(c, th.stiff, th.tensor, th.mass, th.grad, th.div,
+                                                 sizeof(TT) == 8 ? PXG_FP64 : PXG_FP32))
+                   return;
                  unsigned const grid = (unsigned)((E + G - 1) / G);
                  k_element_th<TT, Sh><<<grid, kBlock, sh, s>>>(E, G, T.nq, c->xyz.p, c->conn.p, T.d_geoGrad.p,
                                                               T.d_w.p, gradTab, phiTab, th, c->elemMap.p,

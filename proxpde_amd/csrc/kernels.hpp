@@ -23,6 +23,10 @@ void ensureVal32(pxg_ctx * c);
 void ensureVal64(pxg_ctx * c);
 // element_mfma.hip: fused geometry + MFMA element kernel; false if not applicable
 bool elementMfma(pxg_ctx * c, int flags, double cS, double cSF, double cM, double cMF, int precision);
+// element_th_mfma.hip: Taylor-Hood Q2-Q1 hexahedron on the matrix cores,
+// coefficients of stiffness, tensor stiffness, mass, grad, div; false if the
+// context holds another element
+bool elementThMfma(pxg_ctx * c, double kS, double kT, double kM, double kG, double kD, int precision);
 // element_exact.hip: linear elements (P1 line, Q1 quad, Q1 hex, P1 tet)
 // computed in FP64 in the reference's evaluation order, geometry fused, into
 // keBuf (PXG_FP64) or rounded into keBuf32 (PXG_FP32, exact-arithmetic zeros
