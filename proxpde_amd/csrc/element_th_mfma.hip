@@ -14,21 +14,27 @@
 //   pressure k x velocity (e, j):   k_d sum_q s_q psi_k G_q[j][e]
 // Three products on v_mfma_f64_16x16x4f64 (FP32: f32) tiles, one wavefront
 // per element:
-//   S (27 x 27, K = 3 x 28 points + 28 mass rows): the per-component block,
-//     staged in LDS and added on the three diagonal blocks at the store;
-//   T (velocity rows and columns laid out as 6 tiles of 16, K = 28 points):
-//     the coupled block, upper tiles only (it is symmetric), and in the same
-//     pass the 6 tiles against the pressure tile (the grad block, without
-//     its coefficient); the div block is the grad block mirrored with k_d.
+//   S (27 x 27 as 2 x 2 tiles, K = 3 x 28 points + 28 mass rows): the
+//     per-component block, staged in LDS;
+//   T (K = 28 points): the velocity rows and columns as 6 tiles of 16, tile t
+//     holding component t / 2 and functions 16 (t % 2) + 0..15 (the odd
+//     tiles 11 of them), so one tile's operands G_q[a][t / 2] use one column
+//     of P.  The coupled block in upper tiles only (it is symmetric), and in
+//     the same pass the tiles against the pressure tile (the grad block,
+//     without its coefficient); the div block is the grad block mirrored
+//     with k_d.  Tiles of one component get the S tile added at the store.
 // Operand values are formed in registers from the per-point P (LDS, per
 // wavefront) and the reference tables (LDS, per block); points padded to 28
 // have P = 0 and JxW = 0.  The epilogue applies the per-element Dirichlet
 // elimination of the reference Builder (constrained rows become unit rows,
-// constrained columns are cleared into the lifting Fe = -Ke h), with the
-// lifting sums in a fixed order.
+// constrained columns are cleared into the lifting Fe = -Ke h); the lifting
+// sums come from the point operands (sum_j Ke_ij h_j = sum_q A_iq (B h)_q for
+// the product blocks, rows of S for the per-component block), in a fixed
+// order.
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <cstddef>
 #include <type_traits>
 #include <utility>
 
@@ -40,8 +46,15 @@ namespace
 
 constexpr int kThWaves = 4;
 constexpr int kNV = 27, kNP = 8, kN = 3 * kNV + kNP, kNQ = 27, kNQP = 28, kNPE = 8;
-// tiled index of the T product: velocity 0..80, pressure 96..103
-constexpr int kPT = 6; // the pressure tile
+// Tiled velocity index: tile t (0..5) holds component t / 2, functions
+// 16 (t % 2) + 0..15 (odd tiles: 11 functions, then padding); tile 6 holds
+// the pressure functions 0..7.  Every tile has one component, so a tile's
+// operands share one column of P and the per-component block is added in
+// whole tiles.
+constexpr int kVT = 6, kPT = 6, kHi = kNV - 16;
+
+// local index (0..88) of row 0 of tile t
+__host__ __device__ constexpr int thBase(int t) { return t < kVT ? kNV * (t >> 1) + 16 * (t & 1) : 3 * kNV; }
 
 template <typename T>
 struct ThMfma;
@@ -70,21 +83,39 @@ __device__ inline void thSync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// local index (0..88) of a tiled index, -1 for padding
-__device__ inline int thLocal(int r)
+// a double moved across lanes by a DPP pattern within rows of 16 lanes
+template <int CTRL>
+__device__ inline double thDpp(double x)
 {
-  return r < 3 * kNV ? r : (r >= 16 * kPT && r < 16 * kPT + kNP ? 3 * kNV + (r - 16 * kPT) : -1);
+  unsigned long long const b = (unsigned long long)__double_as_longlong(x);
+  int const lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  int const hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// sum over the 16 lanes of a row (pairs, quads, halves, rows: every lane
+// gets the same bits, IEEE addition being commutative)
+__device__ inline double thRowSum(double x)
+{
+  x += thDpp<0xB1>(x);  // quad_perm [1, 0, 3, 2]
+  x += thDpp<0x4E>(x);  // quad_perm [2, 3, 0, 1]
+  x += thDpp<0x141>(x); // row_half_mirror
+  x += thDpp<0x140>(x); // row_mirror
+  return x;
 }
 
 struct ThWaveLds
 {
   double geo[kNQP][10]; // P by columns (P[b][c] at c * 3 + b), JxW
-  double S[kNV][kNV];   // per-component block
+  double S[kNV][kNV];   // per-component block (reads of the padded rows and
+                        // columns 27..31 stay inside this struct and are
+                        // never stored)
   double x[kNPE * 3];
   double bcv[kN];
   double f[kN]; // lifting sums
   int flag[kN];
 };
+static_assert(offsetof(ThWaveLds, S) + sizeof(double) * (31 * kNV + 31) < sizeof(ThWaveLds), "padded S reads");
 
 template <typename T>
 __global__ void __launch_bounds__(64 * kThWaves, 2)
@@ -97,10 +128,10 @@ __global__ void __launch_bounds__(64 * kThWaves, 2)
 {
   using M = ThMfma<T>;
   using Acc = typename M::Acc;
-  __shared__ double sGeo[kNQ * kNPE * 3];     // mapping gradients [vertex][component][q]
-  __shared__ double sGrad[kNQP * (kNV + 1) * 3]; // reference gradients of the Q2 functions [q][a][b], a = 27 zero
-  __shared__ double sPhi[kNQP * kNV];         // Q2 values [q][a]
-  __shared__ double sPsi[kNQP * 16];          // Q1 pressure values [q][k] (k < 8)
+  __shared__ double sGeo[kNQ * kNPE * 3];        // mapping gradients [vertex][component][q]
+  __shared__ double sGrad[kNQP * (kNV + 1) * 3]; // Q2 reference gradients [q][a][b], a = 27 the zero function
+  __shared__ double sPhi[kNQP * (kNV + 1)];      // Q2 values [q][a], a = 27 zero
+  __shared__ double sPsi[kNQP * 16];             // Q1 pressure values [q][k] (zero past 8)
   __shared__ double sW[kNQ];
   __shared__ ThWaveLds lds[kThWaves];
   int const tid = threadIdx.x;
@@ -114,10 +145,10 @@ __global__ void __launch_bounds__(64 * kThWaves, 2)
     int const q = k / ((kNV + 1) * 3), ab = k - q * ((kNV + 1) * 3); // (a, b) of the first component's functions
     sGrad[k] = q < kNQ && ab / 3 < kNV ? gradG[(q * kN + ab / 3) * 3 + ab % 3] : 0.;
   }
-  for (int k = tid; k < kNQP * kNV; k += blockDim.x)
+  for (int k = tid; k < kNQP * (kNV + 1); k += blockDim.x)
   {
-    int const q = k / kNV, a = k - q * kNV;
-    sPhi[k] = q < kNQ ? phiG[q * kN + a] : 0.;
+    int const q = k / (kNV + 1), a = k - q * (kNV + 1);
+    sPhi[k] = q < kNQ && a < kNV ? phiG[q * kN + a] : 0.;
   }
   for (int k = tid; k < kNQP * 16; k += blockDim.x)
   {
@@ -132,10 +163,15 @@ __global__ void __launch_bounds__(64 * kThWaves, 2)
     W.geo[kNQP - 1][lane] = 0.; // the padded point
   __syncthreads();
   int const kr = lane >> 4, jc = lane & 15;
-  bool const doS = kS + kT != 0. || kM != 0.;
+  int const aHi = jc < kHi ? 16 + jc : kNV; // the lane's function in odd tiles (the zero function past 26)
+  int const lrow = M::row(0, kr);           // the lane's row offset within a tile
   // the element data (vertex coordinates; dof ids, flags and values of the
-  // two local dofs lane and lane + 64) is gathered one element ahead: its
-  // dependent loads are in flight while the previous element computes
+  // two local dofs lane and lane + 64) is gathered ahead: the ids two
+  // elements ahead, the values they point to one element ahead, both issued
+  // at the top of an element and waited for before its first store.  Vector
+  // memory operations complete in order, so a wait for a load issued after
+  // this element's stores (the top of the next element) would also wait for
+  // all of them.
   int64_t const stride = (int64_t)gridDim.x * kThWaves;
   struct Pre
   {
@@ -166,10 +202,31 @@ __global__ void __launch_bounds__(64 * kThWaves, 2)
       }
     }
   };
+  // physical gradients of the lane's functions at point q for the first NT
+  // tiles: gd[t] = G_q[a][t / 2] with a = 16 (t % 2) + jc
+  auto formG = [&](int q, double (&gd)[kVT], auto ntC)
+  {
+    constexpr int NT = decltype(ntC)::value;
+    double const * gl = sGrad + (q * (kNV + 1) + jc) * 3;
+    double const * gh = sGrad + (q * (kNV + 1) + aHi) * 3;
+    double const * pc = W.geo[q];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+    {
+      double const * gr = (t & 1) ? gh : gl;
+      int const c = t >> 1;
+      gd[t] = gr[0] * pc[3 * c] + gr[1] * pc[3 * c + 1] + gr[2] * pc[3 * c + 2];
+    }
+  };
   int64_t e = (int64_t)blockIdx.x * kThWaves + wave;
-  Pre cur, nxt;
+  Pre cur, nxt, ids2;
   gatherIds(e, cur);
   gatherVals(e, cur);
+  gatherIds(e + stride, nxt);
+  // (so that no load is pending at the loop head either)
+  asm volatile("" ::"v"(cur.x), "v"(cur.b0), "v"(cur.b1), "v"(cur.f0), "v"(cur.f1), "v"(nxt.pid), "v"(nxt.id0),
+               "v"(nxt.id1)
+               : "memory");
   for (; e < E; e += stride)
   {
     // ---- 1. element data
@@ -177,21 +234,21 @@ __global__ void __launch_bounds__(64 * kThWaves, 2)
       W.x[lane] = cur.x;
     W.flag[lane] = cur.f0;
     W.bcv[lane] = cur.f0 ? cur.b0 : 0.;
-    W.f[lane] = 0.;
     if (lane + 64 < kN)
     {
       W.flag[lane + 64] = cur.f1;
       W.bcv[lane + 64] = cur.f1 ? cur.b1 : 0.;
-      W.f[lane + 64] = 0.;
     }
+    bool const anyBC = __any(cur.f0 != 0) || __any(lane + 64 < kN && cur.f1 != 0);
+    gatherVals(e + stride, nxt);
+    gatherIds(e + 2 * stride, ids2);
     thSync();
-    gatherIds(e + stride, nxt);
     // ---- 2. geometry, one point per lane: P = (J^T J)^-1 J^T, JxW
     if (lane < kNQ)
     {
       int const q = lane;
       double J[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
-#pragma unroll 1
+#pragma unroll
       for (int v = 0; v < kNPE; ++v)
       {
         double gv[3];
@@ -234,213 +291,206 @@ __global__ void __launch_bounds__(64 * kThWaves, 2)
       W.geo[q][9] = sqrt(det) * sW[q];
     }
     thSync();
-    gatherVals(e + stride, nxt);
-    // physical gradient G_q[a][c] of Q2 function a at the point of the lane's
-    // row group
-    auto grad = [&](int q, int a, double const (&P)[3][3], int c) -> double
+    // ---- 3. S, the per-component block (functions a x j of one component),
+    //         K over (point, component) and the mass rows; in the same loop,
+    //         on elements with constrained dofs, the lifting sums of the T,
+    //         grad and div blocks: with h the constrained values and
+    //         U_q = sum_(e,j) G_q[j][e] h_(e,j), P_q = sum_k JxW_q psi_k h_k,
+    //         row (d, a) gets sum_q G_q[a][d] (k_t JxW_q U_q + k_g P_q) and
+    //         pressure row k gets k_d sum_q JxW_q psi_k U_q
+    double hv[kVT], hp = 0., lift[kVT], liftP = 0.;
+#pragma unroll
+    for (int t = 0; t < kVT; ++t)
     {
-      double const * g = sGrad + (q * (kNV + 1) + a) * 3;
-      return g[0] * P[0][c] + g[1] * P[1][c] + g[2] * P[2][c];
-    };
-    auto loadP = [&](int q, double (&P)[3][3]) -> double
+      hv[t] = (!(t & 1) || jc < kHi) ? W.bcv[thBase(t) + jc] : 0.;
+      lift[t] = 0.;
+    }
+    if (jc < kNP)
+      hp = W.bcv[3 * kNV + jc];
     {
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          P[a][c] = W.geo[q][c * 3 + a];
-      return W.geo[q][9];
-    };
-    // ---- 3. S: the per-component block (function a x function j), K over
-    //         (point, component) and the mass rows
-    if (doS)
-    {
-      Acc s[2][2];
-#pragma unroll
-      for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < 2; ++tj)
-          s[ti][tj] = Acc{0, 0, 0, 0};
+      Acc s00 = Acc{0, 0, 0, 0}, s01 = Acc{0, 0, 0, 0}, s11 = Acc{0, 0, 0, 0};
 #pragma unroll 1
       for (int qb = 0; qb < kNQP / 4; ++qb)
       {
         int const q = 4 * qb + kr;
-        double P[3][3];
-        double const jw = loadP(q, P);
-        double const cs = (kS + kT) * jw, cm = kM * jw;
-        T g[2][3], h[2][3], p[2], pm[2];
-#pragma unroll
-        for (int t = 0; t < 2; ++t)
-        {
-          int const a = 16 * t + jc, aa = a < kNV ? a : kNV - 1;
-          bool const v = a < kNV;
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-          {
-            double const gv = v ? grad(q, aa, P, c) : 0.;
-            g[t][c] = (T)gv;
-            h[t][c] = (T)(cs * gv);
-          }
-          double const ph = v ? sPhi[q * kNV + aa] : 0.;
-          p[t] = (T)ph;
-          pm[t] = (T)(cm * ph);
-        }
+        double gd[kVT];
+        formG(q, gd, std::integral_constant<int, kVT>{});
+        double const jw = W.geo[q][9];
+        double const cs = (kS + kT) * jw;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-            for (int ti = 0; ti <= tj; ++ti)
-              s[ti][tj] = M::mma(g[ti][c], h[tj][c], s[ti][tj]);
+        {
+          T const g0 = (T)gd[2 * c], g1 = (T)gd[2 * c + 1];
+          T const h0 = (T)(cs * gd[2 * c]), h1 = (T)(cs * gd[2 * c + 1]);
+          s00 = M::mma(g0, h0, s00);
+          s01 = M::mma(g0, h1, s01);
+          s11 = M::mma(g1, h1, s11);
+        }
         if (kM != 0.)
         {
+          double const cm = kM * jw;
+          double const p0 = sPhi[q * (kNV + 1) + jc], p1 = sPhi[q * (kNV + 1) + aHi];
+          s00 = M::mma((T)p0, (T)(cm * p0), s00);
+          s01 = M::mma((T)p0, (T)(cm * p1), s01);
+          s11 = M::mma((T)p1, (T)(cm * p1), s11);
+        }
+        if (anyBC)
+        {
+          double const ps = jw * sPsi[q * 16 + jc];
+          double u = 0.;
 #pragma unroll
-          for (int tj = 0; tj < 2; ++tj)
+          for (int t = 0; t < kVT; ++t)
+            u += gd[t] * hv[t];
+          u = thRowSum(u);
+          double const pp = thRowSum(ps * hp);
+          double const w = kT * jw * u + kG * pp;
 #pragma unroll
-            for (int ti = 0; ti <= tj; ++ti)
-              s[ti][tj] = M::mma(p[ti], pm[tj], s[ti][tj]);
+          for (int t = 0; t < kVT; ++t)
+            lift[t] += gd[t] * w;
+          liftP += ps * u;
         }
       }
+      // stage S (27 x 27, both triangles)
+      auto stage = [&](Acc const & s, int ti, int tj)
+      {
 #pragma unroll
-      for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-        for (int ti = 0; ti <= tj; ++ti)
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
+        for (int r = 0; r < 4; ++r)
+        {
+          int const a = 16 * ti + M::row(r, kr), j = 16 * tj + jc;
+          if (a < kNV && j < kNV)
           {
-            int const a = 16 * ti + M::row(r, kr), j = 16 * tj + jc;
-            if (a < kNV && j < kNV)
-            {
-              W.S[a][j] = (double)s[ti][tj][r];
-              if (ti != tj)
-                W.S[j][a] = (double)s[ti][tj][r];
-            }
+            W.S[a][j] = (double)s[r];
+            if (ti != tj)
+              W.S[j][a] = (double)s[r];
           }
+        }
+      };
+      stage(s00, 0, 0);
+      stage(s01, 0, 1);
+      stage(s11, 1, 1);
     }
-    else
+    if (anyBC)
     {
-      for (int k = lane; k < kNV * kNV; k += 64)
-        (&W.S[0][0])[k] = 0.;
+      // the four point groups' sums, then rows (t, jc) and pressure jc
+#pragma unroll
+      for (int t = 0; t < kVT; ++t)
+      {
+        lift[t] += __shfl_xor(lift[t], 16);
+        lift[t] += __shfl_xor(lift[t], 32);
+      }
+      liftP += __shfl_xor(liftP, 16);
+      liftP += __shfl_xor(liftP, 32);
+      if (kr == 0)
+      {
+#pragma unroll
+        for (int t = 0; t < kVT; ++t)
+          if (!(t & 1) || jc < kHi)
+            W.f[thBase(t) + jc] = lift[t];
+        if (jc < kNP)
+          W.f[3 * kNV + jc] = kD * liftP;
+      }
     }
     thSync();
+    if (anyBC)
+    {
+      // the per-component block's share: row (d, a) += sum_j S[a][j] h_(d, j)
+      for (int i = lane; i < 3 * kNV; i += 64)
+      {
+        int const d = (i >= kNV) + (i >= 2 * kNV), a = i - d * kNV;
+        double sum = 0.;
+#pragma unroll 9
+        for (int j = 0; j < kNV; ++j)
+          sum += W.S[a][j] * W.bcv[d * kNV + j];
+        W.f[i] += sum;
+      }
+    }
     // ---- 4. T: coupled velocity block (upper tiles) and the grad block, in
     //         three passes over column tiles (5 and the pressure tile, 3-4,
     //         0-2) so at most 12 accumulator tiles are live; each pass forms
     //         its operands again, runs its tiles and stores them (5.)
-    bool const anyBC = __any(lane < kN && W.flag[lane] != 0) || __any(lane + 64 < kN && W.flag[lane + 64] != 0);
     constexpr size_t ns = (size_t)keRowStride(kN);
     T * const out = ke + (size_t)e * kN * ns;
-    // per-lane operand rows of the six velocity tiles: function a and
-    // component d of tiled row 16 t + jc (padding: the zero function)
-    int gOff[6], pOff[6];
-#pragma unroll
-    for (int t = 0; t < 6; ++t)
-    {
-      int const r = 16 * t + jc;
-      bool const v = r < 3 * kNV;
-      int const d = v ? r / kNV : 0;
-      gOff[t] = (v ? r - d * kNV : kNV) * 3;
-      pOff[t] = d * 3;
-    }
-    int const lrow = M::row(0, kr); // a lane's row offset within a tile
+    int const sLane = lrow * kNV + jc; // the lane's entry of S tiles, less the tile offsets
     auto pass = [&]<int... Cs>(std::integer_sequence<int, Cs...>)
     {
       constexpr int cols[] = {Cs...}; // the pass's column tiles
       constexpr int NC = sizeof...(Cs);
-      constexpr int RT = std::max({(Cs < kPT ? Cs + 1 : kPT)...}); // row tiles 0..RT-1
+      constexpr int RT = std::max({(Cs < kPT ? Cs + 1 : kVT)...}); // row tiles 0..RT-1
       Acc acc[RT][NC];
 #pragma unroll
       for (int ti = 0; ti < RT; ++ti)
 #pragma unroll
         for (int u = 0; u < NC; ++u)
           acc[ti][u] = Acc{0, 0, 0, 0};
-      {
+      // no coefficient tests in here: a zero k_t gives zero operands h, zero
+      // k_g and k_d scale the pressure tile at the store
 #pragma unroll 1
-        for (int qb = 0; qb < kNQP / 4; ++qb)
+      for (int qb = 0; qb < kNQP / 4; ++qb)
+      {
+        int const q = 4 * qb + kr;
+        double gd[kVT];
+        formG(q, gd, std::integral_constant<int, RT>{});
+        double const jw = W.geo[q][9];
+        double const ct = kT * jw;
+        T g[RT];
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+          g[t] = (T)gd[t];
+        T const ps = (T)(jw * sPsi[q * 16 + jc]); // pressure column jc (zero past 8)
+#pragma unroll
+        for (int u = 0; u < NC; ++u)
         {
-          int const q = 4 * qb + kr;
-          double const jw = W.geo[q][9];
-          double const ct = kT * jw;
-          double const * gq = sGrad + q * ((kNV + 1) * 3);
-          T g[RT], h[RT];
-#pragma unroll
-          for (int t = 0; t < RT; ++t)
+          int const tj = cols[u];
+          if (tj < kPT)
           {
-            double const * gr = gq + gOff[t];
-            double const * pc = &W.geo[q][pOff[t]];
-            double const gv = gr[0] * pc[0] + gr[1] * pc[1] + gr[2] * pc[2];
-            g[t] = (T)gv;
-            h[t] = (T)(ct * gv);
+            T const h = (T)(ct * gd[tj < RT ? tj : 0]);
+#pragma unroll
+            for (int ti = 0; ti < RT; ++ti)
+              if (ti <= tj)
+                acc[ti][u] = M::mma(g[ti], h, acc[ti][u]);
           }
-          T const ps = (T)(jw * sPsi[q * 16 + jc]); // pressure column jc (zero past 8)
-          // no coefficient tests in here: a branch around the products makes
-          // the compiler move the accumulators out of and back into the
-          // matrix registers on every point block (a zero k_t gives zero
-          // operands h, zero k_g and k_d scale the pressure tile at the store)
-#pragma unroll
-          for (int u = 0; u < NC; ++u)
+          else
           {
-            int const tj = cols[u];
-            if (tj < kPT)
-            {
 #pragma unroll
-              for (int ti = 0; ti < RT; ++ti)
-                if (ti <= tj)
-                  acc[ti][u] = M::mma(g[ti], h[tj < RT ? tj : 0], acc[ti][u]);
-            }
-            else
-            {
-#pragma unroll
-              for (int ti = 0; ti < RT; ++ti)
-                acc[ti][u] = M::mma(g[ti], ps, acc[ti][u]);
-            }
+            for (int ti = 0; ti < RT; ++ti)
+              acc[ti][u] = M::mma(g[ti], ps, acc[ti][u]);
           }
         }
       }
-      // ---- 5. values, Dirichlet elimination, lifting, stores.  Tile (ti, tj):
-      //         value r of the lane at tiled (16 ti + row(r, kr), 16 tj + jc);
-      //         off-diagonal tiles are stored again mirrored (the pressure
-      //         column tile mirrored is the div block).  Rows of the velocity
-      //         tiles are local rows; the index arithmetic per entry is a
-      //         constant plus the lane's offsets.
+      // ---- 5. values, Dirichlet elimination, stores.  Tile (ti, tj): value
+      //         r of the lane at local (thBase(ti) + row(r, kr), thBase(tj) +
+      //         jc); tiles of one component get the S tile added; off-diagonal
+      //         tiles are stored again mirrored (the pressure column tile
+      //         mirrored is the div block).  Addresses are the lane's base
+      //         plus constants.
 #pragma unroll
       for (int u = 0; u < NC; ++u)
       {
         int const tj = cols[u];
-        int const j = tj < kPT ? 16 * tj + jc : 3 * kNV + jc; // local column
-        bool const jv = tj < kPT ? j < 3 * kNV : jc < kNP;
-        int const jj = jv ? j : 0;
-        int const dj = jj / kNV, aj = jj - dj * kNV; // (velocity columns)
-        int const fj = jv ? W.flag[jj] : 0;
-        double const bj = jv ? W.bcv[jj] : 0.;
-        T * const colBase = out + (size_t)lrow * ns + jj;    // + row(r, 0) + 16 ti rows
-        T * const mirBase = out + (size_t)jj * ns + lrow;    // + row(r, 0) + 16 ti columns
-        double colPart = 0.;
+        int const j = thBase(tj) + jc; // local column
+        bool const jv = tj < kPT ? (!(tj & 1) || jc < kHi) : jc < kNP;
+        int const fj = anyBC ? W.flag[tj < kPT || jv ? j : 0] : 0;
+        T * const colBase = out + (size_t)lrow * ns + j;
+        T * const mirBase = out + (size_t)j * ns + lrow;
 #pragma unroll
         for (int ti = 0; ti < RT; ++ti)
         {
           if (ti > tj)
             continue;
-          // does the tile meet a diagonal (component) block of S?
-          bool const nearDiag = tj < kPT && (16 * tj) / kNV <= (16 * ti + 15) / kNV && (16 * ti) / kNV <= (16 * tj + 15) / kNV;
+          bool const sameComp = tj < kPT && (ti >> 1) == (tj >> 1);
 #pragma unroll
           for (int r = 0; r < 4; ++r)
           {
-            int const i0 = 16 * ti + M::row(r, 0); // the entry's row less the lane offset
-            int const i = i0 + lrow;
-            bool const iv = i0 + M::row(0, 3) < 3 * kNV || i < 3 * kNV;
-            bool const valid = iv && jv;
+            int const i0 = thBase(ti) + M::row(r, 0); // the entry's row less the lane offset
+            int const rr = M::row(r, kr);
+            bool const valid = (!(ti & 1) || rr < kHi) && jv;
             double v = (double)acc[ti][u][r];
-            double vm = v; // the mirrored entry (j, i)
+            double vm;
             if (tj < kPT)
             {
-              if (nearDiag)
-              {
-                int const di = (i >= kNV) + (i >= 2 * kNV);
-                if (valid && di == dj)
-                  v += W.S[i - di * kNV][aj];
-                vm = v;
-              }
+              if (sameComp)
+                v += (&W.S[0][0])[sLane + (16 * (ti & 1) + M::row(r, 0)) * kNV + 16 * (tj & 1)];
+              vm = v;
             }
             else
             {
@@ -449,22 +499,10 @@ __global__ void __launch_bounds__(64 * kThWaves, 2)
             }
             if (anyBC)
             {
-              int const ii = iv ? i : 0;
-              v = valid ? v : 0.;
-              vm = valid ? vm : 0.;
-              // row i collects Ke_ij bcv_j, row j the mirrored Ke_ji bcv_i
-              double rp = v * bj;
-#pragma unroll
-              for (int off = 8; off > 0; off >>= 1)
-                rp += __shfl_xor(rp, off);
-              if (jc == 0 && iv && !W.flag[ii])
-                W.f[ii] += rp;
-              if (ti != tj)
-                colPart += vm * W.bcv[ii];
-              int const fi = W.flag[ii];
+              int const fi = W.flag[i0 + lrow];
               if (fi | fj)
               {
-                v = (fi && i == j) ? 1. : 0.;
+                v = (fi && ti == tj && rr == jc) ? 1. : 0.;
                 vm = 0.;
               }
             }
@@ -476,16 +514,12 @@ __global__ void __launch_bounds__(64 * kThWaves, 2)
             }
           }
         }
-        if (anyBC)
-        {
-          double cp = colPart;
-          cp += __shfl_xor(cp, 16);
-          cp += __shfl_xor(cp, 32);
-          if (kr == 0 && jv && !fj)
-            W.f[jj] += cp;
-        }
       }
     };
+    // the gathered data is taken here, before this element's first store
+    asm volatile("" ::"v"(nxt.x), "v"(nxt.b0), "v"(nxt.b1), "v"(nxt.f0), "v"(nxt.f1), "v"(ids2.pid), "v"(ids2.id0),
+                 "v"(ids2.id1)
+                 : "memory");
     pass(std::integer_sequence<int, 5, 6>{});
     pass(std::integer_sequence<int, 3, 4>{});
     pass(std::integer_sequence<int, 0, 1, 2>{});
@@ -495,11 +529,13 @@ __global__ void __launch_bounds__(64 * kThWaves, 2)
       int const i = 3 * kNV + k / kNP, j = 3 * kNV + k % kNP;
       out[(size_t)i * ns + j] = (T)((i == j && W.flag[i]) ? 1. : 0.);
     }
-    thSync();
     for (int i = lane; i < kN; i += 64)
       fe[e * kN + i] = W.flag[i] ? W.bcv[i] : (anyBC ? 0. - W.f[i] : 0.);
     thSync(); // the wave's LDS is rewritten by its next element
     cur = nxt;
+    nxt.pid = ids2.pid;
+    nxt.id0 = ids2.id0;
+    nxt.id1 = ids2.id1;
   }
 }
 
