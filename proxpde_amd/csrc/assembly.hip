@@ -1659,12 +1659,8 @@ void buildPattern(pxg_ctx * c)
 // PXG_TH_MFMA=0 keeps the Q2-Q1 hexahedron on the vector kernel (comparisons)
 bool thMfmaOn()
 {
-  static bool const on = []
-  {
-    char const * e = std::getenv("PXG_TH_MFMA");
-    return !(e && e[0] == '0');
-  }();
-  return on;
+  char const * e = std::getenv("PXG_TH_MFMA");
+  return !(e && e[0] == '0');
 }
 
 void assembleLhs(pxg_ctx * c, int nterms, pxg_term const * terms, int precision)

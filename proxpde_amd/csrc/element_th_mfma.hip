@@ -35,6 +35,7 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <cstdlib>
 #include <type_traits>
 #include <utility>
 
@@ -568,7 +569,11 @@ bool elementThMfma(pxg_ctx * c, double kS, double kT, double kM, double kG, doub
     int perCu = 0;
     PXG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kern, 64 * kThWaves, 0));
     int64_t const need = (c->nelem + kThWaves - 1) / kThWaves;
-    unsigned const grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)ncu * std::max(perCu, 1) * 4));
+    // PXG_TH_MFMA_BLOCKS caps the grid (a wavefront then takes many
+    // elements: the gathers ahead of the current element are exercised)
+    char const * cap = std::getenv("PXG_TH_MFMA_BLOCKS");
+    int64_t const most = cap && *cap ? std::max<int64_t>(1, std::atoll(cap)) : (int64_t)ncu * std::max(perCu, 1) * 4;
+    unsigned const grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(need, most));
     kern<<<grid, 64 * kThWaves, 0, c->stream>>>(c->nelem, c->xyz.p, c->conn.p, c->elemMap.p, c->bcFlag.p, c->bcVal.p,
                                                 c->hasBC ? 1 : 0, tab.d_geoGrad.p, tab.d_w.p, tab.d_phi.p, tab.d_grad.p,
                                                 kS, kT, kM, kG, kD, ke, c->feBuf.p, c->status.p);

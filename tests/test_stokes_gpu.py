@@ -102,6 +102,29 @@ def _assemble(prob, dofs, vals, coefs, precision=N.FP64):
 @pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{'x'.join(map(str, c[1]))}-{'il' if c[2] else 'blk'}"
                                               for c in CASES])
 def test_stokes_matches_oracle(oracle, gpu, case):
+    _check_case(oracle, case)
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c[0] == "hex_q2q1"],
+                         ids=[f"{c[0]}-{'x'.join(map(str, c[1]))}-{'il' if c[2] else 'blk'}"
+                              for c in CASES if c[0] == "hex_q2q1"])
+def test_stokes_hex_valu_kernel_matches_oracle(oracle, gpu, case, monkeypatch):
+    """The Q2-Q1 hexahedron's VALU element kernel (PXG_TH_MFMA=0; the matrix
+    core kernel is the default) to the same bar."""
+    monkeypatch.setenv("PXG_TH_MFMA", "0")
+    _check_case(oracle, case)
+
+
+@pytest.mark.parametrize("blocks", ["1", "3"])
+def test_stokes_hex_many_elements_per_wavefront(oracle, gpu, blocks, monkeypatch):
+    """The matrix core kernel with its grid capped to 1 or 3 blocks of 4
+    wavefronts: each wavefront walks 7 to 20 elements, its data gathered two
+    and one elements ahead."""
+    monkeypatch.setenv("PXG_TH_MFMA_BLOCKS", blocks)
+    _check_case(oracle, [c for c in CASES if c[0] == "hex_q2q1" and c[1] == (5, 4, 4)][0])
+
+
+def _check_case(oracle, case):
     pair, n, interleaved, pert, coefs, bcs, pins = case
     print(f"stokes case {pair} n={n} interleaved={interleaved} coefs={coefs}", flush=True)
     prob, o = _pair(oracle, n, interleaved, pert, pair)
