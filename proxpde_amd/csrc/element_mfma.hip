@@ -200,8 +200,14 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
   auto grab = [&]() -> int
   {
     int v = 0;
+    // the counter's address goes through a register the compiler takes as
+    // per-lane: an atomic on a uniform address is rewritten into one whose
+    // reply is waited for at once, and this one's reply is needed only
+    // before the element's stores
+    int z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
     if (lane == 0)
-      v = atomicAdd(ctr, 1);
+      v = atomicAdd(ctr + z, 1);
     return v;
   };
   // wave-uniform (scalar registers): lane 0's counter value
@@ -240,7 +246,7 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
       {
         rk = needK ? kappa[id] : 0.;
         rf = hasBC ? (int)bcFlag[id] : 0;
-        rb = rf ? bcVal[id] : 0.;
+        rb = hasBC ? bcVal[id] : 0.; // not dependent on the flag (no serialised load)
       }
     }
   };
@@ -249,9 +255,10 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
     gatherIds(e, pid, id);
     gatherVals(e, pid, id);
   }
+  // (no load pending at the loop head)
+  asm volatile("" ::"v"(rx), "v"(rk), "v"(rb), "v"(rf), "v"(rr) : "memory");
   while (e < E)
   {
-    int const ennReq = grab(); // element after next (lane 0)
     // ---- 1. element data -> LDS
     if (lane < NPE * 3)
       W.x[lane] = rx;
@@ -259,7 +266,7 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
     {
       W.kap[lane] = rk;
       W.flag[lane] = rf;
-      W.bcv[lane] = rb;
+      W.bcv[lane] = rf ? rb : 0.;
       W.f[lane] = 0.;
       if constexpr (PO)
         W.rp[lane] = rr;
@@ -371,8 +378,10 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
       }
     }
     waveSync();
-    // next element's values: their latency hides behind the matrix products
+    // next element's values and the element after next from the work
+    // counter: their latency hides behind the matrix products
     gatherVals(en, npid, nid);
+    int const ennReq = grab();
     // ---- 3. Ke tiles on the matrix cores (upper tiles only); the C operand
     //         C[4s + kr][16 tj + jc] is formed in registers per step
     Acc acc[NT][NT];
@@ -487,6 +496,12 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
         }
       }
     }
+    // the next element's gathered values and the work counter's reply are
+    // taken here, before this element's stores: vector memory operations
+    // complete in order, so a wait for them after the stores (at the top of
+    // the next element) would also wait for all of the stores
+    int64_t const enn = elem(ennReq);
+    asm volatile("" ::"v"(rx), "v"(rk), "v"(rb), "v"(rf), "v"(rr) : "memory");
     // ---- 4. the upper tiles go from the accumulators straight to the
     //         row-major store (the off-diagonal one mirrored): tile (ti, tj),
     //         value r of a lane is D[i][j], i = 16 ti + row(r, lane/16),
@@ -599,7 +614,7 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
       fe[rowOf(lane)] = anyBC ? (W.flag[lane] ? W.bcv[lane] : 0. - W.f[lane]) : 0.;
     waveSync(); // LDS of this element is reused by the next one
     e = en;
-    en = elem(ennReq);
+    en = enn;
   }
 }
 
