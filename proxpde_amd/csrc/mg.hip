@@ -96,10 +96,18 @@ struct pxg_mg
   // orders the cycle stream after work issued on the level contexts' own
   // streams (value casts, packed copies made on first use)
   hipEvent_t join = nullptr;
+  // residual norms read one cycle late: copied to pinned host memory behind
+  // an event, read while the next preconditioner cycle runs
+  double * hNorm = nullptr;
+  hipEvent_t normEv = nullptr;
   ~pxg_mg()
   {
     if (join)
       (void)hipEventDestroy(join);
+    if (normEv)
+      (void)hipEventDestroy(normEv);
+    if (hNorm)
+      (void)hipHostFree(hNorm);
   }
 };
 
@@ -299,6 +307,30 @@ double readScalar(pxg_mg * m, int i)
   PXG_HIP(hipStreamSynchronize(m->s));
   return h;
 }
+
+// an asynchronous read of scalar i (takeScalar waits for it): the host
+// reads the norm of one iteration while the stream runs the cycle after it,
+// so the stream is not left empty while the host decides
+void postScalar(pxg_mg * m, int i)
+{
+  if (!m->hNorm)
+    PXG_HIP(hipHostMalloc((void **)&m->hNorm, sizeof(double), hipHostMallocDefault));
+  if (!m->normEv)
+    PXG_HIP(hipEventCreateWithFlags(&m->normEv, hipEventDisableTiming));
+  PXG_HIP(hipMemcpyAsync(m->hNorm, m->sc.p + i, sizeof(double), hipMemcpyDeviceToHost, m->s));
+  PXG_HIP(hipEventRecord(m->normEv, m->s));
+}
+
+double takeScalar(pxg_mg * m)
+{
+  PXG_HIP(hipEventSynchronize(m->normEv));
+  return *m->hNorm;
+}
+
+// the residual norm expected of the next iteration from the last two (the
+// cycle's contraction); a test that this may pass is made before the cycle
+double nextNormGuess(double r1, double r2) { return r2 > 0. ? r1 * (r1 / r2) : r1; }
+constexpr double kLagSafety = 3.;
 
 void copyScalar(pxg_mg * m, int dst, int src)
 {
@@ -774,6 +806,11 @@ int pcg(pxg_mg * m, T const * b, T * x, double rtol, int maxit, double & relres,
   vcycle<T>(m, 0, r, z);
   PXG_HIP(hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, s));
   dot<T>(n, r, z, sc + 2, m->partial.p, s);
+  // the norm of an iteration is read after its next cycle is queued (the
+  // cycle changes z, p and scalars only, so x is still that iterate when the
+  // test passes); it is read before the cycle when the contraction so far
+  // says the test may pass, which saves the cycle
+  double r1 = rn, r2 = -1.;
   int it = 0;
   while (it < maxit)
   {
@@ -782,15 +819,31 @@ int pcg(pxg_mg * m, T const * b, T * x, double rtol, int maxit, double & relres,
     cgUpdate<T>(n, x, p, r, q, sc, 2, 3, s);
     ++it;
     dot<T>(n, r, r, sc + 1, m->partial.p, s);
-    rn = std::sqrt(readScalar(m, 1));
-    PXG_REQUIRE(std::isfinite(rn), PXG_E_NUMERIC, "CG breakdown (non-finite residual)");
-    relres = rn / bn;
-    if (relres <= rtol)
-      break;
+    bool const lag = nextNormGuess(r1, r2) > kLagSafety * rtol * bn;
+    if (!lag)
+    {
+      rn = std::sqrt(readScalar(m, 1));
+      PXG_REQUIRE(std::isfinite(rn), PXG_E_NUMERIC, "CG breakdown (non-finite residual)");
+      relres = rn / bn;
+      if (relres <= rtol)
+        break;
+    }
+    else
+      postScalar(m, 1);
     vcycle<T>(m, 0, r, z);
     dot<T>(n, r, z, sc + 4, m->partial.p, s);
     cgDirection<T>(n, p, z, sc, 4, 2, s);
     copyScalar(m, 2, 4);
+    if (lag)
+    {
+      rn = std::sqrt(takeScalar(m));
+      PXG_REQUIRE(std::isfinite(rn), PXG_E_NUMERIC, "CG breakdown (non-finite residual)");
+      relres = rn / bn;
+      if (relres <= rtol)
+        break;
+    }
+    r2 = r1;
+    r1 = rn;
   }
   return it;
 }
@@ -850,12 +903,17 @@ int solveIR(pxg_mg * m, double const * b, double * x, double rtol, int maxit, do
   }
   // true residual of x: its norm (the convergence measure) and its FP32
   // rounding (the CG residual from here on)
-  auto trueResidual = [&]() -> double
+  auto trueNorm = [&]() -> double
   {
     residual64(m, b, x, r64);
     dot<double>(n, r64, r64, sc + 6, m->partial.p, s);
     double const rn = std::sqrt(readScalar(m, 6));
     PXG_REQUIRE(std::isfinite(rn), PXG_E_NUMERIC, "non-finite residual in refinement");
+    return rn;
+  };
+  auto trueResidual = [&]() -> double
+  {
+    double const rn = trueNorm();
     convert<float, double>(n, r64, r, s);
     return rn;
   };
@@ -866,6 +924,14 @@ int solveIR(pxg_mg * m, double const * b, double * x, double rtol, int maxit, do
   vcycle<float>(m, 0, r, z);
   PXG_HIP(hipMemcpyAsync(p, z, n * sizeof(float), hipMemcpyDeviceToDevice, s));
   dot<float>(n, r, z, sc + 2, m->partial.p, s);
+  // The recursive norm of an iteration is read after its next cycle is
+  // queued (see pcg), unless the contraction so far says that it may call
+  // for the true residual; a replacement found late waits for the next
+  // iteration (it must come before a cycle), a convergence test found late
+  // is made at once on the true norm (x is still that iterate)
+  auto wanted = [&](double rr) { return rr <= rtol * bn || rr <= kReplace * last; };
+  double r1 = last, r2 = -1.;
+  bool replaceNext = false;
   int it = 0;
   while (it < maxit)
   {
@@ -874,19 +940,47 @@ int solveIR(pxg_mg * m, double const * b, double * x, double rtol, int maxit, do
     cgUpdateMixed(n, x, p, r, q, sc, 2, 3, s);
     ++it;
     dot<float>(n, r, r, sc + 1, m->partial.p, s);
-    double const rr = std::sqrt(readScalar(m, 1));
-    PXG_REQUIRE(std::isfinite(rr), PXG_E_NUMERIC, "CG breakdown (non-finite residual)");
-    if (rr <= rtol * bn || rr <= kReplace * last)
+    bool const lag = !replaceNext && !wanted(nextNormGuess(r1, r2) / kLagSafety);
+    double rr = 0.;
+    if (!lag)
     {
-      last = trueResidual();
-      relres = last / bn;
-      if (relres <= rtol)
-        break;
+      rr = std::sqrt(readScalar(m, 1));
+      PXG_REQUIRE(std::isfinite(rr), PXG_E_NUMERIC, "CG breakdown (non-finite residual)");
+      if (replaceNext || wanted(rr))
+      {
+        replaceNext = false;
+        last = trueResidual();
+        relres = last / bn;
+        if (relres <= rtol)
+          break;
+      }
     }
+    else
+      postScalar(m, 1);
     vcycle<float>(m, 0, r, z);
     dot<float>(n, r, z, sc + 4, m->partial.p, s);
     cgDirection<float>(n, p, z, sc, 4, 2, s);
     copyScalar(m, 2, 4);
+    if (lag)
+    {
+      rr = std::sqrt(takeScalar(m));
+      PXG_REQUIRE(std::isfinite(rr), PXG_E_NUMERIC, "CG breakdown (non-finite residual)");
+      if (rr <= rtol * bn)
+      {
+        double const tn = trueNorm();
+        if (tn <= rtol * bn)
+        {
+          last = tn;
+          relres = tn / bn;
+          break;
+        }
+        replaceNext = true;
+      }
+      else if (rr <= kReplace * last)
+        replaceNext = true;
+    }
+    r2 = r1;
+    r1 = rr;
   }
   return it;
 }
