@@ -23,6 +23,7 @@
 // One wavefront per element; kWaves wavefronts per block share the constant
 // tables in LDS; waves take elements from a shared counter.
 #include "kernels.hpp"
+#include "wave.hpp"
 
 #include <algorithm>
 #include <type_traits>
@@ -575,10 +576,7 @@ __global__ void __launch_bounds__(64 * kWaves, 4)
             int const ii = i < N ? i : N - 1;
             double const dv = valid ? (double)v : 0.;
             // row i collects Ke_ij bcv_j (j >= i), row j collects Ke_ij bcv_i (j > i)
-            double rp = dv * bj;
-#pragma unroll
-            for (int off = 8; off > 0; off >>= 1)
-              rp += __shfl_xor(rp, off);
+            double const rp = rowSum16(dv * bj);
             if (jc == 0 && i < N)
               W.f[i] += rp;
             if (j > i)

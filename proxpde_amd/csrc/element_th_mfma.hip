@@ -32,6 +32,7 @@
 // the product blocks, rows of S for the per-component block), in a fixed
 // order.
 #include "kernels.hpp"
+#include "wave.hpp"
 
 #include <algorithm>
 #include <cstddef>
@@ -82,27 +83,6 @@ __device__ inline void thSync()
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// a double moved across lanes by a DPP pattern within rows of 16 lanes
-template <int CTRL>
-__device__ inline double thDpp(double x)
-{
-  unsigned long long const b = (unsigned long long)__double_as_longlong(x);
-  int const lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
-  int const hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-
-// sum over the 16 lanes of a row (pairs, quads, halves, rows: every lane
-// gets the same bits, IEEE addition being commutative)
-__device__ inline double thRowSum(double x)
-{
-  x += thDpp<0xB1>(x);  // quad_perm [1, 0, 3, 2]
-  x += thDpp<0x4E>(x);  // quad_perm [2, 3, 0, 1]
-  x += thDpp<0x141>(x); // row_half_mirror
-  x += thDpp<0x140>(x); // row_mirror
-  return x;
 }
 
 struct ThWaveLds
@@ -342,8 +322,8 @@ __global__ void __launch_bounds__(64 * kThWaves, 2)
 #pragma unroll
           for (int t = 0; t < kVT; ++t)
             u += gd[t] * hv[t];
-          u = thRowSum(u);
-          double const pp = thRowSum(ps * hp);
+          u = rowSum16(u);
+          double const pp = rowSum16(ps * hp);
           double const w = kT * jw * u + kG * pp;
 #pragma unroll
           for (int t = 0; t < kVT; ++t)
