@@ -680,6 +680,138 @@ __global__ void k_pad_ids(int64_t m, int n, int ns, uint32_t * __restrict__ ids)
   ids[t] = r * (uint32_t)ns + (id - r * (uint32_t)n);
 }
 
+// ---- row-wise pattern build.  The pairs of row r (dofContrib[dofSeg[r] ..
+// dofSeg[r + 1]), element order) give its candidates: candidate k = p n + j
+// of pair p = e n + i is column emap[e n + j] and contribution
+// e n^2 + i n + j.  Sorted by (column, k) in LDS, one wavefront per row,
+// they are the row's contributions in the order of the sort of all entries
+// by (row, column, element), and the first of each column is a structural
+// entry.  Rows with more than kRowCand candidates (or a dof in no element)
+// set *bad and the sort of all entries runs instead.
+constexpr int kRowCand = 1024;
+constexpr int kRowWaves = 4;
+
+__device__ inline void rowSync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void __launch_bounds__(64 * kRowWaves)
+    k_row_sort(int64_t nr, int n, int maxCand, uint32_t const * __restrict__ dofSeg,
+               uint32_t const * __restrict__ dofContrib, uint32_t const * __restrict__ emap,
+               uint32_t * __restrict__ contrib, uint32_t * __restrict__ colTmp, uint32_t * __restrict__ nnzRow,
+               int * __restrict__ bad)
+{
+  __shared__ uint64_t keys[kRowWaves][kRowCand];
+  int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int64_t const r = (int64_t)blockIdx.x * kRowWaves + wave;
+  if (r >= nr) // (whole wavefronts)
+    return;
+  uint64_t * const K = keys[wave];
+  uint32_t const s0 = dofSeg[r], s1 = dofSeg[r + 1];
+  int64_t const cnt = (int64_t)(s1 - s0) * n;
+  if (s1 <= s0 || cnt > maxCand)
+  {
+    if (lane == 0)
+      atomicOr(bad, 1);
+    return;
+  }
+  int size = 64;
+  while (size < cnt)
+    size <<= 1;
+  for (int k = lane; k < size; k += 64)
+  {
+    uint64_t key = ~0ull;
+    if (k < cnt)
+    {
+      int const p = k / n, j = k - p * n;
+      uint32_t const e = dofContrib[s0 + p] / (uint32_t)n;
+      key = ((uint64_t)emap[(size_t)e * n + j] << 32) | (uint32_t)k;
+    }
+    K[k] = key;
+  }
+  rowSync();
+  // bitonic network, ascending: comparator k joins lo = 2k - (k mod st) and
+  // lo + st, descending where lo has the bit len set
+  for (int len = 2; len <= size; len <<= 1)
+    for (int st = len >> 1; st > 0; st >>= 1)
+    {
+      for (int k = lane; k < size / 2; k += 64)
+      {
+        int const lo = 2 * k - (k & (st - 1)), hi = lo + st;
+        bool const up = (lo & len) == 0;
+        uint64_t const a = K[lo], b = K[hi];
+        if ((a > b) == up)
+        {
+          K[lo] = b;
+          K[hi] = a;
+        }
+      }
+      rowSync();
+    }
+  uint32_t const base = s0 * (uint32_t)n, nn = (uint32_t)n * (uint32_t)n;
+  uint32_t heads = 0;
+  for (int k = lane; k < cnt; k += 64)
+  {
+    uint64_t const key = K[k];
+    uint32_t const col = (uint32_t)(key >> 32), kk = (uint32_t)key;
+    uint32_t const p = kk / (uint32_t)n, j = kk - p * (uint32_t)n;
+    uint32_t const pr = dofContrib[s0 + p], e = pr / (uint32_t)n, i = pr - e * (uint32_t)n;
+    contrib[base + k] = e * nn + i * (uint32_t)n + j;
+    colTmp[base + k] = col;
+    heads += (k == 0 || (uint32_t)(K[k - 1] >> 32) != col) ? 1u : 0u;
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    heads += __shfl_xor(heads, o);
+  if (lane == 0)
+    nnzRow[r] = heads;
+}
+
+// structural entries of each row from its sorted candidates (one wavefront
+// per row): columns, first contributions, and the entry of every element-
+// matrix entry
+__global__ void __launch_bounds__(64 * kRowWaves)
+    k_row_fill(int64_t nr, int n, uint32_t const * __restrict__ dofSeg, uint32_t const * __restrict__ colTmp,
+               uint32_t const * __restrict__ contrib, int64_t const * __restrict__ rowptrS,
+               uint32_t * __restrict__ segPtr, int32_t * __restrict__ colS, uint32_t * __restrict__ emapNnz)
+{
+  int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int64_t const r = (int64_t)blockIdx.x * kRowWaves + wave;
+  if (r >= nr)
+    return;
+  uint32_t const s0 = dofSeg[r], s1 = dofSeg[r + 1];
+  int64_t const cnt = (int64_t)(s1 - s0) * n;
+  uint32_t const base = s0 * (uint32_t)n;
+  int64_t running = rowptrS[r] - 1; // entry of the last head before this chunk
+  for (int64_t k0 = 0; k0 < cnt; k0 += 64)
+  {
+    int64_t const k = k0 + lane;
+    bool const in = k < cnt;
+    uint32_t col = 0;
+    bool head = false;
+    if (in)
+    {
+      col = colTmp[base + k];
+      head = k == 0 || colTmp[base + k - 1] != col;
+    }
+    uint64_t const mask = __ballot(head);
+    uint64_t const below = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    int64_t const entry = running + __popcll(mask & below);
+    if (in)
+    {
+      if (head)
+      {
+        colS[entry] = (int32_t)col;
+        segPtr[entry] = base + (uint32_t)k;
+      }
+      emapNnz[contrib[base + k]] = (uint32_t)entry;
+    }
+    running += __popcll(mask);
+  }
+}
+
 __global__ void k_dof_keys(int64_t m, uint32_t const * __restrict__ emap, uint32_t * __restrict__ keys,
                            uint32_t * __restrict__ ids)
 {
@@ -1562,6 +1694,8 @@ void buildDofAdjacency(pxg_ctx * c)
     ids.resize(mn);
     c->dofContrib.resize(mn);
     c->dofSeg.resize(c->ndofs + 1);
+    // (a dof in no element keeps ~0u, which the row-wise pattern build rejects)
+    PXG_HIP(hipMemsetAsync(c->dofSeg.p, 0xff, (c->ndofs + 1) * sizeof(uint32_t), s));
     k_dof_keys<<<gridFor(mn), kBlock, 0, s>>>(mn, c->elemMap.p, keys.p, ids.p);
     int eb = 1;
     while (eb < 32 && ((uint64_t)1 << eb) < (uint64_t)c->ndofs)
@@ -1581,6 +1715,10 @@ void buildDofAdjacency(pxg_ctx * c)
   c->sortTmp.release();
 }
 
+bool patternRowsOn();
+bool buildPatternRows(pxg_ctx * c);
+void finishPattern(pxg_ctx * c);
+
 void buildPattern(pxg_ctx * c)
 {
   int const n = c->fe.n;
@@ -1590,6 +1728,12 @@ void buildPattern(pxg_ctx * c)
               "too many element-matrix entries for 32-bit maps");
   PXG_REQUIRE(c->ndofs > 0 && (double)c->ndofs * (double)c->ndofs < 1.8e19, PXG_E_INVALID, "dof count");
   hipStream_t s = c->stream;
+  buildDofAdjacency(c);
+  if (patternRowsOn() && buildPatternRows(c))
+  {
+    finishPattern(c);
+    return;
+  }
   int endBit = 1;
   while (endBit < 64 && ((uint64_t)1 << endBit) < (uint64_t)c->ndofs * (uint64_t)c->ndofs)
     ++endBit;
@@ -1639,7 +1783,82 @@ void buildPattern(pxg_ctx * c)
     PXG_HIP(hipMemcpyAsync(c->rowptrS.p + c->ndofs, &nz, sizeof(int64_t), hipMemcpyHostToDevice, s));
     PXG_HIP(hipStreamSynchronize(s));
   }
-  buildDofAdjacency(c);
+  finishPattern(c);
+}
+
+// PXG_PATTERN=sort builds every pattern with the sort of all entries;
+// PXG_PATTERN_ROW_MAX lowers the row-wise build's candidate bound (tests:
+// rows past it send the build to the sort of all entries)
+bool patternRowsOn()
+{
+  char const * e = std::getenv("PXG_PATTERN");
+  return !(e && std::string(e) == "sort");
+}
+
+int patternRowMax()
+{
+  char const * e = std::getenv("PXG_PATTERN_ROW_MAX");
+  int const v = e && *e ? std::atoi(e) : kRowCand;
+  return std::max(1, std::min(v, kRowCand));
+}
+
+// the row-wise pattern build (k_row_sort, k_row_fill) from the dof
+// adjacency; false (nothing changed but scratch) when a row is too long
+bool buildPatternRows(pxg_ctx * c)
+{
+  int const n = c->fe.n;
+  int64_t const E = c->nelem, nr = c->ndofs, m = E * (int64_t)n * n;
+  int const ns = keRowStride(n);
+  hipStream_t s = c->stream;
+  DevBuf<uint32_t> colTmp, nnzRow;
+  DevBuf<int> bad;
+  colTmp.resize(m);
+  nnzRow.resize(nr + 1);
+  bad.resize(1);
+  c->contrib.resize(m);
+  c->rowptrS.resize(nr + 1);
+  k_set_int<<<1, 1, 0, s>>>(bad.p, 0);
+  unsigned const g = (unsigned)((nr + kRowWaves - 1) / kRowWaves);
+  k_row_sort<<<g, 64 * kRowWaves, 0, s>>>(nr, n, patternRowMax(), c->dofSeg.p, c->dofContrib.p, c->elemMap.p,
+                                          c->contrib.p, colTmp.p, nnzRow.p, bad.p);
+  PXG_HIP(hipGetLastError());
+  int hb = 1;
+  PXG_HIP(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  PXG_HIP(hipStreamSynchronize(s));
+  if (hb)
+    return false;
+  // rowptr = exclusive sum of the per-row counts (count nr = 0: rowptr[nr] = nnz)
+  PXG_HIP(hipMemsetAsync(nnzRow.p + nr, 0, sizeof(uint32_t), s));
+  size_t tb = 0;
+  PXG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, nnzRow.p, c->rowptrS.p, (int)(nr + 1), s));
+  c->sortTmp.resize(std::max(tb, c->sortTmp.n));
+  PXG_HIP(hipcub::DeviceScan::ExclusiveSum(c->sortTmp.p, tb, nnzRow.p, c->rowptrS.p, (int)(nr + 1), s));
+  int64_t nz = 0;
+  PXG_HIP(hipMemcpyAsync(&nz, c->rowptrS.p + nr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  PXG_HIP(hipStreamSynchronize(s));
+  PXG_REQUIRE(nz > 0 && nz < ((int64_t)1 << 32), PXG_E_INVALID, "structural entry count");
+  c->nnzS = nz;
+  c->segPtr.resize(nz + 1);
+  c->colS.resize(nz);
+  c->emapNnz.resize(m);
+  k_row_fill<<<g, 64 * kRowWaves, 0, s>>>(nr, n, c->dofSeg.p, colTmp.p, c->contrib.p, c->rowptrS.p, c->segPtr.p,
+                                          c->colS.p, c->emapNnz.p);
+  PXG_HIP(hipGetLastError());
+  if (ns != n)
+  {
+    k_pad_ids<<<gridFor(m), kBlock, 0, s>>>(m, n, ns, c->contrib.p);
+    PXG_HIP(hipGetLastError());
+  }
+  uint32_t const mm = (uint32_t)m;
+  PXG_HIP(hipMemcpyAsync(c->segPtr.p + nz, &mm, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  PXG_HIP(hipStreamSynchronize(s));
+  return true;
+}
+
+// what every pattern build ends with
+void finishPattern(pxg_ctx * c)
+{
+  hipStream_t s = c->stream;
   c->sortTmp.release();
   c->diagPos.resize(c->ndofs);
   k_diag_pos<<<gridFor(c->ndofs), kBlock, 0, s>>>(c->ndofs, c->rowptrS.p, c->colS.p, c->diagPos.p);

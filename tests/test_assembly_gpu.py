@@ -121,6 +121,51 @@ def test_repeat_is_bitwise_deterministic(oracle, gpu):
         assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
 
 
+@pytest.mark.parametrize("fe,mesh,n", [("hex_q2", "cube_hex", (5, 4, 3)), ("hex_q1", "cube_hex", (9, 7, 5)),
+                                       ("tet_p2", "cube_tet5", (4, 3, 3)), ("quad_q2", "square", (11, 7)),
+                                       ("tet_p2", "gmsh", None)])
+def test_row_wise_pattern_equals_sort_of_all_entries(gpu, fe, mesh, n, monkeypatch):
+    """The pattern built row by row from the dof adjacency (one wavefront per
+    row sorting its candidate columns) against the sort of all entries by
+    (row, column, element) (PXG_PATTERN=sort), and against the row-wise build
+    sent to that sort by a row bound below the longest row
+    (PXG_PATTERN_ROW_MAX): first build and rebuild give the same matrix and
+    right-hand side bit for bit."""
+    import os
+    if mesh == "gmsh":
+        prob = W.Problem(fe, (0, 0, 0), gmsh=os.path.join(os.path.dirname(__file__), "golden", "cube_uns.msh"))
+        dofs = prob.bc_dofs(("left", "front"))
+    else:
+        prob = W.Problem(fe, n, mesh=mesh)
+        dofs = prob.all_sides()
+    x = prob.dof_coords()
+    kappa = np.exp(W.grf(x, seed=5, std=0.5, ell=0.3, features=64))
+    vals = _bc_values(x)[dofs]
+    out = {}
+    for mode in ("sort", "rows", "rows-bounded"):
+        monkeypatch.setenv("PXG_PATTERN", "sort" if mode == "sort" else "rows")
+        if mode == "rows-bounded":
+            monkeypatch.setenv("PXG_PATTERN_ROW_MAX", str(prob.emap.shape[1] * 2))
+        else:
+            monkeypatch.delenv("PXG_PATTERN_ROW_MAX", raising=False)
+        a = prob.assembler()
+        a.set_dirichlet(dofs, vals)
+        a.set_coefficient(kappa)
+        a.build_pattern()
+        builds = []
+        for _ in range(2):
+            a.assemble([("stiffness_fe", 1.0), ("mass", 0.5)])
+            a.assemble_rhs("const", (1.0,))
+            a.close()
+            builds.append(a.csr())
+            a.clear()
+        out[mode] = builds
+    for mode in ("rows", "rows-bounded"):
+        for k in range(2):
+            for x_, y_ in zip(out["sort"][k], out[mode][k]):
+                assert np.array_equal(np.asarray(x_).view(np.uint8), np.asarray(y_).view(np.uint8)), (mode, k)
+
+
 def test_close_uses_up_the_build(oracle, gpu):
     """closeMatrix consumes the triplets (reference include/builder.hpp:366-371,
     _triplets.clear()) while b keeps accumulating: build(stiffness) -> close ->
