@@ -705,6 +705,7 @@ __global__ void __launch_bounds__(64 * kRowWaves)
                int * __restrict__ bad)
 {
   __shared__ uint64_t keys[kRowWaves][kRowCand];
+  __shared__ uint32_t pairE[kRowWaves][kRowCand / 4], pairI[kRowWaves][kRowCand / 4];
   int const wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int64_t const r = (int64_t)blockIdx.x * kRowWaves + wave;
   if (r >= nr) // (whole wavefronts)
@@ -712,25 +713,40 @@ __global__ void __launch_bounds__(64 * kRowWaves)
   uint64_t * const K = keys[wave];
   uint32_t const s0 = dofSeg[r], s1 = dofSeg[r + 1];
   int64_t const cnt = (int64_t)(s1 - s0) * n;
-  if (s1 <= s0 || cnt > maxCand)
+  if (s1 <= s0 || cnt > maxCand || s1 - s0 > kRowCand / 4)
   {
     if (lane == 0)
       atomicOr(bad, 1);
     return;
   }
+  int const np = (int)(s1 - s0);
+  // element and local row of each pair (one division per pair)
+  for (int p = lane; p < np; p += 64)
+  {
+    uint32_t const pr = dofContrib[s0 + p], e = pr / (uint32_t)n;
+    pairE[wave][p] = e;
+    pairI[wave][p] = pr - e * (uint32_t)n;
+  }
+  rowSync();
   int size = 64;
   while (size < cnt)
     size <<= 1;
+  // candidate k = p n + j of the lane: (p, j) advanced by 64 = q64 n + r64
+  int const q64 = 64 / n, r64 = 64 - q64 * n;
+  int p = lane / n, j = lane - (lane / n) * n;
   for (int k = lane; k < size; k += 64)
   {
     uint64_t key = ~0ull;
-    if (k < cnt)
-    {
-      int const p = k / n, j = k - p * n;
-      uint32_t const e = dofContrib[s0 + p] / (uint32_t)n;
-      key = ((uint64_t)emap[(size_t)e * n + j] << 32) | (uint32_t)k;
-    }
+    if (k < cnt) // low half (p, j) in the candidates' order: p < 256, j < 1024
+      key = ((uint64_t)emap[(size_t)pairE[wave][p] * n + j] << 32) | ((uint32_t)p << 10) | (uint32_t)j;
     K[k] = key;
+    p += q64;
+    j += r64;
+    if (j >= n)
+    {
+      j -= n;
+      ++p;
+    }
   }
   rowSync();
   // bitonic network, ascending: comparator k joins lo = 2k - (k mod st) and
@@ -756,10 +772,8 @@ __global__ void __launch_bounds__(64 * kRowWaves)
   for (int k = lane; k < cnt; k += 64)
   {
     uint64_t const key = K[k];
-    uint32_t const col = (uint32_t)(key >> 32), kk = (uint32_t)key;
-    uint32_t const p = kk / (uint32_t)n, j = kk - p * (uint32_t)n;
-    uint32_t const pr = dofContrib[s0 + p], e = pr / (uint32_t)n, i = pr - e * (uint32_t)n;
-    contrib[base + k] = e * nn + i * (uint32_t)n + j;
+    uint32_t const col = (uint32_t)(key >> 32), pk = ((uint32_t)key) >> 10, jk = (uint32_t)key & 1023u;
+    contrib[base + k] = pairE[wave][pk] * nn + pairI[wave][pk] * (uint32_t)n + jk;
     colTmp[base + k] = col;
     heads += (k == 0 || (uint32_t)(K[k - 1] >> 32) != col) ? 1u : 0u;
   }
@@ -1807,6 +1821,8 @@ int patternRowMax()
 bool buildPatternRows(pxg_ctx * c)
 {
   int const n = c->fe.n;
+  if (n > 1023)
+    return false; // (the candidates' order keys hold j in 10 bits)
   int64_t const E = c->nelem, nr = c->ndofs, m = E * (int64_t)n * n;
   int const ns = keRowStride(n);
   hipStream_t s = c->stream;
